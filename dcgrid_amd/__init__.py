@@ -1,0 +1,77 @@
+"""dcgrid_amd — MI355X (gfx950) DCGrid / uniform-grid simulation step.
+
+Host-side mirror of the reference's step interface (src/fluid_simulation.h:4-26) over the C ABI
+of libdcgrid_amd.so (include/dcgrid_amd.h). The HIP library is required: importing works
+without it, but creating a simulation raises if it has not been built.
+"""
+import ctypes as C
+import os
+
+from ._abi import (  # noqa: F401
+    DCG_OK, DCGError, FIELDS, Info, Params, PROJECT_LOCAL, PROJECT_MULTILEVEL, SCORE_FEATURES,
+    SCORE_GEOMETRIC, SOURCE_NONE, SOURCE_SPHERE_BUOYANT, K_NAMES, Handle, bind, hash_map,
+)
+from . import scenes  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdcgrid_amd.so")
+
+_lib = None
+_fns = None
+
+
+def library():
+    """Load libdcgrid_amd.so (built in-tree by __graft_entry__.build()); raise if absent."""
+    global _lib, _fns
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        lib = C.CDLL(LIB_PATH)
+        fns = bind(lib, "dcg_")
+        lib.dcg_default_params.argtypes = [C.POINTER(Params)]
+        lib.dcg_default_params.restype = None
+        lib.dcg_abi_version.argtypes = []
+        lib.dcg_abi_version.restype = C.c_int32
+        lib.dcg_synchronize.argtypes = [C.c_void_p]
+        lib.dcg_synchronize.restype = C.c_int
+        lib.dcg_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        lib.dcg_set_timing.restype = C.c_int
+        lib.dcg_timing_get.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+        lib.dcg_timing_get.restype = C.c_int
+        lib.dcg_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        lib.dcg_get_params.restype = C.c_int
+        _lib, _fns = lib, fns
+    return _lib
+
+
+def default_params():
+    """Reference defaults (src/data/sim_params.cpp:4-17, rdx = 1/dx as in src/main.cpp:17)."""
+    library()
+    p = Params()
+    _lib.dcg_default_params(C.byref(p))
+    return p
+
+
+class Simulation(Handle):
+    """One simulation on the GPU (FluidSimulationDCGrid or FluidSimulationUniform)."""
+
+    def __init__(self, kind, size, params, max_num_blocks=None):
+        library()
+        self._fns = _fns
+        super().__init__(kind, size, params, max_num_blocks)
+
+    def synchronize(self):
+        self._check(_lib.dcg_synchronize(self._h), "synchronize")
+
+    def set_timing(self, enable):
+        self._check(_lib.dcg_set_timing(self._h, int(bool(enable))), "set_timing")
+
+    def timing(self):
+        """{kernel name: (total ms, launches)} accumulated since timing was enabled."""
+        out = {}
+        for k, name in enumerate(K_NAMES):
+            ms = C.c_double()
+            n = C.c_uint64()
+            self._check(_lib.dcg_timing_get(self._h, k, C.byref(ms), C.byref(n)), "timing_get")
+            out[name] = (ms.value, n.value)
+        return out

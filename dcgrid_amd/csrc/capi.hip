@@ -1,0 +1,281 @@
+// capi.hip — extern "C" entry points of libdcgrid_amd (declared in include/dcgrid_amd.h) and
+// the shared host machinery of SimBase (stream, fills, timing).
+#include <cstring>
+
+#include "dcg_internal.hpp"
+
+namespace dcg {
+
+__global__ void k_fill32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill8(uint8_t* __restrict__ p, uint8_t v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static unsigned fillGrid(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+SimBase::~SimBase() {
+  if (stream) hipStreamSynchronize(stream);
+  for (auto& p : pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (auto e : eventPool) hipEventDestroy(e);
+  for (void* p : allocations) hipFree(p);
+  if (stream) hipStreamDestroy(stream);
+}
+
+int SimBase::createStream() {
+  // Default priority on purpose.
+  DCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  return DCG_OK;
+}
+
+int SimBase::fill32(void* p, uint32_t v, size_t n) {
+  if (n == 0) return DCG_OK;
+  k_fill32<<<fillGrid(n), 256, 0, stream>>>((uint32_t*)p, v, n);
+  return checkLaunch("fill32");
+}
+int SimBase::fill8(void* p, uint8_t v, size_t n) {
+  if (n == 0) return DCG_OK;
+  k_fill8<<<fillGrid(n), 256, 0, stream>>>((uint8_t*)p, v, n);
+  return checkLaunch("fill8");
+}
+int SimBase::sync() {
+  DCG_HIP(hipStreamSynchronize(stream));
+  return DCG_OK;
+}
+
+int SimBase::beginTimed(int k, Pending* ev) {
+  ev->k = -1;
+  if (!(timingMask & (1u << k))) return DCG_OK;
+  for (int i = 0; i < 2; i++) {
+    hipEvent_t e;
+    if (!eventPool.empty()) {
+      e = eventPool.back();
+      eventPool.pop_back();
+    } else {
+      DCG_HIP(hipEventCreate(&e));
+    }
+    (i == 0 ? ev->a : ev->b) = e;
+  }
+  ev->k = k;
+  DCG_HIP(hipEventRecord(ev->a, stream));
+  return DCG_OK;
+}
+int SimBase::endTimed(int, Pending* ev) {
+  if (ev->k < 0) return DCG_OK;
+  DCG_HIP(hipEventRecord(ev->b, stream));
+  pending.push_back(*ev);
+  // Bound the number of outstanding events.
+  if (pending.size() > 4096) return collectTiming();
+  return DCG_OK;
+}
+int SimBase::collectTiming() {
+  if (pending.empty()) return DCG_OK;
+  DCG_HIP(hipStreamSynchronize(stream));
+  for (auto& p : pending) {
+    float ms = 0.f;
+    DCG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    timingMs[p.k] += ms;
+    timingLaunches[p.k] += 1;
+    eventPool.push_back(p.a);
+    eventPool.push_back(p.b);
+  }
+  pending.clear();
+  return DCG_OK;
+}
+
+}  // namespace dcg
+
+using dcg::SimBase;
+
+struct dcg_sim {
+  SimBase* impl;
+};
+
+static std::string g_create_error;
+
+extern "C" {
+
+int32_t dcg_abi_version(void) { return DCG_ABI_VERSION; }
+
+// SimParams::defaultParams, src/data/sim_params.cpp:4-17; rdx = 1/dx as in src/main.cpp:17.
+void dcg_default_params(dcg_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->gx = p->gy = p->gz = 128;
+  p->dx = 10000.f / 128;
+  p->rdx = 1.f / p->dx;
+  p->dt = 3.f;
+  p->velocity_emission_rate = 150.f;
+  p->density_emission_rate = 0.002f;
+  p->emission_radius = 750.f;
+  p->enable_additional_solids = 0;
+  p->block_width = 4;
+  p->max_levels = 0;
+  p->project_mode = DCG_PROJECT_MULTILEVEL;
+  p->project_top_pairs = 5;    // fluid_simulation_dcgrid.cu:274
+  p->project_level_pairs = 5;  // :284
+  p->project_local_pairs = 10; // :301
+  p->score_mode = DCG_SCORE_GEOMETRIC;
+  p->refine_threshold = 1e-4f; // :461
+  p->grad_threshold = 0.05f;
+  p->vort_threshold = 0.5f;
+  p->source_mode = DCG_SOURCE_NONE;
+  p->noise_period = 32.f;
+  p->seed = 20261202u;
+}
+
+int dcg_create_uniform(const int32_t size[3], const dcg_params* p, dcg_sim** out) {
+  if (!size || !p || !out) return DCG_E_ARG;
+  int st = 0;
+  SimBase* s = dcg::makeUniform(size, *p, &st, &g_create_error);
+  if (!s) return st;
+  *out = new dcg_sim{s};
+  return DCG_OK;
+}
+
+int dcg_create_dcgrid(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p, dcg_sim** out) {
+  if (!size || !p || !out) return DCG_E_ARG;
+  int st = 0;
+  SimBase* s = dcg::makeDCGrid(size, max_num_blocks, *p, &st, &g_create_error);
+  if (!s) return st;
+  *out = new dcg_sim{s};
+  return DCG_OK;
+}
+
+void dcg_destroy(dcg_sim* s) {
+  if (!s) return;
+  delete s->impl;
+  delete s;
+}
+
+#define IMPL(s)                \
+  if (!(s)) return DCG_E_ARG;  \
+  SimBase* I = (s)->impl
+
+int dcg_set_params(dcg_sim* s, const dcg_params* p) {
+  IMPL(s);
+  if (!p) return DCG_E_ARG;
+  if (p->gx != I->P.gx || p->gy != I->P.gy || p->gz != I->P.gz || p->block_width != I->P.block_width ||
+      p->max_levels != I->P.max_levels)
+    return I->fail(DCG_E_ARG, "grid size, block width and level cap are fixed at creation");
+  I->P = *p;
+  return DCG_OK;
+}
+int dcg_get_params(const dcg_sim* s, dcg_params* p) {
+  if (!s || !p) return DCG_E_ARG;
+  *p = s->impl->P;
+  return DCG_OK;
+}
+int dcg_init(dcg_sim* s) {
+  IMPL(s);
+  return I->init();
+}
+int dcg_reset(dcg_sim* s) {
+  IMPL(s);
+  return I->reset();
+}
+int dcg_adapt_topology(dcg_sim* s) {
+  IMPL(s);
+  return I->adaptTopology();
+}
+int dcg_advect_velocity(dcg_sim* s) {
+  IMPL(s);
+  return I->advectVelocity();
+}
+int dcg_project(dcg_sim* s) {
+  IMPL(s);
+  return I->project();
+}
+int dcg_project_local(dcg_sim* s) {
+  IMPL(s);
+  return I->projectLocal();
+}
+int dcg_advect_density(dcg_sim* s) {
+  IMPL(s);
+  return I->advectDensity();
+}
+// Step order of src/simulation.cpp:104-111.
+int dcg_step(dcg_sim* s, int32_t n) {
+  IMPL(s);
+  if (n < 0) return I->fail(DCG_E_ARG, "negative step count");
+  for (int i = 0; i < n; i++) {
+    int e;
+    if ((e = I->advectVelocity())) return e;
+    if ((e = I->adaptTopology())) return e;
+    if ((e = (I->P.project_mode == DCG_PROJECT_LOCAL ? I->projectLocal() : I->project()))) return e;
+    if ((e = I->advectDensity())) return e;
+  }
+  return DCG_OK;
+}
+int dcg_debug_stats(dcg_sim* s, double* out) {
+  IMPL(s);
+  if (!out) return DCG_E_ARG;
+  return I->debugStats(out);
+}
+int dcg_seed_scene(dcg_sim* s) {
+  IMPL(s);
+  return I->seedScene();
+}
+int dcg_synchronize(dcg_sim* s) {
+  IMPL(s);
+  return I->sync();
+}
+int dcg_info_get(dcg_sim* s, dcg_info* out) {
+  IMPL(s);
+  if (!out) return DCG_E_ARG;
+  return I->info(out);
+}
+int dcg_field_size(dcg_sim* s, int32_t field, uint64_t* n) {
+  IMPL(s);
+  if (!n) return DCG_E_ARG;
+  return I->fieldSize(field, n);
+}
+int dcg_export_field(dcg_sim* s, int32_t field, float* dst, uint64_t n) {
+  IMPL(s);
+  if (!dst) return DCG_E_ARG;
+  return I->exportField(field, dst, n);
+}
+int dcg_import_field(dcg_sim* s, int32_t field, const float* src, uint64_t n) {
+  IMPL(s);
+  if (!src) return DCG_E_ARG;
+  return I->importField(field, src, n);
+}
+int dcg_export_tables(dcg_sim* s, dcg_tables* t) {
+  IMPL(s);
+  if (!t) return DCG_E_ARG;
+  return I->exportTables(t);
+}
+int dcg_set_timing(dcg_sim* s, int32_t mask) {
+  IMPL(s);
+  int e = I->collectTiming();
+  if (e) return e;
+  I->timingMask = (uint32_t)mask;
+  for (int k = 0; k < DCG_NUM_KERNEL_IDS; k++) {
+    I->timingMs[k] = 0;
+    I->timingLaunches[k] = 0;
+  }
+  return DCG_OK;
+}
+int dcg_timing_get(dcg_sim* s, int32_t kernel, double* total_ms, uint64_t* launches) {
+  IMPL(s);
+  if (kernel < 0 || kernel >= DCG_NUM_KERNEL_IDS || !total_ms || !launches) return DCG_E_ARG;
+  int e = I->collectTiming();
+  if (e) return e;
+  *total_ms = I->timingMs[kernel];
+  *launches = I->timingLaunches[kernel];
+  return DCG_OK;
+}
+const char* dcg_last_error(const dcg_sim* s) { return s ? s->impl->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// dcg_device.hpp — device-side helpers shared by the gfx950 kernels.
+//
+// Float expressions are written in the evaluation order of the reference's kernels and the
+// library is compiled with -ffp-contract=off, so each float result is the IEEE result of the
+// same operation sequence on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dcgrid_amd.h"
+
+namespace dcg {
+
+static constexpr uint32_t NFU = 0xFFFFFFFFu;  // "not found" block / subblock index
+static constexpr uint8_t FREE = 0xFF;         // blockLevels value of a free block
+static constexpr uint8_t MOVED = 1, REFINED = 2;  // DCGrid::blockMoved / blockRefined
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Morton index of a local cell (x most significant bit of each triple); BB = log2(BW).
+template <int BB>
+__device__ __forceinline__ uint32_t loc(int x, int y, int z) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < BB; b++)
+    r |= ((uint32_t)((x >> b) & 1) << (3 * b + 2)) | ((uint32_t)((y >> b) & 1) << (3 * b + 1)) |
+         ((uint32_t)((z >> b) & 1) << (3 * b));
+  return r;
+}
+template <int BB>
+__device__ __forceinline__ void locPos(uint32_t l, int& x, int& y, int& z) {
+  x = y = z = 0;
+#pragma unroll
+  for (int b = 0; b < BB; b++) {
+    x |= (int)((l >> (3 * b + 2)) & 1) << b;
+    y |= (int)((l >> (3 * b + 1)) & 1) << b;
+    z |= (int)((l >> (3 * b)) & 1) << b;
+  }
+}
+
+// Inlet / outside tests of velocityBndCond and densityBndCond (src/utils/sim_utils.cu:24-55).
+__device__ __forceinline__ bool inletHit(const dcg_params& P, int x, int y, int z, int scale) {
+  if (y >= 0) return false;
+  const float a = (float)(x * scale) - .5f * (float)P.gx;
+  const float b = (float)(z * scale) - .5f * (float)P.gz;
+  return __builtin_sqrtf(a * a + b * b) < P.emission_radius * P.rdx;
+}
+__device__ __forceinline__ bool outside(const dcg_params& P, int x, int y, int z, int scale) {
+  return x < 0 || y < 0 || z < 0 || x * scale >= P.gx || y * scale >= P.gy || z * scale >= P.gz;
+}
+__device__ __forceinline__ float3 velocityBC(const dcg_params& P, float3 v, int x, int y, int z, int scale) {
+  if (inletHit(P, x, y, z, scale)) return make_float3(0.f, P.velocity_emission_rate, 0.f);
+  if (outside(P, x, y, z, scale)) return make_float3(0.f, 0.f, 0.f);
+  return v;
+}
+__device__ __forceinline__ float densityBC(const dcg_params& P, float q, int x, int y, int z, int scale) {
+  if (inletHit(P, x, y, z, scale)) return P.density_emission_rate;
+  if (outside(P, x, y, z, scale)) return 0.f;
+  return q;
+}
+__device__ __forceinline__ float temperatureBC(const dcg_params& P, float q, int x, int y, int z, int scale) {
+  if (inletHit(P, x, y, z, scale)) return P.t_ambient;
+  if (outside(P, x, y, z, scale)) return P.t_ambient;
+  return q;
+}
+// getCellFluidity (src/utils/sim_utils.cu:11-22) with the sphere of src/sdf.cuh:8-18.
+__device__ __forceinline__ float cellFluidity(const dcg_params& P, int x, int y, int z, int scale) {
+  const float sqrt3 = 1.73205f;
+  if (!P.enable_additional_solids) return 1.f;
+  const float px = ((float)x + .5f) * (float)scale;
+  const float py = ((float)y + .5f) * (float)scale;
+  const float pz = ((float)z + .5f) * (float)scale;
+  const float cx = (float)P.gx * .5f, cy = (float)P.gy * .45f, cz = (float)P.gz * .5f;
+  const float r = 2000.f * P.rdx;
+  const float dx = px - cx, dy = py - cy, dz = pz - cz;
+  const float d = __builtin_sqrtf(dx * dx + dy * dy + dz * dz) - r;
+  const float overlap = clampf(.5f - d / ((float)scale * sqrt3), 0.f, 1.f);
+  return 1.f - overlap;
+}
+
+// ---- seeded synthetic inputs: counter-based hash and value noise ----
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ float hash01(uint32_t seed, int x, int y, int z, int w) {
+  uint32_t h = mix32(seed ^ 0x9e3779b9u);
+  h = mix32(h ^ (uint32_t)x);
+  h = mix32(h ^ (uint32_t)y ^ 0x68e31da4u);
+  h = mix32(h ^ (uint32_t)z ^ 0xb5297a4du);
+  h = mix32(h ^ (uint32_t)w ^ 0x1b56c4e9u);
+  return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ float valueNoise(float x, float y, float z, uint32_t seed) {
+  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+  const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
+  const float fx = x - xf, fy = y - yf, fz = z - zf;
+  const float sx = fx * fx * (3.f - 2.f * fx);
+  const float sy = fy * fy * (3.f - 2.f * fy);
+  const float sz = fz * fz * (3.f - 2.f * fz);
+  float v[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    v[c] = 2.f * hash01(seed, ix + ((c >> 2) & 1), iy + ((c >> 1) & 1), iz + (c & 1), 0) - 1.f;
+  const float a00 = v[0] + sx * (v[4] - v[0]);
+  const float a10 = v[2] + sx * (v[6] - v[2]);
+  const float a01 = v[1] + sx * (v[5] - v[1]);
+  const float a11 = v[3] + sx * (v[7] - v[3]);
+  const float b0 = a00 + sy * (a10 - a00);
+  const float b1 = a01 + sy * (a11 - a01);
+  return b0 + sz * (b1 - b0);
+}
+__device__ __forceinline__ float fbm3(float x, float y, float z, float period, uint32_t seed) {
+  float n = 0.f, amp = 1.f, freq = 1.f / period;
+  for (int o = 0; o < 3; o++) {
+    n = n + amp * valueNoise(x * freq, y * freq, z * freq, seed + 1013u * (uint32_t)o);
+    amp = amp * .5f;
+    freq = freq * 2.f;
+  }
+  return n / 1.75f;
+}
+__device__ __forceinline__ float3 initVelocity(const dcg_params& P, float cx, float cy, float cz) {
+  float u[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (P.init_velocity_noise == 0.f) {
+      u[c] = 0.f;
+      continue;
+    }
+    const uint32_t s = P.seed ^ (0x51ed270bu * (uint32_t)(c + 1));
+    u[c] = P.init_velocity_noise * fbm3(cx, cy, cz, P.noise_period, s);
+  }
+  return make_float3(u[0], u[1], u[2]);
+}
+__device__ __forceinline__ bool inSource(const dcg_params& P, float cx, float cy, float cz) {
+  const float dx = cx - P.src_center[0], dy = cy - P.src_center[1], dz = cz - P.src_center[2];
+  return dx * dx + dy * dy + dz * dz < P.src_radius * P.src_radius;
+}
+__device__ __forceinline__ float sourceTemperature(const dcg_params& P, int x, int y, int z, int level) {
+  const float u = hash01(P.seed ^ 0x2545f491u, x, y, z, level);
+  return P.src_temperature + P.src_noise * (2.f * u - 1.f);
+}
+
+// float3 helpers with explicit evaluation order
+__device__ __forceinline__ float3 f3mul(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+
+// XCD-aware remap of a linear workgroup id: consecutive ids are dispatched round-robin over the
+// 8 XCDs of MI355X; this maps them so each XCD works on a contiguous range (shared L2 reuse).
+__device__ __forceinline__ uint32_t xcdRemap(uint32_t id, uint32_t n) {
+  const uint32_t xcd = id & 7u, k = id >> 3;
+  const uint32_t per = n >> 3, rem = n & 7u;
+  // XCDs 0..rem-1 own per+1 ids, the others own per ids
+  const uint32_t base = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+  const uint32_t own = xcd < rem ? per + 1 : per;
+  return k < own ? base + k : id;  // ids beyond a full round keep their position
+}
+
+}  // namespace dcg

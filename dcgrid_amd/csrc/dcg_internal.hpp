@@ -1,0 +1,106 @@
+// dcg_internal.hpp — host-side internals of libdcgrid_amd: the simulation base class behind
+// the C ABI, error handling and per-kernel event timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dcgrid_amd.h"
+
+namespace dcg {
+
+#define DCG_HIP(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(DCG_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define DCG_TRY(expr)     \
+  do {                    \
+    int s_ = (expr);      \
+    if (s_) return s_;    \
+  } while (0)
+
+struct SimBase {
+  dcg_params P{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocations;
+
+  // Per-kernel timing with events on the handle's stream.
+  uint32_t timingMask = 0;
+  struct Pending {
+    int k;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> eventPool;
+  double timingMs[DCG_NUM_KERNEL_IDS] = {};
+  uint64_t timingLaunches[DCG_NUM_KERNEL_IDS] = {};
+
+  virtual ~SimBase();
+  virtual int kind() const = 0;
+  virtual int init() = 0;
+  virtual int reset() = 0;
+  virtual int adaptTopology() = 0;
+  virtual int advectVelocity() = 0;
+  virtual int project() = 0;
+  virtual int projectLocal() = 0;
+  virtual int advectDensity() = 0;
+  virtual int debugStats(double* out) = 0;
+  virtual int seedScene() = 0;
+  virtual int info(dcg_info* out) = 0;
+  virtual int fieldSize(int field, uint64_t* n) = 0;
+  virtual int exportField(int field, float* dst, uint64_t n) = 0;
+  virtual int importField(int field, const float* src, uint64_t n) = 0;
+  virtual int exportTables(dcg_tables* t) = 0;
+
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+  int createStream();
+  // Device allocation owned by the handle (freed in the destructor).
+  template <class T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(DCG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocations.push_back(q);
+    *p = (T*)q;
+    return DCG_OK;
+  }
+  // Fill with a kernel (never hipMemsetAsync, so the step stays graph-capturable).
+  int fill32(void* p, uint32_t value, size_t count);
+  int fill8(void* p, uint8_t value, size_t count);
+  int sync();
+
+  // Launch bracket: records events around the launch when kernel id k is timed.
+  int beginTimed(int k, Pending* ev);
+  int endTimed(int k, Pending* ev);
+  int collectTiming();
+  int checkLaunch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(DCG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return DCG_OK;
+  }
+};
+
+// Launch `expr` on the handle's stream, bracketed by timing events for kernel id K.
+#define DCG_LAUNCH(K, NAME, expr)        \
+  do {                                   \
+    SimBase::Pending ev_;                \
+    DCG_TRY(beginTimed((K), &ev_));      \
+    expr;                                \
+    DCG_TRY(checkLaunch(NAME));          \
+    DCG_TRY(endTimed((K), &ev_));        \
+  } while (0)
+
+SimBase* makeUniform(const int32_t size[3], const dcg_params& p, int* status, std::string* err);
+SimBase* makeDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p, int* status,
+                    std::string* err);
+
+}  // namespace dcg

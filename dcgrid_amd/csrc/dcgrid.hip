@@ -1,0 +1,514 @@
+// dcgrid.hip — host driver of the DCGrid back-end, the counterpart of FluidSimulationDCGrid
+// (src/dcgrid/fluid_simulation_dcgrid.{h,cu}): level/budget planner, allocation, reset/init and
+// the launch schedule of the step methods. Nothing here waits for the device: block counts,
+// move limits and selection results stay in device control words, and kernels whose work depends
+// on them are launched over the static per-level capacity and exit early.
+#include <algorithm>
+#include <cstring>
+
+#include "dcg_internal.hpp"
+#include "dcgrid_fluid.hpp"
+#include "dcgrid_topo.hpp"
+
+namespace dcg {
+
+static inline int iDivUpH(int a, int b) { return (a % b != 0) ? (a / b + 1) : (a / b); }
+static inline unsigned blocksFor(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+template <int BW>
+struct DCGridSim : SimBase {
+  using G = Geo<BW>;
+  int gx = 0, gy = 0, gz = 0, L = 0, S = 0;
+  uint32_t B = 0, H = 0;
+  size_t C = 0;
+  uint32_t maxL[DCG_MAX_LEVELS] = {}, fullL[DCG_MAX_LEVELS] = {}, offs[DCG_MAX_LEVELS] = {};
+  uint32_t hsize[DCG_MAX_LEVELS] = {}, hoff[DCG_MAX_LEVELS] = {};
+  DG g{};
+  float* sbs = nullptr;   // 8B + 1 (last entry stays -FLT_MAX)
+  float* bs = nullptr;    // B
+  float* feat = nullptr;  // 8B
+  unsigned long long* cand = nullptr;
+  uint32_t* mcBuf = nullptr;  // sorted move candidates, per-level region at offs[l]
+  uint32_t* dcBuf = nullptr;  // sorted destination subblocks, region at offs[l] + l
+  uint32_t* refList = nullptr;
+  float* stats = nullptr;
+  bool customFluidity = false;
+
+  int kind() const override { return DCG_KIND_DCGRID; }
+  bool fluidityOn() const { return P.enable_additional_solids || customFluidity; }
+
+  // Planner: fluid_simulation_dcgrid.cu:9-140.
+  int create(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p) {
+    P = p;
+    gx = size[0];
+    gy = size[1];
+    gz = size[2];
+    if (gx <= 0 || gy <= 0 || gz <= 0 || gx != P.gx || gy != P.gy || gz != P.gz)
+      return fail(DCG_E_ARG, "size must be positive and equal params gx,gy,gz");
+    const int minDim = std::min(gx, std::min(gy, gz));
+    int cellSize = 2;
+    L = 1;
+    while (gx % cellSize == 0 && gy % cellSize == 0 && gz % cellSize == 0 && cellSize * BW <= minDim) {
+      L++;
+      cellSize *= 2;
+    }
+    if (P.max_levels > 0 && L > P.max_levels) L = P.max_levels;
+    if (L > DCG_MAX_LEVELS) return fail(DCG_E_UNSUPPORTED, "too many levels");
+    const int top = BW << (L - 1);
+    if (gx % top || gy % top || gz % top)
+      return fail(DCG_E_UNSUPPORTED, "domain must be a multiple of the coarsest block extent");
+    uint64_t full64[DCG_MAX_LEVELS];
+    for (int l = 0, cs = 1; l < L; l++, cs *= 2)
+      full64[l] = (uint64_t)iDivUpH(gx, cs * BW) * iDivUpH(gy, cs * BW) * iDivUpH(gz, cs * BW);
+    if (maxBlocks < full64[L - 1]) return fail(DCG_E_BUDGET, "too few blocks to fill lowest resolution");
+    if (maxBlocks * (uint64_t)G::BV >= 0xFFFFFFFFull || maxBlocks * 8 >= 0xFFFFFFFFull)
+      return fail(DCG_E_UNSUPPORTED, "budget exceeds 32-bit cell indexing");
+    B = (uint32_t)maxBlocks;
+    uint64_t mx[DCG_MAX_LEVELS];
+    mx[L - 1] = full64[L - 1];
+    uint64_t left = maxBlocks - mx[L - 1];
+    for (int l = L - 2; l >= 0; l--) {
+      mx[l] = std::min(left / (uint64_t)(l + 1), full64[l]);
+      left -= mx[l];
+    }
+    if (mx[0] == 0) return fail(DCG_E_BUDGET, "too few blocks to reach highest resolution");
+    for (int l = 0; l < L; l++)
+      if (mx[l] == 0) return fail(DCG_E_BUDGET, "a level between the finest and the coarsest gets no blocks");
+    for (int l = 0; l < L; l++) {
+      maxL[l] = (uint32_t)mx[l];
+      fullL[l] = (uint32_t)full64[l];
+    }
+    offs[0] = 0;
+    for (int l = 1; l < L; l++) offs[l] = offs[l - 1] + maxL[l - 1];
+    S = 0;
+    for (int l = 0; l < L; l++)
+      if (maxL[l] < fullL[l]) S = l + 1;
+    H = 4 * B;
+    for (int l = 0; l < L; l++) hsize[l] = 4 * maxL[l];
+    hoff[0] = 0;
+    for (int l = 1; l < L; l++) hoff[l] = hoff[l - 1] + hsize[l - 1];
+    C = (size_t)B * G::BV;
+
+    g.gx = gx;
+    g.gy = gy;
+    g.gz = gz;
+    g.L = L;
+    g.S = S;
+    g.B = B;
+    for (int l = 0; l < L; l++) {
+      g.offs[l] = offs[l];
+      g.maxL[l] = maxL[l];
+      g.fullL[l] = fullL[l];
+      g.rx[l] = iDivUpH(gx, BW << l);
+      g.ry[l] = iDivUpH(gy, BW << l);
+      g.rz[l] = iDivUpH(gz, BW << l);
+      g.hoff[l] = hoff[l];
+      g.hsize[l] = hsize[l];
+    }
+    DCG_TRY(createStream());
+    DCG_TRY(alloc(&g.pos, B));
+    DCG_TRY(alloc(&g.lev, B));
+    DCG_TRY(alloc(&g.flg, B));
+    DCG_TRY(alloc(&g.edge, B));
+    DCG_TRY(alloc(&g.par, B));
+    DCG_TRY(alloc(&g.chi, (size_t)8 * B));
+    DCG_TRY(alloc(&g.nbr, (size_t)27 * B));
+    DCG_TRY(alloc(&g.hkey, H));
+    DCG_TRY(alloc(&g.hval, H));
+    DCG_TRY(alloc(&g.ctl, CTL_WORDS));
+    DCG_TRY(alloc(&g.den, C));
+    DCG_TRY(alloc(&g.tem, C));
+    DCG_TRY(alloc(&g.flu, C));
+    DCG_TRY(alloc(&g.div, C));
+    DCG_TRY(alloc(&g.pre, C));
+    DCG_TRY(alloc(&g.tpre, C));
+    DCG_TRY(alloc(&g.tden, C));
+    DCG_TRY(alloc(&g.ttem, C));
+    DCG_TRY(alloc(&g.vel, C));
+    DCG_TRY(alloc(&g.tvel, C));
+    DCG_TRY(alloc(&g.vort, C));
+    DCG_TRY(alloc(&sbs, (size_t)8 * B + 1));
+    DCG_TRY(alloc(&bs, B));
+    DCG_TRY(alloc(&feat, (size_t)8 * B));
+    DCG_TRY(alloc(&cand, (size_t)8 * B + 16));
+    DCG_TRY(alloc(&mcBuf, (size_t)B + 16));
+    DCG_TRY(alloc(&dcBuf, (size_t)B + 16));
+    DCG_TRY(alloc(&refList, (size_t)8 * B));
+    DCG_TRY(alloc(&stats, B));
+    const float negMax = -FLT_MAX;
+    uint32_t negBits;
+    std::memcpy(&negBits, &negMax, 4);
+    DCG_TRY(fill32(sbs, negBits, (size_t)8 * B + 1));
+    return reset();
+  }
+
+  // reset(), fluid_simulation_dcgrid.cu:212-261
+  int reset() override {
+    DCG_TRY(fill32(g.hkey, 0xFFFFFFFFu, H));
+    DCG_TRY(fill32(g.hval, NFU, H));
+    DCG_TRY(fill32(g.pos, 0, (size_t)3 * B));
+    DCG_TRY(fill8(g.lev, FREE, B));
+    DCG_TRY(fill32(g.flg, 0, B));
+    DCG_TRY(fill8(g.edge, 0, B));
+    DCG_TRY(fill32(g.par, NFU, B));
+    DCG_TRY(fill32(g.chi, NFU, (size_t)8 * B));
+    DCG_TRY(fill32(g.nbr, NFU, (size_t)27 * B));
+    DCG_TRY(fill32(g.den, 0, C));
+    DCG_TRY(fill32(g.tem, 0, C));
+    DCG_TRY(fill32(g.vel, 0, 3 * C));
+    DCG_TRY(fill32(g.flu, 0, C));
+    DCG_TRY(fill32(g.div, 0, C));
+    DCG_TRY(fill32(g.pre, 0, C));
+    DCG_TRY(fill32(g.tpre, 0, C));
+    DCG_TRY(fill32(g.vort, 0, 3 * C));
+    DCG_TRY(fill32(g.ctl, 0, CTL_WORDS));
+    uint32_t loads[DCG_MAX_LEVELS] = {};
+    for (int l = 0; l < L; l++) loads[l] = maxL[l] == fullL[l] ? maxL[l] : 0;
+    DCG_HIP(hipMemcpyAsync(g.ctl + CTL_LOADS, loads, sizeof(loads), hipMemcpyHostToDevice, stream));
+    DCG_TRY(sync());
+    customFluidity = false;
+    return init();
+  }
+  // init(), fluid_simulation_dcgrid.cu:190-210
+  int init() override {
+    for (int level = S; level < L; level++) {
+      const size_t n = (size_t)g.rx[level] * g.ry[level] * g.rz[level];
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_activate_dense",
+                 (k_activate_dense<BW><<<blocksFor(n, 256), 256, 0, stream>>>(g, P, level)));
+    }
+    for (int i = 0; i < 5; i++) DCG_TRY(adaptTopology());
+    return DCG_OK;
+  }
+
+  int subScores() {
+    if (P.score_mode == DCG_SCORE_FEATURES)
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_sub_scores_feat",
+                 (k_sub_scores_feat<<<blocksFor((size_t)8 * B, 256), 256, 0, stream>>>(g, feat, sbs)));
+    else
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_sub_scores_geo",
+                 (k_sub_scores_geo<BW><<<blocksFor((size_t)8 * B, 256), 256, 0, stream>>>(g, P, sbs)));
+    return DCG_OK;
+  }
+
+  // adaptTopology(), fluid_simulation_dcgrid.cu:320-346
+  int adaptTopology() override {
+    const unsigned nWG = blocksFor(B, G::BPW);
+    DCG_TRY(fill32(g.flg, 0, B));
+    if (P.score_mode == DCG_SCORE_FEATURES) {
+      if (fluidityOn())
+        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
+      else
+        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_sub_features", (k_sub_features<BW><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
+    }
+    // moveBlocks (:348-437)
+    DCG_TRY(subScores());
+    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_block_scores", (k_block_scores<<<blocksFor(B, 256), 256, 0, stream>>>(g, sbs, bs)));
+    for (int l = 0; l < L - 1; l++) {
+      uint32_t* mc = mcBuf + offs[l];
+      uint32_t* dc = dcBuf + offs[l] + l;
+      const unsigned rk = blocksFor((size_t)maxL[l] + 1, 256);
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_select<0>", (k_select<0><<<1, 1024, 0, stream>>>(g, l, bs, cand)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_rank<0>", (k_rank<0><<<rk, 256, 0, stream>>>(g, l, cand, mc)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_select<1>", (k_select<1><<<1, 1024, 0, stream>>>(g, l, sbs, cand)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_rank<1>", (k_rank<1><<<rk, 256, 0, stream>>>(g, l, cand, dc)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_match", (k_match<<<1, 1024, 0, stream>>>(g, l, mc, dc, bs, sbs)));
+    }
+    for (int l = L - 2; l >= 0; l--)
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_move",
+                 (k_move<BW><<<blocksFor(maxL[l], 256), 256, 0, stream>>>(g, l, mcBuf + offs[l], dcBuf + offs[l] + l)));
+    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_total_moves", (k_total_moves<<<1, 64, 0, stream>>>(g)));
+    // hash rebuild after moves (:326-332)
+    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_hash_clear", (k_hash_clear<<<blocksFor(H, 256) > 2048 ? 2048 : blocksFor(H, 256), 256, 0, stream>>>(g, H)));
+    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_hash_refill", (k_hash_refill<BW><<<blocksFor(B, 256), 256, 0, stream>>>(g)));
+    // refineSubblocks (:439-483)
+    if (L > 1) {
+      DCG_TRY(subScores());
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refine_select", (k_refine_select<<<L - 1, 1024, 0, stream>>>(g, P, sbs, refList)));
+      uint32_t mxc = 0;
+      for (int l = 0; l < L - 1; l++) mxc = std::max(mxc, maxL[l]);
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refine_apply",
+                 (k_refine_apply<BW><<<dim3(blocksFor(mxc, 256), L - 1), 256, 0, stream>>>(g, refList)));
+    }
+    // apron refresh and value propagation (:336-345)
+    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refresh", (k_refresh<BW><<<blocksFor((size_t)27 * B, 256), 256, 0, stream>>>(g)));
+    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    for (int l = L - 2; l >= 0; l--) {
+      const unsigned n = blocksFor(maxL[l], G::BPW);
+      if (src)
+        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_propagate", (k_propagate<BW, true><<<n, G::WG, 0, stream>>>(g, P, l, mcBuf)));
+      else
+        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_propagate", (k_propagate<BW, false><<<n, G::WG, 0, stream>>>(g, P, l, mcBuf)));
+    }
+    return DCG_OK;
+  }
+
+  template <class T>
+  int restrictAll(T* ch) {
+    for (int l = 0; l < L - 1; l++) {
+      const size_t groups = (size_t)maxL[l] * (G::BV / 8);
+      DCG_LAUNCH(DCG_K_RESTRICT, "k_restrict", (k_restrict<BW, T><<<blocksFor(groups, 256), 256, 0, stream>>>(g, l, ch)));
+    }
+    return DCG_OK;
+  }
+
+  // advectVelocity(), fluid_simulation_dcgrid.cu:263-268
+  int advectVelocity() override {
+    const unsigned nWG = blocksFor(B, G::BPW);
+    const bool buoy = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT, fl = fluidityOn();
+#define ADV_V(F, Bu) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_advect_velocity", (k_advect_velocity<BW, F, Bu><<<nWG, G::WG, 0, stream>>>(g, P)))
+    if (fl && buoy) ADV_V(true, true);
+    else if (fl) ADV_V(true, false);
+    else if (buoy) ADV_V(false, true);
+    else ADV_V(false, false);
+#undef ADV_V
+    std::swap(g.vel, g.tvel);  // replaces the D2D copy (:265)
+    return restrictAll(g.vel);
+  }
+  // advectDensity(), fluid_simulation_dcgrid.cu:313-318
+  int advectDensity() override {
+    const unsigned nWG = blocksFor(B, G::BPW);
+    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT, fl = fluidityOn();
+#define ADV_D(F, Sr) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_advect_density", (k_advect_density<BW, F, Sr><<<nWG, G::WG, 0, stream>>>(g, P)))
+    if (fl && src) ADV_D(true, true);
+    else if (fl) ADV_D(true, false);
+    else if (src) ADV_D(false, true);
+    else ADV_D(false, false);
+#undef ADV_D
+    std::swap(g.den, g.tden);
+    DCG_TRY(restrictAll(g.den));
+    if (src) {
+      std::swap(g.tem, g.ttem);
+      DCG_TRY(restrictAll(g.tem));
+    }
+    return DCG_OK;
+  }
+  int divergence() {
+    const unsigned nWG = blocksFor(B, G::BPW);
+    if (fluidityOn())
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
+    else
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
+    return restrictAll(g.div);
+  }
+  int jacobiPairs(int level, int pairs) {
+    const int s = 1 << level;
+    const float alpha = (float)(s * s) * P.dx * P.dx;  // dcgrid_multigrid_solver.cu:11
+    const unsigned n = blocksFor(maxL[level], G::BPW);
+    for (int i = 0; i < pairs; i++) {
+      DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, g.pre, g.tpre, alpha)));
+      DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, g.tpre, g.pre, alpha)));
+    }
+    return DCG_OK;
+  }
+  int applyPressure() {
+    const unsigned nWG = blocksFor(B, G::BPW);
+    if (fluidityOn())
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
+    else
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
+    return restrictAll(g.vel);
+  }
+  // project(), fluid_simulation_dcgrid.cu:270-294
+  int project() override {
+    DCG_TRY(divergence());
+    DCG_TRY(jacobiPairs(L - 1, P.project_top_pairs));
+    for (int l = L - 2; l >= 0; l--) {
+      DCG_LAUNCH(DCG_K_PROLONGATE, "k_prolongate",
+                 (k_prolongate<BW><<<blocksFor(maxL[l], G::BPW), G::WG, 0, stream>>>(g, l)));
+      DCG_TRY(jacobiPairs(l, P.project_level_pairs));
+    }
+    return applyPressure();
+  }
+  // projectLocal(), fluid_simulation_dcgrid.cu:296-311
+  int projectLocal() override {
+    DCG_TRY(divergence());
+    for (int l = L - 1; l >= 0; l--) DCG_TRY(jacobiPairs(l, P.project_local_pairs));
+    return applyPressure();
+  }
+  // debugStats(), fluid_simulation_dcgrid.cu:517-528 (residual of the last projection)
+  int debugStats(double* out) override {
+    DCG_LAUNCH(DCG_K_OTHER, "k_residual", (k_residual<BW><<<blocksFor(B, 64), 64, 0, stream>>>(g, P, stats)));
+    std::vector<float> h(B);
+    DCG_HIP(hipMemcpyAsync(h.data(), stats, (size_t)B * sizeof(float), hipMemcpyDeviceToHost, stream));
+    DCG_TRY(sync());
+    float sum = 0.f;
+    for (uint32_t b = 0; b < B; b++) sum = sum + h[b];
+    *out = (double)sum;
+    return DCG_OK;
+  }
+  int seedScene() override {
+    DCG_LAUNCH(DCG_K_OTHER, "k_seed", (k_seed<BW><<<blocksFor(B, G::BPW), G::WG, 0, stream>>>(g, P)));
+    DCG_TRY(restrictAll(g.vel));
+    DCG_TRY(restrictAll(g.den));
+    DCG_TRY(restrictAll(g.tem));
+    return DCG_OK;
+  }
+
+  int readCtl(uint32_t* h) {
+    DCG_HIP(hipMemcpyAsync(h, g.ctl, CTL_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    DCG_TRY(sync());
+    if (h[CTL_ERROR] & ERR_HASH_FULL) return fail(DCG_E_HASH_FULL, "hash table fully loaded");
+    return DCG_OK;
+  }
+  int info(dcg_info* o) override {
+    uint32_t ctl[CTL_WORDS];
+    DCG_TRY(readCtl(ctl));
+    std::memset(o, 0, sizeof(*o));
+    o->kind = DCG_KIND_DCGRID;
+    o->size[0] = gx;
+    o->size[1] = gy;
+    o->size[2] = gz;
+    o->block_width = BW;
+    o->levels = L;
+    o->sparse_levels = S;
+    o->apron_width = G::A;
+    o->max_num_blocks = B;
+    o->num_cells = C;
+    o->hash_table_size = H;
+    uint64_t act = 0;
+    for (int l = 0; l < L; l++) {
+      o->max_blocks_level[l] = maxL[l];
+      o->full_blocks_level[l] = fullL[l];
+      o->level_offsets[l] = offs[l];
+      o->block_loads[l] = ctl[CTL_LOADS + l];
+      o->move_limit[l] = ctl[CTL_MOVE_LIMIT + l];
+      o->hash_size_level[l] = hsize[l];
+      o->hash_offset_level[l] = hoff[l];
+      act += (uint64_t)ctl[CTL_LOADS + l] * G::BV;
+    }
+    o->active_cells = act;
+    return DCG_OK;
+  }
+  int fieldSize(int f, uint64_t* n) override {
+    if (f < 0 || f >= DCG_NUM_FIELDS) return fail(DCG_E_ARG, "unknown field");
+    *n = (f == DCG_FIELD_VELOCITY || f == DCG_FIELD_VORTICITY) ? 3 * C : C;
+    return DCG_OK;
+  }
+  float* fieldPtr(int f) {
+    switch (f) {
+      case DCG_FIELD_DENSITY: return g.den;
+      case DCG_FIELD_VELOCITY: return (float*)g.vel;
+      case DCG_FIELD_FLUIDITY: return g.flu;
+      case DCG_FIELD_PRESSURE: return g.pre;
+      case DCG_FIELD_T_PRESSURE: return g.tpre;
+      case DCG_FIELD_DIVERGENCE: return g.div;
+      case DCG_FIELD_TEMPERATURE: return g.tem;
+      case DCG_FIELD_VORTICITY: return (float*)g.vort;
+      default: return nullptr;
+    }
+  }
+  int exportField(int f, float* dst, uint64_t n) override {
+    uint64_t need;
+    DCG_TRY(fieldSize(f, &need));
+    if (n != need) return fail(DCG_E_ARG, "size mismatch");
+    DCG_HIP(hipMemcpyAsync(dst, fieldPtr(f), n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return sync();
+  }
+  int importField(int f, const float* src, uint64_t n) override {
+    uint64_t need;
+    DCG_TRY(fieldSize(f, &need));
+    if (n != need) return fail(DCG_E_ARG, "size mismatch");
+    DCG_HIP(hipMemcpyAsync(fieldPtr(f), src, n * sizeof(float), hipMemcpyHostToDevice, stream));
+    if (f == DCG_FIELD_FLUIDITY) customFluidity = true;
+    return sync();
+  }
+
+  // Host version of apronCell (dcgrid_dev.hpp) used to rebuild the reference's apron table.
+  static uint32_t hloc(int x, int y, int z) {
+    uint32_t r = 0;
+    for (int b = 0; b < G::BB; b++)
+      r |= ((uint32_t)((x >> b) & 1) << (3 * b + 2)) | ((uint32_t)((y >> b) & 1) << (3 * b + 1)) |
+           ((uint32_t)((z >> b) & 1) << (3 * b));
+    return r;
+  }
+  int exportTables(dcg_tables* t) override {
+    std::vector<int32_t> pos(3 * (size_t)B);
+    std::vector<uint8_t> lev(B), edge(B);
+    std::vector<uint32_t> flg(B), par(B), chi(8 * (size_t)B), nbr(27 * (size_t)B), hk(H), hv(H);
+    DCG_HIP(hipMemcpyAsync(pos.data(), g.pos, pos.size() * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(lev.data(), g.lev, B, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(edge.data(), g.edge, B, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(flg.data(), g.flg, (size_t)B * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(par.data(), g.par, (size_t)B * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(chi.data(), g.chi, chi.size() * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(nbr.data(), g.nbr, nbr.size() * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(hk.data(), g.hkey, (size_t)H * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(hv.data(), g.hval, (size_t)H * 4, hipMemcpyDeviceToHost, stream));
+    uint32_t ctl[CTL_WORDS];
+    DCG_TRY(readCtl(ctl));
+    auto u64 = [](uint32_t v) { return v == NFU ? UINT64_MAX : (uint64_t)v; };
+    if (t->block_positions) std::copy(pos.begin(), pos.end(), t->block_positions);
+    if (t->block_levels) std::copy(lev.begin(), lev.end(), t->block_levels);
+    if (t->block_flags)
+      for (uint32_t i = 0; i < B; i++) t->block_flags[i] = (uint8_t)flg[i];
+    if (t->parent_indices)
+      for (uint32_t i = 0; i < B; i++) t->parent_indices[i] = u64(par[i]);
+    if (t->child_indices)
+      for (size_t i = 0; i < chi.size(); i++) t->child_indices[i] = u64(chi[i]);
+    if (t->hash_key) std::copy(hk.begin(), hk.end(), t->hash_key);
+    if (t->hash_val)
+      for (uint32_t i = 0; i < H; i++) t->hash_val[i] = u64(hv[i]);
+    if (t->cell_indices) {
+      constexpr int A = G::A, A3 = G::A3, BV = G::BV;
+      for (uint32_t b = 0; b < B; b++) {
+        uint64_t* e = t->cell_indices + (size_t)b * A3;
+        const int x0 = pos[3 * b], y0 = pos[3 * b + 1], z0 = pos[3 * b + 2];
+        for (int i = 0; i < A; i++)
+          for (int j = 0; j < A; j++)
+            for (int k = 0; k < A; k++) {
+              uint64_t& out = e[(i * A + j) * A + k];
+              const int dx = i == 0 ? -1 : (i == BW + 1 ? 1 : 0);
+              const int dy = j == 0 ? -1 : (j == BW + 1 ? 1 : 0);
+              const int dz = k == 0 ? -1 : (k == BW + 1 ? 1 : 0);
+              if ((dx | dy | dz) == 0) {
+                out = (uint64_t)b * BV + hloc(i - 1, j - 1, k - 1);
+                continue;
+              }
+              if (lev[b] == FREE) {
+                out = UINT64_MAX;  // k_dcgrid_init_apron_indices leaves borders notFound
+                continue;
+              }
+              const uint32_t nb = nbr[(size_t)b * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+              if (nb == b) {
+                const int ci = std::clamp(i, 1, BW), cj = std::clamp(j, 1, BW), ck = std::clamp(k, 1, BW);
+                out = (uint64_t)b * BV + (edge[b] ? hloc(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1))
+                                                  : hloc(ci - 1, cj - 1, ck - 1));
+                continue;
+              }
+              const int d = (int)lev[nb] - (int)lev[b];
+              out = (uint64_t)nb * BV + hloc(((x0 + i - 1) >> d) - pos[3 * nb], ((y0 + j - 1) >> d) - pos[3 * nb + 1],
+                                             ((z0 + k - 1) >> d) - pos[3 * nb + 2]);
+            }
+      }
+    }
+    return DCG_OK;
+  }
+};
+
+SimBase* makeDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p, int* status, std::string* err) {
+  SimBase* s;
+  int st;
+  if (p.block_width == 4) {
+    auto* d = new DCGridSim<4>();
+    st = d->create(size, maxBlocks, p);
+    s = d;
+  } else if (p.block_width == 8) {
+    auto* d = new DCGridSim<8>();
+    st = d->create(size, maxBlocks, p);
+    s = d;
+  } else {
+    *status = DCG_E_UNSUPPORTED;
+    *err = "block_width must be 4 or 8";
+    return nullptr;
+  }
+  *status = st;
+  if (st != DCG_OK) {
+    *err = s->err;
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+}  // namespace dcg

@@ -1,0 +1,148 @@
+// dcgrid_dev.hpp — device-side data structure and lookups of the DCGrid back-end.
+//
+// Layout (reference semantics: src/dcgrid/dcgrid.h:5-73, generalised to BW in {4,8}):
+//  * fields are indexed by global cell = block*BW^3 + Morton(local); velocity is float3 AoS;
+//  * block tables: positions (level-local cell origin), level (0xFF = free), flags, parent
+//    subblock (8*parentBlock + top Morton triple), 8 child blocks per block;
+//  * instead of the reference's (BW+2)^3 size_t apron table (27 B/cell at BW=4) every block keeps
+//    27 neighbour-block ids (1.7 B/cell at BW=4, 0.2 B/cell at BW=8). Each apron direction of the
+//    reference table always points into a single block (the finest block at the same or a
+//    coarser level containing that region), so the reference table is a pure function of this
+//    compact table, the block positions/levels and one "edge" bit per block that records the
+//    clamp rule used for out-of-domain directions. dcg_export_tables rebuilds the reference table.
+#pragma once
+
+#include "dcg_device.hpp"
+
+namespace dcg {
+
+template <int BW>
+struct Geo {
+  static constexpr int BB = BW == 4 ? 2 : 3;    // log2(BW)
+  static constexpr int BV = BW * BW * BW;       // cells per block
+  static constexpr int A = BW + 2;              // apron width
+  static constexpr int A3 = A * A * A;
+  static constexpr int SBB = 3 * (BB - 1);      // log2(cells per subblock)
+  static constexpr int BPW = BW == 4 ? 4 : 1;   // blocks per workgroup
+  static constexpr int WG = BPW * BV;           // workgroup size
+  static constexpr int FACE = 6 * BW * BW;      // face cells of a block's apron
+};
+
+struct DG {
+  int gx, gy, gz, L, S;
+  uint32_t B;
+  uint32_t offs[DCG_MAX_LEVELS], maxL[DCG_MAX_LEVELS], fullL[DCG_MAX_LEVELS];
+  int rx[DCG_MAX_LEVELS], ry[DCG_MAX_LEVELS], rz[DCG_MAX_LEVELS];
+  uint32_t hoff[DCG_MAX_LEVELS], hsize[DCG_MAX_LEVELS];
+  int3* pos;
+  uint8_t* lev;
+  uint32_t* flg;
+  uint8_t* edge;
+  uint32_t* par;
+  uint32_t* chi;
+  uint32_t* nbr;  // 27 per block, direction (dx+1)*9+(dy+1)*3+(dz+1)
+  uint32_t* hkey;
+  uint32_t* hval;
+  uint32_t* ctl;  // device control words, see CTL_* below
+  float* den;
+  float* tem;
+  float* flu;
+  float* div;
+  float* pre;
+  float* tpre;
+  float* tden;
+  float* ttem;
+  float3* vel;
+  float3* tvel;
+  float3* vort;
+};
+
+// Control words (uint32 each) kept on the device so the step never waits for the host.
+enum : int {
+  CTL_LOADS = 0,                          // [16] blocks allocated per level (blockLoads)
+  CTL_MOVE_LIMIT = CTL_LOADS + 16,        // [16] moveLimit
+  CTL_LIM = CTL_MOVE_LIMIT + 16,          // [16] move limit of the current adaptation
+  CTL_MOVES = CTL_LIM + 16,               // [16] matched moves per level
+  CTL_REFINES = CTL_MOVES + 16,           // [16] refinements of subblocks of level l (children at l-1)
+  CTL_LOADS_BEFORE = CTL_REFINES + 16,    // [16] loads before refinement
+  CTL_CAND = CTL_LOADS_BEFORE + 16,       // candidate count of the current selection
+  CTL_TOTAL_MOVES = CTL_CAND + 1,
+  CTL_ERROR = CTL_TOTAL_MOVES + 1,        // sticky error bits
+  CTL_WORDS = CTL_ERROR + 1
+};
+enum : uint32_t { ERR_HASH_FULL = 1, ERR_KEY_EXISTS = 2 };
+
+// Morton key of a block position, gridHash (src/dcgrid/dcgrid_utils.cuh:83-96).
+__device__ __forceinline__ uint32_t bitSpread3(uint32_t data) {
+  uint32_t res = 0;
+  for (uint32_t mask = 1u; mask; mask <<= 3, data <<= 2) res |= data & mask;
+  return res;
+}
+template <int BW>
+__device__ __forceinline__ uint32_t gridHash(int x, int y, int z) {
+  return (bitSpread3((uint32_t)(x / BW)) << 2) | (bitSpread3((uint32_t)(y / BW)) << 1) | bitSpread3((uint32_t)(z / BW));
+}
+
+__device__ __forceinline__ int dirIndex(int dx, int dy, int dz) { return (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1); }
+
+// Finest allocated block at level >= l0 whose region contains the level-l0 cell q (q inside
+// the domain). Equivalent to getBlockIndexDeep (dcgrid_utils.cuh:201-232), which walks the
+// per-level hash tables upward from l0; here the walk goes down the child links from the first
+// dense level, which reach exactly the allocated blocks.
+template <int BW>
+__device__ __forceinline__ uint32_t findBlock(const DG& g, int qx, int qy, int qz, int l0) {
+  constexpr int BB = Geo<BW>::BB;
+  int lv = l0 > g.S ? l0 : g.S;
+  int sh = lv - l0;
+  const int bx = (qx >> sh) >> BB, by = (qy >> sh) >> BB, bz = (qz >> sh) >> BB;
+  uint32_t b = g.offs[lv] + (uint32_t)((bx * g.ry[lv] + by) * g.rz[lv] + bz);
+  while (lv > l0) {
+    sh = lv - l0;
+    const int t = ((((qx >> sh) >> (BB - 1)) & 1) << 2) | ((((qy >> sh) >> (BB - 1)) & 1) << 1) |
+                  (((qz >> sh) >> (BB - 1)) & 1);
+    const uint32_t c = g.chi[8 * b + t];
+    if (c == NFU) break;
+    b = c;
+    lv--;
+  }
+  return b;
+}
+
+// Global cell index of apron entry (ai,aj,ak) in [0,BW+1]^3 of block b (level lb, origin p0):
+// the value the reference's cellIndices table holds for that entry.
+template <int BW>
+__device__ __forceinline__ uint32_t apronCell(const DG& g, uint32_t b, int3 p0, int lb, int ai, int aj, int ak) {
+  constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
+  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
+  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
+  if ((dx | dy | dz) == 0) return b * BV + loc<BB>(ai - 1, aj - 1, ak - 1);
+  const uint32_t nb = g.nbr[b * 27 + dirIndex(dx, dy, dz)];
+  if (nb == b) {
+    // Out-of-domain entry: own clamped cell. Blocks placed by a move or a refinement use the
+    // reference's SPREAD(clamp(i,1,BW)) form (dcgrid_structure.cu:54-60), dense blocks the
+    // clamped interior cell (dcgrid_structure.cu:157-167).
+    const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
+    if (g.edge[b]) return b * BV + loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
+    return b * BV + loc<BB>(ci - 1, cj - 1, ck - 1);
+  }
+  const int d = (int)g.lev[nb] - lb;
+  const int3 np = g.pos[nb];
+  return nb * BV + loc<BB>(((p0.x + ai - 1) >> d) - np.x, ((p0.y + aj - 1) >> d) - np.y, ((p0.z + ak - 1) >> d) - np.z);
+}
+
+// Apron coordinates of face cell f in [0, 6*BW*BW): faces x-,x+,y-,y+,z-,z+.
+template <int BW>
+__device__ __forceinline__ void faceCell(int f, int& ai, int& aj, int& ak) {
+  const int face = f / (BW * BW), uv = f % (BW * BW), u = 1 + uv / BW, v = 1 + uv % BW;
+  switch (face) {
+    case 0: ai = 0; aj = u; ak = v; break;
+    case 1: ai = BW + 1; aj = u; ak = v; break;
+    case 2: ai = u; aj = 0; ak = v; break;
+    case 3: ai = u; aj = BW + 1; ak = v; break;
+    case 4: ai = u; aj = v; ak = 0; break;
+    default: ai = u; aj = v; ak = BW + 1; break;
+  }
+}
+
+}  // namespace dcg

@@ -1,0 +1,538 @@
+// dcgrid_fluid.hpp — gfx950 fluid kernels of the DCGrid back-end: cross-level semi-Lagrangian
+// advection (src/dcgrid/dcgrid_fluid.cu:7-144), vorticity (:146-172), divergence (:174-230),
+// gradient subtraction (:232-259), Jacobi half-sweeps and prolongation
+// (src/dcgrid/dcgrid_multigrid_solver.cu), restriction (src/dcgrid/dcgrid_structure.cu:188-222),
+// the residual of debugStats (:224-251) and value propagation into new blocks
+// (src/dcgrid/dcgrid_adaptation.cu:92-143).
+//
+// Stencil kernels run one workgroup per block (BW=8: 512 threads) or per four blocks (BW=4:
+// 4x64 threads), one thread per cell. The block's cells are contiguous in HBM (coalesced) and
+// are staged with the six face layers of the apron in LDS, so each neighbour read is an LDS
+// read; face cells are fetched through the block's neighbour table.
+#pragma once
+
+#include "dcgrid_dev.hpp"
+
+namespace dcg {
+
+// Block handled by this thread's sub-group and the thread's cell inside it.
+template <int BW>
+struct BlockCtx {
+  uint32_t b;  // global block index (may be >= B for tail threads)
+  int t;       // local cell (Morton) index
+  int sub;     // sub-group (block slot) inside the workgroup
+  bool valid;
+};
+template <int BW>
+__device__ __forceinline__ BlockCtx<BW> blockCtx(uint32_t firstBlock, uint32_t count) {
+  constexpr int BV = Geo<BW>::BV, BPW = Geo<BW>::BPW;
+  BlockCtx<BW> c;
+  c.sub = threadIdx.x / BV;
+  c.t = threadIdx.x % BV;
+  const uint32_t slot = xcdRemap(blockIdx.x, gridDim.x) * BPW + c.sub;
+  c.b = firstBlock + slot;
+  c.valid = slot < count;
+  return c;
+}
+
+template <int BW>
+__device__ __forceinline__ int apronIndexOfLocal(int t) {
+  constexpr int A = Geo<BW>::A, BB = Geo<BW>::BB;
+  int x, y, z;
+  locPos<BB>((uint32_t)t, x, y, z);
+  return ((x + 1) * A + (y + 1)) * A + (z + 1);
+}
+
+// Stage one scalar field of a block (interior + 6 face layers) into tile[A^3].
+template <int BW>
+__device__ __forceinline__ void stageScalar(const DG& g, const float* __restrict__ f, float* tile, uint32_t b, int3 p0,
+                                            int lb, int t) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE;
+  tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
+  for (int fc = t; fc < FACE; fc += BV) {
+    int ai, aj, ak;
+    faceCell<BW>(fc, ai, aj, ak);
+    tile[(ai * A + aj) * A + ak] = f[apronCell<BW>(g, b, p0, lb, ai, aj, ak)];
+  }
+}
+
+// Fluidity of the six apron neighbours of local cell t (1 when the field is known to be 1).
+template <int BW, bool FLUIDITY>
+__device__ __forceinline__ void neighbourFluidity(const DG& g, uint32_t b, int3 p0, int lb, int t, float w[6]) {
+  if (!FLUIDITY) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) w[k] = 1.f;
+    return;
+  }
+  constexpr int BB = Geo<BW>::BB;
+  int x, y, z;
+  locPos<BB>((uint32_t)t, x, y, z);
+  const int ai = x + 1, aj = y + 1, ak = z + 1;
+  w[0] = g.flu[apronCell<BW>(g, b, p0, lb, ai - 1, aj, ak)];  // l
+  w[1] = g.flu[apronCell<BW>(g, b, p0, lb, ai + 1, aj, ak)];  // r
+  w[2] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj - 1, ak)];  // d
+  w[3] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj + 1, ak)];  // u
+  w[4] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj, ak - 1)];  // b
+  w[5] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj, ak + 1)];  // f
+}
+
+// ------------------------------------------------------------------ Jacobi half-sweep
+// k_dcgrid_jacobi / k_dcgrid_jacobi_inv (dcgrid_multigrid_solver.cu:5-41):
+// out = (sum of 6 apron neighbours of `in` - (scale*dx)^2 * div) / 6 on every cell of every
+// block of the level. Neighbours on coarser levels are read from the same array `in`.
+template <int BW>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(DG g, int level, const float* __restrict__ in,
+                                                         float* __restrict__ out, float alpha) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  __shared__ float tiles[BPW][A3];
+  const BlockCtx<BW> c = blockCtx<BW>(g.offs[level], g.maxL[level]);
+  const bool active = c.valid && g.lev[c.b] == level;
+  float* tile = tiles[c.sub];
+  int3 p0 = make_int3(0, 0, 0);
+  if (active) {
+    p0 = g.pos[c.b];
+    stageScalar<BW>(g, in, tile, c.b, p0, level, c.t);
+  }
+  __syncthreads();
+  if (!active) return;
+  const int a = apronIndexOfLocal<BW>(c.t);
+  const uint32_t cell = c.b * BV + c.t;
+  out[cell] = (tile[a - A * A] + tile[a + A * A] + tile[a - A] + tile[a + A] + tile[a - 1] + tile[a + 1] -
+               alpha * g.div[cell]) /
+              6.f;
+}
+
+// ------------------------------------------------------------------ divergence
+// k_dcgrid_calc_divergence (dcgrid_fluid.cu:174-230): zero p, p~ and div on all cells; stage the
+// velocity with boundary conditions on the apron faces; central divergence on leaf cells.
+template <int BW, bool FLUIDITY>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(DG g, dcg_params P) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
+                BB = Geo<BW>::BB, SBB = Geo<BW>::SBB;
+  __shared__ float3 tiles[BPW][A3];
+  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
+  const bool active = c.valid && g.lev[c.b] != FREE;
+  float3* tile = tiles[c.sub];
+  int3 p0 = make_int3(0, 0, 0);
+  int lb = 0, scale = 1;
+  const uint32_t cell = c.b * BV + c.t;
+  if (active) {
+    lb = g.lev[c.b];
+    scale = 1 << lb;
+    p0 = g.pos[c.b];
+    g.pre[cell] = 0.f;
+    g.tpre[cell] = 0.f;
+    g.div[cell] = 0.f;
+    tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
+    for (int fc = c.t; fc < FACE; fc += BV) {
+      int ai, aj, ak;
+      faceCell<BW>(fc, ai, aj, ak);
+      const float3 v = g.vel[apronCell<BW>(g, c.b, p0, lb, ai, aj, ak)];
+      tile[(ai * A + aj) * A + ak] = velocityBC(P, v, p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
+    }
+  }
+  __syncthreads();
+  if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) return;
+  const float alpha = .5f * P.rdx / (float)scale;
+  float w[6];
+  neighbourFluidity<BW, FLUIDITY>(g, c.b, p0, lb, c.t, w);
+  const int a = apronIndexOfLocal<BW>(c.t);
+  const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
+               vf = tile[a + 1];
+  g.div[cell] = alpha * (w[1] * vr.x - w[0] * vl.x + w[3] * vu.y - w[2] * vd.y + w[5] * vf.z - w[4] * vb.z);
+  (void)BB;
+}
+
+// ------------------------------------------------------------------ gradient subtraction
+// k_dcgrid_apply_pressure (dcgrid_fluid.cu:232-259), leaf cells only.
+template <int BW, bool FLUIDITY>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(DG g, dcg_params P) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, SBB = Geo<BW>::SBB;
+  __shared__ float tiles[BPW][A3];
+  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
+  const bool active = c.valid && g.lev[c.b] != FREE;
+  float* tile = tiles[c.sub];
+  int3 p0 = make_int3(0, 0, 0);
+  int lb = 0;
+  if (active) {
+    lb = g.lev[c.b];
+    p0 = g.pos[c.b];
+    stageScalar<BW>(g, g.pre, tile, c.b, p0, lb, c.t);
+  }
+  __syncthreads();
+  if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) return;
+  const float alpha = .5f * P.rdx / (float)(1 << lb);
+  float w[6];
+  neighbourFluidity<BW, FLUIDITY>(g, c.b, p0, lb, c.t, w);
+  const int a = apronIndexOfLocal<BW>(c.t);
+  const uint32_t cell = c.b * BV + c.t;
+  const float p = tile[a];
+  float3 v = g.vel[cell];
+  v.x = v.x - alpha * (w[1] * (tile[a + A * A] - p) + w[0] * (p - tile[a - A * A]));
+  v.y = v.y - alpha * (w[3] * (tile[a + A] - p) + w[2] * (p - tile[a - A]));
+  v.z = v.z - alpha * (w[5] * (tile[a + 1] - p) + w[4] * (p - tile[a - 1]));
+  g.vel[cell] = v;
+}
+
+// ------------------------------------------------------------------ vorticity
+// k_dcgrid_calc_vorticity (dcgrid_fluid.cu:146-172): curl of fluidity-weighted velocity on the
+// apron, every cell of every allocated block (no boundary conditions).
+template <int BW, bool FLUIDITY>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(DG g, dcg_params P) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE;
+  __shared__ float3 tiles[BPW][A3];
+  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
+  const bool active = c.valid && g.lev[c.b] != FREE;
+  float3* tile = tiles[c.sub];
+  int3 p0 = make_int3(0, 0, 0);
+  int lb = 0;
+  const uint32_t cell = c.b * BV + c.t;
+  if (active) {
+    lb = g.lev[c.b];
+    p0 = g.pos[c.b];
+    tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
+    for (int fc = c.t; fc < FACE; fc += BV) {
+      int ai, aj, ak;
+      faceCell<BW>(fc, ai, aj, ak);
+      tile[(ai * A + aj) * A + ak] = g.vel[apronCell<BW>(g, c.b, p0, lb, ai, aj, ak)];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const float alpha = .5f * P.rdx / (float)(1 << lb);
+  float w[6];
+  neighbourFluidity<BW, FLUIDITY>(g, c.b, p0, lb, c.t, w);
+  const int a = apronIndexOfLocal<BW>(c.t);
+  const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
+               vf = tile[a + 1];
+  const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
+  g.vort[cell] = make_float3(alpha * ((wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y)),
+                             alpha * ((wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z)),
+                             alpha * ((wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)));
+}
+
+// ------------------------------------------------------------------ restriction
+// accumulate<T> (dcgrid_structure.cu:188-222): a parent cell becomes the mean of the 8 child
+// cells of one restriction group (the lowest Morton triple of the child block). One thread per
+// group; reads are 8 consecutive cells.
+template <class T>
+__device__ __forceinline__ T zeroOf();
+template <>
+__device__ __forceinline__ float zeroOf<float>() { return 0.f; }
+template <>
+__device__ __forceinline__ float3 zeroOf<float3>() { return make_float3(0.f, 0.f, 0.f); }
+__device__ __forceinline__ float addv(float a, float b) { return a + b; }
+__device__ __forceinline__ float3 addv(float3 a, float3 b) { return f3add(a, b); }
+__device__ __forceinline__ float mulv(float a, float s) { return a * s; }
+__device__ __forceinline__ float3 mulv(float3 a, float s) { return f3mul(a, s); }
+
+template <int BW, class T>
+__global__ void __launch_bounds__(256) k_restrict(DG g, int level, T* __restrict__ ch) {
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, GPB = BV / 8;
+  const uint32_t gi = blockIdx.x * 256 + threadIdx.x;  // group within the level slice
+  const uint32_t bl = gi / GPB;
+  if (bl >= g.maxL[level]) return;
+  const uint32_t b = g.offs[level] + bl;
+  if (g.lev[b] != level) return;
+  const uint32_t ps = g.par[b];
+  if (ps == NFU) return;
+  const uint32_t grp = gi % GPB;
+  const uint32_t parentCell = (ps / 8) * BV + ((ps % 8) << SBB) + grp;
+  const T* src = ch + (size_t)b * BV + 8 * grp;
+  T acc = zeroOf<T>();
+#pragma unroll
+  for (int i = 0; i < 8; i++) acc = addv(acc, src[i]);
+  ch[parentCell] = mulv(acc, .125f);
+}
+
+// ------------------------------------------------------------------ parent stencil
+// Cells of the parent's apron around child cell (x,y,z) used by propagate and prolongate
+// (dcgrid_adaptation.cu:104-126, dcgrid_multigrid_solver.cu:51-63); order 000,001,010,100,011,
+// 101,110,111 of the reference.
+template <int BW>
+__device__ __forceinline__ void parentStencil(const DG& g, uint32_t ps, int x, int y, int z, uint32_t q[8]) {
+  const uint32_t pb = ps / 8;
+  const int3 pp = g.pos[pb];
+  const int pl = g.lev[pb];
+  const int ai = 1 + (x / 2) % BW, aj = 1 + (y / 2) % BW, ak = 1 + (z / 2) % BW;
+  const int si = x % 2 ? 1 : -1, sj = y % 2 ? 1 : -1, sk = z % 2 ? 1 : -1;
+  q[0] = apronCell<BW>(g, pb, pp, pl, ai, aj, ak);
+  q[1] = apronCell<BW>(g, pb, pp, pl, ai, aj, ak + sk);
+  q[2] = apronCell<BW>(g, pb, pp, pl, ai, aj + sj, ak);
+  q[3] = apronCell<BW>(g, pb, pp, pl, ai + si, aj, ak);
+  q[4] = apronCell<BW>(g, pb, pp, pl, ai, aj + sj, ak + sk);
+  q[5] = apronCell<BW>(g, pb, pp, pl, ai + si, aj, ak + sk);
+  q[6] = apronCell<BW>(g, pb, pp, pl, ai + si, aj + sj, ak);
+  q[7] = apronCell<BW>(g, pb, pp, pl, ai + si, aj + sj, ak + sk);
+}
+
+// k_dcgrid_prolongate (dcgrid_multigrid_solver.cu:43-76)
+template <int BW>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(DG g, int level) {
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
+  const BlockCtx<BW> c = blockCtx<BW>(g.offs[level], g.maxL[level]);
+  if (!c.valid || g.lev[c.b] != level) return;
+  const uint32_t ps = g.par[c.b];
+  if (ps == NFU) return;
+  const int3 p0 = g.pos[c.b];
+  int lx, ly, lz;
+  locPos<BB>((uint32_t)c.t, lx, ly, lz);
+  uint32_t q[8];
+  parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
+  const float* p = g.pre;
+  g.pre[c.b * BV + c.t] = (27.f * p[q[0]] + 9.f * (p[q[1]] + p[q[2]] + p[q[3]]) + 3.f * (p[q[4]] + p[q[5]] + p[q[6]]) +
+                           p[q[7]]) /
+                          64.f;
+}
+
+// k_dcgrid_propagate_values (dcgrid_adaptation.cu:92-143) for the blocks of `level` touched by
+// this adaptation: the moved blocks (list) and the new children (a contiguous index range).
+template <int BW, bool SRC>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_propagate(DG g, dcg_params P, int level,
+                                                            const uint32_t* __restrict__ moved) {
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW;
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  const uint32_t e = blockIdx.x * BPW + sub;
+  const uint32_t nm = g.ctl[CTL_MOVES + level];
+  const uint32_t nc = level + 1 < g.L ? g.ctl[CTL_REFINES + level + 1] : 0;
+  uint32_t b;
+  if (e < nm)
+    b = moved[g.offs[level] + e];
+  else if (e < nm + nc)
+    b = g.offs[level] + g.ctl[CTL_LOADS_BEFORE + level] + (e - nm);
+  else
+    return;
+  if (g.lev[b] != level) return;
+  const uint32_t ps = g.par[b];
+  if (ps == NFU) return;
+  const int3 p0 = g.pos[b];
+  int lx, ly, lz;
+  locPos<BB>((uint32_t)t, lx, ly, lz);
+  const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
+  uint32_t q[8];
+  parentStencil<BW>(g, ps, x, y, z, q);
+  const float w27 = 27.f / 64.f, w9 = 9.f / 64.f, w3 = 3.f / 64.f, w1 = 1.f / 64.f;
+  const uint32_t cell = b * BV + t;
+  const float* d = g.den;
+  g.den[cell] = w27 * d[q[0]] + w9 * (d[q[1]] + d[q[2]] + d[q[3]]) + w3 * (d[q[4]] + d[q[5]] + d[q[6]]) + w1 * d[q[7]];
+  if (SRC) {
+    const float* tt = g.tem;
+    g.tem[cell] =
+        w27 * tt[q[0]] + w9 * (tt[q[1]] + tt[q[2]] + tt[q[3]]) + w3 * (tt[q[4]] + tt[q[5]] + tt[q[6]]) + w1 * tt[q[7]];
+  }
+  const float3* v = g.vel;
+  const float3 a0 = f3mul(v[q[0]], w27);
+  const float3 a1 = f3mul(f3add(f3add(v[q[1]], v[q[2]]), v[q[3]]), w9);
+  const float3 a2 = f3mul(f3add(f3add(v[q[4]], v[q[5]]), v[q[6]]), w3);
+  const float3 a3 = f3mul(v[q[7]], w1);
+  g.vel[cell] = f3add(f3add(f3add(a0, a1), a2), a3);
+  g.flu[cell] = cellFluidity(P, x, y, z, 1 << level);
+}
+
+// ------------------------------------------------------------------ advection
+// INIT_SAMPLE (dcgrid_fluid.cu:7-72): block containing the foot point, its apron 2x2x2 stencil
+// (clamped into the apron), fluidity-weighted normalised trilinear weights.
+template <int BW, bool FLUIDITY>
+struct CrossSample {
+  uint32_t i[8];
+  float w[8];
+  int x0, y0, z0, scale;
+  bool zero;
+  __device__ __forceinline__ void init(const DG& g, float px, float py, float pz) {
+    const int ix = clampi((int)floorf(px), 0, g.gx - 1);
+    const int iy = clampi((int)floorf(py), 0, g.gy - 1);
+    const int iz = clampi((int)floorf(pz), 0, g.gz - 1);
+    const uint32_t sb = findBlock<BW>(g, ix, iy, iz, 0);
+    const int sl = g.lev[sb];
+    const int3 sp = g.pos[sb];
+    scale = 1 << sl;
+    const float scaleInv = 1.f / (float)scale;
+    const float x = px * scaleInv - .5f, y = py * scaleInv - .5f, z = pz * scaleInv - .5f;
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    const float dx = x - xf, dy = y - yf, dz = z - zf;
+    x0 = (int)xf;
+    y0 = (int)yf;
+    z0 = (int)zf;
+    const int ai = clampi(x0 + 1 - sp.x, 0, BW), aj = clampi(y0 + 1 - sp.y, 0, BW), ak = clampi(z0 + 1 - sp.z, 0, BW);
+#pragma unroll
+    for (int c = 0; c < 8; c++) i[c] = apronCell<BW>(g, sb, sp, sl, ai + ((c >> 2) & 1), aj + ((c >> 1) & 1), ak + (c & 1));
+    const float Dx = 1.f - dx, Dy = 1.f - dy, Dz = 1.f - dz;
+    float ww[8];
+    if (FLUIDITY) {
+      ww[0] = g.flu[i[0]] * Dx * Dy * Dz;
+      ww[1] = g.flu[i[1]] * Dx * Dy * dz;
+      ww[2] = g.flu[i[2]] * Dx * dy * Dz;
+      ww[3] = g.flu[i[3]] * Dx * dy * dz;
+      ww[4] = g.flu[i[4]] * dx * Dy * Dz;
+      ww[5] = g.flu[i[5]] * dx * Dy * dz;
+      ww[6] = g.flu[i[6]] * dx * dy * Dz;
+      ww[7] = g.flu[i[7]] * dx * dy * dz;
+    } else {
+      ww[0] = Dx * Dy * Dz;
+      ww[1] = Dx * Dy * dz;
+      ww[2] = Dx * dy * Dz;
+      ww[3] = Dx * dy * dz;
+      ww[4] = dx * Dy * Dz;
+      ww[5] = dx * Dy * dz;
+      ww[6] = dx * dy * Dz;
+      ww[7] = dx * dy * dz;
+    }
+    const float wAcc = ww[0] + ww[1] + ww[2] + ww[3] + ww[4] + ww[5] + ww[6] + ww[7];
+    const float wInv = 1.f / wAcc;
+#pragma unroll
+    for (int c = 0; c < 8; c++) w[c] = ww[c] * wInv;
+    zero = wAcc < 1e-6f;
+  }
+};
+
+// Backtraced foot point of a leaf cell (dcgrid_fluid.cu:115-122).
+template <int BW>
+__device__ __forceinline__ void footPoint(const DG& g, const dcg_params& P, uint32_t b, int t, float3 v, float& px,
+                                          float& py, float& pz, int& x, int& y, int& z, int& level) {
+  constexpr int BB = Geo<BW>::BB;
+  const int3 p0 = g.pos[b];
+  level = g.lev[b];
+  const float scale = (float)(1 << level);
+  const float alpha = P.dt * P.rdx;
+  int lx, ly, lz;
+  locPos<BB>((uint32_t)t, lx, ly, lz);
+  x = p0.x | lx;
+  y = p0.y | ly;
+  z = p0.z | lz;
+  px = ((float)x + .5f) * scale - v.x * alpha;
+  py = ((float)y + .5f) * scale - v.y * alpha;
+  pz = ((float)z + .5f) * scale - v.z * alpha;
+}
+
+// k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension).
+template <int BW, bool FLUIDITY, bool BUOY>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_params P) {
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
+  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
+  if (!c.valid || g.lev[c.b] == FREE) return;
+  const uint32_t cell = c.b * BV + c.t;
+  if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
+    g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
+    return;
+  }
+  float px, py, pz;
+  int x, y, z, level;
+  footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
+  CrossSample<BW, FLUIDITY> s;
+  s.init(g, px, py, pz);
+  float3 acc = make_float3(0.f, 0.f, 0.f);
+  if (!s.zero) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const float3 vk = velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
+      acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
+    }
+  }
+  if (BUOY) {
+    const float bf = P.dt * (P.buoy_beta * (g.tem[cell] - P.t_ambient) - P.buoy_alpha * g.den[cell]);
+    acc.y = acc.y + bf;
+  }
+  g.tvel[cell] = acc;
+}
+
+// k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension).
+template <int BW, bool FLUIDITY, bool SRC>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params P) {
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
+  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
+  if (!c.valid || g.lev[c.b] == FREE) return;
+  const uint32_t cell = c.b * BV + c.t;
+  if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
+    g.tden[cell] = 0.f;
+    if (SRC) g.ttem[cell] = 0.f;
+    return;
+  }
+  float px, py, pz;
+  int x, y, z, level;
+  footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
+  CrossSample<BW, FLUIDITY> s;
+  s.init(g, px, py, pz);
+  float q = 0.f, tq = 0.f;
+  if (!s.zero) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
+      const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
+      q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
+      if (SRC) {
+        const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
+        tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
+      }
+    }
+  }
+  if (SRC) {
+    const float sc = (float)(1 << level);
+    if (inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
+      q = P.src_density;
+      tq = sourceTemperature(P, x, y, z, level);
+    }
+    g.ttem[cell] = tq;
+  }
+  g.tden[cell] = q;
+}
+
+// ------------------------------------------------------------------ residual (debugStats)
+// k_dcgrid_debug_stats (dcgrid_structure.cu:224-251): per block, sum over leaf cells in apron
+// [i][j][k] order of scale*|div - (sum p_n - 6p)*rdx^2/scale^2|. One thread per block keeps the
+// reference's summation order.
+template <int BW>
+__global__ void __launch_bounds__(64) k_residual(DG g, dcg_params P, float* __restrict__ stats) {
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
+  const uint32_t b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= g.B) return;
+  if (g.lev[b] == FREE) {
+    stats[b] = 0.f;
+    return;
+  }
+  const int lb = g.lev[b];
+  const int scale = 1 << lb;
+  const int3 p0 = g.pos[b];
+  const float alpha = P.rdx * P.rdx / (float)(scale * scale);
+  float s = 0.f;
+  for (int i = 1; i <= BW; i++)
+    for (int j = 1; j <= BW; j++)
+      for (int k = 1; k <= BW; k++) {
+        const uint32_t cidx = apronCell<BW>(g, b, p0, lb, i, j, k);
+        if (g.chi[cidx >> SBB] != NFU) continue;
+        const float* p = g.pre;
+        const float pc = p[cidx];
+        const float pl = p[apronCell<BW>(g, b, p0, lb, i - 1, j, k)], pr = p[apronCell<BW>(g, b, p0, lb, i + 1, j, k)];
+        const float pd = p[apronCell<BW>(g, b, p0, lb, i, j - 1, k)], pu = p[apronCell<BW>(g, b, p0, lb, i, j + 1, k)];
+        const float pb = p[apronCell<BW>(g, b, p0, lb, i, j, k - 1)], pf = p[apronCell<BW>(g, b, p0, lb, i, j, k + 1)];
+        const float r = g.div[cidx] - (pl + pr + pd + pu + pb + pf - 6.f * pc) * alpha;
+        s = s + (float)scale * fabsf(r);
+      }
+  stats[b] = s;
+  (void)BV;
+}
+
+// ------------------------------------------------------------------ seeded scene (extension)
+template <int BW>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_seed(DG g, dcg_params P) {
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
+  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
+  if (!c.valid || g.lev[c.b] == FREE) return;
+  const int level = g.lev[c.b];
+  const float sc = (float)(1 << level);
+  const int3 p0 = g.pos[c.b];
+  int lx, ly, lz;
+  locPos<BB>((uint32_t)c.t, lx, ly, lz);
+  const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
+  const float cx = ((float)x + .5f) * sc, cy = ((float)y + .5f) * sc, cz = ((float)z + .5f) * sc;
+  const uint32_t cell = c.b * BV + c.t;
+  g.vel[cell] = initVelocity(P, cx, cy, cz);
+  if (P.source_mode == DCG_SOURCE_SPHERE_BUOYANT && inSource(P, cx, cy, cz)) {
+    g.den[cell] = P.src_density;
+    g.tem[cell] = sourceTemperature(P, x, y, z, level);
+  } else {
+    g.den[cell] = 0.f;
+    g.tem[cell] = P.t_ambient;
+  }
+}
+
+}  // namespace dcg

@@ -1,0 +1,538 @@
+// uniform.hip — gfx950 kernels and host driver of the dense (uniform) grid back-end, the
+// counterpart of FluidSimulationUniform (src/uniformgrid/fluid_simulation_uniform.cu) and its
+// kernels (src/uniformgrid/uniformgrid_fluid.cu, uniformgrid_structure.cu).
+//
+// Layout: x-fastest SoA scalars, float3 AoS velocity (src/utils/grid_math.cuh:10); pressure,
+// fluidity and divergence carry the mip pyramid (grid_math.cuh:24-52). One thread per cell,
+// 256-thread workgroups over the linear index so every wavefront reads 256 contiguous bytes.
+#include <algorithm>
+#include <cstring>
+
+#include "dcg_device.hpp"
+#include "dcg_internal.hpp"
+
+namespace dcg {
+
+struct UGrid {
+  int gx, gy, gz;
+  float* den;
+  float* tem;
+  float3* vel;
+  float* flu;  // mip
+  float* pre;  // mip
+  float* tpre; // mip
+  float* div;  // mip
+  float* tden;
+  float* ttem;
+  float3* tvel;
+};
+
+struct UDims {
+  int w, h, d;
+  size_t off;
+};
+
+__device__ __forceinline__ size_t uidx(const UGrid& g, int x, int y, int z) {
+  return ((size_t)z * g.gy + y) * g.gx + x;
+}
+
+// k_uniform_set_solidity_ratio, uniformgrid_structure.cu:23-31
+__global__ void __launch_bounds__(256) k_u_fluidity(dcg_params P, UGrid g, UDims L, int scale) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+  g.flu[L.off + i] = cellFluidity(P, x, y, z, scale);
+}
+
+// INIT_SAMPLE (uniformgrid_fluid.cu:7-48): 8 clamped corner indices and normalised weights.
+struct USample {
+  size_t i[8];
+  float w[8];
+  int x0, y0, z0;
+  bool zero;
+};
+template <bool FLUIDITY>
+__device__ __forceinline__ void usample(const UGrid& g, float px, float py, float pz, USample& s) {
+  const float x = px - .5f, y = py - .5f, z = pz - .5f;
+  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+  const int x0 = (int)xf, y0 = (int)yf, z0 = (int)zf;
+  const int x1 = x0 + 1, y1 = y0 + 1, z1 = z0 + 1;
+  const int x0c = clampi(x0, 0, g.gx - 1), y0c = clampi(y0, 0, g.gy - 1), z0c = clampi(z0, 0, g.gz - 1);
+  const int x1c = clampi(x1, 0, g.gx - 1), y1c = clampi(y1, 0, g.gy - 1), z1c = clampi(z1, 0, g.gz - 1);
+  s.i[0] = uidx(g, x0c, y0c, z0c);
+  s.i[1] = uidx(g, x0c, y0c, z1c);
+  s.i[2] = uidx(g, x0c, y1c, z0c);
+  s.i[3] = uidx(g, x0c, y1c, z1c);
+  s.i[4] = uidx(g, x1c, y0c, z0c);
+  s.i[5] = uidx(g, x1c, y0c, z1c);
+  s.i[6] = uidx(g, x1c, y1c, z0c);
+  s.i[7] = uidx(g, x1c, y1c, z1c);
+  const float dx = x - xf, dy = y - yf, dz = z - zf;
+  const float Dx = 1.f - dx, Dy = 1.f - dy, Dz = 1.f - dz;
+  float w[8];
+  if (FLUIDITY) {
+    w[0] = g.flu[s.i[0]] * Dx * Dy * Dz;
+    w[1] = g.flu[s.i[1]] * Dx * Dy * dz;
+    w[2] = g.flu[s.i[2]] * Dx * dy * Dz;
+    w[3] = g.flu[s.i[3]] * Dx * dy * dz;
+    w[4] = g.flu[s.i[4]] * dx * Dy * Dz;
+    w[5] = g.flu[s.i[5]] * dx * Dy * dz;
+    w[6] = g.flu[s.i[6]] * dx * dy * Dz;
+    w[7] = g.flu[s.i[7]] * dx * dy * dz;
+  } else {  // fluidity == 1 exactly: 1*a == a, so the products are the same floats
+    w[0] = Dx * Dy * Dz;
+    w[1] = Dx * Dy * dz;
+    w[2] = Dx * dy * Dz;
+    w[3] = Dx * dy * dz;
+    w[4] = dx * Dy * Dz;
+    w[5] = dx * Dy * dz;
+    w[6] = dx * dy * Dz;
+    w[7] = dx * dy * dz;
+  }
+  const float wAcc = w[0] + w[1] + w[2] + w[3] + w[4] + w[5] + w[6] + w[7];
+  const float wInv = 1.f / wAcc;
+#pragma unroll
+  for (int c = 0; c < 8; c++) s.w[c] = w[c] * wInv;
+  s.zero = wAcc < 1e-6f;
+  s.x0 = x0;
+  s.y0 = y0;
+  s.z0 = z0;
+}
+
+// k_uniform_advect_velocity (uniformgrid_fluid.cu:50-67,88-95) + buoyancy (extension).
+template <bool FLUIDITY>
+__global__ void __launch_bounds__(256) k_u_advect_velocity(dcg_params P, UGrid g) {
+  const size_t n = (size_t)g.gx * g.gy * g.gz;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  const float3 v = g.vel[i];
+  const float px = ((float)x + .5f) - v.x * P.dt * P.rdx;
+  const float py = ((float)y + .5f) - v.y * P.dt * P.rdx;
+  const float pz = ((float)z + .5f) - v.z * P.dt * P.rdx;
+  USample s;
+  usample<FLUIDITY>(g, px, py, pz, s);
+  float3 acc = make_float3(0.f, 0.f, 0.f);
+  if (!s.zero) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const float3 vc = velocityBC(P, g.vel[s.i[c]], s.x0 + ((c >> 2) & 1), s.y0 + ((c >> 1) & 1), s.z0 + (c & 1), 1);
+      acc = c == 0 ? f3mul(vc, s.w[0]) : f3add(acc, f3mul(vc, s.w[c]));
+    }
+  }
+  if (P.source_mode == DCG_SOURCE_SPHERE_BUOYANT) {
+    const float b = P.dt * (P.buoy_beta * (g.tem[i] - P.t_ambient) - P.buoy_alpha * g.den[i]);
+    acc.y = acc.y + b;
+  }
+  g.tvel[i] = acc;
+}
+
+// k_uniform_advect_density (uniformgrid_fluid.cu:69-86,97-105) + temperature and the sphere
+// source (extension).
+template <bool FLUIDITY, bool SRC>
+__global__ void __launch_bounds__(256) k_u_advect_density(dcg_params P, UGrid g) {
+  const size_t n = (size_t)g.gx * g.gy * g.gz;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  const float3 v = g.vel[i];
+  const float px = ((float)x + .5f) - v.x * P.dt * P.rdx;
+  const float py = ((float)y + .5f) - v.y * P.dt * P.rdx;
+  const float pz = ((float)z + .5f) - v.z * P.dt * P.rdx;
+  USample s;
+  usample<FLUIDITY>(g, px, py, pz, s);
+  float q = 0.f, t = 0.f;
+  if (!s.zero) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int cx = s.x0 + ((c >> 2) & 1), cy = s.y0 + ((c >> 1) & 1), cz = s.z0 + (c & 1);
+      const float qc = densityBC(P, g.den[s.i[c]], cx, cy, cz, 1);
+      q = c == 0 ? qc * s.w[0] : q + qc * s.w[c];
+      if (SRC) {
+        const float tc = temperatureBC(P, g.tem[s.i[c]], cx, cy, cz, 1);
+        t = c == 0 ? tc * s.w[0] : t + tc * s.w[c];
+      }
+    }
+  }
+  if (SRC) {
+    if (inSource(P, (float)x + .5f, (float)y + .5f, (float)z + .5f)) {
+      q = P.src_density;
+      t = sourceTemperature(P, x, y, z, 0);
+    }
+    g.ttem[i] = t;
+  }
+  g.tden[i] = q;
+}
+
+// k_uniform_calc_divergence, uniformgrid_fluid.cu:107-132
+template <bool FLUIDITY>
+__global__ void __launch_bounds__(256) k_u_divergence(dcg_params P, UGrid g) {
+  const size_t n = (size_t)g.gx * g.gy * g.gz;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  const size_t pl = (size_t)g.gx * g.gy;
+  const size_t il = x > 0 ? i - 1 : i, ir = x < g.gx - 1 ? i + 1 : i;
+  const size_t id = y > 0 ? i - g.gx : i, iu = y < g.gy - 1 ? i + g.gx : i;
+  const size_t ib = z > 0 ? i - pl : i, iF = z < g.gz - 1 ? i + pl : i;
+  g.pre[i] = 0.f;
+  g.tpre[i] = 0.f;
+  const float wl = FLUIDITY ? g.flu[il] : 1.f, wr = FLUIDITY ? g.flu[ir] : 1.f;
+  const float wd = FLUIDITY ? g.flu[id] : 1.f, wu = FLUIDITY ? g.flu[iu] : 1.f;
+  const float wb = FLUIDITY ? g.flu[ib] : 1.f, wf = FLUIDITY ? g.flu[iF] : 1.f;
+  const float3 vl = velocityBC(P, g.vel[il], x - 1, y, z, 1);
+  const float3 vr = velocityBC(P, g.vel[ir], x + 1, y, z, 1);
+  const float3 vd = velocityBC(P, g.vel[id], x, y - 1, z, 1);
+  const float3 vu = velocityBC(P, g.vel[iu], x, y + 1, z, 1);
+  const float3 vb = velocityBC(P, g.vel[ib], x, y, z - 1, 1);
+  const float3 vf = velocityBC(P, g.vel[iF], x, y, z + 1, 1);
+  g.div[i] = (.5f * P.rdx) * (wr * vr.x - wl * vl.x + wu * vu.y - wd * vd.y + wf * vf.z - wb * vb.z);
+}
+
+// k_uniform_restrict, uniformgrid_fluid.cu:134-160
+__global__ void __launch_bounds__(256) k_u_restrict(UGrid g, UDims L, UDims Lc) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+  const size_t w = Lc.w, h = Lc.h;
+  const size_t i000 = Lc.off + ((size_t)(2 * z) * h + 2 * y) * w + 2 * x;
+  const size_t i001 = i000 + 1, i010 = i000 + w, i011 = i010 + 1;
+  const size_t i100 = i000 + w * h, i101 = i100 + 1, i110 = i100 + w, i111 = i110 + 1;
+  const size_t o = L.off + i;
+  g.pre[o] = 0.f;
+  g.tpre[o] = 0.f;
+  g.div[o] = .125f * (g.div[i000] + g.div[i001] + g.div[i010] + g.div[i011] + g.div[i100] + g.div[i101] +
+                      g.div[i110] + g.div[i111]);
+}
+
+// calcPressure / k_uniform_jacobi(_inv), uniformgrid_fluid.cu:162-204. The workgroup covers a
+// 64x4 tile of an x-y plane so the y neighbours are in the same L1 lines.
+__global__ void __launch_bounds__(256) k_u_jacobi(const float* __restrict__ in, float* __restrict__ out,
+                                                  const float* __restrict__ div, UDims L, float alpha) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+  const size_t pl = (size_t)L.w * L.h;
+  const size_t il = x > 0 ? i - 1 : i, ir = x < L.w - 1 ? i + 1 : i;
+  const size_t id = y > 0 ? i - L.w : i, iu = y < L.h - 1 ? i + L.w : i;
+  const size_t ib = z > 0 ? i - pl : i, iF = z < L.d - 1 ? i + pl : i;
+  const float* p = in + L.off;
+  out[L.off + i] = (p[il] + p[ir] + p[id] + p[iu] + p[ib] + p[iF] - alpha * div[L.off + i]) / 6.f;
+}
+
+// k_uniform_prolongate, uniformgrid_fluid.cu:206-237
+__global__ void __launch_bounds__(256) k_u_prolongate(float* __restrict__ pre, UDims L, UDims Lp) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+  const int w = L.w, h = L.h, d = L.d;
+  const size_t i000 = Lp.off + ((size_t)(z / 2) * Lp.h + (y / 2)) * Lp.w + (x / 2);
+  const int sx = (x == 0 || x == w - 1) ? 0 : 2 * (x % 2) - 1;
+  const int sy = (y == 0 || y == h - 1) ? 0 : 2 * (y % 2) - 1;
+  const int sz = (z == 0 || z == d - 1) ? 0 : 2 * (z % 2) - 1;
+  const size_t i001 = i000 + sx;
+  const size_t i010 = i000 + sy * w / 2;
+  const size_t i011 = i010 + sx;
+  const size_t i100 = i000 + sz * (w / 2) * (h / 2);
+  const size_t i101 = i100 + sx;
+  const size_t i110 = i100 + sy * (w / 2);
+  const size_t i111 = i110 + sx;
+  pre[L.off + i] = (27.f * pre[i000] + 9.f * (pre[i001] + pre[i010] + pre[i100]) +
+                    3.f * (pre[i011] + pre[i101] + pre[i110]) + pre[i111]) /
+                   64.f;
+}
+
+// k_uniform_apply_pressure, uniformgrid_fluid.cu:239-260
+template <bool FLUIDITY>
+__global__ void __launch_bounds__(256) k_u_apply_pressure(dcg_params P, UGrid g) {
+  const size_t n = (size_t)g.gx * g.gy * g.gz;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  const size_t pl = (size_t)g.gx * g.gy;
+  const size_t il = x > 0 ? i - 1 : i, ir = x < g.gx - 1 ? i + 1 : i;
+  const size_t id = y > 0 ? i - g.gx : i, iu = y < g.gy - 1 ? i + g.gx : i;
+  const size_t ib = z > 0 ? i - pl : i, iF = z < g.gz - 1 ? i + pl : i;
+  const float alpha = .5f * P.rdx;
+  const float p = g.pre[i];
+  const float fl = FLUIDITY ? g.flu[il] : 1.f, fr = FLUIDITY ? g.flu[ir] : 1.f;
+  const float fd = FLUIDITY ? g.flu[id] : 1.f, fu = FLUIDITY ? g.flu[iu] : 1.f;
+  const float fb = FLUIDITY ? g.flu[ib] : 1.f, ff = FLUIDITY ? g.flu[iF] : 1.f;
+  float3 v = g.vel[i];
+  v.x = v.x - alpha * (fr * (g.pre[ir] - p) + fl * (p - g.pre[il]));
+  v.y = v.y - alpha * (fu * (g.pre[iu] - p) + fd * (p - g.pre[id]));
+  v.z = v.z - alpha * (ff * (g.pre[iF] - p) + fb * (p - g.pre[ib]));
+  g.vel[i] = v;
+}
+
+// k_uniform_debug_stats, uniformgrid_structure.cu:33-42: one bin of 256 cells per thread.
+__global__ void __launch_bounds__(256) k_u_stats(UGrid g, float* __restrict__ stats, size_t numBins) {
+  const size_t b = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= numBins) return;
+  float s = 0.f;
+  for (size_t k = 0; k < 256; k++) s = s + g.den[b * 256 + k] * g.flu[b * 256 + k];
+  stats[b] = s;
+}
+
+// Seeded initial condition (extension).
+__global__ void __launch_bounds__(256) k_u_seed(dcg_params P, UGrid g) {
+  const size_t n = (size_t)g.gx * g.gy * g.gz;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  const float cx = (float)x + .5f, cy = (float)y + .5f, cz = (float)z + .5f;
+  g.vel[i] = initVelocity(P, cx, cy, cz);
+  if (P.source_mode == DCG_SOURCE_SPHERE_BUOYANT && inSource(P, cx, cy, cz)) {
+    g.den[i] = P.src_density;
+    g.tem[i] = sourceTemperature(P, x, y, z, 0);
+  } else {
+    g.den[i] = 0.f;
+    g.tem[i] = P.t_ambient;
+  }
+}
+
+static unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+struct UniformSim : SimBase {
+  int gx = 0, gy = 0, gz = 0, L = 0;
+  size_t N = 0, MC = 0;
+  UGrid g{};
+  std::vector<UDims> dims;  // per mip level used by the solver (and one more for allocation)
+  float* stats = nullptr;
+  float* hstats = nullptr;
+  bool customFluidity = false;
+
+  int kind() const override { return DCG_KIND_UNIFORM; }
+
+  // FluidSimulationUniform ctor, fluid_simulation_uniform.cu:6-56
+  int create(const int32_t size[3], const dcg_params& p) {
+    P = p;
+    gx = size[0];
+    gy = size[1];
+    gz = size[2];
+    if (gx <= 0 || gy <= 0 || gz <= 0 || gx != P.gx || gy != P.gy || gz != P.gz)
+      return fail(DCG_E_ARG, "size must be positive and equal params gx,gy,gz");
+    const int minDim = std::min(gx, std::min(gy, gz));
+    int cellSize = 2;
+    L = 1;
+    while (gx % cellSize == 0 && gy % cellSize == 0 && gz % cellSize == 0 && cellSize * 4 <= minDim) {
+      L++;
+      cellSize *= 2;
+    }
+    N = (size_t)gx * gy * gz;
+    MC = 0;
+    for (size_t s = 1; gx % s == 0 && gy % s == 0 && gz % s == 0; s *= 2) MC += N / (s * s * s);
+    size_t off = 0;
+    for (int l = 0; l < L; l++) {
+      dims.push_back(UDims{gx >> l, gy >> l, gz >> l, off});
+      off += N >> (3 * l);
+    }
+    DCG_TRY(createStream());
+    g.gx = gx;
+    g.gy = gy;
+    g.gz = gz;
+    DCG_TRY(alloc(&g.den, N));
+    DCG_TRY(alloc(&g.tem, N));
+    DCG_TRY(alloc(&g.tden, N));
+    DCG_TRY(alloc(&g.ttem, N));
+    DCG_TRY(alloc(&g.vel, N));
+    DCG_TRY(alloc(&g.tvel, N));
+    DCG_TRY(alloc(&g.flu, MC));
+    DCG_TRY(alloc(&g.pre, MC));
+    DCG_TRY(alloc(&g.tpre, MC));
+    DCG_TRY(alloc(&g.div, MC));
+    const size_t nb = N / 256;
+    DCG_TRY(alloc(&stats, nb > 0 ? nb : 1));
+    DCG_TRY(fill32(g.div, 0, MC));
+    DCG_TRY(fill32(g.tpre, 0, MC));
+    return reset();
+  }
+
+  bool fluidityOn() const { return P.enable_additional_solids || customFluidity; }
+
+  // reset(), fluid_simulation_uniform.cu:81-88
+  int reset() override {
+    DCG_TRY(fill32(g.den, 0, N));
+    DCG_TRY(fill32(g.tem, 0, N));
+    DCG_TRY(fill32(g.vel, 0, 3 * N));
+    DCG_TRY(fill32(g.pre, 0, MC));
+    DCG_TRY(fill32(g.flu, 0, MC));
+    customFluidity = false;
+    return init();
+  }
+  // init(), fluid_simulation_uniform.cu:76-79
+  int init() override {
+    DCG_TRY(adaptTopology());
+    DCG_TRY(fill32(g.den, 0, N));
+    DCG_TRY(fill32(g.vel, 0, 3 * N));
+    uint32_t ta;
+    std::memcpy(&ta, &P.t_ambient, 4);
+    DCG_TRY(fill32(g.tem, ta, N));
+    return DCG_OK;
+  }
+  // adaptTopology(): fluidity per level, fluid_simulation_uniform.cu:143-147
+  int adaptTopology() override {
+    for (int l = 0; l < L; l++) {
+      const UDims& D = dims[l];
+      const size_t n = (size_t)D.w * D.h * D.d;
+      DCG_LAUNCH(DCG_K_OTHER, "k_u_fluidity", (k_u_fluidity<<<nblk(n), 256, 0, stream>>>(P, g, D, 1 << l)));
+    }
+    customFluidity = false;
+    return DCG_OK;
+  }
+  int advectVelocity() override {
+    if (fluidityOn())
+      DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_u_advect_velocity", (k_u_advect_velocity<true><<<nblk(N), 256, 0, stream>>>(P, g)));
+    else
+      DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_u_advect_velocity", (k_u_advect_velocity<false><<<nblk(N), 256, 0, stream>>>(P, g)));
+    std::swap(g.vel, g.tvel);  // replaces the D2D copy of uniform.cu:91-92
+    return DCG_OK;
+  }
+  int advectDensity() override {
+    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const bool fl = fluidityOn();
+#define U_ADV_D(F, S) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_u_advect_density", (k_u_advect_density<F, S><<<nblk(N), 256, 0, stream>>>(P, g)))
+    if (fl && src) U_ADV_D(true, true);
+    else if (fl) U_ADV_D(true, false);
+    else if (src) U_ADV_D(false, true);
+    else U_ADV_D(false, false);
+#undef U_ADV_D
+    std::swap(g.den, g.tden);
+    if (src) std::swap(g.tem, g.ttem);
+    return DCG_OK;
+  }
+  int divergence() {
+    if (fluidityOn())
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_u_divergence", (k_u_divergence<true><<<nblk(N), 256, 0, stream>>>(P, g)));
+    else
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_u_divergence", (k_u_divergence<false><<<nblk(N), 256, 0, stream>>>(P, g)));
+    return DCG_OK;
+  }
+  int jacobiPairs(int level, int pairs) {
+    const UDims& D = dims[level];
+    const size_t n = (size_t)D.w * D.h * D.d;
+    const int scale = 1 << level;
+    // alpha = dx*dx*scale*scale, uniformgrid_fluid.cu:171
+    const float alpha = P.dx * P.dx * (float)scale * (float)scale;
+    for (int i = 0; i < pairs; i++) {
+      DCG_LAUNCH(DCG_K_JACOBI, "k_u_jacobi", (k_u_jacobi<<<nblk(n), 256, 0, stream>>>(g.pre, g.tpre, g.div, D, alpha)));
+      DCG_LAUNCH(DCG_K_JACOBI, "k_u_jacobi", (k_u_jacobi<<<nblk(n), 256, 0, stream>>>(g.tpre, g.pre, g.div, D, alpha)));
+    }
+    return DCG_OK;
+  }
+  int applyPressure() {
+    if (fluidityOn())
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_u_apply_pressure", (k_u_apply_pressure<true><<<nblk(N), 256, 0, stream>>>(P, g)));
+    else
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_u_apply_pressure", (k_u_apply_pressure<false><<<nblk(N), 256, 0, stream>>>(P, g)));
+    return DCG_OK;
+  }
+  // project(), fluid_simulation_uniform.cu:96-124
+  int project() override {
+    DCG_TRY(divergence());
+    for (int l = 1; l < L; l++) {
+      const UDims& D = dims[l];
+      DCG_LAUNCH(DCG_K_RESTRICT, "k_u_restrict", (k_u_restrict<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(g, D, dims[l - 1])));
+    }
+    DCG_TRY(jacobiPairs(L - 1, P.project_top_pairs));
+    for (int l = L - 2; l >= 0; l--) {
+      const UDims& D = dims[l];
+      DCG_LAUNCH(DCG_K_PROLONGATE, "k_u_prolongate", (k_u_prolongate<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(g.pre, D, dims[l + 1])));
+      DCG_TRY(jacobiPairs(l, P.project_level_pairs));
+    }
+    return applyPressure();
+  }
+  // projectLocal(), fluid_simulation_uniform.cu:126-135
+  int projectLocal() override {
+    DCG_TRY(divergence());
+    DCG_TRY(jacobiPairs(0, P.project_local_pairs));
+    return applyPressure();
+  }
+  // debugStats(), fluid_simulation_uniform.cu:160-175
+  int debugStats(double* out) override {
+    const size_t nb = N / 256;
+    float sum = 0.f;
+    if (nb > 0) {
+      DCG_LAUNCH(DCG_K_OTHER, "k_u_stats", (k_u_stats<<<nblk(nb), 256, 0, stream>>>(g, stats, nb)));
+      std::vector<float> h(nb);
+      DCG_HIP(hipMemcpyAsync(h.data(), stats, nb * sizeof(float), hipMemcpyDeviceToHost, stream));
+      DCG_TRY(sync());
+      for (size_t b = 0; b < nb; b++) sum = sum + h[b];
+    }
+    *out = (double)sum;
+    return DCG_OK;
+  }
+  int seedScene() override {
+    DCG_LAUNCH(DCG_K_OTHER, "k_u_seed", (k_u_seed<<<nblk(N), 256, 0, stream>>>(P, g)));
+    return DCG_OK;
+  }
+  int info(dcg_info* o) override {
+    std::memset(o, 0, sizeof(*o));
+    o->kind = DCG_KIND_UNIFORM;
+    o->size[0] = gx;
+    o->size[1] = gy;
+    o->size[2] = gz;
+    o->levels = L;
+    o->num_cells = N;
+    o->active_cells = N;
+    return DCG_OK;
+  }
+  int fieldSize(int f, uint64_t* n) override {
+    switch (f) {
+      case DCG_FIELD_DENSITY:
+      case DCG_FIELD_TEMPERATURE: *n = N; return DCG_OK;
+      case DCG_FIELD_VELOCITY: *n = 3 * N; return DCG_OK;
+      case DCG_FIELD_PRESSURE:
+      case DCG_FIELD_T_PRESSURE:
+      case DCG_FIELD_DIVERGENCE:
+      case DCG_FIELD_FLUIDITY: *n = MC; return DCG_OK;
+      default: return fail(DCG_E_ARG, "field not available on the uniform grid");
+    }
+  }
+  float* fieldPtr(int f) {
+    switch (f) {
+      case DCG_FIELD_DENSITY: return g.den;
+      case DCG_FIELD_TEMPERATURE: return g.tem;
+      case DCG_FIELD_VELOCITY: return (float*)g.vel;
+      case DCG_FIELD_PRESSURE: return g.pre;
+      case DCG_FIELD_T_PRESSURE: return g.tpre;
+      case DCG_FIELD_DIVERGENCE: return g.div;
+      case DCG_FIELD_FLUIDITY: return g.flu;
+      default: return nullptr;
+    }
+  }
+  int exportField(int f, float* dst, uint64_t n) override {
+    uint64_t need;
+    DCG_TRY(fieldSize(f, &need));
+    if (n != need) return fail(DCG_E_ARG, "size mismatch");
+    DCG_HIP(hipMemcpyAsync(dst, fieldPtr(f), n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return sync();
+  }
+  int importField(int f, const float* src, uint64_t n) override {
+    uint64_t need;
+    DCG_TRY(fieldSize(f, &need));
+    if (n != need) return fail(DCG_E_ARG, "size mismatch");
+    DCG_HIP(hipMemcpyAsync(fieldPtr(f), src, n * sizeof(float), hipMemcpyHostToDevice, stream));
+    if (f == DCG_FIELD_FLUIDITY) customFluidity = true;
+    return sync();
+  }
+  int exportTables(dcg_tables*) override { return fail(DCG_E_ARG, "uniform grid has no block tables"); }
+};
+
+SimBase* makeUniform(const int32_t size[3], const dcg_params& p, int* status, std::string* err) {
+  auto* s = new UniformSim();
+  int st = s->create(size, p);
+  *status = st;
+  if (st != DCG_OK) {
+    *err = s->err;
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+}  // namespace dcg

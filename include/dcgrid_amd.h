@@ -1,0 +1,240 @@
+/*
+ * dcgrid_amd.h — C ABI of the MI355X (gfx950) DCGrid / uniform-grid simulation step.
+ *
+ * This is the drop-in boundary for the reference's abstract step interface
+ * `FluidSimulation` (src/fluid_simulation.h:4-26) and its two back-ends
+ * `FluidSimulationDCGrid` (src/dcgrid/fluid_simulation_dcgrid.h:5-61) and
+ * `FluidSimulationUniform` (src/uniformgrid/fluid_simulation_uniform.h:5-39),
+ * plus the global parameter upload `copySimParamsToDevice` (src/utils/sim_utils.h:6).
+ *
+ * Conventions
+ *  - Plain C: opaque handle, POD parameters, caller-owned host buffers, int status codes.
+ *  - No entry point ever calls exit(); failures return a negative DCG_E_* code and
+ *    dcg_last_error() describes the last failure of that handle.
+ *  - One host thread per handle. Calls are ordered on the handle's own HIP stream; calls
+ *    that return host data synchronise that stream.
+ *  - Parameters are per handle (the reference keeps them in one process-global
+ *    __constant__ block, src/utils/sim_utils.cu:6-9).
+ */
+#ifndef DCGRID_AMD_H
+#define DCGRID_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DCG_ABI_VERSION 1
+#define DCG_MAX_LEVELS 16
+
+/* Status codes. */
+enum {
+  DCG_OK = 0,
+  DCG_E_ARG = -1,         /* bad argument (null pointer, unknown field, bad size) */
+  DCG_E_BUDGET = -2,      /* block budget too small: src/dcgrid/fluid_simulation_dcgrid.cu:35-53 */
+  DCG_E_HIP = -3,         /* a HIP runtime call failed */
+  DCG_E_UNSUPPORTED = -4, /* configuration outside what this build implements */
+  DCG_E_HASH_FULL = -5,   /* per-level hash table full: src/dcgrid/dcgrid_utils.cuh:138 */
+  DCG_E_NOMEM = -6        /* device allocation failed */
+};
+
+/* Grid kinds reported by dcg_info. */
+enum { DCG_KIND_UNIFORM = 0, DCG_KIND_DCGRID = 1 };
+
+/* Field identifiers for export/import. Sizes are reported by dcg_field_size (in floats).
+ * DCGrid fields are indexed by global cell index block*BW^3 + Morton(local) (reference layout,
+ * src/dcgrid/dcgrid.h:29-43); uniform fields are x-fastest (src/utils/grid_math.cuh:10),
+ * pressure/fluidity/divergence of the uniform grid carry the whole mip pyramid
+ * (src/utils/grid_math.cuh:24-52). VELOCITY and VORTICITY are float3 AoS (3 floats per cell). */
+enum {
+  DCG_FIELD_DENSITY = 0,
+  DCG_FIELD_VELOCITY = 1,
+  DCG_FIELD_FLUIDITY = 2,
+  DCG_FIELD_PRESSURE = 3,
+  DCG_FIELD_T_PRESSURE = 4,
+  DCG_FIELD_DIVERGENCE = 5,
+  DCG_FIELD_TEMPERATURE = 6,
+  DCG_FIELD_VORTICITY = 7,
+  DCG_NUM_FIELDS = 8
+};
+
+/* Projection schedule used by dcg_step. */
+enum { DCG_PROJECT_MULTILEVEL = 0, DCG_PROJECT_LOCAL = 1 };
+
+/* Topology score used by dcg_adapt_topology. */
+enum {
+  DCG_SCORE_GEOMETRIC = 0, /* reference: src/dcgrid/dcgrid_adaptation.cu:26-34 */
+  DCG_SCORE_FEATURES = 1   /* density-gradient / vorticity thresholds (extension) */
+};
+
+/* Sources. */
+enum { DCG_SOURCE_NONE = 0, DCG_SOURCE_SPHERE_BUOYANT = 1 };
+
+/*
+ * Simulation parameters. The first block equals the simulation fields of the reference's
+ * SimParams (src/data/sim_params.h:14-30; render fields are out of scope). Everything
+ * after `enable_additional_solids` is a knob the reference hard-codes, or an extension.
+ */
+typedef struct dcg_params {
+  float dt;
+  int32_t gx, gy, gz;
+  float dx, rdx; /* rdx must equal 1/dx (the reference sets it in src/main.cpp:17) */
+  float velocity_emission_rate;
+  float density_emission_rate;
+  float emission_radius;
+  int32_t enable_additional_solids;
+
+  /* Block grid shape (DCGrid only). */
+  int32_t block_width; /* 4 = reference layout (src/dcgrid/dcgrid.h:52), or 8 */
+  int32_t max_levels;  /* 0 = reference rule (fluid_simulation_dcgrid.cu:12-22); else a cap */
+
+  /* Projection schedule (counts of jacobi/jacobi_inv pairs). */
+  int32_t project_mode;        /* DCG_PROJECT_* used by dcg_step */
+  int32_t project_top_pairs;   /* DCGrid 5 (fluid_simulation_dcgrid.cu:274), uniform 2 (uniform.cu:103) */
+  int32_t project_level_pairs; /* DCGrid 5 (:284), uniform 1 (uniform.cu:115) */
+  int32_t project_local_pairs; /* DCGrid 10 (:301), uniform 5 (uniform.cu:129) */
+
+  /* Topology scoring (DCGrid only). */
+  int32_t score_mode;     /* DCG_SCORE_* */
+  float refine_threshold; /* 1e-4 in the reference (fluid_simulation_dcgrid.cu:461) */
+  float grad_threshold;   /* DCG_SCORE_FEATURES: |grad density| threshold */
+  float vort_threshold;   /* DCG_SCORE_FEATURES: |vorticity| threshold */
+
+  /* Sources (extension; none exist in the reference). */
+  int32_t source_mode;   /* DCG_SOURCE_* */
+  float src_center[3];   /* in finest-level cell units */
+  float src_radius;      /* in finest-level cell units */
+  float src_density;     /* density inside the sphere */
+  float src_temperature; /* temperature inside the sphere (plus noise) */
+  float src_noise;       /* amplitude of the seeded uniform temperature noise */
+  float buoy_alpha;      /* buoyancy: u_y += dt*(beta*(T - t_ambient) - alpha*density) */
+  float buoy_beta;
+  float t_ambient;
+  float init_velocity_noise; /* amplitude of the seeded 3-octave value-noise initial velocity */
+  float noise_period;        /* lattice period of the first octave, finest cells */
+  uint32_t seed;
+  int32_t reserved[8];
+} dcg_params;
+
+/* Shape/state summary of a simulation. Per-level arrays are indexed by level (0 finest). */
+typedef struct dcg_info {
+  int32_t kind;        /* DCG_KIND_* */
+  int32_t size[3];
+  int32_t block_width; /* DCGrid */
+  int32_t levels;      /* DCGrid levels or uniform mip levels */
+  int32_t sparse_levels;
+  int32_t apron_width; /* BW + 2 */
+  uint64_t max_num_blocks;
+  uint64_t num_cells;     /* cells of the field arrays (B*BW^3 or gx*gy*gz) */
+  uint64_t active_cells;  /* sum over levels of block_loads*BW^3, or gx*gy*gz */
+  uint64_t hash_table_size;
+  uint64_t max_blocks_level[DCG_MAX_LEVELS];
+  uint64_t full_blocks_level[DCG_MAX_LEVELS];
+  uint64_t level_offsets[DCG_MAX_LEVELS];
+  uint64_t block_loads[DCG_MAX_LEVELS];
+  uint64_t move_limit[DCG_MAX_LEVELS];
+  uint64_t hash_size_level[DCG_MAX_LEVELS];
+  uint64_t hash_offset_level[DCG_MAX_LEVELS];
+} dcg_info;
+
+/* Block tables in the reference's semantics (src/dcgrid/dcgrid.h:5-27), for parity checks.
+ * Caller-owned host buffers; a NULL member is skipped. notFound = UINT64_MAX.
+ * cell_indices is the (BW+2)^3 apron table per block in [i][j][k] x-major order
+ * (src/dcgrid/dcgrid.h:73). The hash is reported slot by slot; slot placement follows
+ * insertion order and is only meaningful as a map (level, key) -> block. */
+typedef struct dcg_tables {
+  int32_t* block_positions; /* 3*B */
+  uint8_t* block_levels;    /* B, 0xFF = free */
+  uint8_t* block_flags;     /* B */
+  uint64_t* parent_indices; /* B */
+  uint64_t* child_indices;  /* 8*B */
+  uint64_t* cell_indices;   /* B*(BW+2)^3 */
+  uint32_t* hash_key;       /* hash_table_size */
+  uint64_t* hash_val;       /* hash_table_size */
+} dcg_tables;
+
+typedef struct dcg_sim dcg_sim;
+
+/* Reference defaults: src/data/sim_params.cpp:4-17, rdx = 1/dx (src/main.cpp:17). */
+void dcg_default_params(dcg_params* p);
+
+/* Replaces FluidSimulationUniform::FluidSimulationUniform (fluid_simulation_uniform.cu:6-56):
+ * allocate, then reset() -> init(). */
+int dcg_create_uniform(const int32_t size[3], const dcg_params* p, dcg_sim** out);
+
+/* Replaces FluidSimulationDCGrid::FluidSimulationDCGrid(size, maxNumBlocks)
+ * (fluid_simulation_dcgrid.cu:9-140): level/budget planning, allocation, reset() -> init(). */
+int dcg_create_dcgrid(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p,
+                      dcg_sim** out);
+
+/* Replaces the destructors (fluid_simulation_dcgrid.cu:142-188, uniform.cu:58-73). */
+void dcg_destroy(dcg_sim* s);
+
+/* Replaces copySimParamsToDevice (src/utils/sim_utils.cu:7-9), per handle. Grid size,
+ * block width and level cap are fixed at creation and must not change. */
+int dcg_set_params(dcg_sim* s, const dcg_params* p);
+int dcg_get_params(const dcg_sim* s, dcg_params* p);
+
+/* FluidSimulation::init / reset (src/fluid_simulation.h:9-10). */
+int dcg_init(dcg_sim* s);
+int dcg_reset(dcg_sim* s);
+/* FluidSimulation::adaptTopology (src/fluid_simulation.h:11). */
+int dcg_adapt_topology(dcg_sim* s);
+/* FluidSimulation::advectVelocity / project / projectLocal / advectDensity
+ * (src/fluid_simulation.h:13-16). */
+int dcg_advect_velocity(dcg_sim* s);
+int dcg_project(dcg_sim* s);
+int dcg_project_local(dcg_sim* s);
+int dcg_advect_density(dcg_sim* s);
+/* n steps in the order of src/simulation.cpp:104-111:
+ * advectVelocity -> adaptTopology -> project (or projectLocal) -> advectDensity. */
+int dcg_step(dcg_sim* s, int32_t n);
+/* FluidSimulation::debugStats (src/fluid_simulation.h:22): DCGrid returns the residual sum
+ * (dcgrid_structure.cu:224-251) of the last projection, uniform returns the smoke mass
+ * (uniformgrid_structure.cu:33-42). Synchronises. */
+int dcg_debug_stats(dcg_sim* s, double* out);
+
+/* Sets the seeded initial condition selected by the parameters (sphere source, value-noise
+ * velocity) on the current topology. Extension; no counterpart in the reference. */
+int dcg_seed_scene(dcg_sim* s);
+
+/* Waits for all work queued on the handle. */
+int dcg_synchronize(dcg_sim* s);
+
+int dcg_info_get(dcg_sim* s, dcg_info* out);
+int dcg_field_size(dcg_sim* s, int32_t field, uint64_t* nfloats);
+/* Host <-> device copies of one field (synchronising). */
+int dcg_export_field(dcg_sim* s, int32_t field, float* dst, uint64_t nfloats);
+int dcg_import_field(dcg_sim* s, int32_t field, const float* src, uint64_t nfloats);
+/* Block tables in reference semantics (DCGrid only). */
+int dcg_export_tables(dcg_sim* s, dcg_tables* t);
+
+/* Time, in milliseconds, of the kernels the last call on this handle queued, when
+ * timing is enabled (0 disables). Used by the benchmark for per-kernel roofline figures. */
+int dcg_set_timing(dcg_sim* s, int32_t enable);
+int dcg_timing_get(dcg_sim* s, int32_t kernel, double* total_ms, uint64_t* launches);
+
+/* Kernel ids for dcg_timing_get. */
+enum {
+  DCG_K_ADVECT_VELOCITY = 0,
+  DCG_K_ADVECT_DENSITY = 1,
+  DCG_K_DIVERGENCE = 2,
+  DCG_K_JACOBI = 3,
+  DCG_K_PROLONGATE = 4,
+  DCG_K_APPLY_PRESSURE = 5,
+  DCG_K_RESTRICT = 6,
+  DCG_K_TOPOLOGY = 7,
+  DCG_K_OTHER = 8,
+  DCG_NUM_KERNEL_IDS = 9
+};
+
+const char* dcg_last_error(const dcg_sim* s);
+int32_t dcg_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCGRID_AMD_H */

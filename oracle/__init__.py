@@ -1,0 +1,79 @@
+"""CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+A C++ restatement of the reference's simulation step (src/dcgrid/*.cu, src/uniformgrid/*.cu,
+src/utils/sim_utils.cu) built into oracle/liboracle.so by oracle/Makefile. Only tests/,
+__graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, as the checker — never as
+the thing measured or shipped.
+
+Parity status of each restated function is documented in oracle_dcgrid.cpp / oracle_uniform.cpp
+and pinned by tests/test_oracle_*.py.
+"""
+import ctypes as C
+import os
+
+from dcgrid_amd._abi import Handle, bind
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+_fns = None
+
+
+def library():
+    global _lib, _fns
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} missing (make -C oracle)")
+        lib = C.CDLL(LIB_PATH)
+        _fns = bind(lib, "orc_", extra_create_arg=True)
+        lib.orc_set_threads.argtypes = [C.c_int32]
+        lib.orc_set_threads.restype = None
+        lib.orc_get_threads.argtypes = []
+        lib.orc_get_threads.restype = C.c_int32
+        lib.orc_export_field_f64.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64]
+        lib.orc_export_field_f64.restype = C.c_int
+        for name, args, res in [
+            ("orc_bit_spread3", [C.c_uint32], C.c_uint32),
+            ("orc_grid_hash", [C.c_int32] * 4, C.c_uint32),
+            ("orc_local_index", [C.c_int32] * 4, C.c_uint32),
+            ("orc_mipmap_cells", [C.c_uint64] * 3, C.c_uint64),
+            ("orc_mipmap_idx", [C.c_int32] * 3 + [C.c_uint64] * 4, C.c_uint64),
+            ("orc_value_noise", [C.c_float] * 3 + [C.c_uint32], C.c_float),
+            ("orc_hash01", [C.c_uint32] + [C.c_int32] * 4, C.c_float),
+        ]:
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+def set_threads(n):
+    library().orc_set_threads(int(n))
+
+
+def get_threads():
+    return library().orc_get_threads()
+
+
+class OracleSim(Handle):
+    """Same surface as dcgrid_amd.Simulation, computed on the CPU in float (32) or double (64)."""
+
+    def __init__(self, kind, size, params, max_num_blocks=None, precision=64):
+        library()
+        self._fns = _fns
+        self._extra = (C.c_int32(precision),)
+        self.precision = precision
+        super().__init__(kind, size, params, max_num_blocks)
+
+    def export_field_f64(self, field):
+        import numpy as np
+        from dcgrid_amd._abi import FIELDS, FIELD_VELOCITY, FIELD_VORTICITY
+        f = FIELDS.get(field, field)
+        n = self.field_size(f)
+        out = np.empty(n, dtype=np.float64)
+        st = _lib.orc_export_field_f64(self._h, f, out.ctypes.data_as(C.POINTER(C.c_double)), n)
+        self._check(st, "export_field_f64")
+        if f in (FIELD_VELOCITY, FIELD_VORTICITY):
+            out = out.reshape(-1, 3)
+        return out

@@ -1,0 +1,66 @@
+/*
+ * oracle.h — C API of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. The oracle is a CPU restatement of the reference's simulation
+ * step (wouterraateland/dcgrid, the src/dcgrid and src/uniformgrid kernels and src/utils/sim_utils.cu),
+ * templated on the field precision (float or double). It exists to check the HIP library;
+ * only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, and never
+ * as the thing measured or shipped.
+ *
+ * The API mirrors include/dcgrid_amd.h with an `orc_` prefix and an extra `precision`
+ * argument (32 or 64) at creation, so tests drive both sides with identical calls.
+ */
+#ifndef DCGRID_ORACLE_H
+#define DCGRID_ORACLE_H
+
+#include "../include/dcgrid_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_sim orc_sim;
+
+int orc_create_uniform(const int32_t size[3], const dcg_params* p, int32_t precision,
+                       orc_sim** out);
+int orc_create_dcgrid(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p,
+                      int32_t precision, orc_sim** out);
+void orc_destroy(orc_sim* s);
+int orc_set_params(orc_sim* s, const dcg_params* p);
+int orc_init(orc_sim* s);
+int orc_reset(orc_sim* s);
+int orc_adapt_topology(orc_sim* s);
+int orc_advect_velocity(orc_sim* s);
+int orc_project(orc_sim* s);
+int orc_project_local(orc_sim* s);
+int orc_advect_density(orc_sim* s);
+int orc_step(orc_sim* s, int32_t n);
+int orc_debug_stats(orc_sim* s, double* out);
+int orc_seed_scene(orc_sim* s);
+int orc_info_get(orc_sim* s, dcg_info* out);
+int orc_field_size(orc_sim* s, int32_t field, uint64_t* nfloats);
+/* Fields are exported/imported as float (FP64 instances round on export) or as double. */
+int orc_export_field(orc_sim* s, int32_t field, float* dst, uint64_t nfloats);
+int orc_export_field_f64(orc_sim* s, int32_t field, double* dst, uint64_t nfloats);
+int orc_import_field(orc_sim* s, int32_t field, const float* src, uint64_t nfloats);
+int orc_export_tables(orc_sim* s, dcg_tables* t);
+/* Number of OpenMP threads used by the oracle (0 = runtime default). */
+void orc_set_threads(int32_t n);
+int32_t orc_get_threads(void);
+const char* orc_last_error(const orc_sim* s);
+
+/* Scalar helpers exposed for closed-form tests of the restated index algebra. */
+uint32_t orc_bit_spread3(uint32_t v);                             /* dcgrid_utils.cuh:83-90 */
+uint32_t orc_grid_hash(int32_t x, int32_t y, int32_t z, int32_t bw); /* dcgrid_utils.cuh:92-96 */
+uint32_t orc_local_index(int32_t x, int32_t y, int32_t z, int32_t bw);
+uint64_t orc_mipmap_cells(uint64_t w, uint64_t h, uint64_t d);   /* grid_math.cuh:24-30 */
+uint64_t orc_mipmap_idx(int32_t x, int32_t y, int32_t z, uint64_t w, uint64_t h, uint64_t d,
+                        uint64_t scale);                          /* grid_math.cuh:43-52 */
+float orc_value_noise(float x, float y, float z, uint32_t seed);
+float orc_hash01(uint32_t seed, int32_t x, int32_t y, int32_t z, int32_t w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
