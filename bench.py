@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the DCGrid simulation step on one MI355X.
+
+Workload (default): the adaptive smoke plume on a 256^3 virtual domain of 8^3-cell blocks, 3
+levels, a 4096-block budget, feature-driven refinement (density gradient 0.05, vorticity 0.5),
+the buoyant sphere source scaled 4x from the 64^3 plume, and the seeded 3-octave value-noise
+initial velocity of amplitude 0.2. Block tables are rebuilt every step. Synthetic, seeded inputs.
+
+One step = advectVelocity -> adaptTopology -> project -> advectDensity (src/simulation.cpp:104-111)
+through the C ABI. Prints one JSON line (see README of the contract in DESIGN.md):
+  value          active cell-steps/s (sum of allocated cells over the timed steps / time)
+  roofline       the Jacobi half-sweep kernel (the projection's hot loop): algorithmic bytes
+                 (12 B per cell per sweep: read p and div, write p~) over its HIP-event-timed
+                 average duration, against the 8 TB/s HBM peak
+  cpu_baseline   the CPU oracle (FP32 restatement, OpenMP over all host cores) on a bounded
+                 sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def jacobi_bytes(info, params):
+    """Algorithmic bytes and launch count of the Jacobi half-sweeps of one project() call."""
+    bv = int(info.block_width) ** 3
+    L = int(info.levels)
+    total_bytes = 0
+    launches = 0
+    for level in range(L):
+        pairs = params.project_top_pairs if level == L - 1 else params.project_level_pairs
+        cells = int(info.block_loads[level]) * bv
+        total_bytes += 2 * pairs * cells * 12
+        launches += 2 * pairs
+    return total_bytes, launches
+
+
+def step_bytes(info, params):
+    """Algorithmic HBM bytes of one step (per-stage compulsory field traffic, FP32, no solids)."""
+    bv = int(info.block_width) ** 3
+    L = int(info.levels)
+    V, S = 12, 4
+    src = params.source_mode != 0
+    k = 2 if src else 1  # advected scalars
+    act = sum(int(info.block_loads[l]) for l in range(L)) * bv
+    below = sum(int(info.block_loads[l]) for l in range(L - 1)) * bv
+    jb, _ = jacobi_bytes(info, params)
+    b = act * (2 * V + (2 * S if src else 0))          # advect u (+ buoyancy)
+    b += below * 9 * V // 8 * 2                         # restrict u twice
+    b += act * (V + 4) + below * 9 * 4 // 8             # divergence + restrict div
+    b += jb                                             # Jacobi sweeps
+    b += sum(int(info.block_loads[l]) for l in range(L - 1)) * bv * 4 * 9 // 8  # prolongate
+    b += act * (2 * V + 4)                              # gradient subtraction
+    b += act * (V + 2 * k * S) + below * 9 * k * S // 8  # advect scalars + restrict
+    return b
+
+
+def make_sim(cls, w, **kw):
+    s = cls(w.kind, w.size, w.params, w.max_num_blocks if w.kind == "dcgrid" else None, **kw)
+    s.seed_scene()
+    return s
+
+
+def cpu_baseline(w, warm_steps, budget_s):
+    """The oracle (FP32) on the same seeded workload: warm_steps untimed, then steps until
+    about budget_s seconds of CPU work; reports active cell-steps/s."""
+    import oracle
+    oracle.library()
+    cores = oracle.get_threads()
+    c = make_sim(oracle.OracleSim, w, precision=32)
+    c.step(warm_steps)
+    n = 0
+    cells = 0
+    t0 = time.perf_counter()
+    while True:
+        inf = c.info()
+        act = int(inf.active_cells)
+        c.step(1)
+        n += 1
+        cells += act
+        if time.perf_counter() - t0 > budget_s or n >= 2000:
+            break
+    dt = time.perf_counter() - t0
+    c.close()
+    return {
+        "value": cells / dt,
+        "unit": "active cell-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{n} steps of {w.name} after {warm_steps} untimed steps from the same seeded scene "
+                  f"({cells / max(n, 1):.0f} active cells/step), FP32 oracle, OpenMP {cores} threads",
+        "seconds": round(dt, 3),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    # default: the configuration's full 200-step run (20 untimed + 180 timed)
+    ap.add_argument("--steps", type=int, default=180)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(the sharding design is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    import dcgrid_amd
+    from dcgrid_amd import scenes
+    w = scenes.config(args.config)
+
+    log(f"[bench] creating {w.name} {w.size} budget={w.max_num_blocks}")
+    sim = make_sim(dcgrid_amd.Simulation, w)
+    log(f"[bench] warmup {args.warmup} steps")
+    sim.step(args.warmup)
+    sim.synchronize()
+    info0 = sim.info()
+    act0 = int(info0.active_cells)
+    if not args.no_timing:
+        sim.set_timing(["jacobi"])
+    log(f"[bench] timing {args.steps} steps (active cells {act0})")
+    cs0 = sim.cell_steps()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    cell_steps = sim.cell_steps() - cs0  # active cells of every timed step, counted on the device
+    info1 = sim.info()
+    act1 = int(info1.active_cells)
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = cell_steps / elapsed
+
+    roof = None
+    if not args.no_timing:
+        tm = sim.timing()
+        j_ms, j_n = tm["jacobi"]
+        p = w.params
+        if p.project_top_pairs == p.project_level_pairs:
+            # every active cell is swept 2*pairs times per step: exact over the varying topology
+            jb = 2 * p.project_level_pairs * 12 * cell_steps
+            jl = j_n
+        else:
+            jb, jl = jacobi_bytes(info1, p)
+        if j_n > 0 and j_ms > 0:
+            per_launch_bytes = jb / jl
+            avg_ms = j_ms / j_n
+            achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+            roof = {
+                "kernel": "k_jacobi (all levels)",
+                "bound": "hbm",
+                "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "avg_launch_us": round(avg_ms * 1e3, 3),
+                "launches": int(j_n),
+                "algorithmic_bytes_per_launch": int(per_launch_bytes),
+            }
+    sb = step_bytes(info1, w.params)
+    out = {
+        "metric": "active cell-steps/s",
+        "value": round(value, 1),
+        "unit": "active cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded sphere source + value-noise velocity)",
+        "config": {
+            "workload": w.name,
+            "domain": list(w.size),
+            "block_width": w.params.block_width,
+            "levels": int(info1.levels),
+            "budget_blocks": w.max_num_blocks,
+            "active_cells_first_last": [act0, act1],
+            "mean_active_cells": round(cell_steps / args.steps, 1),
+            "block_loads": info1.levels_list("block_loads"),
+            "step_algorithmic_bytes": sb,
+            "step_algorithmic_GBps": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
+        },
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    sim.close()
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline (~{args.cpu_seconds}s)")
+        out["cpu_baseline"] = cpu_baseline(w, 2, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

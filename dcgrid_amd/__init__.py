@@ -40,6 +40,8 @@ def library():
         lib.dcg_timing_get.restype = C.c_int
         lib.dcg_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
         lib.dcg_get_params.restype = C.c_int
+        lib.dcg_cell_steps.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        lib.dcg_cell_steps.restype = C.c_int
         _lib, _fns = lib, fns
     return _lib
 
@@ -63,8 +65,18 @@ class Simulation(Handle):
     def synchronize(self):
         self._check(_lib.dcg_synchronize(self._h), "synchronize")
 
-    def set_timing(self, enable):
-        self._check(_lib.dcg_set_timing(self._h, int(bool(enable))), "set_timing")
+    def cell_steps(self):
+        """Active cells summed over the steps run by step() since creation or reset."""
+        v = C.c_uint64()
+        self._check(_lib.dcg_cell_steps(self._h, C.byref(v)), "cell_steps")
+        return v.value
+
+    def set_timing(self, kernels):
+        """Time every launch of the named kernel classes (see K_NAMES) with HIP events."""
+        mask = 0
+        for k in kernels:
+            mask |= 1 << K_NAMES.index(k)
+        self._check(_lib.dcg_set_timing(self._h, mask), "set_timing")
 
     def timing(self):
         """{kernel name: (total ms, launches)} accumulated since timing was enabled."""

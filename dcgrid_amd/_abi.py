@@ -50,6 +50,14 @@ PROJECT_MULTILEVEL, PROJECT_LOCAL = 0, 1
 SCORE_GEOMETRIC, SCORE_FEATURES = 0, 1
 SOURCE_NONE, SOURCE_SPHERE_BUOYANT = 0, 1
 
+STAGE_DIVERGENCE = 0
+STAGE_RESTRICT = 1
+STAGE_JACOBI = 2
+STAGE_JACOBI_INV = 3
+STAGE_PROLONGATE = 4
+STAGE_APPLY_PRESSURE = 5
+STAGE_VORTICITY = 6
+
 K_NAMES = ["advect_velocity", "advect_density", "divergence", "jacobi", "prolongate",
            "apply_pressure", "restrict", "topology", "other"]
 
@@ -181,6 +189,7 @@ def bind(lib, prefix, extra_create_arg=False):
         "export_field": ([sim, C.c_int32, P(C.c_float), C.c_uint64], C.c_int),
         "import_field": ([sim, C.c_int32, P(C.c_float), C.c_uint64], C.c_int),
         "export_tables": ([sim, P(Tables)], C.c_int),
+        "run_stage": ([sim, C.c_int32, C.c_int32], C.c_int),
         "last_error": ([sim], C.c_char_p),
     }
     fns = {}
@@ -276,6 +285,10 @@ class Handle:
 
     def seed_scene(self):
         self._check(self._fns["seed_scene"](self._h), "seed_scene")
+
+    def run_stage(self, stage, arg=0):
+        """One stage of the step (STAGE_*); arg = level, or field id for STAGE_RESTRICT."""
+        self._check(self._fns["run_stage"](self._h, int(stage), int(FIELDS.get(arg, arg))), "run_stage")
 
     def info(self):
         inf = Info()

@@ -56,7 +56,7 @@ int SimBase::sync() {
 
 int SimBase::beginTimed(int k, Pending* ev) {
   ev->k = -1;
-  if (!(timingMask & (1u << k))) return DCG_OK;
+  if (!(timingMask & (1u << k)) || batchK == k) return DCG_OK;
   for (int i = 0; i < 2; i++) {
     hipEvent_t e;
     if (!eventPool.empty()) {
@@ -71,8 +71,9 @@ int SimBase::beginTimed(int k, Pending* ev) {
   DCG_HIP(hipEventRecord(ev->a, stream));
   return DCG_OK;
 }
-int SimBase::endTimed(int, Pending* ev) {
+int SimBase::endTimed(int, Pending* ev, uint32_t launches) {
   if (ev->k < 0) return DCG_OK;
+  ev->n = launches;
   DCG_HIP(hipEventRecord(ev->b, stream));
   pending.push_back(*ev);
   // Bound the number of outstanding events.
@@ -86,7 +87,7 @@ int SimBase::collectTiming() {
     float ms = 0.f;
     DCG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
     timingMs[p.k] += ms;
-    timingLaunches[p.k] += 1;
+    timingLaunches[p.k] += p.n;
     eventPool.push_back(p.a);
     eventPool.push_back(p.b);
   }
@@ -213,6 +214,7 @@ int dcg_step(dcg_sim* s, int32_t n) {
     int e;
     if ((e = I->advectVelocity())) return e;
     if ((e = I->adaptTopology())) return e;
+    if ((e = I->countStep())) return e;
     if ((e = (I->P.project_mode == DCG_PROJECT_LOCAL ? I->projectLocal() : I->project()))) return e;
     if ((e = I->advectDensity())) return e;
   }
@@ -255,6 +257,15 @@ int dcg_export_tables(dcg_sim* s, dcg_tables* t) {
   IMPL(s);
   if (!t) return DCG_E_ARG;
   return I->exportTables(t);
+}
+int dcg_run_stage(dcg_sim* s, int32_t stage, int32_t arg) {
+  IMPL(s);
+  return I->runStage(stage, arg);
+}
+int dcg_cell_steps(dcg_sim* s, uint64_t* total) {
+  IMPL(s);
+  if (!total) return DCG_E_ARG;
+  return I->cellSteps(total);
 }
 int dcg_set_timing(dcg_sim* s, int32_t mask) {
   IMPL(s);

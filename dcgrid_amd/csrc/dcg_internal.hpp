@@ -33,8 +33,10 @@ struct SimBase {
   uint32_t timingMask = 0;
   struct Pending {
     int k;
+    uint32_t n;  // launches covered by the event pair
     hipEvent_t a, b;
   };
+  int batchK = -1;  // kernel class currently timed as one batch (no per-launch events)
   std::vector<Pending> pending;
   std::vector<hipEvent_t> eventPool;
   double timingMs[DCG_NUM_KERNEL_IDS] = {};
@@ -56,6 +58,10 @@ struct SimBase {
   virtual int exportField(int field, float* dst, uint64_t n) = 0;
   virtual int importField(int field, const float* src, uint64_t n) = 0;
   virtual int exportTables(dcg_tables* t) = 0;
+  // Active cells of one step, accumulated by dcg_step after adaptTopology.
+  virtual int countStep() = 0;
+  virtual int cellSteps(uint64_t* out) = 0;
+  virtual int runStage(int stage, int arg) = 0;
 
   int fail(int code, const std::string& m) {
     err = m;
@@ -80,7 +86,7 @@ struct SimBase {
 
   // Launch bracket: records events around the launch when kernel id k is timed.
   int beginTimed(int k, Pending* ev);
-  int endTimed(int k, Pending* ev);
+  int endTimed(int k, Pending* ev, uint32_t launches = 1);
   int collectTiming();
   int checkLaunch(const char* what) {
     hipError_t e = hipGetLastError();
@@ -89,7 +95,8 @@ struct SimBase {
   }
 };
 
-// Launch `expr` on the handle's stream, bracketed by timing events for kernel id K.
+// Launch `expr` on the handle's stream, bracketed by timing events for kernel id K (unless a
+// batch of K launches is being timed as a whole).
 #define DCG_LAUNCH(K, NAME, expr)        \
   do {                                   \
     SimBase::Pending ev_;                \

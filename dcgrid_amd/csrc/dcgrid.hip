@@ -32,6 +32,7 @@ struct DCGridSim : SimBase {
   uint32_t* dcBuf = nullptr;  // sorted destination subblocks, region at offs[l] + l
   uint32_t* refList = nullptr;
   float* stats = nullptr;
+  unsigned long long* cellStepCounter = nullptr;
   bool customFluidity = false;
 
   int kind() const override { return DCG_KIND_DCGRID; }
@@ -135,6 +136,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&dcBuf, (size_t)B + 16));
     DCG_TRY(alloc(&refList, (size_t)8 * B));
     DCG_TRY(alloc(&stats, B));
+    DCG_TRY(alloc(&cellStepCounter, 1));
     const float negMax = -FLT_MAX;
     uint32_t negBits;
     std::memcpy(&negBits, &negMax, 4);
@@ -162,6 +164,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(fill32(g.tpre, 0, C));
     DCG_TRY(fill32(g.vort, 0, 3 * C));
     DCG_TRY(fill32(g.ctl, 0, CTL_WORDS));
+    DCG_TRY(fill32(cellStepCounter, 0, 2));
     uint32_t loads[DCG_MAX_LEVELS] = {};
     for (int l = 0; l < L; l++) loads[l] = maxL[l] == fullL[l] ? maxL[l] : 0;
     DCG_HIP(hipMemcpyAsync(g.ctl + CTL_LOADS, loads, sizeof(loads), hipMemcpyHostToDevice, stream));
@@ -283,30 +286,98 @@ struct DCGridSim : SimBase {
     }
     return DCG_OK;
   }
-  int divergence() {
+  int divergenceKernel() {
     const unsigned nWG = blocksFor(B, G::BPW);
     if (fluidityOn())
       DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
     else
       DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
+    return DCG_OK;
+  }
+  int divergence() {
+    DCG_TRY(divergenceKernel());
     return restrictAll(g.div);
+  }
+  int jacobiOne(int level, bool fwd) {
+    const int s = 1 << level;
+    const float alpha = (float)(s * s) * P.dx * P.dx;
+    const unsigned n = blocksFor(maxL[level], G::BPW);
+    DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi",
+               (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, fwd ? g.pre : g.tpre, fwd ? g.tpre : g.pre, alpha)));
+    return DCG_OK;
+  }
+  int prolongate(int level) {
+    DCG_LAUNCH(DCG_K_PROLONGATE, "k_prolongate",
+               (k_prolongate<BW><<<blocksFor(maxL[level], G::BPW), G::WG, 0, stream>>>(g, level)));
+    return DCG_OK;
+  }
+  int vorticity() {
+    const unsigned nWG = blocksFor(B, G::BPW);
+    if (fluidityOn())
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
+    else
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
+    return DCG_OK;
+  }
+  int runStage(int stage, int arg) override {
+    const bool perLevel = stage == DCG_STAGE_JACOBI || stage == DCG_STAGE_JACOBI_INV || stage == DCG_STAGE_PROLONGATE;
+    if (perLevel && (arg < 0 || arg >= L || (stage == DCG_STAGE_PROLONGATE && arg >= L - 1)))
+      return fail(DCG_E_ARG, "level out of range");
+    switch (stage) {
+      case DCG_STAGE_DIVERGENCE: return divergenceKernel();
+      case DCG_STAGE_RESTRICT:
+        switch (arg) {
+          case DCG_FIELD_DENSITY: return restrictAll(g.den);
+          case DCG_FIELD_TEMPERATURE: return restrictAll(g.tem);
+          case DCG_FIELD_DIVERGENCE: return restrictAll(g.div);
+          case DCG_FIELD_PRESSURE: return restrictAll(g.pre);
+          case DCG_FIELD_VELOCITY: return restrictAll(g.vel);
+          default: return fail(DCG_E_ARG, "field cannot be restricted");
+        }
+      case DCG_STAGE_JACOBI: return jacobiOne(arg, true);
+      case DCG_STAGE_JACOBI_INV: return jacobiOne(arg, false);
+      case DCG_STAGE_PROLONGATE: return prolongate(arg);
+      case DCG_STAGE_APPLY_PRESSURE: return applyKernel();
+      case DCG_STAGE_VORTICITY: return vorticity();
+      default: return fail(DCG_E_ARG, "unknown stage");
+    }
   }
   int jacobiPairs(int level, int pairs) {
     const int s = 1 << level;
     const float alpha = (float)(s * s) * P.dx * P.dx;  // dcgrid_multigrid_solver.cu:11
     const unsigned n = blocksFor(maxL[level], G::BPW);
+    if (pairs <= 0) return DCG_OK;
+    Pending batch;  // the run of back-to-back sweeps of one level is timed as one batch
+    DCG_TRY(beginTimed(DCG_K_JACOBI, &batch));
+    batchK = DCG_K_JACOBI;
     for (int i = 0; i < pairs; i++) {
       DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, g.pre, g.tpre, alpha)));
       DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, g.tpre, g.pre, alpha)));
     }
+    batchK = -1;
+    return endTimed(DCG_K_JACOBI, &batch, 2 * pairs);
+  }
+  int countStep() override {
+    DCG_LAUNCH(DCG_K_OTHER, "k_count_step", (k_count_step<BW><<<1, 64, 0, stream>>>(g, cellStepCounter)));
     return DCG_OK;
   }
-  int applyPressure() {
+  int cellSteps(uint64_t* out) override {
+    unsigned long long v = 0;
+    DCG_HIP(hipMemcpyAsync(&v, cellStepCounter, sizeof(v), hipMemcpyDeviceToHost, stream));
+    DCG_TRY(sync());
+    *out = v;
+    return DCG_OK;
+  }
+  int applyKernel() {
     const unsigned nWG = blocksFor(B, G::BPW);
     if (fluidityOn())
       DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
     else
       DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
+    return DCG_OK;
+  }
+  int applyPressure() {
+    DCG_TRY(applyKernel());
     return restrictAll(g.vel);
   }
   // project(), fluid_simulation_dcgrid.cu:270-294

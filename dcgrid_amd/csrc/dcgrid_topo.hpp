@@ -379,6 +379,15 @@ __global__ void __launch_bounds__(256) k_move(DG g, int level, const uint32_t* _
   atomicOr(&g.flg[pb], (uint32_t)REFINED);
 }
 
+// Adds the active cells of this step (all cells of allocated blocks) to a 64-bit counter.
+template <int BW>
+__global__ void __launch_bounds__(64) k_count_step(DG g, unsigned long long* __restrict__ counter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long s = 0;
+  for (int l = 0; l < g.L; l++) s += (unsigned long long)g.ctl[CTL_LOADS + l] * Geo<BW>::BV;
+  *counter += s;
+}
+
 __global__ void __launch_bounds__(64) k_total_moves(DG g) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t s = 0;

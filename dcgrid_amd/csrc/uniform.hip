@@ -305,6 +305,7 @@ struct UniformSim : SimBase {
   float* stats = nullptr;
   float* hstats = nullptr;
   bool customFluidity = false;
+  uint64_t cellStepCount = 0;
 
   int kind() const override { return DCG_KIND_UNIFORM; }
 
@@ -362,6 +363,7 @@ struct UniformSim : SimBase {
     DCG_TRY(fill32(g.pre, 0, MC));
     DCG_TRY(fill32(g.flu, 0, MC));
     customFluidity = false;
+    cellStepCount = 0;
     return init();
   }
   // init(), fluid_simulation_uniform.cu:76-79
@@ -418,10 +420,23 @@ struct UniformSim : SimBase {
     const int scale = 1 << level;
     // alpha = dx*dx*scale*scale, uniformgrid_fluid.cu:171
     const float alpha = P.dx * P.dx * (float)scale * (float)scale;
+    if (pairs <= 0) return DCG_OK;
+    Pending batch;  // the run of back-to-back sweeps is timed as one batch
+    DCG_TRY(beginTimed(DCG_K_JACOBI, &batch));
+    batchK = DCG_K_JACOBI;
     for (int i = 0; i < pairs; i++) {
       DCG_LAUNCH(DCG_K_JACOBI, "k_u_jacobi", (k_u_jacobi<<<nblk(n), 256, 0, stream>>>(g.pre, g.tpre, g.div, D, alpha)));
       DCG_LAUNCH(DCG_K_JACOBI, "k_u_jacobi", (k_u_jacobi<<<nblk(n), 256, 0, stream>>>(g.tpre, g.pre, g.div, D, alpha)));
     }
+    batchK = -1;
+    return endTimed(DCG_K_JACOBI, &batch, 2 * pairs);
+  }
+  int countStep() override {
+    cellStepCount += N;
+    return DCG_OK;
+  }
+  int cellSteps(uint64_t* out) override {
+    *out = cellStepCount;
     return DCG_OK;
   }
   int applyPressure() {
@@ -521,6 +536,35 @@ struct UniformSim : SimBase {
     return sync();
   }
   int exportTables(dcg_tables*) override { return fail(DCG_E_ARG, "uniform grid has no block tables"); }
+  int runStage(int stage, int arg) override {
+    const bool perLevel = stage == DCG_STAGE_JACOBI || stage == DCG_STAGE_JACOBI_INV || stage == DCG_STAGE_PROLONGATE;
+    if (perLevel && (arg < 0 || arg >= L || (stage == DCG_STAGE_PROLONGATE && arg >= L - 1)))
+      return fail(DCG_E_ARG, "level out of range");
+    if (stage == DCG_STAGE_DIVERGENCE) return divergence();
+    if (stage == DCG_STAGE_RESTRICT) {
+      for (int l = 1; l < L; l++) {
+        const UDims& D = dims[l];
+        DCG_LAUNCH(DCG_K_RESTRICT, "k_u_restrict", (k_u_restrict<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(g, D, dims[l - 1])));
+      }
+      return DCG_OK;
+    }
+    if (stage == DCG_STAGE_JACOBI || stage == DCG_STAGE_JACOBI_INV) {
+      const UDims& D = dims[arg];
+      const int scale = 1 << arg;
+      const float alpha = P.dx * P.dx * (float)scale * (float)scale;
+      const bool fwd = stage == DCG_STAGE_JACOBI;
+      DCG_LAUNCH(DCG_K_JACOBI, "k_u_jacobi",
+                 (k_u_jacobi<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(fwd ? g.pre : g.tpre, fwd ? g.tpre : g.pre, g.div, D, alpha)));
+      return DCG_OK;
+    }
+    if (stage == DCG_STAGE_PROLONGATE) {
+      const UDims& D = dims[arg];
+      DCG_LAUNCH(DCG_K_PROLONGATE, "k_u_prolongate", (k_u_prolongate<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(g.pre, D, dims[arg + 1])));
+      return DCG_OK;
+    }
+    if (stage == DCG_STAGE_APPLY_PRESSURE) return applyPressure();
+    return fail(DCG_E_ARG, "stage not available on the uniform grid");
+  }
 };
 
 SimBase* makeUniform(const int32_t size[3], const dcg_params& p, int* status, std::string* err) {

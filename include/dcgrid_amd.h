@@ -200,6 +200,23 @@ int dcg_debug_stats(dcg_sim* s, double* out);
  * velocity) on the current topology. Extension; no counterpart in the reference. */
 int dcg_seed_scene(dcg_sim* s);
 
+/* Active cells summed over the steps run by dcg_step since creation or the last reset (for a
+ * DCGrid, the cells of all allocated blocks after each step's adaptTopology). Synchronises. */
+int dcg_cell_steps(dcg_sim* s, uint64_t* total);
+
+/* One stage of the step, for tests and custom schedules. `arg` is the level for the per-level
+ * stages and the field id for DCG_STAGE_RESTRICT (restricts that field on all levels). */
+enum {
+  DCG_STAGE_DIVERGENCE = 0,     /* k_dcgrid_calc_divergence / k_uniform_calc_divergence (no restriction) */
+  DCG_STAGE_RESTRICT = 1,       /* accumulate<T> (DCGrid) / k_uniform_restrict of the divergence (uniform) */
+  DCG_STAGE_JACOBI = 2,         /* one half-sweep pressure -> t_pressure on level `arg` */
+  DCG_STAGE_JACOBI_INV = 3,     /* one half-sweep t_pressure -> pressure on level `arg` */
+  DCG_STAGE_PROLONGATE = 4,     /* level `arg` from level arg+1 */
+  DCG_STAGE_APPLY_PRESSURE = 5, /* gradient subtraction (no restriction) */
+  DCG_STAGE_VORTICITY = 6       /* DCGrid only */
+};
+int dcg_run_stage(dcg_sim* s, int32_t stage, int32_t arg);
+
 /* Waits for all work queued on the handle. */
 int dcg_synchronize(dcg_sim* s);
 
@@ -211,9 +228,11 @@ int dcg_import_field(dcg_sim* s, int32_t field, const float* src, uint64_t nfloa
 /* Block tables in reference semantics (DCGrid only). */
 int dcg_export_tables(dcg_sim* s, dcg_tables* t);
 
-/* Time, in milliseconds, of the kernels the last call on this handle queued, when
- * timing is enabled (0 disables). Used by the benchmark for per-kernel roofline figures. */
-int dcg_set_timing(dcg_sim* s, int32_t enable);
+/* Per-kernel timing with HIP events recorded on the handle's stream around every launch of the
+ * kernel classes selected by `mask` (bit k = kernel id k below; 0 disables). Setting the mask
+ * clears the accumulated totals; dcg_timing_get synchronises and returns the total duration and
+ * launch count of one class since then. Used by the benchmark for roofline figures. */
+int dcg_set_timing(dcg_sim* s, int32_t mask);
 int dcg_timing_get(dcg_sim* s, int32_t kernel, double* total_ms, uint64_t* launches);
 
 /* Kernel ids for dcg_timing_get. */

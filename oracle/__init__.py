@@ -11,7 +11,7 @@ and pinned by tests/test_oracle_*.py.
 import ctypes as C
 import os
 
-from dcgrid_amd._abi import Handle, bind
+from dcgrid_amd._abi import Handle, Info, Params, bind
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
@@ -32,6 +32,10 @@ def library():
         lib.orc_get_threads.restype = C.c_int32
         lib.orc_export_field_f64.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64]
         lib.orc_export_field_f64.restype = C.c_int
+        lib.orc_lookup.argtypes = [C.c_void_p] + [C.c_int32] * 4
+        lib.orc_lookup.restype = C.c_int64
+        lib.orc_plan.argtypes = [C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Params), C.POINTER(Info)]
+        lib.orc_plan.restype = C.c_int
         for name, args, res in [
             ("orc_bit_spread3", [C.c_uint32], C.c_uint32),
             ("orc_grid_hash", [C.c_int32] * 4, C.c_uint32),
@@ -52,6 +56,17 @@ def set_threads(n):
     library().orc_set_threads(int(n))
 
 
+def plan(size, max_num_blocks, params):
+    """The DCGrid level/budget planner alone (no allocation); returns an Info."""
+    lib = library()
+    inf = Info()
+    st = lib.orc_plan((C.c_int32 * 3)(*size), int(max_num_blocks), C.byref(params), C.byref(inf))
+    if st != 0:
+        from dcgrid_amd._abi import DCGError
+        raise DCGError(st, "plan", "")
+    return inf
+
+
 def get_threads():
     return library().orc_get_threads()
 
@@ -65,6 +80,10 @@ class OracleSim(Handle):
         self._extra = (C.c_int32(precision),)
         self.precision = precision
         super().__init__(kind, size, params, max_num_blocks)
+
+    def lookup(self, x, y, z, level=0):
+        """getBlockIndexDeep on the oracle's hash tables (-1 when nothing contains the cell)."""
+        return int(_lib.orc_lookup(self._h, x, y, z, level))
 
     def export_field_f64(self, field):
         import numpy as np

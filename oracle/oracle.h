@@ -44,6 +44,12 @@ int orc_export_field(orc_sim* s, int32_t field, float* dst, uint64_t nfloats);
 int orc_export_field_f64(orc_sim* s, int32_t field, double* dst, uint64_t nfloats);
 int orc_import_field(orc_sim* s, int32_t field, const float* src, uint64_t nfloats);
 int orc_export_tables(orc_sim* s, dcg_tables* t);
+/* One stage of the step (DCG_STAGE_* of dcgrid_amd.h). */
+int orc_run_stage(orc_sim* s, int32_t stage, int32_t arg);
+/* getBlockIndexDeep (dcgrid_utils.cuh:201-232) on the oracle's hash tables; -1 if none. */
+int64_t orc_lookup(orc_sim* s, int32_t x, int32_t y, int32_t z, int32_t level);
+/* Level/budget planner of the DCGrid constructor only (no allocation). */
+int orc_plan(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p, dcg_info* out);
 /* Number of OpenMP threads used by the oracle (0 = runtime default). */
 void orc_set_threads(int32_t n);
 int32_t orc_get_threads(void);

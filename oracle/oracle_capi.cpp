@@ -7,6 +7,7 @@ namespace orc {
 Sim* makeUniform(const int32_t size[3], const dcg_params& P, int precision, int* status, std::string* err);
 Sim* makeDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& P, int precision, int* status,
                 std::string* err);
+int planDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& P, dcg_info* out, std::string* err);
 }  // namespace orc
 
 struct orc_sim {
@@ -137,6 +138,18 @@ int orc_export_tables(orc_sim* s, dcg_tables* t) {
   IMPL(s);
   if (!t) return DCG_E_ARG;
   return I->exportTables(t);
+}
+int orc_run_stage(orc_sim* s, int32_t stage, int32_t arg) {
+  IMPL(s);
+  return I->runStage(stage, arg);
+}
+int64_t orc_lookup(orc_sim* s, int32_t x, int32_t y, int32_t z, int32_t level) {
+  if (!s) return -1;
+  return s->impl->lookup(x, y, z, level);
+}
+int orc_plan(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p, dcg_info* out) {
+  if (!size || !p || !out) return DCG_E_ARG;
+  return orc::planDCGrid(size, max_num_blocks, *p, out, &g_create_error);
 }
 void orc_set_threads(int32_t n) {
   if (n > 0) omp_set_num_threads(n);

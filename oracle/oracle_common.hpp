@@ -224,6 +224,10 @@ struct Sim {
   virtual int exportField(int field, double* dst, uint64_t n) = 0;
   virtual int importField(int field, const float* src, uint64_t n) = 0;
   virtual int exportTables(dcg_tables* t) = 0;
+  virtual int runStage(int stage, int arg) = 0;
+  // getBlockIndexDeep (DCGrid only): finest allocated block at level >= `level` containing the
+  // level-local cell (x,y,z); -1 when not applicable.
+  virtual int64_t lookup(int x, int y, int z, int level) { return -1; }
   int fail(int code, const std::string& m) {
     err = m;
     return code;

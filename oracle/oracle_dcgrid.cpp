@@ -55,6 +55,10 @@ struct DCGrid : Sim {
   // ---------------------------------------------------------------- planner
   // FluidSimulationDCGrid ctor, fluid_simulation_dcgrid.cu:9-140.
   int create(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p) {
+    if (int e = plan(size, maxBlocks, p)) return e;
+    return allocate();
+  }
+  int plan(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p) {
     P = p;
     gx = size[0];
     gy = size[1];
@@ -104,6 +108,10 @@ struct DCGrid : Sim {
     hoff[0] = 0;
     for (int l = 1; l < L; l++) hoff[l] = hoff[l - 1] + hsize[l - 1];
     C = B * (size_t)BV;
+    for (int l = 0; l < L; l++) loads[l] = maxL[l] == fullL[l] ? maxL[l] : 0;
+    return DCG_OK;
+  }
+  int allocate() {
     hkey.assign(H, HASH_EMPTY);
     hval.assign(H, NF);
     pos.assign(3 * B, 0);
@@ -1103,6 +1111,35 @@ struct DCGrid : Sim {
     for (size_t i = 0; i < n; i++) (*v)[i] = (R)src[i];
     return DCG_OK;
   }
+  int runStage(int stage, int arg) override {
+    const bool perLevel = stage == DCG_STAGE_JACOBI || stage == DCG_STAGE_JACOBI_INV || stage == DCG_STAGE_PROLONGATE;
+    if (perLevel && (arg < 0 || arg >= L || (stage == DCG_STAGE_PROLONGATE && arg >= L - 1)))
+      return fail(DCG_E_ARG, "level out of range");
+    switch (stage) {
+      case DCG_STAGE_DIVERGENCE: calcDivergence(); return DCG_OK;
+      case DCG_STAGE_RESTRICT:
+        switch (arg) {
+          case DCG_FIELD_DENSITY: accumulate(den, (R)0); return DCG_OK;
+          case DCG_FIELD_TEMPERATURE: accumulate(tem, (R)0); return DCG_OK;
+          case DCG_FIELD_DIVERGENCE: accumulate(dive, (R)0); return DCG_OK;
+          case DCG_FIELD_PRESSURE: accumulate(pre, (R)0); return DCG_OK;
+          case DCG_FIELD_VELOCITY: accumulate(vel, V3<R>{0, 0, 0}); return DCG_OK;
+          default: return fail(DCG_E_ARG, "field cannot be restricted");
+        }
+      case DCG_STAGE_JACOBI: jacobi(arg, pre, t_pre); return DCG_OK;
+      case DCG_STAGE_JACOBI_INV: jacobi(arg, t_pre, pre); return DCG_OK;
+      case DCG_STAGE_PROLONGATE: prolongate(arg); return DCG_OK;
+      case DCG_STAGE_APPLY_PRESSURE: applyPressure(); return DCG_OK;
+      case DCG_STAGE_VORTICITY: calcVorticity(); return DCG_OK;
+      default: return fail(DCG_E_ARG, "unknown stage");
+    }
+  }
+  int64_t lookup(int x, int y, int z, int level) override {
+    if (level < 0 || level > S) return -1;
+    int l = level;
+    const size_t b = getBlockIndexDeep(x, y, z, l);
+    return b == NF ? -1 : (int64_t)b;
+  }
   int exportTables(dcg_tables* t) override {
     auto u64 = [](size_t v) { return v == NF ? UINT64_MAX : (uint64_t)v; };
     if (t->block_positions) std::copy(pos.begin(), pos.end(), t->block_positions);
@@ -1143,4 +1180,17 @@ Sim* makeDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& P, 
   return s;
 }
 
+}  // namespace orc
+
+namespace orc {
+// Level/budget planner only (fluid_simulation_dcgrid.cu:9-140), without allocation.
+int planDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& P, dcg_info* out, std::string* err) {
+  DCGrid<float> g;
+  const int st = g.plan(size, maxBlocks, P);
+  if (st != DCG_OK) {
+    *err = g.err;
+    return st;
+  }
+  return g.info(out);
+}
 }  // namespace orc

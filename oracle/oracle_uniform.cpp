@@ -437,6 +437,22 @@ struct Uniform : Sim {
     return DCG_OK;
   }
   int exportTables(dcg_tables*) override { return fail(DCG_E_ARG, "uniform grid has no block tables"); }
+  int runStage(int stage, int arg) override {
+    const bool perLevel = stage == DCG_STAGE_JACOBI || stage == DCG_STAGE_JACOBI_INV || stage == DCG_STAGE_PROLONGATE;
+    if (perLevel && (arg < 0 || arg >= L || (stage == DCG_STAGE_PROLONGATE && arg >= L - 1)))
+      return fail(DCG_E_ARG, "level out of range");
+    switch (stage) {
+      case DCG_STAGE_DIVERGENCE: calcDivergence(); return DCG_OK;
+      case DCG_STAGE_RESTRICT:
+        for (int level = 1; level < L; level++) restrictLevel(level);
+        return DCG_OK;
+      case DCG_STAGE_JACOBI: jacobi(arg, pre, t_pre); return DCG_OK;
+      case DCG_STAGE_JACOBI_INV: jacobi(arg, t_pre, pre); return DCG_OK;
+      case DCG_STAGE_PROLONGATE: prolongate(arg); return DCG_OK;
+      case DCG_STAGE_APPLY_PRESSURE: applyPressure(); return DCG_OK;
+      default: return fail(DCG_E_ARG, "stage not available on the uniform grid");
+    }
+  }
 };
 
 Sim* makeUniform(const int32_t size[3], const dcg_params& P, int precision, int* status,
