@@ -62,7 +62,7 @@ struct DCGridSim : SimBase {
     for (int l = 0, cs = 1; l < L; l++, cs *= 2)
       full64[l] = (uint64_t)iDivUpH(gx, cs * BW) * iDivUpH(gy, cs * BW) * iDivUpH(gz, cs * BW);
     if (maxBlocks < full64[L - 1]) return fail(DCG_E_BUDGET, "too few blocks to fill lowest resolution");
-    if (maxBlocks * (uint64_t)G::BV >= 0xFFFFFFFFull || maxBlocks * 8 >= 0xFFFFFFFFull)
+    if (maxBlocks * (uint64_t)G::BV >= 0xFFFFFFFFull || maxBlocks > ND_BLOCK_MASK)
       return fail(DCG_E_UNSUPPORTED, "budget exceeds 32-bit cell indexing");
     B = (uint32_t)maxBlocks;
     uint64_t mx[DCG_MAX_LEVELS];
@@ -110,10 +110,9 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&g.pos, B));
     DCG_TRY(alloc(&g.lev, B));
     DCG_TRY(alloc(&g.flg, B));
-    DCG_TRY(alloc(&g.edge, B));
     DCG_TRY(alloc(&g.par, B));
     DCG_TRY(alloc(&g.chi, (size_t)8 * B));
-    DCG_TRY(alloc(&g.nbr, (size_t)27 * B));
+    DCG_TRY(alloc(&g.nd, (size_t)27 * B));
     DCG_TRY(alloc(&g.hkey, H));
     DCG_TRY(alloc(&g.hval, H));
     DCG_TRY(alloc(&g.ctl, CTL_WORDS));
@@ -151,10 +150,9 @@ struct DCGridSim : SimBase {
     DCG_TRY(fill32(g.pos, 0, (size_t)3 * B));
     DCG_TRY(fill8(g.lev, FREE, B));
     DCG_TRY(fill32(g.flg, 0, B));
-    DCG_TRY(fill8(g.edge, 0, B));
     DCG_TRY(fill32(g.par, NFU, B));
     DCG_TRY(fill32(g.chi, NFU, (size_t)8 * B));
-    DCG_TRY(fill32(g.nbr, NFU, (size_t)27 * B));
+    DCG_TRY(fill32(g.nd, NFU, (size_t)4 * 27 * B));
     DCG_TRY(fill32(g.den, 0, C));
     DCG_TRY(fill32(g.tem, 0, C));
     DCG_TRY(fill32(g.vel, 0, 3 * C));
@@ -189,7 +187,7 @@ struct DCGridSim : SimBase {
                  (k_sub_scores_feat<<<blocksFor((size_t)8 * B, 256), 256, 0, stream>>>(g, feat, sbs)));
     else
       DCG_LAUNCH(DCG_K_TOPOLOGY, "k_sub_scores_geo",
-                 (k_sub_scores_geo<BW><<<blocksFor((size_t)8 * B, 256), 256, 0, stream>>>(g, P, sbs)));
+                 (k_sub_scores_geo<BW><<<blocksFor((size_t)8 * B, 256), 256, 0, stream>>>(g, sbs)));
     return DCG_OK;
   }
 
@@ -495,15 +493,15 @@ struct DCGridSim : SimBase {
   }
   int exportTables(dcg_tables* t) override {
     std::vector<int32_t> pos(3 * (size_t)B);
-    std::vector<uint8_t> lev(B), edge(B);
-    std::vector<uint32_t> flg(B), par(B), chi(8 * (size_t)B), nbr(27 * (size_t)B), hk(H), hv(H);
+    std::vector<uint8_t> lev(B);
+    std::vector<uint32_t> flg(B), par(B), chi(8 * (size_t)B), hk(H), hv(H);
+    std::vector<int32_t> nd(4 * 27 * (size_t)B);
     DCG_HIP(hipMemcpyAsync(pos.data(), g.pos, pos.size() * 4, hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(lev.data(), g.lev, B, hipMemcpyDeviceToHost, stream));
-    DCG_HIP(hipMemcpyAsync(edge.data(), g.edge, B, hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(flg.data(), g.flg, (size_t)B * 4, hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(par.data(), g.par, (size_t)B * 4, hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(chi.data(), g.chi, chi.size() * 4, hipMemcpyDeviceToHost, stream));
-    DCG_HIP(hipMemcpyAsync(nbr.data(), g.nbr, nbr.size() * 4, hipMemcpyDeviceToHost, stream));
+    DCG_HIP(hipMemcpyAsync(nd.data(), g.nd, nd.size() * 4, hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(hk.data(), g.hkey, (size_t)H * 4, hipMemcpyDeviceToHost, stream));
     DCG_HIP(hipMemcpyAsync(hv.data(), g.hval, (size_t)H * 4, hipMemcpyDeviceToHost, stream));
     uint32_t ctl[CTL_WORDS];
@@ -524,7 +522,6 @@ struct DCGridSim : SimBase {
       constexpr int A = G::A, A3 = G::A3, BV = G::BV;
       for (uint32_t b = 0; b < B; b++) {
         uint64_t* e = t->cell_indices + (size_t)b * A3;
-        const int x0 = pos[3 * b], y0 = pos[3 * b + 1], z0 = pos[3 * b + 2];
         for (int i = 0; i < A; i++)
           for (int j = 0; j < A; j++)
             for (int k = 0; k < A; k++) {
@@ -540,16 +537,16 @@ struct DCGridSim : SimBase {
                 out = UINT64_MAX;  // k_dcgrid_init_apron_indices leaves borders notFound
                 continue;
               }
-              const uint32_t nb = nbr[(size_t)b * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
-              if (nb == b) {
+              // same arithmetic as apronCell (dcgrid_dev.hpp)
+              const int32_t* d = &nd[4 * ((size_t)b * 27 + (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))];
+              const uint32_t sh = (uint32_t)d[0] >> 27, nb = (uint32_t)d[0] & ND_BLOCK_MASK;
+              if (sh == ND_OUT) {
                 const int ci = std::clamp(i, 1, BW), cj = std::clamp(j, 1, BW), ck = std::clamp(k, 1, BW);
-                out = (uint64_t)b * BV + (edge[b] ? hloc(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1))
-                                                  : hloc(ci - 1, cj - 1, ck - 1));
+                out = (uint64_t)b * BV + (d[1] ? hloc(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1))
+                                               : hloc(ci - 1, cj - 1, ck - 1));
                 continue;
               }
-              const int d = (int)lev[nb] - (int)lev[b];
-              out = (uint64_t)nb * BV + hloc(((x0 + i - 1) >> d) - pos[3 * nb], ((y0 + j - 1) >> d) - pos[3 * nb + 1],
-                                             ((z0 + k - 1) >> d) - pos[3 * nb + 2]);
+              out = (uint64_t)nb * BV + hloc((i - 1 - d[1]) >> sh, (j - 1 - d[2]) >> sh, (k - 1 - d[3]) >> sh);
             }
       }
     }

@@ -5,11 +5,12 @@
 //  * block tables: positions (level-local cell origin), level (0xFF = free), flags, parent
 //    subblock (8*parentBlock + top Morton triple), 8 child blocks per block;
 //  * instead of the reference's (BW+2)^3 size_t apron table (27 B/cell at BW=4) every block keeps
-//    27 neighbour-block ids (1.7 B/cell at BW=4, 0.2 B/cell at BW=8). Each apron direction of the
-//    reference table always points into a single block (the finest block at the same or a
-//    coarser level containing that region), so the reference table is a pure function of this
-//    compact table, the block positions/levels and one "edge" bit per block that records the
-//    clamp rule used for out-of-domain directions. dcg_export_tables rebuilds the reference table.
+//    27 neighbour descriptors of 16 B (6.75 B/cell at BW=4, 0.84 B/cell at BW=8). All entries of
+//    one apron direction of the reference table point into a single block (the finest block at
+//    the same or a coarser level containing that region, or the block itself outside the
+//    domain), so each entry is that block's cell at a fixed, shifted offset: the descriptor holds
+//    the block, the level shift and the origin offset, and the reference table is a pure function
+//    of the descriptors (dcg_export_tables rebuilds it).
 #pragma once
 
 #include "dcg_device.hpp"
@@ -37,10 +38,9 @@ struct DG {
   int3* pos;
   uint8_t* lev;
   uint32_t* flg;
-  uint8_t* edge;
   uint32_t* par;
   uint32_t* chi;
-  uint32_t* nbr;  // 27 per block, direction (dx+1)*9+(dy+1)*3+(dz+1)
+  int4* nd;  // 27 neighbour descriptors per block, direction (dx+1)*9+(dy+1)*3+(dz+1)
   uint32_t* hkey;
   uint32_t* hval;
   uint32_t* ctl;  // device control words, see CTL_* below
@@ -108,27 +108,39 @@ __device__ __forceinline__ uint32_t findBlock(const DG& g, int qx, int qy, int q
   return b;
 }
 
-// Global cell index of apron entry (ai,aj,ak) in [0,BW+1]^3 of block b (level lb, origin p0):
-// the value the reference's cellIndices table holds for that entry.
+// Neighbour descriptor of one apron direction of a block: x = neighbour block | shift << 27,
+// (y,z,w) = origin offset o = npos * 2^shift - p0 (block origins in level-local cells, shift =
+// level(neighbour) - level(block)). The cell of apron entry (ai,aj,ak) in that direction is
+// neighbour*BW^3 + Morton(((a-1) - o) >> shift), one load and integer arithmetic.
+// Out-of-domain directions have shift ND_OUT and y = clamp rule of the block (see apronCell).
+static constexpr uint32_t ND_OUT = 31;
+static constexpr uint32_t ND_BLOCK_MASK = 0x07FFFFFFu;  // blocks are < 2^27
+__device__ __forceinline__ uint32_t ndBlock(int4 d) { return (uint32_t)d.x & ND_BLOCK_MASK; }
+__device__ __forceinline__ uint32_t ndShift(int4 d) { return (uint32_t)d.x >> 27; }
+__device__ __forceinline__ int4 makeNd(uint32_t nb, uint32_t shift, int ox, int oy, int oz) {
+  return make_int4((int)(nb | (shift << 27)), ox, oy, oz);
+}
+
+// Global cell index of apron entry (ai,aj,ak) in [0,BW+1]^3 of block b: the value the
+// reference's cellIndices table holds for that entry.
 template <int BW>
-__device__ __forceinline__ uint32_t apronCell(const DG& g, uint32_t b, int3 p0, int lb, int ai, int aj, int ak) {
+__device__ __forceinline__ uint32_t apronCell(const DG& g, uint32_t b, int ai, int aj, int ak) {
   constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
   const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
   const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
   const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
   if ((dx | dy | dz) == 0) return b * BV + loc<BB>(ai - 1, aj - 1, ak - 1);
-  const uint32_t nb = g.nbr[b * 27 + dirIndex(dx, dy, dz)];
-  if (nb == b) {
+  const int4 d = g.nd[b * 27 + dirIndex(dx, dy, dz)];
+  const uint32_t sh = ndShift(d);
+  if (sh == ND_OUT) {
     // Out-of-domain entry: own clamped cell. Blocks placed by a move or a refinement use the
     // reference's SPREAD(clamp(i,1,BW)) form (dcgrid_structure.cu:54-60), dense blocks the
     // clamped interior cell (dcgrid_structure.cu:157-167).
     const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
-    if (g.edge[b]) return b * BV + loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
+    if (d.y) return b * BV + loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
     return b * BV + loc<BB>(ci - 1, cj - 1, ck - 1);
   }
-  const int d = (int)g.lev[nb] - lb;
-  const int3 np = g.pos[nb];
-  return nb * BV + loc<BB>(((p0.x + ai - 1) >> d) - np.x, ((p0.y + aj - 1) >> d) - np.y, ((p0.z + ak - 1) >> d) - np.z);
+  return ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> sh, (aj - 1 - d.z) >> sh, (ak - 1 - d.w) >> sh);
 }
 
 // Apron coordinates of face cell f in [0, 6*BW*BW): faces x-,x+,y-,y+,z-,z+.

@@ -45,20 +45,19 @@ __device__ __forceinline__ int apronIndexOfLocal(int t) {
 
 // Stage one scalar field of a block (interior + 6 face layers) into tile[A^3].
 template <int BW>
-__device__ __forceinline__ void stageScalar(const DG& g, const float* __restrict__ f, float* tile, uint32_t b, int3 p0,
-                                            int lb, int t) {
+__device__ __forceinline__ void stageScalar(const DG& g, const float* __restrict__ f, float* tile, uint32_t b, int t) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE;
   tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
   for (int fc = t; fc < FACE; fc += BV) {
     int ai, aj, ak;
     faceCell<BW>(fc, ai, aj, ak);
-    tile[(ai * A + aj) * A + ak] = f[apronCell<BW>(g, b, p0, lb, ai, aj, ak)];
+    tile[(ai * A + aj) * A + ak] = f[apronCell<BW>(g, b, ai, aj, ak)];
   }
 }
 
 // Fluidity of the six apron neighbours of local cell t (1 when the field is known to be 1).
 template <int BW, bool FLUIDITY>
-__device__ __forceinline__ void neighbourFluidity(const DG& g, uint32_t b, int3 p0, int lb, int t, float w[6]) {
+__device__ __forceinline__ void neighbourFluidity(const DG& g, uint32_t b, int t, float w[6]) {
   if (!FLUIDITY) {
 #pragma unroll
     for (int k = 0; k < 6; k++) w[k] = 1.f;
@@ -68,12 +67,12 @@ __device__ __forceinline__ void neighbourFluidity(const DG& g, uint32_t b, int3 
   int x, y, z;
   locPos<BB>((uint32_t)t, x, y, z);
   const int ai = x + 1, aj = y + 1, ak = z + 1;
-  w[0] = g.flu[apronCell<BW>(g, b, p0, lb, ai - 1, aj, ak)];  // l
-  w[1] = g.flu[apronCell<BW>(g, b, p0, lb, ai + 1, aj, ak)];  // r
-  w[2] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj - 1, ak)];  // d
-  w[3] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj + 1, ak)];  // u
-  w[4] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj, ak - 1)];  // b
-  w[5] = g.flu[apronCell<BW>(g, b, p0, lb, ai, aj, ak + 1)];  // f
+  w[0] = g.flu[apronCell<BW>(g, b, ai - 1, aj, ak)];  // l
+  w[1] = g.flu[apronCell<BW>(g, b, ai + 1, aj, ak)];  // r
+  w[2] = g.flu[apronCell<BW>(g, b, ai, aj - 1, ak)];  // d
+  w[3] = g.flu[apronCell<BW>(g, b, ai, aj + 1, ak)];  // u
+  w[4] = g.flu[apronCell<BW>(g, b, ai, aj, ak - 1)];  // b
+  w[5] = g.flu[apronCell<BW>(g, b, ai, aj, ak + 1)];  // f
 }
 
 // ------------------------------------------------------------------ Jacobi half-sweep
@@ -85,14 +84,11 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(DG g, int level, const f
                                                          float* __restrict__ out, float alpha) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float tiles[BPW][A3];
-  const BlockCtx<BW> c = blockCtx<BW>(g.offs[level], g.maxL[level]);
-  const bool active = c.valid && g.lev[c.b] == level;
+  // the allocated blocks of a level are the prefix [offs, offs + loads) of its slice
+  const BlockCtx<BW> c = blockCtx<BW>(g.offs[level], g.ctl[CTL_LOADS + level]);
+  const bool active = c.valid;
   float* tile = tiles[c.sub];
-  int3 p0 = make_int3(0, 0, 0);
-  if (active) {
-    p0 = g.pos[c.b];
-    stageScalar<BW>(g, in, tile, c.b, p0, level, c.t);
-  }
+  if (active) stageScalar<BW>(g, in, tile, c.b, c.t);
   __syncthreads();
   if (!active) return;
   const int a = apronIndexOfLocal<BW>(c.t);
@@ -127,7 +123,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(DG g, dcg_params P) 
     for (int fc = c.t; fc < FACE; fc += BV) {
       int ai, aj, ak;
       faceCell<BW>(fc, ai, aj, ak);
-      const float3 v = g.vel[apronCell<BW>(g, c.b, p0, lb, ai, aj, ak)];
+      const float3 v = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
       tile[(ai * A + aj) * A + ak] = velocityBC(P, v, p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
     }
   }
@@ -135,7 +131,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(DG g, dcg_params P) 
   if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) return;
   const float alpha = .5f * P.rdx / (float)scale;
   float w[6];
-  neighbourFluidity<BW, FLUIDITY>(g, c.b, p0, lb, c.t, w);
+  neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
   const int a = apronIndexOfLocal<BW>(c.t);
   const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
                vf = tile[a + 1];
@@ -152,18 +148,16 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(DG g, dcg_params
   const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
   const bool active = c.valid && g.lev[c.b] != FREE;
   float* tile = tiles[c.sub];
-  int3 p0 = make_int3(0, 0, 0);
   int lb = 0;
   if (active) {
     lb = g.lev[c.b];
-    p0 = g.pos[c.b];
-    stageScalar<BW>(g, g.pre, tile, c.b, p0, lb, c.t);
+    stageScalar<BW>(g, g.pre, tile, c.b, c.t);
   }
   __syncthreads();
   if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) return;
   const float alpha = .5f * P.rdx / (float)(1 << lb);
   float w[6];
-  neighbourFluidity<BW, FLUIDITY>(g, c.b, p0, lb, c.t, w);
+  neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
   const int a = apronIndexOfLocal<BW>(c.t);
   const uint32_t cell = c.b * BV + c.t;
   const float p = tile[a];
@@ -184,24 +178,22 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(DG g, dcg_params P) {
   const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
   const bool active = c.valid && g.lev[c.b] != FREE;
   float3* tile = tiles[c.sub];
-  int3 p0 = make_int3(0, 0, 0);
   int lb = 0;
   const uint32_t cell = c.b * BV + c.t;
   if (active) {
     lb = g.lev[c.b];
-    p0 = g.pos[c.b];
     tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
     for (int fc = c.t; fc < FACE; fc += BV) {
       int ai, aj, ak;
       faceCell<BW>(fc, ai, aj, ak);
-      tile[(ai * A + aj) * A + ak] = g.vel[apronCell<BW>(g, c.b, p0, lb, ai, aj, ak)];
+      tile[(ai * A + aj) * A + ak] = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
     }
   }
   __syncthreads();
   if (!active) return;
   const float alpha = .5f * P.rdx / (float)(1 << lb);
   float w[6];
-  neighbourFluidity<BW, FLUIDITY>(g, c.b, p0, lb, c.t, w);
+  neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
   const int a = apronIndexOfLocal<BW>(c.t);
   const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
                vf = tile[a + 1];
@@ -252,18 +244,16 @@ __global__ void __launch_bounds__(256) k_restrict(DG g, int level, T* __restrict
 template <int BW>
 __device__ __forceinline__ void parentStencil(const DG& g, uint32_t ps, int x, int y, int z, uint32_t q[8]) {
   const uint32_t pb = ps / 8;
-  const int3 pp = g.pos[pb];
-  const int pl = g.lev[pb];
   const int ai = 1 + (x / 2) % BW, aj = 1 + (y / 2) % BW, ak = 1 + (z / 2) % BW;
   const int si = x % 2 ? 1 : -1, sj = y % 2 ? 1 : -1, sk = z % 2 ? 1 : -1;
-  q[0] = apronCell<BW>(g, pb, pp, pl, ai, aj, ak);
-  q[1] = apronCell<BW>(g, pb, pp, pl, ai, aj, ak + sk);
-  q[2] = apronCell<BW>(g, pb, pp, pl, ai, aj + sj, ak);
-  q[3] = apronCell<BW>(g, pb, pp, pl, ai + si, aj, ak);
-  q[4] = apronCell<BW>(g, pb, pp, pl, ai, aj + sj, ak + sk);
-  q[5] = apronCell<BW>(g, pb, pp, pl, ai + si, aj, ak + sk);
-  q[6] = apronCell<BW>(g, pb, pp, pl, ai + si, aj + sj, ak);
-  q[7] = apronCell<BW>(g, pb, pp, pl, ai + si, aj + sj, ak + sk);
+  q[0] = apronCell<BW>(g, pb, ai, aj, ak);
+  q[1] = apronCell<BW>(g, pb, ai, aj, ak + sk);
+  q[2] = apronCell<BW>(g, pb, ai, aj + sj, ak);
+  q[3] = apronCell<BW>(g, pb, ai + si, aj, ak);
+  q[4] = apronCell<BW>(g, pb, ai, aj + sj, ak + sk);
+  q[5] = apronCell<BW>(g, pb, ai + si, aj, ak + sk);
+  q[6] = apronCell<BW>(g, pb, ai + si, aj + sj, ak);
+  q[7] = apronCell<BW>(g, pb, ai + si, aj + sj, ak + sk);
 }
 
 // k_dcgrid_prolongate (dcgrid_multigrid_solver.cu:43-76)
@@ -355,7 +345,7 @@ struct CrossSample {
     z0 = (int)zf;
     const int ai = clampi(x0 + 1 - sp.x, 0, BW), aj = clampi(y0 + 1 - sp.y, 0, BW), ak = clampi(z0 + 1 - sp.z, 0, BW);
 #pragma unroll
-    for (int c = 0; c < 8; c++) i[c] = apronCell<BW>(g, sb, sp, sl, ai + ((c >> 2) & 1), aj + ((c >> 1) & 1), ak + (c & 1));
+    for (int c = 0; c < 8; c++) i[c] = apronCell<BW>(g, sb, ai + ((c >> 2) & 1), aj + ((c >> 1) & 1), ak + (c & 1));
     const float Dx = 1.f - dx, Dy = 1.f - dy, Dz = 1.f - dz;
     float ww[8];
     if (FLUIDITY) {
@@ -404,12 +394,105 @@ __device__ __forceinline__ void footPoint(const DG& g, const dcg_params& P, uint
   pz = ((float)z + .5f) * scale - v.z * alpha;
 }
 
-// k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension).
+// Same sample as CrossSample when the foot point's finest block is the thread's own block b:
+// corners and fluidity come from the block's LDS tile (interior + full apron, raw values).
+// Returns false (and the caller takes the global path) when the foot point lies in another
+// block or in a refined subblock of b.
+template <int BW, bool FLUIDITY>
+struct TileSample {
+  int a0;  // tile index of corner 000
+  float w[8];
+  int x0, y0, z0, scale;
+  bool zero;
+  __device__ __forceinline__ bool init(const DG& g, float px, float py, float pz, uint32_t b, int3 p0, int lb,
+                                       const float* fluTile) {
+    constexpr int A = Geo<BW>::A, BB = Geo<BW>::BB;
+    const int ix = clampi((int)floorf(px), 0, g.gx - 1);
+    const int iy = clampi((int)floorf(py), 0, g.gy - 1);
+    const int iz = clampi((int)floorf(pz), 0, g.gz - 1);
+    const int qx = (ix >> lb) - p0.x, qy = (iy >> lb) - p0.y, qz = (iz >> lb) - p0.z;
+    if ((unsigned)qx >= (unsigned)BW || (unsigned)qy >= (unsigned)BW || (unsigned)qz >= (unsigned)BW) return false;
+    const int t = (((qx >> (BB - 1)) & 1) << 2) | (((qy >> (BB - 1)) & 1) << 1) | ((qz >> (BB - 1)) & 1);
+    if (g.chi[8 * b + t] != NFU) return false;
+    scale = 1 << lb;
+    const float scaleInv = 1.f / (float)scale;
+    const float x = px * scaleInv - .5f, y = py * scaleInv - .5f, z = pz * scaleInv - .5f;
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    const float dx = x - xf, dy = y - yf, dz = z - zf;
+    x0 = (int)xf;
+    y0 = (int)yf;
+    z0 = (int)zf;
+    const int ai = clampi(x0 + 1 - p0.x, 0, BW), aj = clampi(y0 + 1 - p0.y, 0, BW), ak = clampi(z0 + 1 - p0.z, 0, BW);
+    a0 = (ai * A + aj) * A + ak;
+    const float Dx = 1.f - dx, Dy = 1.f - dy, Dz = 1.f - dz;
+    float ww[8];
+    if (FLUIDITY) {
+      ww[0] = fluTile[a0] * Dx * Dy * Dz;
+      ww[1] = fluTile[a0 + 1] * Dx * Dy * dz;
+      ww[2] = fluTile[a0 + A] * Dx * dy * Dz;
+      ww[3] = fluTile[a0 + A + 1] * Dx * dy * dz;
+      ww[4] = fluTile[a0 + A * A] * dx * Dy * Dz;
+      ww[5] = fluTile[a0 + A * A + 1] * dx * Dy * dz;
+      ww[6] = fluTile[a0 + A * A + A] * dx * dy * Dz;
+      ww[7] = fluTile[a0 + A * A + A + 1] * dx * dy * dz;
+    } else {
+      ww[0] = Dx * Dy * Dz;
+      ww[1] = Dx * Dy * dz;
+      ww[2] = Dx * dy * Dz;
+      ww[3] = Dx * dy * dz;
+      ww[4] = dx * Dy * Dz;
+      ww[5] = dx * Dy * dz;
+      ww[6] = dx * dy * Dz;
+      ww[7] = dx * dy * dz;
+    }
+    const float wAcc = ww[0] + ww[1] + ww[2] + ww[3] + ww[4] + ww[5] + ww[6] + ww[7];
+    const float wInv = 1.f / wAcc;
+#pragma unroll
+    for (int c = 0; c < 8; c++) w[c] = ww[c] * wInv;
+    zero = wAcc < 1e-6f;
+    return true;
+  }
+  static __device__ __forceinline__ int off(int c) {
+    constexpr int A = Geo<BW>::A;
+    return ((c >> 2) & 1) * A * A + ((c >> 1) & 1) * A + (c & 1);
+  }
+};
+
+// Stage a block's values over its whole apron ((BW+2)^3 entries, edges and corners included):
+// tile[a] = field[cell of apron entry a], raw (boundary conditions are applied per sample).
+template <int BW, class T>
+__device__ __forceinline__ void stageApron(const DG& g, const T* __restrict__ f, T* tile, uint32_t b, int t) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3;
+  tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
+  for (int a = t; a < A3; a += BV) {
+    const int ai = a / (A * A), aj = (a / A) % A, ak = a % A;
+    if (ai > 0 && ai <= BW && aj > 0 && aj <= BW && ak > 0 && ak <= BW) continue;
+    tile[a] = f[apronCell<BW>(g, b, ai, aj, ak)];
+  }
+}
+
+// k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension). The block's
+// velocity (and fluidity) with the full apron are staged in LDS; samples whose foot point stays
+// in the block read the tile, the others gather through the neighbour tables.
 template <int BW, bool FLUIDITY, bool BUOY>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_params P) {
-  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  __shared__ float3 vtiles[BPW][A3];
+  __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  if (!c.valid || g.lev[c.b] == FREE) return;
+  const bool active = c.valid && g.lev[c.b] != FREE;
+  float3* vt = vtiles[c.sub];
+  float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+  int3 p0 = make_int3(0, 0, 0);
+  int lb = 0;
+  if (active) {
+    lb = g.lev[c.b];
+    p0 = g.pos[c.b];
+    stageApron<BW, float3>(g, g.vel, vt, c.b, c.t);
+    if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
+  }
+  __syncthreads();
+  if (!active) return;
   const uint32_t cell = c.b * BV + c.t;
   if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
     g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
@@ -417,15 +500,28 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_param
   }
   float px, py, pz;
   int x, y, z, level;
-  footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
-  CrossSample<BW, FLUIDITY> s;
-  s.init(g, px, py, pz);
+  footPoint<BW>(g, P, c.b, c.t, vt[apronIndexOfLocal<BW>(c.t)], px, py, pz, x, y, z, level);
   float3 acc = make_float3(0.f, 0.f, 0.f);
-  if (!s.zero) {
+  TileSample<BW, FLUIDITY> ts;
+  if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
+    if (!ts.zero) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const float3 vk = velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
-      acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
+      for (int k = 0; k < 8; k++) {
+        const float3 vk = velocityBC(P, vt[ts.a0 + ts.off(k)], ts.x0 + ((k >> 2) & 1), ts.y0 + ((k >> 1) & 1),
+                                     ts.z0 + (k & 1), ts.scale);
+        acc = k == 0 ? f3mul(vk, ts.w[0]) : f3add(acc, f3mul(vk, ts.w[k]));
+      }
+    }
+  } else {
+    CrossSample<BW, FLUIDITY> s;
+    s.init(g, px, py, pz);
+    if (!s.zero) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const float3 vk =
+            velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
+        acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
+      }
     }
   }
   if (BUOY) {
@@ -435,12 +531,30 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_param
   g.tvel[cell] = acc;
 }
 
-// k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension).
+// k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
+// with the same LDS fast path as the velocity.
 template <int BW, bool FLUIDITY, bool SRC>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params P) {
-  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  __shared__ float dtiles[BPW][A3];
+  __shared__ float ttiles[SRC ? BPW : 1][SRC ? A3 : 1];
+  __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  if (!c.valid || g.lev[c.b] == FREE) return;
+  const bool active = c.valid && g.lev[c.b] != FREE;
+  float* dt_ = dtiles[c.sub];
+  float* tt = SRC ? ttiles[c.sub] : nullptr;
+  float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+  int3 p0 = make_int3(0, 0, 0);
+  int lb = 0;
+  if (active) {
+    lb = g.lev[c.b];
+    p0 = g.pos[c.b];
+    stageApron<BW, float>(g, g.den, dt_, c.b, c.t);
+    if (SRC) stageApron<BW, float>(g, g.tem, tt, c.b, c.t);
+    if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
+  }
+  __syncthreads();
+  if (!active) return;
   const uint32_t cell = c.b * BV + c.t;
   if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
     g.tden[cell] = 0.f;
@@ -450,18 +564,34 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
   float px, py, pz;
   int x, y, z, level;
   footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
-  CrossSample<BW, FLUIDITY> s;
-  s.init(g, px, py, pz);
   float q = 0.f, tq = 0.f;
-  if (!s.zero) {
+  TileSample<BW, FLUIDITY> ts;
+  if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
+    if (!ts.zero) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
-      const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
-      q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
-      if (SRC) {
-        const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
-        tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
+      for (int k = 0; k < 8; k++) {
+        const int cx = ts.x0 + ((k >> 2) & 1), cy = ts.y0 + ((k >> 1) & 1), cz = ts.z0 + (k & 1);
+        const float qk = densityBC(P, dt_[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+        q = k == 0 ? qk * ts.w[0] : q + qk * ts.w[k];
+        if (SRC) {
+          const float tk = temperatureBC(P, tt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+          tq = k == 0 ? tk * ts.w[0] : tq + tk * ts.w[k];
+        }
+      }
+    }
+  } else {
+    CrossSample<BW, FLUIDITY> s;
+    s.init(g, px, py, pz);
+    if (!s.zero) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
+        const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
+        q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
+        if (SRC) {
+          const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
+          tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
+        }
       }
     }
   }
@@ -491,19 +621,18 @@ __global__ void __launch_bounds__(64) k_residual(DG g, dcg_params P, float* __re
   }
   const int lb = g.lev[b];
   const int scale = 1 << lb;
-  const int3 p0 = g.pos[b];
   const float alpha = P.rdx * P.rdx / (float)(scale * scale);
   float s = 0.f;
   for (int i = 1; i <= BW; i++)
     for (int j = 1; j <= BW; j++)
       for (int k = 1; k <= BW; k++) {
-        const uint32_t cidx = apronCell<BW>(g, b, p0, lb, i, j, k);
+        const uint32_t cidx = apronCell<BW>(g, b, i, j, k);
         if (g.chi[cidx >> SBB] != NFU) continue;
         const float* p = g.pre;
         const float pc = p[cidx];
-        const float pl = p[apronCell<BW>(g, b, p0, lb, i - 1, j, k)], pr = p[apronCell<BW>(g, b, p0, lb, i + 1, j, k)];
-        const float pd = p[apronCell<BW>(g, b, p0, lb, i, j - 1, k)], pu = p[apronCell<BW>(g, b, p0, lb, i, j + 1, k)];
-        const float pb = p[apronCell<BW>(g, b, p0, lb, i, j, k - 1)], pf = p[apronCell<BW>(g, b, p0, lb, i, j, k + 1)];
+        const float pl = p[apronCell<BW>(g, b, i - 1, j, k)], pr = p[apronCell<BW>(g, b, i + 1, j, k)];
+        const float pd = p[apronCell<BW>(g, b, i, j - 1, k)], pu = p[apronCell<BW>(g, b, i, j + 1, k)];
+        const float pb = p[apronCell<BW>(g, b, i, j, k - 1)], pf = p[apronCell<BW>(g, b, i, j, k + 1)];
         const float r = g.div[cidx] - (pl + pr + pd + pu + pb + pf - 6.f * pc) * alpha;
         s = s + (float)scale * fabsf(r);
       }

@@ -34,10 +34,8 @@ __global__ void __launch_bounds__(256) k_activate_dense(DG g, dcg_params P, int 
   const int3 p = make_int3(BW * bx, BW * by, BW * bz);
   g.pos[b] = p;
   g.lev[b] = (uint8_t)level;
-  g.edge[b] = 0;
   if (level < g.L - 1) {
-    const int rx1 = g.rx[level + 1], ry1 = g.ry[level + 1], rz1 = g.rz[level + 1];
-    (void)rx1;
+    const int ry1 = g.ry[level + 1], rz1 = g.rz[level + 1];
     const uint32_t pb = g.offs[level + 1] + (uint32_t)(((bx >> 1) * ry1 + (by >> 1)) * rz1 + (bz >> 1));
     g.par[b] = 8 * pb + 4 * (bx & 1) + 2 * (by & 1) + (bz & 1);
   }
@@ -50,12 +48,12 @@ __global__ void __launch_bounds__(256) k_activate_dense(DG g, dcg_params P, int 
     for (int dy = -1; dy <= 1; dy++)
       for (int dz = -1; dz <= 1; dz++) {
         const int nx = bx + dx, ny = by + dy, nz = bz + dz;
-        uint32_t nb;
+        int4 d;
         if (nx < 0 || ny < 0 || nz < 0 || nx >= rx || ny >= ry || nz >= rz)
-          nb = b;
+          d = makeNd(b, ND_OUT, 0, 0, 0);  // clamped interior cell (dcgrid_structure.cu:157-167)
         else
-          nb = g.offs[level] + (uint32_t)((nx * ry + ny) * rz + nz);
-        g.nbr[b * 27 + dirIndex(dx, dy, dz)] = nb;
+          d = makeNd(g.offs[level] + (uint32_t)((nx * ry + ny) * rz + nz), 0, BW * dx, BW * dy, BW * dz);
+        g.nd[b * 27 + dirIndex(dx, dy, dz)] = d;
       }
   const int scale = 1 << level;
   for (int t = 0; t < BV; t++) {
@@ -77,7 +75,7 @@ __device__ __forceinline__ bool levelBlocked(const DG& g, int level) {
 
 // k_dcgrid_calc_subblock_scores, geometric score (dcgrid_adaptation.cu:10-34).
 template <int BW>
-__global__ void __launch_bounds__(256) k_sub_scores_geo(DG g, dcg_params P, float* __restrict__ sbs) {
+__global__ void __launch_bounds__(256) k_sub_scores_geo(DG g, float* __restrict__ sbs) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= 8 * g.B) return;
   const uint32_t b = s / 8;
@@ -115,15 +113,13 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(DG g, dcg_params P
   const int level = valid ? (int)g.lev[b] : FREE;
   const bool active = valid && level != FREE;
   float* tile = tiles[sub];
-  int3 p0 = make_int3(0, 0, 0);
   if (t < 8) smax[sub][t] = 0u;
   if (active) {
-    p0 = g.pos[b];
     tile[apronIndexOfLocal<BW>(t)] = g.den[b * BV + t];
     for (int fc = t; fc < FACE; fc += BV) {
       int ai, aj, ak;
       faceCell<BW>(fc, ai, aj, ak);
-      tile[(ai * A + aj) * A + ak] = g.den[apronCell<BW>(g, b, p0, level, ai, aj, ak)];
+      tile[(ai * A + aj) * A + ak] = g.den[apronCell<BW>(g, b, ai, aj, ak)];
     }
   }
   __syncthreads();
@@ -506,7 +502,7 @@ __global__ void __launch_bounds__(256) k_refresh(DG g) {
   constexpr int BB = Geo<BW>::BB;
   const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
   const uint32_t b = gid / 27, d = gid % 27;
-  if (b >= g.B || d == 13) return;
+  if (b >= g.B) return;
   const int level = g.lev[b];
   if (level == FREE) return;
   const int dx = (int)(d / 9) - 1, dy = (int)((d / 3) % 3) - 1, dz = (int)(d % 3) - 1;
@@ -516,30 +512,41 @@ __global__ void __launch_bounds__(256) k_refresh(DG g) {
   const int ay = p0.y + (dy < 0 ? -1 : (dy > 0 ? BW : 0));
   const int az = p0.z + (dz < 0 ? -1 : (dz > 0 ? BW : 0));
   const uint32_t f = g.flg[b];
-  uint32_t* e = &g.nbr[b * 27 + d];
+  int4* e = &g.nd[b * 27 + d];
+  uint32_t nb;
   if (f & MOVED) {
-    if (d == 0) g.edge[b] = 1;
-    const int scale = 1 << level;
-    if (ax < 0 || ay < 0 || az < 0 || ax * scale >= g.gx || ay * scale >= g.gy || az * scale >= g.gz) {
-      *e = b;
+    if (d == 13) {
+      *e = makeNd(b, 0, 0, 0, 0);
       return;
     }
-    *e = findBlock<BW>(g, ax, ay, az, level);
-    return;
+    const int scale = 1 << level;
+    if (ax < 0 || ay < 0 || az < 0 || ax * scale >= g.gx || ay * scale >= g.gy || az * scale >= g.gz) {
+      *e = makeNd(b, ND_OUT, 1, 0, 0);  // SPREAD(clamp(i,1,BW)) form, dcgrid_structure.cu:54-60
+      return;
+    }
+    nb = findBlock<BW>(g, ax, ay, az, level);
+  } else {
+    if (d == 13) return;
+    const int4 old = *e;
+    if (ndShift(old) == ND_OUT) return;
+    const uint32_t prev = ndBlock(old);
+    const uint32_t pf = g.flg[prev];
+    if (pf & MOVED) {
+      nb = findBlock<BW>(g, ax, ay, az, level);
+    } else if ((pf & REFINED) && (int)g.lev[prev] > level) {
+      const int dl = (int)g.lev[prev] - level;
+      const int3 pp = g.pos[prev];
+      const int lx = (ax >> dl) - pp.x, ly = (ay >> dl) - pp.y, lz = (az >> dl) - pp.z;
+      const int t = (((lx >> (BB - 1)) & 1) << 2) | (((ly >> (BB - 1)) & 1) << 1) | ((lz >> (BB - 1)) & 1);
+      nb = g.chi[8 * prev + t];
+      if (nb == NFU) return;
+    } else {
+      return;  // neither block moved: the descriptor is still exact
+    }
   }
-  const uint32_t prev = *e;
-  if (prev == b) return;
-  const uint32_t pf = g.flg[prev];
-  if (pf & MOVED) {
-    *e = findBlock<BW>(g, ax, ay, az, level);
-  } else if ((pf & REFINED) && (int)g.lev[prev] > level) {
-    const int dl = (int)g.lev[prev] - level;
-    const int3 pp = g.pos[prev];
-    const int lx = (ax >> dl) - pp.x, ly = (ay >> dl) - pp.y, lz = (az >> dl) - pp.z;
-    const int t = (((lx >> (BB - 1)) & 1) << 2) | (((ly >> (BB - 1)) & 1) << 1) | ((lz >> (BB - 1)) & 1);
-    const uint32_t c = g.chi[8 * prev + t];
-    if (c != NFU) *e = c;
-  }
+  const int sh = (int)g.lev[nb] - level;
+  const int3 np = g.pos[nb];
+  *e = makeNd(nb, (uint32_t)sh, (np.x << sh) - p0.x, (np.y << sh) - p0.y, (np.z << sh) - p0.z);
 }
 
 }  // namespace dcg
