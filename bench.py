@@ -65,6 +65,36 @@ def step_bytes(info, params):
     return b
 
 
+def copy_probe(nbytes=1 << 30):
+    """A plain device copy of known size (read nbytes, write nbytes), three times: the reference
+    that rocprofv3's FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: 3 x {nbytes} B copied")
+    del x, y
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 counter summary committed for this
+    workload (profiles/traffic_<kernel>_<workload>.txt, 'key: value' lines), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        return int(float(kv["traffic_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
+
+
 def make_sim(cls, w, **kw):
     s = cls(w.kind, w.size, w.params, w.max_num_blocks if w.kind == "dcgrid" else None, **kw)
     s.seed_scene()
@@ -103,6 +133,69 @@ def cpu_baseline(w, warm_steps, budget_s):
     }
 
 
+def adjoint_bench(w, args):
+    """Differentiable control workload (uniform 128^3, 64 steps, checkpoints every 8): time of
+    the plain forward rollout, of the recorded forward (with checkpoints) and of the backward
+    pass (segment recomputation + adjoint kernels); ratio = backward / plain forward."""
+    import torch
+    import dcgrid_amd
+    from dcgrid_amd import scenes
+    n = w.size[0]
+    steps, k = w.steps, 8
+    den, vel, tgt = scenes.control_inputs(n)
+    sim = dcgrid_amd.Simulation(w.kind, w.size, w.params)
+    dev = torch.device("cuda")
+    d0 = torch.as_tensor(den, device=dev)
+    v0 = torch.as_tensor(vel, device=dev)
+    target = torch.as_tensor(tgt, device=dev)
+    N = d0.numel()
+
+    def load():
+        torch.cuda.synchronize()
+        sim.import_field_device("density", d0.data_ptr(), N)
+        sim.import_field_device("velocity", v0.data_ptr(), 3 * N)
+
+    load()
+    sim.step(2)  # warm-up
+    sim.synchronize()
+    load()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    t_fwd = time.perf_counter() - t0
+    load()
+    t0 = time.perf_counter()
+    sim.tape_forward(steps, k)
+    sim.synchronize()
+    t_tape = time.perf_counter() - t0
+    dN = torch.empty_like(d0)
+    sim.export_field_device("density", dN.data_ptr(), N)
+    g = (dN - target).contiguous()
+    gv = torch.zeros_like(v0)
+    od, ov = torch.empty_like(d0), torch.empty_like(v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.tape_backward(g.data_ptr(), gv.data_ptr(), None, od.data_ptr(), ov.data_ptr(), None)
+    t_bwd = time.perf_counter() - t0
+    sim.close()
+    return {
+        "metric": "adjoint/forward time ratio",
+        "value": round(t_bwd / t_fwd, 3),
+        "unit": "ratio",
+        "n_gpus": 1,
+        "steps": steps,
+        "higher_is_better": False,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian velocity, Gaussian blob density, 4-blob target)",
+        "config": {"workload": w.name, "domain": list(w.size), "checkpoint_every": k},
+        "forward_ms": round(1e3 * t_fwd, 3),
+        "recorded_forward_ms": round(1e3 * t_tape, 3),
+        "backward_ms": round(1e3 * t_bwd, 3),
+        "forward_cell_steps_per_s": round(N * steps / t_fwd, 1),
+        "grad_norm_velocity": float(torch.linalg.vector_norm(ov)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +206,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -122,7 +217,12 @@ def main():
     import dcgrid_amd
     from dcgrid_amd import scenes
     w = scenes.config(args.config)
+    if args.config == 3:
+        print(json.dumps(adjoint_bench(w, args)), flush=True)
+        return
 
+    if args.copy_probe:
+        copy_probe()
     log(f"[bench] creating {w.name} {w.size} budget={w.max_num_blocks}")
     sim = make_sim(dcgrid_amd.Simulation, w)
     log(f"[bench] warmup {args.warmup} steps")
@@ -168,7 +268,7 @@ def main():
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None,
+                "traffic": measured_traffic(w.name, "k_jacobi"),
                 "avg_launch_us": round(avg_ms * 1e3, 3),
                 "launches": int(j_n),
                 "algorithmic_bytes_per_launch": int(per_launch_bytes),

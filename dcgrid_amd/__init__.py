@@ -42,6 +42,14 @@ def library():
         lib.dcg_get_params.restype = C.c_int
         lib.dcg_cell_steps.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         lib.dcg_cell_steps.restype = C.c_int
+        for name in ("dcg_export_field_device", "dcg_import_field_device"):
+            f = getattr(lib, name)
+            f.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]
+            f.restype = C.c_int
+        lib.dcg_tape_forward.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        lib.dcg_tape_forward.restype = C.c_int
+        lib.dcg_tape_backward.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        lib.dcg_tape_backward.restype = C.c_int
         _lib, _fns = lib, fns
     return _lib
 
@@ -87,3 +95,25 @@ class Simulation(Handle):
             self._check(_lib.dcg_timing_get(self._h, k, C.byref(ms), C.byref(n)), "timing_get")
             out[name] = (ms.value, n.value)
         return out
+
+    # -- device-pointer access and the differentiable rollout (uniform grid)
+    def export_field_device(self, field, ptr, nfloats):
+        """Copy a field into caller-owned device memory at `ptr` (an int address)."""
+        f = FIELDS.get(field, field)
+        self._check(_lib.dcg_export_field_device(self._h, f, C.c_void_p(ptr), int(nfloats)), "export_field_device")
+
+    def import_field_device(self, field, ptr, nfloats):
+        f = FIELDS.get(field, field)
+        self._check(_lib.dcg_import_field_device(self._h, f, C.c_void_p(ptr), int(nfloats)), "import_field_device")
+
+    def tape_forward(self, steps, checkpoint_every):
+        """Run `steps` steps from the current state, checkpointing every `checkpoint_every`."""
+        self._check(_lib.dcg_tape_forward(self._h, int(steps), int(checkpoint_every)), "tape_forward")
+
+    def tape_backward(self, g_density, g_velocity, g_temperature, d_density, d_velocity, d_temperature):
+        """Device addresses (ints; temperature ones may be None): adjoint of the final state in,
+        gradient with respect to the rollout's initial state out."""
+        vp = lambda p: C.c_void_p(p) if p else None  # noqa: E731
+        st = _lib.dcg_tape_backward(self._h, vp(g_density), vp(g_velocity), vp(g_temperature), vp(d_density),
+                                    vp(d_velocity), vp(d_temperature))
+        self._check(st, "tape_backward")

@@ -253,6 +253,26 @@ int dcg_import_field(dcg_sim* s, int32_t field, const float* src, uint64_t n) {
   if (!src) return DCG_E_ARG;
   return I->importField(field, src, n);
 }
+int dcg_export_field_device(dcg_sim* s, int32_t field, float* dst, uint64_t n) {
+  IMPL(s);
+  if (!dst) return DCG_E_ARG;
+  return I->exportFieldDevice(field, dst, n);
+}
+int dcg_import_field_device(dcg_sim* s, int32_t field, const float* src, uint64_t n) {
+  IMPL(s);
+  if (!src) return DCG_E_ARG;
+  return I->importFieldDevice(field, src, n);
+}
+int dcg_tape_forward(dcg_sim* s, int32_t steps, int32_t checkpoint_every) {
+  IMPL(s);
+  return I->tapeForward(steps, checkpoint_every);
+}
+int dcg_tape_backward(dcg_sim* s, const float* g_density, const float* g_velocity, const float* g_temperature,
+                      float* d_density, float* d_velocity, float* d_temperature) {
+  IMPL(s);
+  if (!g_density || !g_velocity || !d_density || !d_velocity) return DCG_E_ARG;
+  return I->tapeBackward(g_density, g_velocity, g_temperature, d_density, d_velocity, d_temperature);
+}
 int dcg_export_tables(dcg_sim* s, dcg_tables* t) {
   IMPL(s);
   if (!t) return DCG_E_ARG;

@@ -62,6 +62,21 @@ struct SimBase {
   virtual int countStep() = 0;
   virtual int cellSteps(uint64_t* out) = 0;
   virtual int runStage(int stage, int arg) = 0;
+  // Device-pointer field copies (caller-owned device memory; synchronising).
+  virtual int exportFieldDevice(int field, float* dst, uint64_t n) {
+    return fail(DCG_E_UNSUPPORTED, "device field copies are not available for this grid");
+  }
+  virtual int importFieldDevice(int field, const float* src, uint64_t n) {
+    return fail(DCG_E_UNSUPPORTED, "device field copies are not available for this grid");
+  }
+  // Differentiable rollout (extension).
+  virtual int tapeForward(int steps, int checkpointEvery) {
+    return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
+  }
+  virtual int tapeBackward(const float* gden, const float* gvel, const float* gtem, float* oden, float* ovel,
+                           float* otem) {
+    return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
+  }
 
   int fail(int code, const std::string& m) {
     err = m;
@@ -78,6 +93,16 @@ struct SimBase {
     allocations.push_back(q);
     *p = (T*)q;
     return DCG_OK;
+  }
+  // Free one allocation made by alloc().
+  void release(void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < allocations.size(); i++)
+      if (allocations[i] == p) {
+        hipFree(p);
+        allocations.erase(allocations.begin() + i);
+        return;
+      }
   }
   // Fill with a kernel (never hipMemsetAsync, so the step stays graph-capturable).
   int fill32(void* p, uint32_t value, size_t count);

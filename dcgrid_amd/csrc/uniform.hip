@@ -295,6 +295,12 @@ __global__ void __launch_bounds__(256) k_u_seed(dcg_params P, UGrid g) {
   }
 }
 
+}  // namespace dcg
+
+#include "uniform_adjoint.hpp"
+
+namespace dcg {
+
 static unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
 struct UniformSim : SimBase {
@@ -306,6 +312,19 @@ struct UniformSim : SimBase {
   float* hstats = nullptr;
   bool customFluidity = false;
   uint64_t cellStepCount = 0;
+
+  // Differentiable rollout: checkpoints of the state (density, temperature, velocity) every
+  // tapeK steps plus the final state, a segment of tapeK + 1 recomputed states, and the
+  // adjoint fields.
+  int tapeN = 0, tapeK = 0, tapeCk = 0, tapeSeg = 0;
+  bool tapeValid = false;
+  float* ckBuf = nullptr;   // tapeCk states of 5N floats
+  float* segBuf = nullptr;  // tapeSeg states of 5N floats
+  int* winBuf = nullptr;    // window radius per segment state
+  int* adjErr = nullptr;
+  float *lq = nullptr, *lt = nullptr, *lq0 = nullptr, *lt0 = nullptr;
+  float3 *lu = nullptr, *lu0 = nullptr;
+  float *lp = nullptr, *ltp = nullptr, *ld = nullptr;
 
   int kind() const override { return DCG_KIND_UNIFORM; }
 
@@ -536,6 +555,188 @@ struct UniformSim : SimBase {
     return sync();
   }
   int exportTables(dcg_tables*) override { return fail(DCG_E_ARG, "uniform grid has no block tables"); }
+  int exportFieldDevice(int f, float* dst, uint64_t n) override {
+    uint64_t need;
+    DCG_TRY(fieldSize(f, &need));
+    if (n != need || !dst) return fail(DCG_E_ARG, "size mismatch");
+    DCG_HIP(hipMemcpyAsync(dst, fieldPtr(f), n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    return sync();
+  }
+  int importFieldDevice(int f, const float* src, uint64_t n) override {
+    uint64_t need;
+    DCG_TRY(fieldSize(f, &need));
+    if (n != need || !src) return fail(DCG_E_ARG, "size mismatch");
+    DCG_HIP(hipMemcpyAsync(fieldPtr(f), src, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    if (f == DCG_FIELD_FLUIDITY) customFluidity = true;
+    return sync();
+  }
+
+  // ---------------------------------------------------------------- differentiable rollout
+  float* stateDen(float* base, int k) { return base + (size_t)k * 5 * N; }
+  float* stateTem(float* base, int k) { return base + (size_t)k * 5 * N + N; }
+  float3* stateVel(float* base, int k) { return (float3*)(base + (size_t)k * 5 * N + 2 * N); }
+  int saveState(float* base, int k) {
+    DCG_HIP(hipMemcpyAsync(stateDen(base, k), g.den, N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(stateTem(base, k), g.tem, N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(stateVel(base, k), g.vel, N * 12, hipMemcpyDeviceToDevice, stream));
+    return DCG_OK;
+  }
+  int loadState(float* base, int k) {
+    DCG_HIP(hipMemcpyAsync(g.den, stateDen(base, k), N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(g.tem, stateTem(base, k), N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(g.vel, stateVel(base, k), N * 12, hipMemcpyDeviceToDevice, stream));
+    return DCG_OK;
+  }
+  // One step of the differentiable rollout: the dcg_step order without the cell-step count.
+  int plainStep() {
+    DCG_TRY(advectVelocity());
+    DCG_TRY(adaptTopology());
+    DCG_TRY(P.project_mode == DCG_PROJECT_LOCAL ? projectLocal() : project());
+    return advectDensity();
+  }
+  int tapeForward(int steps, int K) override {
+    if (steps < 1 || K < 1) return fail(DCG_E_ARG, "steps and checkpoint interval must be positive");
+    if (K > steps) K = steps;
+    const int ck = (steps + K - 1) / K + 1;  // states 0, K, 2K, ... below steps, and the final one
+    if (ck != tapeCk || K + 1 != tapeSeg) {
+      release(ckBuf);
+      release(segBuf);
+      release(winBuf);
+      ckBuf = segBuf = nullptr;
+      winBuf = nullptr;
+      tapeCk = tapeSeg = 0;
+      DCG_TRY(alloc(&ckBuf, (size_t)ck * 5 * N));
+      DCG_TRY(alloc(&segBuf, (size_t)(K + 1) * 5 * N));
+      DCG_TRY(alloc(&winBuf, (size_t)(K + 1)));
+      tapeCk = ck;
+      tapeSeg = K + 1;
+    }
+    if (!lq) {
+      DCG_TRY(alloc(&adjErr, 1));
+      DCG_TRY(alloc(&lq, N));
+      DCG_TRY(alloc(&lt, N));
+      DCG_TRY(alloc(&lq0, N));
+      DCG_TRY(alloc(&lt0, N));
+      DCG_TRY(alloc(&lu, N));
+      DCG_TRY(alloc(&lu0, N));
+      DCG_TRY(alloc(&lp, MC));
+      DCG_TRY(alloc(&ltp, MC));
+      DCG_TRY(alloc(&ld, MC));
+    }
+    tapeValid = false;
+    DCG_TRY(saveState(ckBuf, 0));
+    for (int k = 0; k < steps; k++) {
+      DCG_TRY(plainStep());
+      if ((k + 1) % K == 0 && k + 1 < steps) DCG_TRY(saveState(ckBuf, (k + 1) / K));
+    }
+    DCG_TRY(saveState(ckBuf, ck - 1));
+    tapeN = steps;
+    tapeK = K;
+    tapeValid = true;
+    return DCG_OK;
+  }
+  int window(int slot) {
+    DCG_TRY(fill32(winBuf + slot, 0, 1));
+    DCG_LAUNCH(DCG_K_OTHER, "k_ua_window", (k_ua_window<<<nblk(N), 256, 0, stream>>>(P, g, stateVel(segBuf, slot), winBuf + slot)));
+    return DCG_OK;
+  }
+  // Adjoint of one step between segment states s (before) and s + 1 (after).
+  int adjointStep(int s) {
+    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const bool fl = fluidityOn();
+    const float* den0 = stateDen(segBuf, s);
+    const float* tem0 = stateTem(segBuf, s);
+    const float3* vel0 = stateVel(segBuf, s);
+    const float3* vel1 = stateVel(segBuf, s + 1);
+    // advectDensity: lq -> lq0 (and lt -> lt0 with the source), foot terms into lu
+#define UA_ADV_D(F, S) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_advect_density", (k_ua_advect_density<F, S><<<nblk(N), 256, 0, stream>>>(P, g, den0, tem0, vel1, lq, lt, lq0, lt0, lu, winBuf + s + 1, adjErr)))
+    if (fl && src) UA_ADV_D(true, true);
+    else if (fl) UA_ADV_D(true, false);
+    else if (src) UA_ADV_D(false, true);
+    else UA_ADV_D(false, false);
+#undef UA_ADV_D
+    // projection: lu (after) -> lu (before)
+    if (fl)
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_ua_grad_T", (k_ua_grad_T<true><<<nblk(N), 256, 0, stream>>>(P, g, lu, lp)));
+    else
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_ua_grad_T", (k_ua_grad_T<false><<<nblk(N), 256, 0, stream>>>(P, g, lu, lp)));
+    DCG_TRY(fill32(ld, 0, MC));
+    auto pairsT = [&](int level, int pairs) -> int {
+      const UDims& D = dims[level];
+      const size_t n = (size_t)D.w * D.h * D.d;
+      const int scale = 1 << level;
+      const float alpha = P.dx * P.dx * (float)scale * (float)scale;
+      for (int i = 0; i < pairs; i++) {
+        DCG_LAUNCH(DCG_K_JACOBI, "k_ua_jacobi", (k_ua_jacobi<<<nblk(n), 256, 0, stream>>>(lp, ltp, ld, D, alpha)));
+        DCG_LAUNCH(DCG_K_JACOBI, "k_ua_jacobi", (k_ua_jacobi<<<nblk(n), 256, 0, stream>>>(ltp, lp, ld, D, alpha)));
+      }
+      return DCG_OK;
+    };
+    if (P.project_mode == DCG_PROJECT_LOCAL) {
+      DCG_TRY(pairsT(0, P.project_local_pairs));
+    } else {
+      for (int l = 0; l <= L - 2; l++) {
+        DCG_TRY(pairsT(l, P.project_level_pairs));
+        const UDims& C = dims[l + 1];
+        DCG_LAUNCH(DCG_K_PROLONGATE, "k_ua_prolongate_T", (k_ua_prolongate_T<<<nblk((size_t)C.w * C.h * C.d), 256, 0, stream>>>(lp, dims[l], C)));
+      }
+      DCG_TRY(pairsT(L - 1, P.project_top_pairs));
+      for (int l = L - 1; l >= 1; l--) {
+        const UDims& D = dims[l - 1];
+        DCG_LAUNCH(DCG_K_RESTRICT, "k_ua_restrict_T", (k_ua_restrict_T<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(ld, D, dims[l])));
+      }
+    }
+    if (fl)
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_ua_div_T", (k_ua_div_T<true><<<nblk(N), 256, 0, stream>>>(P, g, ld, lu)));
+    else
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_ua_div_T", (k_ua_div_T<false><<<nblk(N), 256, 0, stream>>>(P, g, ld, lu)));
+    // advectVelocity: lu -> lu0, buoyancy into lq0 / lt0
+    float* ltOut = src ? lt0 : lt;  // without the source temperature is not advected
+#define UA_ADV_V(F, B) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_ua_advect_velocity", (k_ua_advect_velocity<F, B><<<nblk(N), 256, 0, stream>>>(P, g, vel0, lu, lu0, lq0, ltOut, winBuf + s, adjErr)))
+    if (fl && src) UA_ADV_V(true, true);
+    else if (fl) UA_ADV_V(true, false);
+    else if (src) UA_ADV_V(false, true);
+    else UA_ADV_V(false, false);
+#undef UA_ADV_V
+    std::swap(lq, lq0);
+    std::swap(lu, lu0);
+    if (src) std::swap(lt, lt0);
+    return DCG_OK;
+  }
+  int tapeBackward(const float* gden, const float* gvel, const float* gtem, float* oden, float* ovel,
+                   float* otem) override {
+    if (!tapeValid) return fail(DCG_E_ARG, "no recorded rollout (dcg_tape_forward)");
+    DCG_HIP(hipMemcpyAsync(lq, gden, N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(lu, gvel, N * 12, hipMemcpyDeviceToDevice, stream));
+    if (gtem)
+      DCG_HIP(hipMemcpyAsync(lt, gtem, N * 4, hipMemcpyDeviceToDevice, stream));
+    else
+      DCG_TRY(fill32(lt, 0, N));
+    DCG_TRY(fill32(adjErr, 0, 1));
+    const int segs = (tapeN + tapeK - 1) / tapeK;
+    for (int sg = segs - 1; sg >= 0; sg--) {
+      const int k0 = sg * tapeK, k1 = std::min(tapeN, k0 + tapeK);
+      DCG_TRY(loadState(ckBuf, sg));
+      DCG_TRY(saveState(segBuf, 0));
+      DCG_TRY(window(0));
+      for (int k = k0; k < k1; k++) {
+        DCG_TRY(plainStep());
+        DCG_TRY(saveState(segBuf, k - k0 + 1));
+        DCG_TRY(window(k - k0 + 1));
+      }
+      for (int k = k1 - 1; k >= k0; k--) DCG_TRY(adjointStep(k - k0));
+    }
+    DCG_HIP(hipMemcpyAsync(oden, lq, N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_HIP(hipMemcpyAsync(ovel, lu, N * 12, hipMemcpyDeviceToDevice, stream));
+    if (otem) DCG_HIP(hipMemcpyAsync(otem, lt, N * 4, hipMemcpyDeviceToDevice, stream));
+    DCG_TRY(loadState(ckBuf, tapeCk - 1));  // the live state is the rollout's final state again
+    int herr = 0;
+    DCG_HIP(hipMemcpyAsync(&herr, adjErr, 4, hipMemcpyDeviceToHost, stream));
+    DCG_TRY(sync());
+    if (herr) return fail(DCG_E_UNSUPPORTED, "a foot travels more than 4 cells in one step; the adjoint gather supports at most 4");
+    return DCG_OK;
+  }
+
   int runStage(int stage, int arg) override {
     const bool perLevel = stage == DCG_STAGE_JACOBI || stage == DCG_STAGE_JACOBI_INV || stage == DCG_STAGE_PROLONGATE;
     if (perLevel && (arg < 0 || arg >= L || (stage == DCG_STAGE_PROLONGATE && arg >= L - 1)))

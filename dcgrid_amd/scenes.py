@@ -134,3 +134,27 @@ def config(i, scale=1):
         full = sum((gx // (8 << l)) * (gy // (8 << l)) * (gz // (8 << l)) for l in range(4))
         return Workload("dcgrid1024x256x1024_bw8_l4", "dcgrid", (gx, gy, gz), p, 300, full)
     raise ValueError(i)
+
+
+def control_inputs(n, seed=SEED_BASE + 3):
+    """Seeded inputs of the differentiable density-control workload on an n^3 uniform grid
+    (x-fastest cell order): initial velocity N(0, 0.1^2) per component, initial density a
+    centred Gaussian blob (sigma 16 n/128 cells, peak 1), and the target density: 4 isotropic
+    Gaussian blobs (sigma 6 n/128 cells, peak 1) at seeded centres. Returns float32
+    (density0[N], velocity0[N,3], target[N])."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = n ** 3
+    vel = rng.normal(0.0, 0.1, size=(N, 3)).astype(np.float32)
+    c = np.arange(n) + 0.5
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+
+    def blob(cx, cy, cz, sigma):
+        return np.exp(-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (2.0 * sigma ** 2))
+
+    den = blob(n / 2, n / 2, n / 2, 16.0 * n / 128).reshape(-1).astype(np.float32)
+    centres = rng.uniform(0.25 * n, 0.75 * n, size=(4, 3))
+    tgt = np.zeros((n, n, n))
+    for cx, cy, cz in centres:
+        tgt = np.maximum(tgt, blob(cx, cy, cz, 6.0 * n / 128))
+    return den, vel, tgt.reshape(-1).astype(np.float32)

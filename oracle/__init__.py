@@ -32,6 +32,12 @@ def library():
         lib.orc_get_threads.restype = C.c_int32
         lib.orc_export_field_f64.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64]
         lib.orc_export_field_f64.restype = C.c_int
+        lib.orc_import_field_f64.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_uint64]
+        lib.orc_import_field_f64.restype = C.c_int
+        lib.orc_tape_forward.argtypes = [C.c_void_p, C.c_int32]
+        lib.orc_tape_forward.restype = C.c_int
+        lib.orc_tape_backward.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 6
+        lib.orc_tape_backward.restype = C.c_int
         lib.orc_lookup.argtypes = [C.c_void_p] + [C.c_int32] * 4
         lib.orc_lookup.restype = C.c_int64
         lib.orc_plan.argtypes = [C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Params), C.POINTER(Info)]
@@ -96,3 +102,37 @@ class OracleSim(Handle):
         if f in (FIELD_VELOCITY, FIELD_VORTICITY):
             out = out.reshape(-1, 3)
         return out
+
+    def import_field_f64(self, field, values):
+        import numpy as np
+        from dcgrid_amd._abi import FIELDS
+        f = FIELDS.get(field, field)
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        st = _lib.orc_import_field_f64(self._h, f, a.ctypes.data_as(C.POINTER(C.c_double)), a.size)
+        self._check(st, "import_field_f64")
+
+    def tape_forward(self, steps):
+        """Record `steps` steps from the current state (differentiable rollout, uniform grid)."""
+        self._check(_lib.orc_tape_forward(self._h, int(steps)), "tape_forward")
+
+    def tape_backward(self, g_density, g_velocity, g_temperature=None):
+        """Gradient of <g, final state> with respect to the recorded initial state:
+        returns (d_density, d_velocity[N,3], d_temperature or None)."""
+        import numpy as np
+        P = C.POINTER(C.c_double)
+        gd = np.ascontiguousarray(g_density, dtype=np.float64).reshape(-1)
+        gv = np.ascontiguousarray(g_velocity, dtype=np.float64).reshape(-1)
+        n = gd.size
+        od = np.empty(n)
+        ov = np.empty(3 * n)
+        if g_temperature is not None:
+            gt = np.ascontiguousarray(g_temperature, dtype=np.float64).reshape(-1)
+            ot = np.empty(n)
+            gtp, otp = gt.ctypes.data_as(P), ot.ctypes.data_as(P)
+        else:
+            ot = None
+            gtp = otp = None
+        st = _lib.orc_tape_backward(self._h, gd.ctypes.data_as(P), gv.ctypes.data_as(P), gtp,
+                                    od.ctypes.data_as(P), ov.ctypes.data_as(P), otp)
+        self._check(st, "tape_backward")
+        return od, ov.reshape(-1, 3), ot

@@ -43,7 +43,15 @@ int orc_field_size(orc_sim* s, int32_t field, uint64_t* nfloats);
 int orc_export_field(orc_sim* s, int32_t field, float* dst, uint64_t nfloats);
 int orc_export_field_f64(orc_sim* s, int32_t field, double* dst, uint64_t nfloats);
 int orc_import_field(orc_sim* s, int32_t field, const float* src, uint64_t nfloats);
+int orc_import_field_f64(orc_sim* s, int32_t field, const double* src, uint64_t nfloats);
 int orc_export_tables(orc_sim* s, dcg_tables* t);
+/* Differentiable rollout on the uniform grid (extension): record n steps from the current
+ * state, then map the adjoint of the final state (density, velocity xyz-interleaved,
+ * temperature; gtem/otem may be NULL) to the gradient with respect to the recorded initial
+ * state. FP64 in the FP64 instance: the reference-logic gradient. */
+int orc_tape_forward(orc_sim* s, int32_t n);
+int orc_tape_backward(orc_sim* s, const double* gden, const double* gvel, const double* gtem, double* oden,
+                      double* ovel, double* otem);
 /* One stage of the step (DCG_STAGE_* of dcgrid_amd.h). */
 int orc_run_stage(orc_sim* s, int32_t stage, int32_t arg);
 /* getBlockIndexDeep (dcgrid_utils.cuh:201-232) on the oracle's hash tables; -1 if none. */

@@ -132,7 +132,23 @@ int orc_export_field(orc_sim* s, int32_t field, float* dst, uint64_t n) {
 int orc_import_field(orc_sim* s, int32_t field, const float* src, uint64_t n) {
   IMPL(s);
   if (!src) return DCG_E_ARG;
+  std::vector<double> tmp(src, src + n);
+  return I->importField(field, tmp.data(), n);
+}
+int orc_import_field_f64(orc_sim* s, int32_t field, const double* src, uint64_t n) {
+  IMPL(s);
+  if (!src) return DCG_E_ARG;
   return I->importField(field, src, n);
+}
+int orc_tape_forward(orc_sim* s, int32_t n) {
+  IMPL(s);
+  return I->tapeForward(n);
+}
+int orc_tape_backward(orc_sim* s, const double* gden, const double* gvel, const double* gtem, double* oden,
+                      double* ovel, double* otem) {
+  IMPL(s);
+  if (!gden || !gvel || !oden || !ovel) return DCG_E_ARG;
+  return I->tapeBackward(gden, gvel, gtem, oden, ovel, otem);
 }
 int orc_export_tables(orc_sim* s, dcg_tables* t) {
   IMPL(s);

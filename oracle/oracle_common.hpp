@@ -222,12 +222,20 @@ struct Sim {
   virtual int info(dcg_info* out) = 0;
   virtual int fieldSize(int field, uint64_t* n) = 0;
   virtual int exportField(int field, double* dst, uint64_t n) = 0;
-  virtual int importField(int field, const float* src, uint64_t n) = 0;
+  virtual int importField(int field, const double* src, uint64_t n) = 0;
   virtual int exportTables(dcg_tables* t) = 0;
   virtual int runStage(int stage, int arg) = 0;
   // getBlockIndexDeep (DCGrid only): finest allocated block at level >= `level` containing the
   // level-local cell (x,y,z); -1 when not applicable.
   virtual int64_t lookup(int x, int y, int z, int level) { return -1; }
+  // Differentiable rollout (extension; uniform grid only): record n steps, then pull the
+  // adjoint of the final state back to the recorded initial state. Null temperature pointers
+  // mean a zero seed / no output.
+  virtual int tapeForward(int n) { return fail(DCG_E_UNSUPPORTED, "adjoint: uniform grid only"); }
+  virtual int tapeBackward(const double* gden, const double* gvel, const double* gtem, double* oden,
+                           double* ovel, double* otem) {
+    return fail(DCG_E_UNSUPPORTED, "adjoint: uniform grid only");
+  }
   int fail(int code, const std::string& m) {
     err = m;
     return code;

@@ -1098,7 +1098,7 @@ struct DCGrid : Sim {
     for (size_t i = 0; i < n; i++) dst[i] = (double)(*v)[i];
     return DCG_OK;
   }
-  int importField(int f, const float* src, uint64_t n) override {
+  int importField(int f, const double* src, uint64_t n) override {
     uint64_t need;
     if (int e = fieldSize(f, &need)) return e;
     if (n != need) return fail(DCG_E_ARG, "size mismatch");

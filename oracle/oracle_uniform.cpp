@@ -340,6 +340,314 @@ struct Uniform : Sim {
     applyPressure();
     return DCG_OK;
   }
+  // ------------------------------------------------------------------ adjoint (extension)
+  // Reverse mode of the step above on a fixed fluidity field. Each forward loop `out = f(in)`
+  // is mirrored, in reverse order, by `lam_in += (df/din)^T lam_out` followed by
+  // `lam_out = 0` (the output overwrote its previous value). Written as plain scatters.
+  struct Snap {
+    std::vector<R> den, tem;
+    std::vector<V3<R>> vel;
+  };
+  std::vector<Snap> tape;
+
+  int tapeForward(int n) override {
+    if (n < 0) return fail(DCG_E_ARG, "negative step count");
+    tape.clear();
+    tape.push_back(Snap{den, tem, vel});
+    for (int k = 0; k < n; k++) {
+      advectVelocity();
+      adaptTopology();
+      if (P.project_mode == DCG_PROJECT_LOCAL)
+        projectLocal();
+      else
+        project();
+      advectDensity();
+      tape.push_back(Snap{den, tem, vel});
+    }
+    return DCG_OK;
+  }
+
+  bool inDomain(int x, int y, int z) const { return x >= 0 && y >= 0 && z >= 0 && x < gx && y < gy && z < gz; }
+
+  // Sample of the foot of cell (x,y,z) under velocity v, with the unnormalised corner weights
+  // f_c t_c and d(t_c)/d(d_a) for the derivative of the normalised interpolant.
+  struct FootD {
+    Sample s;
+    R f[8], t[8], dtd[8][3], W;
+  };
+  void footD(int x, int y, int z, const V3<R>& v, FootD& F) const {
+    const R dt = (R)P.dt, rdx = (R)P.rdx;
+    const R px = ((R)x + (R).5) - v.x * dt * rdx;
+    const R py = ((R)y + (R).5) - v.y * dt * rdx;
+    const R pz = ((R)z + (R).5) - v.z * dt * rdx;
+    sample(px, py, pz, F.s);
+    const R xs = px - (R).5, ys = py - (R).5, zs = pz - (R).5;
+    const R d[3] = {xs - std::floor(xs), ys - std::floor(ys), zs - std::floor(zs)};
+    F.W = 0;
+    for (int c = 0; c < 8; c++) {
+      const int bit[3] = {(c >> 2) & 1, (c >> 1) & 1, c & 1};
+      R tau[3];
+      for (int a = 0; a < 3; a++) tau[a] = bit[a] ? d[a] : (R)1 - d[a];
+      F.t[c] = tau[0] * tau[1] * tau[2];
+      for (int a = 0; a < 3; a++) {
+        R g = bit[a] ? (R)1 : (R)-1;
+        for (int b = 0; b < 3; b++)
+          if (b != a) g *= tau[b];
+        F.dtd[c][a] = g;
+      }
+      F.f[c] = flu[F.s.i[c]];
+      F.W += F.f[c] * F.t[c];
+    }
+  }
+  // d(value)/d(u_a) of the sample with BC-applied corner values vc[8] (value = sum f t vc / W).
+  void footGrad(const FootD& F, const R vc[8], R out[3]) const {
+    const R k = -(R)P.dt * (R)P.rdx;
+    R val = 0;
+    for (int c = 0; c < 8; c++) val += F.f[c] * F.t[c] * vc[c];
+    val /= F.W;
+    for (int a = 0; a < 3; a++) {
+      R num = 0, den_ = 0;
+      for (int c = 0; c < 8; c++) {
+        num += F.f[c] * vc[c] * F.dtd[c][a];
+        den_ += F.f[c] * F.dtd[c][a];
+      }
+      out[a] = k * (num - val * den_) / F.W;
+    }
+  }
+  int corner(const Sample& s, int c, int& x, int& y, int& z) const {
+    x = s.x0 + ((c >> 2) & 1);
+    y = s.y0 + ((c >> 1) & 1);
+    z = s.z0 + (c & 1);
+    return inDomain(x, y, z);
+  }
+
+  // Adjoint of project() / projectLocal(): lu holds lambda of the velocity after the
+  // projection on entry and lambda of the velocity before it on exit.
+  void projectAdjoint(std::vector<V3<R>>& lu) {
+    std::vector<R> lp(MC, 0), ltp(MC, 0), ld(MC, 0);
+    const R a = (R).5f * (R)P.rdx;
+    // applyPressure: v -= a (f_r (p_r - p) + f_l (p - p_l)) per axis; v itself passes through
+    for (int z = 0; z < gz; z++)
+      for (int y = 0; y < gy; y++)
+        for (int x = 0; x < gx; x++) {
+          const size_t i = idx(x, y, z);
+          const size_t il = x > 0 ? i - 1 : i, ir = x < gx - 1 ? i + 1 : i;
+          const size_t id = y > 0 ? i - gx : i, iu = y < gy - 1 ? i + gx : i;
+          const size_t ib = z > 0 ? i - (size_t)gx * gy : i, iF = z < gz - 1 ? i + (size_t)gx * gy : i;
+          const size_t nb[3][2] = {{il, ir}, {id, iu}, {ib, iF}};
+          const R lam[3] = {lu[i].x, lu[i].y, lu[i].z};
+          for (int ax = 0; ax < 3; ax++) {
+            const size_t l = nb[ax][0], r = nb[ax][1];
+            const R fl = flu[l], fr = flu[r];
+            lp[r] += -a * fr * lam[ax];
+            lp[i] += a * fr * lam[ax];
+            lp[i] += -a * fl * lam[ax];
+            lp[l] += a * fl * lam[ax];
+          }
+        }
+    // Jacobi half-sweep in -> out on `level`, reversed
+    auto jacobiAdj = [&](int level, std::vector<R>& lin, std::vector<R>& lout) {
+      const int scale = 1 << level;
+      const R alpha = (R)P.dx * (R)P.dx * (R)scale * (R)scale;
+      const int w = gx / scale, h = gy / scale, d = gz / scale;
+      for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+          for (int x = 0; x < w; x++) {
+            const size_t i = mip(x, y, z, scale);
+            const size_t il = x > 0 ? i - 1 : i, ir = x < w - 1 ? i + 1 : i;
+            const size_t id = y > 0 ? i - w : i, iu = y < h - 1 ? i + w : i;
+            const size_t ib = z > 0 ? i - (size_t)w * h : i, iF = z < d - 1 ? i + (size_t)w * h : i;
+            const R g = lout[i] / (R)6;
+            lin[il] += g;
+            lin[ir] += g;
+            lin[id] += g;
+            lin[iu] += g;
+            lin[ib] += g;
+            lin[iF] += g;
+            ld[i] += -alpha * g;
+          }
+      for (int z = 0; z < d; z++)
+        for (int y = 0; y < h; y++)
+          for (int x = 0; x < w; x++) lout[mip(x, y, z, scale)] = 0;
+    };
+    auto pairsAdj = [&](int level, int pairs) {
+      for (int i = 0; i < pairs; i++) {
+        jacobiAdj(level, ltp, lp);  // t_pre -> pre
+        jacobiAdj(level, lp, ltp);  // pre -> t_pre
+      }
+    };
+    auto zeroLevel = [&](std::vector<R>& v, int level) {
+      const int scale = 1 << level;
+      for (int z = 0; z < gz / scale; z++)
+        for (int y = 0; y < gy / scale; y++)
+          for (int x = 0; x < gx / scale; x++) v[mip(x, y, z, scale)] = 0;
+    };
+    if (P.project_mode == DCG_PROJECT_LOCAL) {
+      pairsAdj(0, P.project_local_pairs);
+    } else {
+      for (int level = 0; level <= L - 2; level++) {
+        pairsAdj(level, P.project_level_pairs);
+        // prolongate(level): pre[level] = sum of 27/9/3/1 weights of pre[level + 1]
+        const int scale = 1 << level;
+        const int w = gx / scale, h = gy / scale, d = gz / scale;
+        for (int z = 0; z < d; z++)
+          for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+              const size_t i = mip(x, y, z, scale);
+              const size_t i000 = mip(x / 2, y / 2, z / 2, 2 * scale);
+              const long sx = (x == 0 || x == w - 1) ? 0 : 2 * (x % 2) - 1;
+              const long sy = (y == 0 || y == h - 1) ? 0 : 2 * (y % 2) - 1;
+              const long sz = (z == 0 || z == d - 1) ? 0 : 2 * (z % 2) - 1;
+              const size_t i001 = i000 + sx, i010 = i000 + sy * w / 2, i011 = i010 + sx;
+              const size_t i100 = i000 + sz * (w / 2) * (h / 2), i101 = i100 + sx, i110 = i100 + sy * (w / 2);
+              const size_t i111 = i110 + sx;
+              const R g = lp[i] / (R)64;
+              lp[i000] += (R)27 * g;
+              lp[i001] += (R)9 * g;
+              lp[i010] += (R)9 * g;
+              lp[i100] += (R)9 * g;
+              lp[i011] += (R)3 * g;
+              lp[i101] += (R)3 * g;
+              lp[i110] += (R)3 * g;
+              lp[i111] += g;
+              lp[i] = 0;
+            }
+      }
+      pairsAdj(L - 1, P.project_top_pairs);
+      // restrictLevel(level) for level = L-1 .. 1: zeroes pre/t_pre, div = mean of children
+      for (int level = L - 1; level >= 1; level--) {
+        zeroLevel(lp, level);
+        zeroLevel(ltp, level);
+        const int scale = 1 << level, cs = scale / 2;
+        for (int z = 0; z < gz / scale; z++)
+          for (int y = 0; y < gy / scale; y++)
+            for (int x = 0; x < gx / scale; x++) {
+              const size_t i = mip(x, y, z, scale);
+              const R g = (R).125f * ld[i];
+              for (int c = 0; c < 8; c++)
+                ld[mip(2 * x + ((c >> 2) & 1), 2 * y + ((c >> 1) & 1), 2 * z + (c & 1), cs)] += g;
+              ld[i] = 0;
+            }
+      }
+    }
+    // calcDivergence: div = a (w_r v_r.x - w_l v_l.x + ...) with BC'd neighbour velocities
+    for (int z = 0; z < gz; z++)
+      for (int y = 0; y < gy; y++)
+        for (int x = 0; x < gx; x++) {
+          const size_t i = idx(x, y, z);
+          const R g = a * ld[i];
+          if (x + 1 < gx) lu[i + 1].x += flu[i + 1] * g;
+          if (x - 1 >= 0) lu[i - 1].x += -flu[i - 1] * g;
+          if (y + 1 < gy) lu[i + gx].y += flu[i + gx] * g;
+          if (y - 1 >= 0) lu[i - gx].y += -flu[i - gx] * g;
+          const size_t pl = (size_t)gx * gy;
+          if (z + 1 < gz) lu[i + pl].z += flu[i + pl] * g;
+          if (z - 1 >= 0) lu[i - pl].z += -flu[i - pl] * g;
+        }
+  }
+
+  int tapeBackward(const double* gden, const double* gvel, const double* gtem, double* oden, double* ovel,
+                   double* otem) override {
+    if (tape.empty()) return fail(DCG_E_ARG, "no recorded rollout");
+    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const R dt = (R)P.dt;
+    std::vector<R> lq(N), lt(N, 0);
+    std::vector<V3<R>> lu(N);
+    for (size_t i = 0; i < N; i++) {
+      lq[i] = (R)gden[i];
+      lu[i] = V3<R>{(R)gvel[3 * i], (R)gvel[3 * i + 1], (R)gvel[3 * i + 2]};
+      if (gtem) lt[i] = (R)gtem[i];
+    }
+    const int n = (int)tape.size() - 1;
+    for (int k = n - 1; k >= 0; k--) {
+      const Snap& S0 = tape[k];
+      const Snap& S1 = tape[k + 1];
+      // advectDensity with u_{k+1}: den_{k+1} = A(den_k), tem_{k+1} = A(tem_k) (with sources)
+      std::vector<R> lq0(N, 0), lt0(N, 0);
+      if (!src) lt0 = lt;  // temperature is only advected together with the source
+      for (int z = 0; z < gz; z++)
+        for (int y = 0; y < gy; y++)
+          for (int x = 0; x < gx; x++) {
+            const size_t i = idx(x, y, z);
+            if (src && inSource(P, (float)x + .5f, (float)y + .5f, (float)z + .5f)) continue;
+            FootD F;
+            footD(x, y, z, S1.vel[i], F);
+            if (F.s.zero) continue;
+            R qc[8], tc[8];
+            for (int c = 0; c < 8; c++) {
+              int cx, cy, cz;
+              corner(F.s, c, cx, cy, cz);
+              qc[c] = densityBC(P, S0.den[F.s.i[c]], cx, cy, cz, 1);
+              tc[c] = temperatureBC(P, S0.tem[F.s.i[c]], cx, cy, cz, 1);
+              if (corner(F.s, c, cx, cy, cz)) {
+                lq0[F.s.i[c]] += F.s.w[c] * lq[i];
+                if (src) lt0[F.s.i[c]] += F.s.w[c] * lt[i];
+              }
+            }
+            R g[3];
+            footGrad(F, qc, g);
+            lu[i].x += g[0] * lq[i];
+            lu[i].y += g[1] * lq[i];
+            lu[i].z += g[2] * lq[i];
+            if (src) {
+              footGrad(F, tc, g);
+              lu[i].x += g[0] * lt[i];
+              lu[i].y += g[1] * lt[i];
+              lu[i].z += g[2] * lt[i];
+            }
+          }
+      // project
+      projectAdjoint(lu);
+      // advectVelocity with u_k: u' = A(u_k; u_k) (+ buoyancy from den_k, tem_k)
+      std::vector<V3<R>> lu0(N, V3<R>{0, 0, 0});
+      for (int z = 0; z < gz; z++)
+        for (int y = 0; y < gy; y++)
+          for (int x = 0; x < gx; x++) {
+            const size_t i = idx(x, y, z);
+            if (src) {
+              lq0[i] += -dt * (R)P.buoy_alpha * lu[i].y;
+              lt0[i] += dt * (R)P.buoy_beta * lu[i].y;
+            }
+            FootD F;
+            footD(x, y, z, S0.vel[i], F);
+            if (F.s.zero) continue;
+            R vc[3][8];
+            for (int c = 0; c < 8; c++) {
+              int cx, cy, cz;
+              const bool in = corner(F.s, c, cx, cy, cz);
+              const V3<R> v = velocityBC(P, S0.vel[F.s.i[c]], cx, cy, cz, 1);
+              vc[0][c] = v.x;
+              vc[1][c] = v.y;
+              vc[2][c] = v.z;
+              if (in) {
+                lu0[F.s.i[c]].x += F.s.w[c] * lu[i].x;
+                lu0[F.s.i[c]].y += F.s.w[c] * lu[i].y;
+                lu0[F.s.i[c]].z += F.s.w[c] * lu[i].z;
+              }
+            }
+            const R lam[3] = {lu[i].x, lu[i].y, lu[i].z};
+            for (int comp = 0; comp < 3; comp++) {
+              R g[3];
+              footGrad(F, vc[comp], g);
+              lu0[i].x += g[0] * lam[comp];
+              lu0[i].y += g[1] * lam[comp];
+              lu0[i].z += g[2] * lam[comp];
+            }
+          }
+      lq.swap(lq0);
+      lt.swap(lt0);
+      lu.swap(lu0);
+    }
+    for (size_t i = 0; i < N; i++) {
+      oden[i] = (double)lq[i];
+      ovel[3 * i] = (double)lu[i].x;
+      ovel[3 * i + 1] = (double)lu[i].y;
+      ovel[3 * i + 2] = (double)lu[i].z;
+      if (otem) otem[i] = (double)lt[i];
+    }
+    return DCG_OK;
+  }
+
   // debugStats(): k_uniform_debug_stats + host sum (uniform.cu:160-175; structure.cu:33-42)
   int debugStats(double* out) override {
     const size_t numBins = N / 256;
@@ -424,7 +732,7 @@ struct Uniform : Sim {
     for (size_t i = 0; i < n; i++) dst[i] = (double)(*v)[i];
     return DCG_OK;
   }
-  int importField(int f, const float* src, uint64_t n) override {
+  int importField(int f, const double* src, uint64_t n) override {
     uint64_t need;
     if (int e = fieldSize(f, &need)) return e;
     if (n != need) return fail(DCG_E_ARG, "size mismatch");
