@@ -196,6 +196,51 @@ def adjoint_bench(w, args):
     }
 
 
+def solve_bench(w, args):
+    """Projection solve of the adaptive workload: after `warmup` reference steps, the same
+    velocity field is projected with every level solved to a 1e-8 relative residual by FP64
+    iterative refinement around FP32 multigrid (mixed), and by the all-FP32 and all-FP64
+    variants; wall time of each projection (median of `steps` repetitions from the same
+    state), cycles and reached residual."""
+    import dcgrid_amd
+    from dcgrid_amd._abi import SOLVE_FP32, SOLVE_FP64, SOLVE_MIXED
+    p = w.params.copy()
+    p.solve_tol = 1e-8
+    p.solve_max_cycles = 60
+    sim = make_sim(dcgrid_amd.Simulation, w)
+    sim.set_params(p)
+    log(f"[bench] solve: {args.warmup} reference steps")
+    sim.step(args.warmup)
+    vel = sim.export_field("velocity")
+    inf = sim.info()
+    out = {"metric": "projection solve time to 1e-8 relative residual", "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "data": "synthetic (seeded plume after warmup steps)",
+           "config": {"workload": w.name, "domain": list(w.size), "block_loads": inf.levels_list("block_loads"),
+                      "active_cells": int(inf.active_cells)}}
+    reps = max(1, min(args.steps, 5))
+    for name, prec in (("mixed", SOLVE_MIXED), ("fp32", SOLVE_FP32), ("fp64", SOLVE_FP64)):
+        q = p.copy()
+        q.solve_precision = prec
+        sim.set_params(q)
+        times = []
+        for _ in range(reps):
+            sim.import_field("velocity", vel)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            sim.project_solve()
+            sim.synchronize()
+            times.append(1e3 * (time.perf_counter() - t0))
+        rep = sim.solve_report()
+        times.sort()
+        out[name] = {"ms": round(times[len(times) // 2], 3), "cycles": rep["cycles"],
+                     "rel_residual": [float(f"{r:.3e}") for r in rep["rel_residual"]],
+                     "converged": rep["converged"]}
+        log(f"[bench] {name}: {out[name]}")
+    out["value"] = out["mixed"]["ms"]
+    sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +251,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--solve", action="store_true",
+                    help="time the projection solve to 1e-8 (mixed, FP32, FP64) instead of the step")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device-to-device copy first (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -219,6 +266,9 @@ def main():
     w = scenes.config(args.config)
     if args.config == 3:
         print(json.dumps(adjoint_bench(w, args)), flush=True)
+        return
+    if args.solve:
+        print(json.dumps(solve_bench(w, args)), flush=True)
         return
 
     if args.copy_probe:

@@ -46,7 +46,8 @@ FIELDS = {
     "vorticity": FIELD_VORTICITY,
 }
 
-PROJECT_MULTILEVEL, PROJECT_LOCAL = 0, 1
+PROJECT_MULTILEVEL, PROJECT_LOCAL, PROJECT_SOLVE = 0, 1, 2
+SOLVE_MIXED, SOLVE_FP32, SOLVE_FP64 = 0, 1, 2
 SCORE_GEOMETRIC, SCORE_FEATURES = 0, 1
 SOURCE_NONE, SOURCE_SPHERE_BUOYANT = 0, 1
 
@@ -98,7 +99,11 @@ class Params(C.Structure):
         ("init_velocity_noise", C.c_float),
         ("noise_period", C.c_float),
         ("seed", C.c_uint32),
-        ("reserved", C.c_int32 * 8),
+        ("solve_precision", C.c_int32),
+        ("solve_tol", C.c_float),
+        ("solve_max_cycles", C.c_int32),
+        ("solve_smooth", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def copy(self):
@@ -141,6 +146,23 @@ class Info(C.Structure):
         return [int(v) for v in getattr(self, name)[: self.levels]]
 
 
+class SolveInfo(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32),
+        ("cycles", C.c_int32 * DCG_MAX_LEVELS),
+        ("rel_residual", C.c_double * DCG_MAX_LEVELS),
+        ("initial_residual", C.c_double * DCG_MAX_LEVELS),
+        ("singular", C.c_int32 * DCG_MAX_LEVELS),
+        ("converged", C.c_int32),
+    ]
+
+    def as_dict(self):
+        n = self.levels
+        return {"cycles": list(self.cycles[:n]), "rel_residual": list(self.rel_residual[:n]),
+                "initial_residual": list(self.initial_residual[:n]), "singular": list(self.singular[:n]),
+                "converged": bool(self.converged)}
+
+
 class Tables(C.Structure):
     _fields_ = [
         ("block_positions", C.POINTER(C.c_int32)),
@@ -180,6 +202,8 @@ def bind(lib, prefix, extra_create_arg=False):
         "advect_velocity": ([sim], C.c_int),
         "project": ([sim], C.c_int),
         "project_local": ([sim], C.c_int),
+        "project_solve": ([sim], C.c_int),
+        "solve_report": ([sim, P(SolveInfo)], C.c_int),
         "advect_density": ([sim], C.c_int),
         "step": ([sim, C.c_int32], C.c_int),
         "debug_stats": ([sim, P(C.c_double)], C.c_int),
@@ -271,6 +295,14 @@ class Handle:
 
     def project_local(self):
         self._check(self._fns["project_local"](self._h), "project_local")
+
+    def project_solve(self):
+        self._check(self._fns["project_solve"](self._h), "project_solve")
+
+    def solve_report(self):
+        out = SolveInfo()
+        self._check(self._fns["solve_report"](self._h, C.byref(out)), "solve_report")
+        return out.as_dict()
 
     def advect_density(self):
         self._check(self._fns["advect_density"](self._h), "advect_density")

@@ -134,6 +134,10 @@ void dcg_default_params(dcg_params* p) {
   p->source_mode = DCG_SOURCE_NONE;
   p->noise_period = 32.f;
   p->seed = 20261202u;
+  p->solve_precision = DCG_SOLVE_MIXED;
+  p->solve_tol = 1e-8f;
+  p->solve_max_cycles = 50;
+  p->solve_smooth = 2;
 }
 
 int dcg_create_uniform(const int32_t size[3], const dcg_params* p, dcg_sim** out) {
@@ -202,6 +206,15 @@ int dcg_project_local(dcg_sim* s) {
   IMPL(s);
   return I->projectLocal();
 }
+int dcg_project_solve(dcg_sim* s) {
+  IMPL(s);
+  return I->projectSolve();
+}
+int dcg_solve_report(dcg_sim* s, dcg_solve_info* out) {
+  IMPL(s);
+  if (!out) return DCG_E_ARG;
+  return I->solveReport(out);
+}
 int dcg_advect_density(dcg_sim* s) {
   IMPL(s);
   return I->advectDensity();
@@ -215,7 +228,7 @@ int dcg_step(dcg_sim* s, int32_t n) {
     if ((e = I->advectVelocity())) return e;
     if ((e = I->adaptTopology())) return e;
     if ((e = I->countStep())) return e;
-    if ((e = (I->P.project_mode == DCG_PROJECT_LOCAL ? I->projectLocal() : I->project()))) return e;
+    if ((e = I->projectByMode())) return e;
     if ((e = I->advectDensity())) return e;
   }
   return DCG_OK;

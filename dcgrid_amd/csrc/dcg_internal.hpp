@@ -62,6 +62,16 @@ struct SimBase {
   virtual int countStep() = 0;
   virtual int cellSteps(uint64_t* out) = 0;
   virtual int runStage(int stage, int arg) = 0;
+  // Projection with the level systems solved to a tolerance (extension).
+  virtual int projectSolve() { return fail(DCG_E_UNSUPPORTED, "the projection solve is implemented for the DCGrid"); }
+  virtual int solveReport(dcg_solve_info* out) {
+    return fail(DCG_E_UNSUPPORTED, "the projection solve is implemented for the DCGrid");
+  }
+  int projectByMode() {
+    if (P.project_mode == DCG_PROJECT_LOCAL) return projectLocal();
+    if (P.project_mode == DCG_PROJECT_SOLVE) return projectSolve();
+    return project();
+  }
   // Device-pointer field copies (caller-owned device memory; synchronising).
   virtual int exportFieldDevice(int field, float* dst, uint64_t n) {
     return fail(DCG_E_UNSUPPORTED, "device field copies are not available for this grid");

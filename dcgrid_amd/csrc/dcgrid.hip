@@ -8,12 +8,21 @@
 
 #include "dcg_internal.hpp"
 #include "dcgrid_fluid.hpp"
+#include "dcgrid_solve.hpp"
 #include "dcgrid_topo.hpp"
 
 namespace dcg {
 
 static inline int iDivUpH(int a, int b) { return (a % b != 0) ? (a / b + 1) : (a / b); }
 static inline unsigned blocksFor(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+// Workgroups for a grid-stride walk over at most `blocks` allocated blocks: enough to fill the
+// chip (about four resident workgroups per CU), never more than there are block slots.
+template <int BW>
+static inline unsigned slotGrid(size_t blocks) {
+  const unsigned cap = Geo<BW>::BPW == 1 ? 1024u : 2048u;
+  const unsigned n = blocksFor(blocks, Geo<BW>::BPW);
+  return n == 0 ? 1u : (n < cap ? n : cap);
+}
 
 template <int BW>
 struct DCGridSim : SimBase {
@@ -193,7 +202,7 @@ struct DCGridSim : SimBase {
 
   // adaptTopology(), fluid_simulation_dcgrid.cu:320-346
   int adaptTopology() override {
-    const unsigned nWG = blocksFor(B, G::BPW);
+    const unsigned nWG = slotGrid<BW>(B);
     DCG_TRY(fill32(g.flg, 0, B));
     if (P.score_mode == DCG_SCORE_FEATURES) {
       if (fluidityOn())
@@ -235,7 +244,7 @@ struct DCGridSim : SimBase {
     DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refresh", (k_refresh<BW><<<blocksFor((size_t)27 * B, 256), 256, 0, stream>>>(g)));
     const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
     for (int l = L - 2; l >= 0; l--) {
-      const unsigned n = blocksFor(maxL[l], G::BPW);
+      const unsigned n = std::min(blocksFor(maxL[l], G::BPW), 256u);  // grid-stride over the touched blocks
       if (src)
         DCG_LAUNCH(DCG_K_TOPOLOGY, "k_propagate", (k_propagate<BW, true><<<n, G::WG, 0, stream>>>(g, P, l, mcBuf)));
       else
@@ -248,14 +257,14 @@ struct DCGridSim : SimBase {
   int restrictAll(T* ch) {
     for (int l = 0; l < L - 1; l++) {
       const size_t groups = (size_t)maxL[l] * (G::BV / 8);
-      DCG_LAUNCH(DCG_K_RESTRICT, "k_restrict", (k_restrict<BW, T><<<blocksFor(groups, 256), 256, 0, stream>>>(g, l, ch)));
+      DCG_LAUNCH(DCG_K_RESTRICT, "k_restrict", (k_restrict<BW, T><<<blocksFor(groups, 256) < 2048 ? blocksFor(groups, 256) : 2048, 256, 0, stream>>>(g, l, ch)));
     }
     return DCG_OK;
   }
 
   // advectVelocity(), fluid_simulation_dcgrid.cu:263-268
   int advectVelocity() override {
-    const unsigned nWG = blocksFor(B, G::BPW);
+    const unsigned nWG = slotGrid<BW>(B);
     const bool buoy = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT, fl = fluidityOn();
 #define ADV_V(F, Bu) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_advect_velocity", (k_advect_velocity<BW, F, Bu><<<nWG, G::WG, 0, stream>>>(g, P)))
     if (fl && buoy) ADV_V(true, true);
@@ -268,7 +277,7 @@ struct DCGridSim : SimBase {
   }
   // advectDensity(), fluid_simulation_dcgrid.cu:313-318
   int advectDensity() override {
-    const unsigned nWG = blocksFor(B, G::BPW);
+    const unsigned nWG = slotGrid<BW>(B);
     const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT, fl = fluidityOn();
 #define ADV_D(F, Sr) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_advect_density", (k_advect_density<BW, F, Sr><<<nWG, G::WG, 0, stream>>>(g, P)))
     if (fl && src) ADV_D(true, true);
@@ -285,7 +294,7 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
   int divergenceKernel() {
-    const unsigned nWG = blocksFor(B, G::BPW);
+    const unsigned nWG = slotGrid<BW>(B);
     if (fluidityOn())
       DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
     else
@@ -299,18 +308,18 @@ struct DCGridSim : SimBase {
   int jacobiOne(int level, bool fwd) {
     const int s = 1 << level;
     const float alpha = (float)(s * s) * P.dx * P.dx;
-    const unsigned n = blocksFor(maxL[level], G::BPW);
+    const unsigned n = slotGrid<BW>(maxL[level]);
     DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi",
                (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, fwd ? g.pre : g.tpre, fwd ? g.tpre : g.pre, alpha)));
     return DCG_OK;
   }
   int prolongate(int level) {
     DCG_LAUNCH(DCG_K_PROLONGATE, "k_prolongate",
-               (k_prolongate<BW><<<blocksFor(maxL[level], G::BPW), G::WG, 0, stream>>>(g, level)));
+               (k_prolongate<BW><<<slotGrid<BW>(maxL[level]), G::WG, 0, stream>>>(g, level)));
     return DCG_OK;
   }
   int vorticity() {
-    const unsigned nWG = blocksFor(B, G::BPW);
+    const unsigned nWG = slotGrid<BW>(B);
     if (fluidityOn())
       DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
     else
@@ -343,7 +352,7 @@ struct DCGridSim : SimBase {
   int jacobiPairs(int level, int pairs) {
     const int s = 1 << level;
     const float alpha = (float)(s * s) * P.dx * P.dx;  // dcgrid_multigrid_solver.cu:11
-    const unsigned n = blocksFor(maxL[level], G::BPW);
+    const unsigned n = slotGrid<BW>(maxL[level]);
     if (pairs <= 0) return DCG_OK;
     Pending batch;  // the run of back-to-back sweeps of one level is timed as one batch
     DCG_TRY(beginTimed(DCG_K_JACOBI, &batch));
@@ -367,7 +376,7 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
   int applyKernel() {
-    const unsigned nWG = blocksFor(B, G::BPW);
+    const unsigned nWG = slotGrid<BW>(B);
     if (fluidityOn())
       DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
     else
@@ -384,7 +393,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(jacobiPairs(L - 1, P.project_top_pairs));
     for (int l = L - 2; l >= 0; l--) {
       DCG_LAUNCH(DCG_K_PROLONGATE, "k_prolongate",
-                 (k_prolongate<BW><<<blocksFor(maxL[l], G::BPW), G::WG, 0, stream>>>(g, l)));
+                 (k_prolongate<BW><<<slotGrid<BW>(maxL[l]), G::WG, 0, stream>>>(g, l)));
       DCG_TRY(jacobiPairs(l, P.project_level_pairs));
     }
     return applyPressure();
@@ -395,6 +404,228 @@ struct DCGridSim : SimBase {
     for (int l = L - 1; l >= 0; l--) DCG_TRY(jacobiPairs(l, P.project_local_pairs));
     return applyPressure();
   }
+  // ---------------------------------------------------------------- projection solve (extension)
+  double* xd = nullptr;        // FP64 pressure of all cells (mixed and FP64 solves)
+  double* rhoBuf = nullptr;    // right-hand side / residual in density units (T = float or double)
+  double* eBuf = nullptr;      // multigrid corrections (T)
+  uint8_t* reachBuf = nullptr; // finest level with cells inside each cell
+  double* ueBuf = nullptr;     // pyramid of the top level: corrections, right-hand sides, activity
+  double* urhoBuf = nullptr;
+  uint8_t* uactBuf = nullptr;
+  double* partBuf = nullptr;
+  std::vector<UDim> pyr;       // pyramid grids below the top level (cell size doubling)
+  dcg_solve_info rep{};
+  uint32_t hostLoads[DCG_MAX_LEVELS] = {};
+  static constexpr unsigned kSolveWG = 1024;  // workgroups of the cell loops with reductions
+
+  unsigned cellGrid(int level) const {
+    const unsigned n = blocksFor((size_t)maxL[level] * G::BV, 256);
+    return n < 2048u ? n : 2048u;
+  }
+  int solveSetup() {
+    if (xd) return DCG_OK;
+    const int top = L - 1;
+    const int tx = g.rx[top] * BW, ty = g.ry[top] * BW, tz = g.rz[top] * BW;
+    if ((tx << top) != g.gx || (ty << top) != g.gy || (tz << top) != g.gz)
+      return fail(DCG_E_UNSUPPORTED, "projection solve: the grid size must be a multiple of the top level's blocks");
+    const size_t C = (size_t)B * G::BV;
+    DCG_TRY(alloc(&xd, C));
+    DCG_TRY(alloc(&rhoBuf, C));
+    DCG_TRY(alloc(&eBuf, C));
+    DCG_TRY(alloc(&reachBuf, C));
+    DCG_TRY(alloc(&partBuf, 4 * (size_t)kSolveWG));
+    // pyramid: halve while every size stays even and at least 2
+    size_t off = 0;
+    int w = tx, h = ty, d = tz;
+    while (w % 2 == 0 && h % 2 == 0 && d % 2 == 0 && std::min(w, std::min(h, d)) >= 4) {
+      w /= 2;
+      h /= 2;
+      d /= 2;
+      pyr.push_back(UDim{w, h, d, (uint32_t)off});
+      off += (size_t)w * h * d;
+    }
+    DCG_TRY(alloc(&ueBuf, off + 1));
+    DCG_TRY(alloc(&urhoBuf, off + 1));
+    DCG_TRY(alloc(&uactBuf, off + 1));
+    return DCG_OK;
+  }
+  float levelH2(int level) const {
+    const float s = (float)(1 << level);
+    return s * s * P.dx * P.dx;  // alpha of k_dcgrid_jacobi, dcgrid_multigrid_solver.cu:11
+  }
+  template <class T>
+  int smooth(int level, int lmin, int sweeps) {
+    T* e = (T*)eBuf;
+    const T* rho = (const T*)rhoBuf;
+    for (int s = 0; s < sweeps; s++)
+      for (int color = 0; color < 2; color++)
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_rbgs",
+                   (k_sv_rbgs<BW, T><<<cellGrid(level), 256, 0, stream>>>(g, level, lmin, color, e, rho, reachBuf, levelH2(level))));
+    return DCG_OK;
+  }
+  template <class T>
+  int smoothU(int j, int sweeps) {
+    const UDim& D = pyr[j];
+    const unsigned n = blocksFor((size_t)D.w * D.h * D.d, 256);
+    const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
+    for (int k = 0; k < sweeps; k++)
+      for (int color = 0; color < 2; color++)
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_rbgs", (k_svu_rbgs<T><<<n, 256, 0, stream>>>(D, color, (T*)ueBuf, (const T*)urhoBuf, uactBuf, h2)));
+    return DCG_OK;
+  }
+  // One V-cycle for the correction of level lmin (right-hand side in rho on its cells).
+  template <class T>
+  int vcycle(int lmin) {
+    T* e = (T*)eBuf;
+    T* rho = (T*)rhoBuf;
+    const int nu = std::max(1, std::min(P.solve_smooth, 10));
+    const int top = L - 1;
+    DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, T><<<cellGrid(lmin), 256, 0, stream>>>(g, lmin, e, e)));
+    for (int k = lmin; k < top; k++) {
+      DCG_TRY(smooth<T>(k, lmin, nu));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, T><<<cellGrid(k + 1), 256, 0, stream>>>(g, k + 1, e, rho)));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
+                 (k_sv_restrict<BW, T><<<cellGrid(k), 256, 0, stream>>>(g, k, lmin, e, rho, reachBuf, levelH2(k))));
+    }
+    DCG_TRY(smooth<T>(top, lmin, nu));
+    if (!pyr.empty()) {
+      T* ue = (T*)ueBuf;
+      T* ur = (T*)urhoBuf;
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict_top",
+                 (k_sv_restrict_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, rho, reachBuf, ue, ur, uactBuf, pyr[0], levelH2(top))));
+      const int J = (int)pyr.size();
+      for (int j = 0; j + 1 < J; j++) {
+        DCG_TRY(smoothU<T>(j, nu));
+        const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
+        const UDim& Dc = pyr[j + 1];
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_restrict",
+                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(pyr[j], Dc, ue, ur, uactBuf, h2)));
+      }
+      {
+        const float s = (float)(1 << (L + J - 1)), h2 = s * s * P.dx * P.dx;
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_bottom", (k_svu_bottom<T><<<1, 1024, 0, stream>>>(pyr[J - 1], ue, ur, uactBuf, h2, 40)));
+      }
+      for (int j = J - 2; j >= 0; j--) {
+        const UDim& D = pyr[j];
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_prolong",
+                   (k_svu_prolong<T><<<blocksFor((size_t)D.w * D.h * D.d, 256), 256, 0, stream>>>(D, pyr[j + 1], ue, uactBuf)));
+        DCG_TRY(smoothU<T>(j, nu));
+      }
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_top",
+                 (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, pyr[0])));
+      DCG_TRY(smooth<T>(top, lmin, nu));
+    } else {
+      DCG_TRY(smooth<T>(top, lmin, 20 * nu));
+    }
+    for (int k = top - 1; k >= lmin; k--) {
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong", (k_sv_prolong<BW, T><<<cellGrid(k), 256, 0, stream>>>(g, k, lmin, e, reachBuf)));
+      DCG_TRY(smooth<T>(k, lmin, nu));
+    }
+    return DCG_OK;
+  }
+  // Sums of the partials of the last residual kernel.
+  int readPartials(int count, double* out) {
+    std::vector<double> h((size_t)count * kSolveWG);
+    DCG_HIP(hipMemcpyAsync(h.data(), partBuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    DCG_TRY(sync());
+    for (int k = 0; k < count; k++) out[k] = 0;
+    for (unsigned w = 0; w < kSolveWG; w++)
+      for (int k = 0; k < count; k++) out[k] += h[(size_t)w * count + k];
+    return DCG_OK;
+  }
+  template <class TX, class T>
+  int solveLevel(int l, TX* x) {
+    T* rho = (T*)rhoBuf;
+    const float alpha = levelH2(l);
+    const bool singular = hostLoads[l] == maxL[l] && maxL[l] == fullL[l];
+    const double N = (double)hostLoads[l] * G::BV;
+    double r0 = -1, rr = 0, prev = 0;
+    int cycles = 0, stall = 0;
+    const double tol = (double)P.solve_tol;
+    const int maxCycles = std::max(1, P.solve_max_cycles);
+    for (;;) {
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_residual",
+                 (k_sv_residual<BW, TX, T><<<kSolveWG, 256, 0, stream>>>(g, l, x, rho, partBuf, alpha, alpha)));
+      double s[4];
+      DCG_TRY(readPartials(4, s));
+      double m = 0;
+      if (singular) {
+        m = s[0] / N;
+        const double m0 = s[2] / N;
+        rr = std::sqrt(std::max(s[1] - N * m * m, 0.0));
+        if (r0 < 0) r0 = std::sqrt(std::max(s[3] - N * m0 * m0, 0.0));
+      } else {
+        rr = std::sqrt(s[1]);
+        if (r0 < 0) r0 = std::sqrt(s[3]);
+      }
+      if (rr <= tol * r0 || cycles >= maxCycles) break;
+      if (cycles > 0 && rr > 0.7 * prev) {
+        if (++stall >= 3) break;  // the working precision is exhausted
+      } else {
+        stall = 0;
+      }
+      prev = rr;
+      if (singular)
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_shift", (k_sv_shift<BW, T><<<cellGrid(l), 256, 0, stream>>>(g, l, rho, m / (double)alpha)));
+      DCG_TRY(vcycle<T>(l));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_update", (k_sv_update<BW, TX, T><<<cellGrid(l), 256, 0, stream>>>(g, l, x, (const T*)eBuf, reachBuf)));
+      cycles++;
+    }
+    if (singular) {  // zero-mean normalisation of a pure Neumann level
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_sum", (k_sv_sum<BW, TX><<<kSolveWG, 256, 0, stream>>>(g, l, x, partBuf)));
+      double sx;
+      DCG_TRY(readPartials(1, &sx));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_shift", (k_sv_shift<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, x, sx / N)));
+    }
+    rep.cycles[l] = cycles;
+    rep.rel_residual[l] = r0 > 0 ? rr / r0 : 0.0;
+    rep.initial_residual[l] = r0;
+    rep.singular[l] = singular ? 1 : 0;
+    if (!(rr <= tol * r0)) rep.converged = 0;
+    return DCG_OK;
+  }
+  template <class TX, class T>
+  int solveAll(TX* x) {
+    for (int l = L - 1; l >= 0; l--) {
+      if (l == L - 1)
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, x, x)));
+      else
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_x", (k_sv_prolong_x<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, x)));
+      DCG_TRY((solveLevel<TX, T>(l, x)));
+    }
+    return DCG_OK;
+  }
+  int projectSolve() override {
+    DCG_TRY(solveSetup());
+    DCG_TRY(divergence());
+    {
+      uint32_t h[DCG_MAX_LEVELS];
+      DCG_HIP(hipMemcpyAsync(h, g.ctl + CTL_LOADS, sizeof(h), hipMemcpyDeviceToHost, stream));
+      DCG_TRY(sync());
+      for (int l = 0; l < L; l++) hostLoads[l] = h[l];
+    }
+    for (int l = 0; l < L; l++)
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_reach", (k_sv_reach<BW><<<cellGrid(l), 256, 0, stream>>>(g, l, reachBuf)));
+    rep = dcg_solve_info{};
+    rep.levels = L;
+    rep.converged = 1;
+    if (P.solve_precision == DCG_SOLVE_FP32) {
+      DCG_TRY((solveAll<float, float>(g.pre)));
+    } else {
+      if (P.solve_precision == DCG_SOLVE_FP64)
+        DCG_TRY((solveAll<double, double>(xd)));
+      else
+        DCG_TRY((solveAll<double, float>(xd)));
+      for (int l = 0; l < L; l++)
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_store", (k_sv_store<BW, double><<<cellGrid(l), 256, 0, stream>>>(g, l, xd)));
+    }
+    return applyPressure();
+  }
+  int solveReport(dcg_solve_info* out) override {
+    *out = rep;
+    return DCG_OK;
+  }
+
   // debugStats(), fluid_simulation_dcgrid.cu:517-528 (residual of the last projection)
   int debugStats(double* out) override {
     DCG_LAUNCH(DCG_K_OTHER, "k_residual", (k_residual<BW><<<blocksFor(B, 64), 64, 0, stream>>>(g, P, stats)));

@@ -35,6 +35,47 @@ __device__ __forceinline__ BlockCtx<BW> blockCtx(uint32_t firstBlock, uint32_t c
   return c;
 }
 
+// Grid-stride walk over allocated blocks. The allocated blocks of level l are the prefix
+// [offs[l], offs[l] + loads[l]) of its slice (blocks are never freed), so a kernel launched with
+// a fixed number of workgroups visits exactly the allocated blocks of one level (level >= 0) or
+// of all levels in level order (level < 0), BPW blocks per workgroup and iteration, with the
+// counts read from the device control words: no workgroup is spent on free blocks and the host
+// needs no block counts.
+__device__ __forceinline__ uint32_t slotTotal(const DG& g, int level) {
+  if (level >= 0) return g.ctl[CTL_LOADS + level];
+  uint32_t n = 0;
+  for (int l = 0; l < g.L; l++) n += g.ctl[CTL_LOADS + l];
+  return n;
+}
+__device__ __forceinline__ uint32_t slotBlock(const DG& g, int level, uint32_t s) {
+  if (level >= 0) return g.offs[level] + s;
+  for (int l = 0; l < g.L; l++) {
+    const uint32_t n = g.ctl[CTL_LOADS + l];
+    if (s < n) return g.offs[l] + s;
+    s -= n;
+  }
+  return 0;
+}
+template <int BW>
+__device__ __forceinline__ uint32_t firstSlot() {
+  return xcdRemap(blockIdx.x, gridDim.x) * Geo<BW>::BPW;
+}
+template <int BW>
+__device__ __forceinline__ uint32_t slotStride() {
+  return gridDim.x * Geo<BW>::BPW;
+}
+template <int BW>
+__device__ __forceinline__ BlockCtx<BW> slotCtx(const DG& g, int level, uint32_t base, uint32_t total) {
+  constexpr int BV = Geo<BW>::BV;
+  BlockCtx<BW> c;
+  c.sub = threadIdx.x / BV;
+  c.t = threadIdx.x % BV;
+  const uint32_t slot = base + c.sub;
+  c.valid = slot < total;
+  c.b = c.valid ? slotBlock(g, level, slot) : 0;
+  return c;
+}
+
 template <int BW>
 __device__ __forceinline__ int apronIndexOfLocal(int t) {
   constexpr int A = Geo<BW>::A, BB = Geo<BW>::BB;
@@ -84,18 +125,20 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(DG g, int level, const f
                                                          float* __restrict__ out, float alpha) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float tiles[BPW][A3];
-  // the allocated blocks of a level are the prefix [offs, offs + loads) of its slice
-  const BlockCtx<BW> c = blockCtx<BW>(g.offs[level], g.ctl[CTL_LOADS + level]);
-  const bool active = c.valid;
-  float* tile = tiles[c.sub];
-  if (active) stageScalar<BW>(g, in, tile, c.b, c.t);
-  __syncthreads();
-  if (!active) return;
-  const int a = apronIndexOfLocal<BW>(c.t);
-  const uint32_t cell = c.b * BV + c.t;
-  out[cell] = (tile[a - A * A] + tile[a + A * A] + tile[a - A] + tile[a + A] + tile[a - 1] + tile[a + 1] -
-               alpha * g.div[cell]) /
-              6.f;
+  const uint32_t total = slotTotal(g, level);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tile is no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, level, base, total);
+    float* tile = tiles[c.sub];
+    if (c.valid) stageScalar<BW>(g, in, tile, c.b, c.t);
+    __syncthreads();
+    if (!c.valid) continue;
+    const int a = apronIndexOfLocal<BW>(c.t);
+    const uint32_t cell = c.b * BV + c.t;
+    out[cell] = (tile[a - A * A] + tile[a + A * A] + tile[a - A] + tile[a + A] + tile[a - 1] + tile[a + 1] -
+                 alpha * g.div[cell]) /
+                6.f;
+  }
 }
 
 // ------------------------------------------------------------------ divergence
@@ -106,37 +149,41 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(DG g, dcg_params P) 
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
                 BB = Geo<BW>::BB, SBB = Geo<BW>::SBB;
   __shared__ float3 tiles[BPW][A3];
-  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  const bool active = c.valid && g.lev[c.b] != FREE;
-  float3* tile = tiles[c.sub];
-  int3 p0 = make_int3(0, 0, 0);
-  int lb = 0, scale = 1;
-  const uint32_t cell = c.b * BV + c.t;
-  if (active) {
-    lb = g.lev[c.b];
-    scale = 1 << lb;
-    p0 = g.pos[c.b];
-    g.pre[cell] = 0.f;
-    g.tpre[cell] = 0.f;
-    g.div[cell] = 0.f;
-    tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
-    for (int fc = c.t; fc < FACE; fc += BV) {
-      int ai, aj, ak;
-      faceCell<BW>(fc, ai, aj, ak);
-      const float3 v = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
-      tile[(ai * A + aj) * A + ak] = velocityBC(P, v, p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
+  const uint32_t total = slotTotal(g, -1);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tiles are no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const bool active = c.valid && g.lev[c.b] != FREE;
+    float3* tile = tiles[c.sub];
+    int3 p0 = make_int3(0, 0, 0);
+    int lb = 0, scale = 1;
+    const uint32_t cell = c.b * BV + c.t;
+    if (active) {
+      lb = g.lev[c.b];
+      scale = 1 << lb;
+      p0 = g.pos[c.b];
+      g.pre[cell] = 0.f;
+      g.tpre[cell] = 0.f;
+      g.div[cell] = 0.f;
+      tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
+      for (int fc = c.t; fc < FACE; fc += BV) {
+        int ai, aj, ak;
+        faceCell<BW>(fc, ai, aj, ak);
+        const float3 v = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
+        tile[(ai * A + aj) * A + ak] = velocityBC(P, v, p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
+      }
     }
+    __syncthreads();
+    if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) continue;
+    const float alpha = .5f * P.rdx / (float)scale;
+    float w[6];
+    neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
+    const int a = apronIndexOfLocal<BW>(c.t);
+    const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
+                 vf = tile[a + 1];
+    g.div[cell] = alpha * (w[1] * vr.x - w[0] * vl.x + w[3] * vu.y - w[2] * vd.y + w[5] * vf.z - w[4] * vb.z);
+    (void)BB;
   }
-  __syncthreads();
-  if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) return;
-  const float alpha = .5f * P.rdx / (float)scale;
-  float w[6];
-  neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
-  const int a = apronIndexOfLocal<BW>(c.t);
-  const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
-               vf = tile[a + 1];
-  g.div[cell] = alpha * (w[1] * vr.x - w[0] * vl.x + w[3] * vu.y - w[2] * vd.y + w[5] * vf.z - w[4] * vb.z);
-  (void)BB;
 }
 
 // ------------------------------------------------------------------ gradient subtraction
@@ -145,27 +192,31 @@ template <int BW, bool FLUIDITY>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(DG g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, SBB = Geo<BW>::SBB;
   __shared__ float tiles[BPW][A3];
-  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  const bool active = c.valid && g.lev[c.b] != FREE;
-  float* tile = tiles[c.sub];
-  int lb = 0;
-  if (active) {
-    lb = g.lev[c.b];
-    stageScalar<BW>(g, g.pre, tile, c.b, c.t);
+  const uint32_t total = slotTotal(g, -1);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tiles are no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const bool active = c.valid && g.lev[c.b] != FREE;
+    float* tile = tiles[c.sub];
+    int lb = 0;
+    if (active) {
+      lb = g.lev[c.b];
+      stageScalar<BW>(g, g.pre, tile, c.b, c.t);
+    }
+    __syncthreads();
+    if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) continue;
+    const float alpha = .5f * P.rdx / (float)(1 << lb);
+    float w[6];
+    neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
+    const int a = apronIndexOfLocal<BW>(c.t);
+    const uint32_t cell = c.b * BV + c.t;
+    const float p = tile[a];
+    float3 v = g.vel[cell];
+    v.x = v.x - alpha * (w[1] * (tile[a + A * A] - p) + w[0] * (p - tile[a - A * A]));
+    v.y = v.y - alpha * (w[3] * (tile[a + A] - p) + w[2] * (p - tile[a - A]));
+    v.z = v.z - alpha * (w[5] * (tile[a + 1] - p) + w[4] * (p - tile[a - 1]));
+    g.vel[cell] = v;
   }
-  __syncthreads();
-  if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) return;
-  const float alpha = .5f * P.rdx / (float)(1 << lb);
-  float w[6];
-  neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
-  const int a = apronIndexOfLocal<BW>(c.t);
-  const uint32_t cell = c.b * BV + c.t;
-  const float p = tile[a];
-  float3 v = g.vel[cell];
-  v.x = v.x - alpha * (w[1] * (tile[a + A * A] - p) + w[0] * (p - tile[a - A * A]));
-  v.y = v.y - alpha * (w[3] * (tile[a + A] - p) + w[2] * (p - tile[a - A]));
-  v.z = v.z - alpha * (w[5] * (tile[a + 1] - p) + w[4] * (p - tile[a - 1]));
-  g.vel[cell] = v;
 }
 
 // ------------------------------------------------------------------ vorticity
@@ -175,32 +226,36 @@ template <int BW, bool FLUIDITY>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(DG g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE;
   __shared__ float3 tiles[BPW][A3];
-  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  const bool active = c.valid && g.lev[c.b] != FREE;
-  float3* tile = tiles[c.sub];
-  int lb = 0;
-  const uint32_t cell = c.b * BV + c.t;
-  if (active) {
-    lb = g.lev[c.b];
-    tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
-    for (int fc = c.t; fc < FACE; fc += BV) {
-      int ai, aj, ak;
-      faceCell<BW>(fc, ai, aj, ak);
-      tile[(ai * A + aj) * A + ak] = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
+  const uint32_t total = slotTotal(g, -1);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tiles are no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const bool active = c.valid && g.lev[c.b] != FREE;
+    float3* tile = tiles[c.sub];
+    int lb = 0;
+    const uint32_t cell = c.b * BV + c.t;
+    if (active) {
+      lb = g.lev[c.b];
+      tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
+      for (int fc = c.t; fc < FACE; fc += BV) {
+        int ai, aj, ak;
+        faceCell<BW>(fc, ai, aj, ak);
+        tile[(ai * A + aj) * A + ak] = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
+      }
     }
+    __syncthreads();
+    if (!active) continue;
+    const float alpha = .5f * P.rdx / (float)(1 << lb);
+    float w[6];
+    neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
+    const int a = apronIndexOfLocal<BW>(c.t);
+    const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
+                 vf = tile[a + 1];
+    const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
+    g.vort[cell] = make_float3(alpha * ((wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y)),
+                               alpha * ((wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z)),
+                               alpha * ((wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)));
   }
-  __syncthreads();
-  if (!active) return;
-  const float alpha = .5f * P.rdx / (float)(1 << lb);
-  float w[6];
-  neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
-  const int a = apronIndexOfLocal<BW>(c.t);
-  const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
-               vf = tile[a + 1];
-  const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
-  g.vort[cell] = make_float3(alpha * ((wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y)),
-                             alpha * ((wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z)),
-                             alpha * ((wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)));
 }
 
 // ------------------------------------------------------------------ restriction
@@ -221,20 +276,19 @@ __device__ __forceinline__ float3 mulv(float3 a, float s) { return f3mul(a, s); 
 template <int BW, class T>
 __global__ void __launch_bounds__(256) k_restrict(DG g, int level, T* __restrict__ ch) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, GPB = BV / 8;
-  const uint32_t gi = blockIdx.x * 256 + threadIdx.x;  // group within the level slice
-  const uint32_t bl = gi / GPB;
-  if (bl >= g.maxL[level]) return;
-  const uint32_t b = g.offs[level] + bl;
-  if (g.lev[b] != level) return;
-  const uint32_t ps = g.par[b];
-  if (ps == NFU) return;
-  const uint32_t grp = gi % GPB;
-  const uint32_t parentCell = (ps / 8) * BV + ((ps % 8) << SBB) + grp;
-  const T* src = ch + (size_t)b * BV + 8 * grp;
-  T acc = zeroOf<T>();
+  const uint32_t groups = g.ctl[CTL_LOADS + level] * GPB;  // allocated prefix of the level slice
+  for (uint32_t gi = blockIdx.x * 256 + threadIdx.x; gi < groups; gi += gridDim.x * 256) {
+    const uint32_t b = g.offs[level] + gi / GPB;
+    const uint32_t ps = g.par[b];
+    if (ps == NFU) continue;
+    const uint32_t grp = gi % GPB;
+    const uint32_t parentCell = (ps / 8) * BV + ((ps % 8) << SBB) + grp;
+    const T* src = ch + (size_t)b * BV + 8 * grp;
+    T acc = zeroOf<T>();
 #pragma unroll
-  for (int i = 0; i < 8; i++) acc = addv(acc, src[i]);
-  ch[parentCell] = mulv(acc, .125f);
+    for (int i = 0; i < 8; i++) acc = addv(acc, src[i]);
+    ch[parentCell] = mulv(acc, .125f);
+  }
 }
 
 // ------------------------------------------------------------------ parent stencil
@@ -260,19 +314,22 @@ __device__ __forceinline__ void parentStencil(const DG& g, uint32_t ps, int x, i
 template <int BW>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(DG g, int level) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
-  const BlockCtx<BW> c = blockCtx<BW>(g.offs[level], g.maxL[level]);
-  if (!c.valid || g.lev[c.b] != level) return;
-  const uint32_t ps = g.par[c.b];
-  if (ps == NFU) return;
-  const int3 p0 = g.pos[c.b];
-  int lx, ly, lz;
-  locPos<BB>((uint32_t)c.t, lx, ly, lz);
-  uint32_t q[8];
-  parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
-  const float* p = g.pre;
-  g.pre[c.b * BV + c.t] = (27.f * p[q[0]] + 9.f * (p[q[1]] + p[q[2]] + p[q[3]]) + 3.f * (p[q[4]] + p[q[5]] + p[q[6]]) +
-                           p[q[7]]) /
-                          64.f;
+  const uint32_t total = slotTotal(g, level);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    const BlockCtx<BW> c = slotCtx<BW>(g, level, base, total);
+    if (!c.valid) continue;
+    const uint32_t ps = g.par[c.b];
+    if (ps == NFU) continue;
+    const int3 p0 = g.pos[c.b];
+    int lx, ly, lz;
+    locPos<BB>((uint32_t)c.t, lx, ly, lz);
+    uint32_t q[8];
+    parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
+    const float* p = g.pre;
+    g.pre[c.b * BV + c.t] = (27.f * p[q[0]] + 9.f * (p[q[1]] + p[q[2]] + p[q[3]]) +
+                             3.f * (p[q[4]] + p[q[5]] + p[q[6]]) + p[q[7]]) /
+                            64.f;
+  }
 }
 
 // k_dcgrid_propagate_values (dcgrid_adaptation.cu:92-143) for the blocks of `level` touched by
@@ -282,41 +339,36 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_propagate(DG g, dcg_params P, i
                                                             const uint32_t* __restrict__ moved) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW;
   const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
-  const uint32_t e = blockIdx.x * BPW + sub;
   const uint32_t nm = g.ctl[CTL_MOVES + level];
   const uint32_t nc = level + 1 < g.L ? g.ctl[CTL_REFINES + level + 1] : 0;
-  uint32_t b;
-  if (e < nm)
-    b = moved[g.offs[level] + e];
-  else if (e < nm + nc)
-    b = g.offs[level] + g.ctl[CTL_LOADS_BEFORE + level] + (e - nm);
-  else
-    return;
-  if (g.lev[b] != level) return;
-  const uint32_t ps = g.par[b];
-  if (ps == NFU) return;
-  const int3 p0 = g.pos[b];
-  int lx, ly, lz;
-  locPos<BB>((uint32_t)t, lx, ly, lz);
-  const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
-  uint32_t q[8];
-  parentStencil<BW>(g, ps, x, y, z, q);
-  const float w27 = 27.f / 64.f, w9 = 9.f / 64.f, w3 = 3.f / 64.f, w1 = 1.f / 64.f;
-  const uint32_t cell = b * BV + t;
-  const float* d = g.den;
-  g.den[cell] = w27 * d[q[0]] + w9 * (d[q[1]] + d[q[2]] + d[q[3]]) + w3 * (d[q[4]] + d[q[5]] + d[q[6]]) + w1 * d[q[7]];
-  if (SRC) {
-    const float* tt = g.tem;
-    g.tem[cell] =
-        w27 * tt[q[0]] + w9 * (tt[q[1]] + tt[q[2]] + tt[q[3]]) + w3 * (tt[q[4]] + tt[q[5]] + tt[q[6]]) + w1 * tt[q[7]];
+  for (uint32_t e = blockIdx.x * BPW + sub; e < nm + nc; e += gridDim.x * BPW) {
+    const uint32_t b = e < nm ? moved[g.offs[level] + e] : g.offs[level] + g.ctl[CTL_LOADS_BEFORE + level] + (e - nm);
+    if (g.lev[b] != level) continue;
+    const uint32_t ps = g.par[b];
+    if (ps == NFU) continue;
+    const int3 p0 = g.pos[b];
+    int lx, ly, lz;
+    locPos<BB>((uint32_t)t, lx, ly, lz);
+    const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
+    uint32_t q[8];
+    parentStencil<BW>(g, ps, x, y, z, q);
+    const float w27 = 27.f / 64.f, w9 = 9.f / 64.f, w3 = 3.f / 64.f, w1 = 1.f / 64.f;
+    const uint32_t cell = b * BV + t;
+    const float* d = g.den;
+    g.den[cell] = w27 * d[q[0]] + w9 * (d[q[1]] + d[q[2]] + d[q[3]]) + w3 * (d[q[4]] + d[q[5]] + d[q[6]]) + w1 * d[q[7]];
+    if (SRC) {
+      const float* tt = g.tem;
+      g.tem[cell] =
+          w27 * tt[q[0]] + w9 * (tt[q[1]] + tt[q[2]] + tt[q[3]]) + w3 * (tt[q[4]] + tt[q[5]] + tt[q[6]]) + w1 * tt[q[7]];
+    }
+    const float3* v = g.vel;
+    const float3 a0 = f3mul(v[q[0]], w27);
+    const float3 a1 = f3mul(f3add(f3add(v[q[1]], v[q[2]]), v[q[3]]), w9);
+    const float3 a2 = f3mul(f3add(f3add(v[q[4]], v[q[5]]), v[q[6]]), w3);
+    const float3 a3 = f3mul(v[q[7]], w1);
+    g.vel[cell] = f3add(f3add(f3add(a0, a1), a2), a3);
+    g.flu[cell] = cellFluidity(P, x, y, z, 1 << level);
   }
-  const float3* v = g.vel;
-  const float3 a0 = f3mul(v[q[0]], w27);
-  const float3 a1 = f3mul(f3add(f3add(v[q[1]], v[q[2]]), v[q[3]]), w9);
-  const float3 a2 = f3mul(f3add(f3add(v[q[4]], v[q[5]]), v[q[6]]), w3);
-  const float3 a3 = f3mul(v[q[7]], w1);
-  g.vel[cell] = f3add(f3add(f3add(a0, a1), a2), a3);
-  g.flu[cell] = cellFluidity(P, x, y, z, 1 << level);
 }
 
 // ------------------------------------------------------------------ advection
@@ -479,56 +531,60 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_param
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float3 vtiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
-  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  const bool active = c.valid && g.lev[c.b] != FREE;
-  float3* vt = vtiles[c.sub];
-  float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
-  int3 p0 = make_int3(0, 0, 0);
-  int lb = 0;
-  if (active) {
-    lb = g.lev[c.b];
-    p0 = g.pos[c.b];
-    stageApron<BW, float3>(g, g.vel, vt, c.b, c.t);
-    if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
-  }
-  __syncthreads();
-  if (!active) return;
-  const uint32_t cell = c.b * BV + c.t;
-  if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
-    g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
-    return;
-  }
-  float px, py, pz;
-  int x, y, z, level;
-  footPoint<BW>(g, P, c.b, c.t, vt[apronIndexOfLocal<BW>(c.t)], px, py, pz, x, y, z, level);
-  float3 acc = make_float3(0.f, 0.f, 0.f);
-  TileSample<BW, FLUIDITY> ts;
-  if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
-    if (!ts.zero) {
+  const uint32_t total = slotTotal(g, -1);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tiles are no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const bool active = c.valid && g.lev[c.b] != FREE;
+    float3* vt = vtiles[c.sub];
+    float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+    int3 p0 = make_int3(0, 0, 0);
+    int lb = 0;
+    if (active) {
+      lb = g.lev[c.b];
+      p0 = g.pos[c.b];
+      stageApron<BW, float3>(g, g.vel, vt, c.b, c.t);
+      if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
+    }
+    __syncthreads();
+    if (!active) continue;
+    const uint32_t cell = c.b * BV + c.t;
+    if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
+      g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
+      continue;
+    }
+    float px, py, pz;
+    int x, y, z, level;
+    footPoint<BW>(g, P, c.b, c.t, vt[apronIndexOfLocal<BW>(c.t)], px, py, pz, x, y, z, level);
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    TileSample<BW, FLUIDITY> ts;
+    if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
+      if (!ts.zero) {
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const float3 vk = velocityBC(P, vt[ts.a0 + ts.off(k)], ts.x0 + ((k >> 2) & 1), ts.y0 + ((k >> 1) & 1),
-                                     ts.z0 + (k & 1), ts.scale);
-        acc = k == 0 ? f3mul(vk, ts.w[0]) : f3add(acc, f3mul(vk, ts.w[k]));
+        for (int k = 0; k < 8; k++) {
+          const float3 vk = velocityBC(P, vt[ts.a0 + ts.off(k)], ts.x0 + ((k >> 2) & 1), ts.y0 + ((k >> 1) & 1),
+                                       ts.z0 + (k & 1), ts.scale);
+          acc = k == 0 ? f3mul(vk, ts.w[0]) : f3add(acc, f3mul(vk, ts.w[k]));
+        }
+      }
+    } else {
+      CrossSample<BW, FLUIDITY> s;
+      s.init(g, px, py, pz);
+      if (!s.zero) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const float3 vk =
+              velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
+          acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
+        }
       }
     }
-  } else {
-    CrossSample<BW, FLUIDITY> s;
-    s.init(g, px, py, pz);
-    if (!s.zero) {
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const float3 vk =
-            velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
-        acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
-      }
+    if (BUOY) {
+      const float bf = P.dt * (P.buoy_beta * (g.tem[cell] - P.t_ambient) - P.buoy_alpha * g.den[cell]);
+      acc.y = acc.y + bf;
     }
+    g.tvel[cell] = acc;
   }
-  if (BUOY) {
-    const float bf = P.dt * (P.buoy_beta * (g.tem[cell] - P.t_ambient) - P.buoy_alpha * g.den[cell]);
-    acc.y = acc.y + bf;
-  }
-  g.tvel[cell] = acc;
 }
 
 // k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
@@ -539,71 +595,75 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
   __shared__ float dtiles[BPW][A3];
   __shared__ float ttiles[SRC ? BPW : 1][SRC ? A3 : 1];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
-  const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
-  const bool active = c.valid && g.lev[c.b] != FREE;
-  float* dt_ = dtiles[c.sub];
-  float* tt = SRC ? ttiles[c.sub] : nullptr;
-  float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
-  int3 p0 = make_int3(0, 0, 0);
-  int lb = 0;
-  if (active) {
-    lb = g.lev[c.b];
-    p0 = g.pos[c.b];
-    stageApron<BW, float>(g, g.den, dt_, c.b, c.t);
-    if (SRC) stageApron<BW, float>(g, g.tem, tt, c.b, c.t);
-    if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
-  }
-  __syncthreads();
-  if (!active) return;
-  const uint32_t cell = c.b * BV + c.t;
-  if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
-    g.tden[cell] = 0.f;
-    if (SRC) g.ttem[cell] = 0.f;
-    return;
-  }
-  float px, py, pz;
-  int x, y, z, level;
-  footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
-  float q = 0.f, tq = 0.f;
-  TileSample<BW, FLUIDITY> ts;
-  if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
-    if (!ts.zero) {
+  const uint32_t total = slotTotal(g, -1);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tiles are no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const bool active = c.valid && g.lev[c.b] != FREE;
+    float* dt_ = dtiles[c.sub];
+    float* tt = SRC ? ttiles[c.sub] : nullptr;
+    float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+    int3 p0 = make_int3(0, 0, 0);
+    int lb = 0;
+    if (active) {
+      lb = g.lev[c.b];
+      p0 = g.pos[c.b];
+      stageApron<BW, float>(g, g.den, dt_, c.b, c.t);
+      if (SRC) stageApron<BW, float>(g, g.tem, tt, c.b, c.t);
+      if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
+    }
+    __syncthreads();
+    if (!active) continue;
+    const uint32_t cell = c.b * BV + c.t;
+    if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
+      g.tden[cell] = 0.f;
+      if (SRC) g.ttem[cell] = 0.f;
+      continue;
+    }
+    float px, py, pz;
+    int x, y, z, level;
+    footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
+    float q = 0.f, tq = 0.f;
+    TileSample<BW, FLUIDITY> ts;
+    if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
+      if (!ts.zero) {
 #pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int cx = ts.x0 + ((k >> 2) & 1), cy = ts.y0 + ((k >> 1) & 1), cz = ts.z0 + (k & 1);
-        const float qk = densityBC(P, dt_[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
-        q = k == 0 ? qk * ts.w[0] : q + qk * ts.w[k];
-        if (SRC) {
-          const float tk = temperatureBC(P, tt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
-          tq = k == 0 ? tk * ts.w[0] : tq + tk * ts.w[k];
+        for (int k = 0; k < 8; k++) {
+          const int cx = ts.x0 + ((k >> 2) & 1), cy = ts.y0 + ((k >> 1) & 1), cz = ts.z0 + (k & 1);
+          const float qk = densityBC(P, dt_[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+          q = k == 0 ? qk * ts.w[0] : q + qk * ts.w[k];
+          if (SRC) {
+            const float tk = temperatureBC(P, tt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+            tq = k == 0 ? tk * ts.w[0] : tq + tk * ts.w[k];
+          }
+        }
+      }
+    } else {
+      CrossSample<BW, FLUIDITY> s;
+      s.init(g, px, py, pz);
+      if (!s.zero) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
+          const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
+          q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
+          if (SRC) {
+            const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
+            tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
+          }
         }
       }
     }
-  } else {
-    CrossSample<BW, FLUIDITY> s;
-    s.init(g, px, py, pz);
-    if (!s.zero) {
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
-        const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
-        q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
-        if (SRC) {
-          const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
-          tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
-        }
+    if (SRC) {
+      const float sc = (float)(1 << level);
+      if (inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
+        q = P.src_density;
+        tq = sourceTemperature(P, x, y, z, level);
       }
+      g.ttem[cell] = tq;
     }
+    g.tden[cell] = q;
   }
-  if (SRC) {
-    const float sc = (float)(1 << level);
-    if (inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
-      q = P.src_density;
-      tq = sourceTemperature(P, x, y, z, level);
-    }
-    g.ttem[cell] = tq;
-  }
-  g.tden[cell] = q;
 }
 
 // ------------------------------------------------------------------ residual (debugStats)

@@ -107,39 +107,43 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(DG g, dcg_params P
                 SBB = Geo<BW>::SBB;
   __shared__ float tiles[BPW][A3];
   __shared__ unsigned int smax[BPW][8];
-  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
-  const uint32_t b = blockIdx.x * BPW + sub;
-  const bool valid = b < g.B;
-  const int level = valid ? (int)g.lev[b] : FREE;
-  const bool active = valid && level != FREE;
-  float* tile = tiles[sub];
-  if (t < 8) smax[sub][t] = 0u;
-  if (active) {
-    tile[apronIndexOfLocal<BW>(t)] = g.den[b * BV + t];
-    for (int fc = t; fc < FACE; fc += BV) {
-      int ai, aj, ak;
-      faceCell<BW>(fc, ai, aj, ak);
-      tile[(ai * A + aj) * A + ak] = g.den[apronCell<BW>(g, b, ai, aj, ak)];
+  const uint32_t total = slotTotal(g, -1);
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's tile and maxima are no longer read
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const int sub = c.sub, t = c.t;
+    const uint32_t b = c.b;
+    const bool active = c.valid;
+    const int level = active ? (int)g.lev[b] : 0;
+    float* tile = tiles[sub];
+    if (t < 8) smax[sub][t] = 0u;
+    if (active) {
+      tile[apronIndexOfLocal<BW>(t)] = g.den[b * BV + t];
+      for (int fc = t; fc < FACE; fc += BV) {
+        int ai, aj, ak;
+        faceCell<BW>(fc, ai, aj, ak);
+        tile[(ai * A + aj) * A + ak] = g.den[apronCell<BW>(g, b, ai, aj, ak)];
+      }
     }
+    __syncthreads();
+    float cs = 0.f;
+    if (active) {
+      const float alpha = .5f * P.rdx / (float)(1 << level);
+      const int a = apronIndexOfLocal<BW>(t);
+      const float gxv = alpha * (tile[a + A * A] - tile[a - A * A]);
+      const float gyv = alpha * (tile[a + A] - tile[a - A]);
+      const float gzv = alpha * (tile[a + 1] - tile[a - 1]);
+      const float gm = __builtin_sqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
+      const float3 w = g.vort[b * BV + t];
+      const float wm = __builtin_sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+      const float fg = gm / P.grad_threshold, fw = wm / P.vort_threshold;
+      cs = fg > fw ? fg : fw;
+    }
+    // scores are >= 0, so their bit patterns order like their values
+    atomicMax(&smax[sub][t >> SBB], __float_as_uint(cs));
+    __syncthreads();
+    if (t < 8 && active) feat[8 * b + t] = __uint_as_float(smax[sub][t]);
   }
-  __syncthreads();
-  float cs = 0.f;
-  if (active) {
-    const float alpha = .5f * P.rdx / (float)(1 << level);
-    const int a = apronIndexOfLocal<BW>(t);
-    const float gxv = alpha * (tile[a + A * A] - tile[a - A * A]);
-    const float gyv = alpha * (tile[a + A] - tile[a - A]);
-    const float gzv = alpha * (tile[a + 1] - tile[a - 1]);
-    const float gm = __builtin_sqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-    const float3 w = g.vort[b * BV + t];
-    const float wm = __builtin_sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
-    const float fg = gm / P.grad_threshold, fw = wm / P.vort_threshold;
-    cs = fg > fw ? fg : fw;
-  }
-  // scores are >= 0, so their bit patterns order like their values
-  atomicMax(&smax[sub][t >> SBB], __float_as_uint(cs));
-  __syncthreads();
-  if (t < 8 && valid) feat[8 * b + t] = __uint_as_float(smax[sub][t]);
 }
 
 // Subblock scores from the feature values: -FLT_MAX where the reference's conditions exclude the
@@ -246,16 +250,23 @@ __global__ void __launch_bounds__(1024) k_select(DG g, int level, const float* _
   const uint32_t k = sK;
   if (k == 0) return;
   const uint32_t T = blockDim.x;
-  const uint32_t seg = (n + T - 1) / T;
-  const uint32_t lo = min(threadIdx.x * seg, n), hi = min(lo + seg, n);
   auto key32 = [&](uint32_t i) -> uint32_t {
     const float s = scores[base + i];
     return MODE == 0 ? blockKey(s) : subKey(s);
   };
-  if (k >= n) {
-    for (uint32_t i = lo; i < hi; i++) cand[i] = ((unsigned long long)key32(i) << 32) | i;
+  // Candidates past the allocated prefix of the slice are free blocks (or their subblocks):
+  // their score is -FLT_MAX, i.e. the largest key, and their positions follow every allocated
+  // one, so they rank after all allocated candidates. Select among the allocated prefix and
+  // append free candidates in position order when k exceeds it.
+  const uint32_t nA = min(n, MODE == 0 ? g.ctl[CTL_LOADS + level] : 8 * g.ctl[CTL_LOADS + level + 1]);
+  if (k >= nA) {
+    const uint32_t kFree = MODE == 0 ? blockKey(-FLT_MAX) : subKey(-FLT_MAX);
+    for (uint32_t i = threadIdx.x; i < k; i += T)
+      cand[i] = ((unsigned long long)(i < nA ? key32(i) : kFree) << 32) | i;
     return;
   }
+  const uint32_t seg = (nA + T - 1) / T;
+  const uint32_t lo = min(threadIdx.x * seg, nA), hi = min(lo + seg, nA);
   // Radix select on key32: 11 + 11 + 10 bits.
   uint32_t prefix = 0, mask = 0, rem = k;
   for (int pass = 0; pass < 3; pass++) {
@@ -440,7 +451,8 @@ __global__ void __launch_bounds__(1024) k_refine_select(DG g, dcg_params P, cons
   const int l = blockIdx.x + 1;
   const uint32_t loadLm1 = g.ctl[CTL_LOADS + l - 1], loadL = g.ctl[CTL_LOADS + l];
   const uint32_t lim = min(g.maxL[l - 1] - loadLm1, 8 * loadL - loadLm1);
-  const uint32_t n = 8 * g.maxL[l], base = 8 * g.offs[l];
+  // subblocks of free blocks score -FLT_MAX: only the allocated prefix of the slice can qualify
+  const uint32_t n = 8 * loadL, base = 8 * g.offs[l];
   const uint32_t T = blockDim.x;
   const uint32_t seg = (n + T - 1) / T;
   const uint32_t lo = min(threadIdx.x * seg, n), hi = min(lo + seg, n);

@@ -596,6 +596,7 @@ struct UniformSim : SimBase {
   }
   int tapeForward(int steps, int K) override {
     if (steps < 1 || K < 1) return fail(DCG_E_ARG, "steps and checkpoint interval must be positive");
+    if (P.project_mode == DCG_PROJECT_SOLVE) return fail(DCG_E_UNSUPPORTED, "the adjoint follows the Jacobi schedules");
     if (K > steps) K = steps;
     const int ck = (steps + K - 1) / K + 1;  // states 0, K, 2K, ... below steps, and the final one
     if (ck != tapeCk || K + 1 != tapeSeg) {

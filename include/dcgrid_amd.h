@@ -60,8 +60,15 @@ enum {
   DCG_NUM_FIELDS = 8
 };
 
-/* Projection schedule used by dcg_step. */
-enum { DCG_PROJECT_MULTILEVEL = 0, DCG_PROJECT_LOCAL = 1 };
+/* Projection schedule used by dcg_step. DCG_PROJECT_SOLVE (extension) solves, level by level
+ * from the coarsest, the fixed point of the Jacobi sweeps of DCG_PROJECT_MULTILEVEL to the
+ * relative residual solve_tol (dcg_project_solve). */
+enum { DCG_PROJECT_MULTILEVEL = 0, DCG_PROJECT_LOCAL = 1, DCG_PROJECT_SOLVE = 2 };
+
+/* Arithmetic of DCG_PROJECT_SOLVE: FP64 residual / norms / correction update around FP32
+ * multigrid cycles with red-black Gauss-Seidel smoothing (MIXED), or everything in FP32 or
+ * in FP64. */
+enum { DCG_SOLVE_MIXED = 0, DCG_SOLVE_FP32 = 1, DCG_SOLVE_FP64 = 2 };
 
 /* Topology score used by dcg_adapt_topology. */
 enum {
@@ -115,8 +122,25 @@ typedef struct dcg_params {
   float init_velocity_noise; /* amplitude of the seeded 3-octave value-noise initial velocity */
   float noise_period;        /* lattice period of the first octave, finest cells */
   uint32_t seed;
-  int32_t reserved[8];
+
+  /* DCG_PROJECT_SOLVE (extension, DCGrid). */
+  int32_t solve_precision;  /* DCG_SOLVE_* */
+  float solve_tol;          /* per-level relative residual |r| / |r(0)| to reach (1e-8) */
+  int32_t solve_max_cycles; /* refinement cycles per level before giving up (50) */
+  int32_t solve_smooth;     /* red-black Gauss-Seidel sweeps before and after each coarse
+                               correction (2; at most 10, so <= 20 per cycle and grid) */
+  int32_t reserved[4];
 } dcg_params;
+
+/* Outcome of the last DCG_PROJECT_SOLVE projection, per level (0 finest). */
+typedef struct dcg_solve_info {
+  int32_t levels;
+  int32_t cycles[DCG_MAX_LEVELS];          /* refinement cycles run on the level */
+  double rel_residual[DCG_MAX_LEVELS];     /* |r| / |r(0)| reached */
+  double initial_residual[DCG_MAX_LEVELS]; /* |r(0)|: residual of a zero level pressure */
+  int32_t singular[DCG_MAX_LEVELS];        /* level covers the domain: pure Neumann, zero-mean */
+  int32_t converged;                       /* every level reached solve_tol */
+} dcg_solve_info;
 
 /* Shape/state summary of a simulation. Per-level arrays are indexed by level (0 finest). */
 typedef struct dcg_info {
@@ -188,6 +212,10 @@ int dcg_advect_velocity(dcg_sim* s);
 int dcg_project(dcg_sim* s);
 int dcg_project_local(dcg_sim* s);
 int dcg_advect_density(dcg_sim* s);
+/* Extension: the projection with the level systems solved to solve_tol (DCG_PROJECT_SOLVE),
+ * and the report of the last such solve (synchronises). DCGrid only. */
+int dcg_project_solve(dcg_sim* s);
+int dcg_solve_report(dcg_sim* s, dcg_solve_info* out);
 /* n steps in the order of src/simulation.cpp:104-111:
  * advectVelocity -> adaptTopology -> project (or projectLocal) -> advectDensity. */
 int dcg_step(dcg_sim* s, int32_t n);

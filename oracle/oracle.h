@@ -34,6 +34,8 @@ int orc_advect_velocity(orc_sim* s);
 int orc_project(orc_sim* s);
 int orc_project_local(orc_sim* s);
 int orc_advect_density(orc_sim* s);
+int orc_project_solve(orc_sim* s);
+int orc_solve_report(orc_sim* s, dcg_solve_info* out);
 int orc_step(orc_sim* s, int32_t n);
 int orc_debug_stats(orc_sim* s, double* out);
 int orc_seed_scene(orc_sim* s);

@@ -80,6 +80,15 @@ int orc_project_local(orc_sim* s) {
   IMPL(s);
   return I->projectLocal();
 }
+int orc_project_solve(orc_sim* s) {
+  IMPL(s);
+  return I->projectSolve();
+}
+int orc_solve_report(orc_sim* s, dcg_solve_info* out) {
+  IMPL(s);
+  if (!out) return DCG_E_ARG;
+  return I->solveReport(out);
+}
 int orc_advect_density(orc_sim* s) {
   IMPL(s);
   return I->advectDensity();
@@ -91,7 +100,7 @@ int orc_step(orc_sim* s, int32_t n) {
     int e;
     if ((e = I->advectVelocity())) return e;
     if ((e = I->adaptTopology())) return e;
-    if ((e = (I->P.project_mode == DCG_PROJECT_LOCAL ? I->projectLocal() : I->project()))) return e;
+    if ((e = I->projectByMode())) return e;
     if ((e = I->advectDensity())) return e;
   }
   return DCG_OK;

@@ -228,6 +228,16 @@ struct Sim {
   // getBlockIndexDeep (DCGrid only): finest allocated block at level >= `level` containing the
   // level-local cell (x,y,z); -1 when not applicable.
   virtual int64_t lookup(int x, int y, int z, int level) { return -1; }
+  // Projection with the level systems solved to solve_tol (extension; DCGrid only).
+  virtual int projectSolve() { return fail(DCG_E_UNSUPPORTED, "the projection solve is implemented for the DCGrid"); }
+  virtual int solveReport(dcg_solve_info* out) {
+    return fail(DCG_E_UNSUPPORTED, "the projection solve is implemented for the DCGrid");
+  }
+  int projectByMode() {
+    if (P.project_mode == DCG_PROJECT_LOCAL) return projectLocal();
+    if (P.project_mode == DCG_PROJECT_SOLVE) return projectSolve();
+    return project();
+  }
   // Differentiable rollout (extension; uniform grid only): record n steps, then pull the
   // adjoint of the final state back to the recorded initial state. Null temperature pointers
   // mean a zero seed / no output.

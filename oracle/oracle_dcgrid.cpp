@@ -975,6 +975,110 @@ struct DCGrid : Sim {
     accumulate(vel, V3<R>{0, 0, 0});
     return DCG_OK;
   }
+  // ---------------------------------------------------------------- projection solve (extension)
+  // The fixed point of the Jacobi sweeps of project(), level by level from the coarsest:
+  //   r_c = (sum of the 6 apron pressures) - 6 p_c - alpha div_c = 0   on every cell of the level,
+  // with apron entries on coarser levels held at their solved values. A level whose loads equal
+  // its full count covers the domain and sees no coarser level: its system is pure Neumann
+  // (singular), so the right-hand side is projected to zero mean and the solution normalised to
+  // zero mean. Solved here by lexicographic SOR (omega 1.8) until |r| <= tol |r(0)| (tol 1e-12 in
+  // the FP64 instance, the reference solution); slow but independent of the GPU's multigrid.
+  dcg_solve_info rep{};
+  bool levelSingular(int l) const { return loads[l] == fullL[l]; }
+  template <class F>
+  void forLevelCells(int level, F f) {
+    for (size_t b = offs[level]; b < offs[level] + maxL[level]; b++)
+      if (lev[b] == level)
+        for (int l = 0; l < BV; l++) f(b, b * BV + l);
+  }
+  // Residual norm of the level with the (projected) right-hand side alpha*div - shift.
+  double levelResidualNorm(int level, R alpha, R shift, bool zeroLevel) {
+    double s = 0;
+    forLevelCells(level, [&](size_t b, size_t c) {
+      const size_t* ci = &cidx[b * A3];
+      const int a = apronOf(c);
+      const size_t nb[6] = {ci[a - A2], ci[a + A2], ci[a - A], ci[a + A], ci[a - 1], ci[a + 1]};
+      R sum = 0;
+      for (int k = 0; k < 6; k++) {
+        const bool unknown = lev[nb[k] / BV] == level;
+        sum += (zeroLevel && unknown) ? (R)0 : pre[nb[k]];
+      }
+      const R r = sum - (zeroLevel ? (R)0 : (R)6 * pre[c]) - (alpha * dive[c] - shift);
+      s += (double)r * (double)r;
+    });
+    return std::sqrt(s);
+  }
+  void solveLevel(int level) {
+    const int scale = 1 << level;
+    const R alpha = (R)(scale * scale) * (R)P.dx * (R)P.dx;
+    const bool singular = levelSingular(level);
+    R shift = 0;
+    size_t n = 0;
+    if (singular) {
+      double m = 0;
+      forLevelCells(level, [&](size_t, size_t c) {
+        m += (double)(alpha * dive[c]);
+        n++;
+      });
+      shift = (R)(m / (double)n);
+    }
+    const double tol = sizeof(R) == 8 ? 1e-12 : std::max((double)P.solve_tol, 1e-6);
+    const double r0 = levelResidualNorm(level, alpha, shift, true);
+    double rr = levelResidualNorm(level, alpha, shift, false);
+    const R omega = (R)1.8;
+    int sweeps = 0;
+    const int maxSweeps = 200000;
+    while (rr > tol * r0 && sweeps < maxSweeps) {
+      for (int it = 0; it < 10; it++, sweeps++)
+        forLevelCells(level, [&](size_t b, size_t c) {
+          const size_t* ci = &cidx[b * A3];
+          const int a = apronOf(c);
+          const size_t nb[6] = {ci[a - A2], ci[a + A2], ci[a - A], ci[a + A], ci[a - 1], ci[a + 1]};
+          R sum = 0;
+          int self = 0;
+          for (int k = 0; k < 6; k++) {
+            if (nb[k] == c)
+              self++;
+            else
+              sum += pre[nb[k]];
+          }
+          const R gs = (sum - (alpha * dive[c] - shift)) / (R)(6 - self);
+          pre[c] = pre[c] + omega * (gs - pre[c]);
+        });
+      rr = levelResidualNorm(level, alpha, shift, false);
+    }
+    if (singular) {
+      double m = 0;
+      forLevelCells(level, [&](size_t, size_t c) { m += (double)pre[c]; });
+      const R mean = (R)(m / (double)n);
+      forLevelCells(level, [&](size_t, size_t c) { pre[c] = pre[c] - mean; });
+      rr = levelResidualNorm(level, alpha, shift, false);
+    }
+    rep.cycles[level] = sweeps;
+    rep.rel_residual[level] = r0 > 0 ? rr / r0 : 0.0;
+    rep.initial_residual[level] = r0;
+    rep.singular[level] = singular ? 1 : 0;
+    if (!(rr <= tol * r0)) rep.converged = 0;
+  }
+  int projectSolve() override {
+    calcDivergence();
+    accumulate(dive, (R)0);
+    rep = dcg_solve_info{};
+    rep.levels = L;
+    rep.converged = 1;
+    for (int level = L - 1; level >= 0; level--) {
+      if (level < L - 1) prolongate(level);  // initial guess from the solved coarser level
+      solveLevel(level);
+    }
+    applyPressure();
+    accumulate(vel, V3<R>{0, 0, 0});
+    return DCG_OK;
+  }
+  int solveReport(dcg_solve_info* out) override {
+    *out = rep;
+    return DCG_OK;
+  }
+
   // k_dcgrid_debug_stats + host sum, dcgrid_structure.cu:224-251, fluid_simulation_dcgrid.cu:517-528
   int debugStats(double* out) override {
     std::vector<R> stats(B, (R)0);

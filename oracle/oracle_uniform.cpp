@@ -359,6 +359,8 @@ struct Uniform : Sim {
       adaptTopology();
       if (P.project_mode == DCG_PROJECT_LOCAL)
         projectLocal();
+      else if (P.project_mode == DCG_PROJECT_SOLVE)
+        return fail(DCG_E_UNSUPPORTED, "the adjoint follows the Jacobi schedules");
       else
         project();
       advectDensity();

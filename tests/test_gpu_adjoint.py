@@ -69,9 +69,9 @@ def gpu_gradient(w, steps, k, den=None, vel=None, seed_scene=False, gd=None, gv=
     return o_d.cpu().numpy(), o_v.cpu().numpy(), o_t.cpu().numpy(), fin, after
 
 
-def oracle_gradient(w, steps, den=None, vel=None, seed_scene=False, gd=None, gv=None, gt=None):
+def oracle_gradient(w, steps, den=None, vel=None, seed_scene=False, gd=None, gv=None, gt=None, precision=64):
     import oracle
-    s = oracle.OracleSim("uniform", w.size, w.params, precision=64)
+    s = oracle.OracleSim("uniform", w.size, w.params, precision=precision)
     if seed_scene:
         s.seed_scene()
     if den is not None:
@@ -87,35 +87,44 @@ def oracle_gradient(w, steps, den=None, vel=None, seed_scene=False, gd=None, gv=
     return out
 
 
+# The FP32 rollout is bit-identical to the oracle's FP32 instance, so the adjoints are compared
+# with it at 1e-5 (only summation order differs: gathers here, scatters there). Against the FP64
+# instance the gradient also carries the FP32 forward's own rounding (the FP32 oracle shows the
+# same 3.2e-4 velocity-gradient distance on the control case), hence 1e-3 there.
+TOL_FP32, TOL_FP64 = 1e-5, 1e-3
+
+
 def test_control32_vs_oracle_adjoint():
     """L = 0.5 |rho_8 - target|^2 on the 32^3 control variant; checkpoints every 3 steps (a
-    partial last segment): both gradients within 1e-4 relative L2 of the FP64 oracle."""
+    partial last segment)."""
     w, den, vel, tgt = control_case(32)
     say("[case] control32 K=3")
     seed = lambda fin: fin[0] - tgt  # noqa: E731
     gd, gv, _, _, _ = gpu_gradient(w, 8, 3, den, vel, gd=seed)
-    od, ov, _ = oracle_gradient(w, 8, den, vel, gd=seed)
-    say("rel l2 density", rel_l2(gd, od), "velocity", rel_l2(gv, ov))
-    assert rel_l2(gd, od) <= 1e-4
-    assert rel_l2(gv, ov) <= 1e-4
+    for prec, tol in ((32, TOL_FP32), (64, TOL_FP64)):
+        od, ov, _ = oracle_gradient(w, 8, den, vel, gd=seed, precision=prec)
+        say(f"fp{prec} rel l2 density", rel_l2(gd, od), "velocity", rel_l2(gv, ov))
+        assert rel_l2(gd, od) <= tol
+        assert rel_l2(gv, ov) <= tol
 
 
 @pytest.mark.parametrize("solids", [False, True])
 def test_plume_with_sources_vs_oracle_adjoint(solids):
     """Buoyant plume with the emitting sphere (temperature, buoyancy, source masking), with and
     without solids (fluidity-weighted sampling, divergence and gradient): random seeds on all
-    three final fields; every gradient within 1e-4 relative L2 of the FP64 oracle."""
+    three final fields."""
     w = plume_case(16, solids)
     say(f"[case] plume16 solids={solids}")
     rng = np.random.Generator(np.random.PCG64(5))
     N = 16 ** 3
     gd0, gv0, gt0 = rng.normal(size=N), rng.normal(size=(N, 3)), rng.normal(size=N)
     gd, gv, gt, _, _ = gpu_gradient(w, 3, 2, seed_scene=True, gd=gd0, gv=gv0, gt=gt0)
-    od, ov, ot = oracle_gradient(w, 3, seed_scene=True, gd=gd0, gv=gv0, gt=gt0)
-    say("rel l2", rel_l2(gd, od), rel_l2(gv, ov), rel_l2(gt, ot))
-    assert rel_l2(gd, od) <= 1e-4
-    assert rel_l2(gv, ov) <= 1e-4
-    assert rel_l2(gt, ot) <= 1e-4
+    for prec, tol in ((32, TOL_FP32), (64, TOL_FP64)):
+        od, ov, ot = oracle_gradient(w, 3, seed_scene=True, gd=gd0, gv=gv0, gt=gt0, precision=prec)
+        say(f"fp{prec} rel l2", rel_l2(gd, od), rel_l2(gv, ov), rel_l2(gt, ot))
+        assert rel_l2(gd, od) <= tol
+        assert rel_l2(gv, ov) <= tol
+        assert rel_l2(gt, ot) <= tol
 
 
 def test_control32_vs_fp64_finite_differences():
