@@ -35,6 +35,7 @@ FIELD_T_PRESSURE = 4
 FIELD_DIVERGENCE = 5
 FIELD_TEMPERATURE = 6
 FIELD_VORTICITY = 7
+FIELD_VAPOR = 8
 FIELDS = {
     "density": FIELD_DENSITY,
     "velocity": FIELD_VELOCITY,
@@ -44,12 +45,14 @@ FIELDS = {
     "divergence": FIELD_DIVERGENCE,
     "temperature": FIELD_TEMPERATURE,
     "vorticity": FIELD_VORTICITY,
+    "vapor": FIELD_VAPOR,
 }
 
 PROJECT_MULTILEVEL, PROJECT_LOCAL, PROJECT_SOLVE = 0, 1, 2
 SOLVE_MIXED, SOLVE_FP32, SOLVE_FP64 = 0, 1, 2
 SCORE_GEOMETRIC, SCORE_FEATURES = 0, 1
 SOURCE_NONE, SOURCE_SPHERE_BUOYANT = 0, 1
+SOLID_SPHERE, SOLID_TERRAIN = 0, 1
 
 STAGE_DIVERGENCE = 0
 STAGE_RESTRICT = 1
@@ -103,7 +106,19 @@ class Params(C.Structure):
         ("solve_tol", C.c_float),
         ("solve_max_cycles", C.c_int32),
         ("solve_smooth", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("solid_mode", C.c_int32),
+        ("terrain_height", C.c_float),
+        ("terrain_period", C.c_float),
+        ("moist", C.c_int32),
+        ("lapse_rate", C.c_float),
+        ("qsat_scale", C.c_float),
+        ("humidity", C.c_float),
+        ("moist_layer", C.c_float),
+        ("cond_rate", C.c_float),
+        ("latent_heat", C.c_float),
+        ("wind", C.c_float * 3),
+        ("adiabatic_rate", C.c_float),
+        ("reserved", C.c_int32 * 3),
     ]
 
     def copy(self):

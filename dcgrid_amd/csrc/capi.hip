@@ -172,8 +172,8 @@ int dcg_set_params(dcg_sim* s, const dcg_params* p) {
   IMPL(s);
   if (!p) return DCG_E_ARG;
   if (p->gx != I->P.gx || p->gy != I->P.gy || p->gz != I->P.gz || p->block_width != I->P.block_width ||
-      p->max_levels != I->P.max_levels)
-    return I->fail(DCG_E_ARG, "grid size, block width and level cap are fixed at creation");
+      p->max_levels != I->P.max_levels || p->moist != I->P.moist)
+    return I->fail(DCG_E_ARG, "grid size, block width, level cap and moist atmosphere are fixed at creation");
   I->P = *p;
   return DCG_OK;
 }

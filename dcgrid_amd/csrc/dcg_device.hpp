@@ -50,9 +50,37 @@ __device__ __forceinline__ bool inletHit(const dcg_params& P, int x, int y, int 
 __device__ __forceinline__ bool outside(const dcg_params& P, int x, int y, int z, int scale) {
   return x < 0 || y < 0 || z < 0 || x * scale >= P.gx || y * scale >= P.gy || z * scale >= P.gz;
 }
+// Deterministic exp: Cody-Waite reduction by ln 2 (split in a float-exact high part and a low
+// part) and the degree-7 Taylor polynomial of exp on |r| <= ln2/2, scaled by 2^n. The same
+// float operations run on the host, so both produce the same bits.
+__device__ __forceinline__ float expDet(float x) {
+  const float n = floorf(x * 1.44269504f + .5f);
+  const float r = (x - n * 0.693145752f) - n * 1.42860677e-06f;
+  const float p =
+      1.f + r * (1.f + r * (.5f + r * (1.f / 6.f + r * (1.f / 24.f + r * (1.f / 120.f + r * (1.f / 720.f + r * (1.f / 5040.f)))))));
+  return ldexpf(p, (int)n);
+}
+// Moist atmosphere (extension): ambient temperature at height hy (finest cells), saturation
+// vapor (Magnus form), ambient vapor profile.
+__device__ __forceinline__ float ambientT(const dcg_params& P, float hy) {
+  return P.moist ? P.t_ambient - P.lapse_rate * hy : P.t_ambient;
+}
+__device__ __forceinline__ float qsat(const dcg_params& P, float T) {
+  return P.qsat_scale * expDet(17.27f * T / (T + 237.3f));
+}
+__device__ __forceinline__ float ambientVapor(const dcg_params& P, float hy) {
+  return hy < P.moist_layer ? P.humidity * qsat(P, ambientT(P, hy)) : 0.f;
+}
+// Height (finest cells) of the centre of level-local cell row y.
+__device__ __forceinline__ float cellHeight(int y, int scale) { return ((float)y + .5f) * (float)scale; }
+
 __device__ __forceinline__ float3 velocityBC(const dcg_params& P, float3 v, int x, int y, int z, int scale) {
   if (inletHit(P, x, y, z, scale)) return make_float3(0.f, P.velocity_emission_rate, 0.f);
-  if (outside(P, x, y, z, scale)) return make_float3(0.f, 0.f, 0.f);
+  if (outside(P, x, y, z, scale)) {
+    // lateral boundaries carry the ambient wind (zero unless configured); floor and lid are walls
+    if (y >= 0 && y * scale < P.gy) return make_float3(P.wind[0], P.wind[1], P.wind[2]);
+    return make_float3(0.f, 0.f, 0.f);
+  }
   return v;
 }
 __device__ __forceinline__ float densityBC(const dcg_params& P, float q, int x, int y, int z, int scale) {
@@ -62,20 +90,43 @@ __device__ __forceinline__ float densityBC(const dcg_params& P, float q, int x, 
 }
 __device__ __forceinline__ float temperatureBC(const dcg_params& P, float q, int x, int y, int z, int scale) {
   if (inletHit(P, x, y, z, scale)) return P.t_ambient;
-  if (outside(P, x, y, z, scale)) return P.t_ambient;
+  if (outside(P, x, y, z, scale)) return ambientT(P, cellHeight(y, scale));
   return q;
 }
-// getCellFluidity (src/utils/sim_utils.cu:11-22) with the sphere of src/sdf.cuh:8-18.
+__device__ __forceinline__ float vaporBC(const dcg_params& P, float q, int x, int y, int z, int scale) {
+  if (outside(P, x, y, z, scale)) return ambientVapor(P, cellHeight(y, scale));
+  return q;
+}
+// Saturation adjustment of one cell: condense a fraction of the supersaturation (or evaporate
+// cloud water into an undersaturated cell), with latent heating.
+__device__ __forceinline__ void saturationAdjust(const dcg_params& P, float& qv, float& qc, float& T) {
+  const float ex = qv - qsat(P, T);
+  float c = P.cond_rate * ex;
+  if (c < -qc) c = -qc;
+  qv = qv - c;
+  qc = qc + c;
+  T = T + P.latent_heat * c;
+}
+
+// getCellFluidity (src/utils/sim_utils.cu:11-22) with the sphere of src/sdf.cuh:8-18, or with a
+// terrain heightfield (extension): signed distance = height of the cell centre above the
+// terrain.
+__device__ __forceinline__ float terrainHeight(const dcg_params& P, float x, float z);
 __device__ __forceinline__ float cellFluidity(const dcg_params& P, int x, int y, int z, int scale) {
   const float sqrt3 = 1.73205f;
   if (!P.enable_additional_solids) return 1.f;
   const float px = ((float)x + .5f) * (float)scale;
   const float py = ((float)y + .5f) * (float)scale;
   const float pz = ((float)z + .5f) * (float)scale;
-  const float cx = (float)P.gx * .5f, cy = (float)P.gy * .45f, cz = (float)P.gz * .5f;
-  const float r = 2000.f * P.rdx;
-  const float dx = px - cx, dy = py - cy, dz = pz - cz;
-  const float d = __builtin_sqrtf(dx * dx + dy * dy + dz * dz) - r;
+  float d;
+  if (P.solid_mode == DCG_SOLID_TERRAIN) {
+    d = py - terrainHeight(P, px, pz);
+  } else {
+    const float cx = (float)P.gx * .5f, cy = (float)P.gy * .45f, cz = (float)P.gz * .5f;
+    const float r = 2000.f * P.rdx;
+    const float dx = px - cx, dy = py - cy, dz = pz - cz;
+    d = __builtin_sqrtf(dx * dx + dy * dy + dz * dz) - r;
+  }
   const float overlap = clampf(.5f - d / ((float)scale * sqrt3), 0.f, 1.f);
   return 1.f - overlap;
 }
@@ -125,6 +176,17 @@ __device__ __forceinline__ float fbm3(float x, float y, float z, float period, u
   }
   return n / 1.75f;
 }
+// Terrain heightfield (finest cells): 5-octave fBm of value noise mapped to [0, terrain_height].
+__device__ __forceinline__ float terrainHeight(const dcg_params& P, float x, float z) {
+  float n = 0.f, amp = 1.f, freq = 1.f / P.terrain_period;
+  const uint32_t seed = P.seed ^ 0x3c6ef372u;
+  for (int o = 0; o < 5; o++) {
+    n = n + amp * valueNoise(x * freq, 0.f, z * freq, seed + 7919u * (uint32_t)o);
+    amp = amp * .5f;
+    freq = freq * 2.f;
+  }
+  return P.terrain_height * (.5f + .5f * (n / 1.9375f));
+}
 __device__ __forceinline__ float3 initVelocity(const dcg_params& P, float cx, float cy, float cz) {
   float u[3];
 #pragma unroll
@@ -136,7 +198,7 @@ __device__ __forceinline__ float3 initVelocity(const dcg_params& P, float cx, fl
     const uint32_t s = P.seed ^ (0x51ed270bu * (uint32_t)(c + 1));
     u[c] = P.init_velocity_noise * fbm3(cx, cy, cz, P.noise_period, s);
   }
-  return make_float3(u[0], u[1], u[2]);
+  return make_float3(u[0] + P.wind[0], u[1] + P.wind[1], u[2] + P.wind[2]);
 }
 __device__ __forceinline__ bool inSource(const dcg_params& P, float cx, float cy, float cz) {
   const float dx = cx - P.src_center[0], dy = cy - P.src_center[1], dz = cz - P.src_center[2];

@@ -46,6 +46,8 @@ struct DCGridSim : SimBase {
 
   int kind() const override { return DCG_KIND_DCGRID; }
   bool fluidityOn() const { return P.enable_additional_solids || customFluidity; }
+  // temperature is advected and drives buoyancy: sphere source or moist atmosphere
+  bool thermo() const { return P.source_mode == DCG_SOURCE_SPHERE_BUOYANT || P.moist; }
 
   // Planner: fluid_simulation_dcgrid.cu:9-140.
   int create(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p) {
@@ -136,6 +138,10 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&g.vel, C));
     DCG_TRY(alloc(&g.tvel, C));
     DCG_TRY(alloc(&g.vort, C));
+    if (P.moist) {  // the vapor field exists only for a moist atmosphere (fixed at creation)
+      DCG_TRY(alloc(&g.vap, C));
+      DCG_TRY(alloc(&g.tvap, C));
+    }
     DCG_TRY(alloc(&sbs, (size_t)8 * B + 1));
     DCG_TRY(alloc(&bs, B));
     DCG_TRY(alloc(&feat, (size_t)8 * B));
@@ -170,6 +176,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(fill32(g.pre, 0, C));
     DCG_TRY(fill32(g.tpre, 0, C));
     DCG_TRY(fill32(g.vort, 0, 3 * C));
+    if (g.vap) DCG_TRY(fill32(g.vap, 0, C));
     DCG_TRY(fill32(g.ctl, 0, CTL_WORDS));
     DCG_TRY(fill32(cellStepCounter, 0, 2));
     uint32_t loads[DCG_MAX_LEVELS] = {};
@@ -242,7 +249,7 @@ struct DCGridSim : SimBase {
     }
     // apron refresh and value propagation (:336-345)
     DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refresh", (k_refresh<BW><<<blocksFor((size_t)27 * B, 256), 256, 0, stream>>>(g)));
-    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const bool src = thermo();
     for (int l = L - 2; l >= 0; l--) {
       const unsigned n = std::min(blocksFor(maxL[l], G::BPW), 256u);  // grid-stride over the touched blocks
       if (src)
@@ -265,7 +272,7 @@ struct DCGridSim : SimBase {
   // advectVelocity(), fluid_simulation_dcgrid.cu:263-268
   int advectVelocity() override {
     const unsigned nWG = slotGrid<BW>(B);
-    const bool buoy = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT, fl = fluidityOn();
+    const bool buoy = thermo(), fl = fluidityOn();
 #define ADV_V(F, Bu) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_advect_velocity", (k_advect_velocity<BW, F, Bu><<<nWG, G::WG, 0, stream>>>(g, P)))
     if (fl && buoy) ADV_V(true, true);
     else if (fl) ADV_V(true, false);
@@ -278,18 +285,25 @@ struct DCGridSim : SimBase {
   // advectDensity(), fluid_simulation_dcgrid.cu:313-318
   int advectDensity() override {
     const unsigned nWG = slotGrid<BW>(B);
-    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT, fl = fluidityOn();
-#define ADV_D(F, Sr) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_advect_density", (k_advect_density<BW, F, Sr><<<nWG, G::WG, 0, stream>>>(g, P)))
-    if (fl && src) ADV_D(true, true);
-    else if (fl) ADV_D(true, false);
-    else if (src) ADV_D(false, true);
-    else ADV_D(false, false);
+    const bool src = thermo(), moist = P.moist != 0, fl = fluidityOn();
+#define ADV_D(F, Sr, M) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_advect_density", (k_advect_density<BW, F, Sr, M><<<nWG, G::WG, 0, stream>>>(g, P)))
+    if (moist) {
+      if (fl) ADV_D(true, true, true);
+      else ADV_D(false, true, true);
+    } else if (fl && src) ADV_D(true, true, false);
+    else if (fl) ADV_D(true, false, false);
+    else if (src) ADV_D(false, true, false);
+    else ADV_D(false, false, false);
 #undef ADV_D
     std::swap(g.den, g.tden);
     DCG_TRY(restrictAll(g.den));
     if (src) {
       std::swap(g.tem, g.ttem);
       DCG_TRY(restrictAll(g.tem));
+    }
+    if (moist) {
+      std::swap(g.vap, g.tvap);
+      DCG_TRY(restrictAll(g.vap));
     }
     return DCG_OK;
   }
@@ -512,7 +526,7 @@ struct DCGridSim : SimBase {
         DCG_TRY(smoothU<T>(j, nu));
       }
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_top",
-                 (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, pyr[0])));
+                 (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, uactBuf, pyr[0])));
       DCG_TRY(smooth<T>(top, lmin, nu));
     } else {
       DCG_TRY(smooth<T>(top, lmin, 20 * nu));
@@ -642,6 +656,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(restrictAll(g.vel));
     DCG_TRY(restrictAll(g.den));
     DCG_TRY(restrictAll(g.tem));
+    if (g.vap) DCG_TRY(restrictAll(g.vap));
     return DCG_OK;
   }
 
@@ -682,6 +697,7 @@ struct DCGridSim : SimBase {
   }
   int fieldSize(int f, uint64_t* n) override {
     if (f < 0 || f >= DCG_NUM_FIELDS) return fail(DCG_E_ARG, "unknown field");
+    if (f == DCG_FIELD_VAPOR && !g.vap) return fail(DCG_E_ARG, "no vapor field (moist atmosphere off)");
     *n = (f == DCG_FIELD_VELOCITY || f == DCG_FIELD_VORTICITY) ? 3 * C : C;
     return DCG_OK;
   }
@@ -695,6 +711,7 @@ struct DCGridSim : SimBase {
       case DCG_FIELD_DIVERGENCE: return g.div;
       case DCG_FIELD_TEMPERATURE: return g.tem;
       case DCG_FIELD_VORTICITY: return (float*)g.vort;
+      case DCG_FIELD_VAPOR: return g.vap;
       default: return nullptr;
     }
   }

@@ -52,6 +52,8 @@ struct DG {
   float* tpre;
   float* tden;
   float* ttem;
+  float* vap;   // water vapor (moist atmosphere; allocated when enabled at creation)
+  float* tvap;
   float3* vel;
   float3* tvel;
   float3* vort;

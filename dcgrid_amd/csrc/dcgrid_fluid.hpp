@@ -360,6 +360,11 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_propagate(DG g, dcg_params P, i
       const float* tt = g.tem;
       g.tem[cell] =
           w27 * tt[q[0]] + w9 * (tt[q[1]] + tt[q[2]] + tt[q[3]]) + w3 * (tt[q[4]] + tt[q[5]] + tt[q[6]]) + w1 * tt[q[7]];
+      if (P.moist) {
+        const float* vv = g.vap;
+        g.vap[cell] =
+            w27 * vv[q[0]] + w9 * (vv[q[1]] + vv[q[2]] + vv[q[3]]) + w3 * (vv[q[4]] + vv[q[5]] + vv[q[6]]) + w1 * vv[q[7]];
+      }
     }
     const float3* v = g.vel;
     const float3 a0 = f3mul(v[q[0]], w27);
@@ -580,7 +585,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_param
       }
     }
     if (BUOY) {
-      const float bf = P.dt * (P.buoy_beta * (g.tem[cell] - P.t_ambient) - P.buoy_alpha * g.den[cell]);
+      const float ta = ambientT(P, cellHeight(y, 1 << level));
+      const float bf = P.dt * (P.buoy_beta * (g.tem[cell] - ta) - P.buoy_alpha * g.den[cell]);
       acc.y = acc.y + bf;
     }
     g.tvel[cell] = acc;
@@ -589,12 +595,16 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_param
 
 // k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
 // with the same LDS fast path as the velocity.
-template <int BW, bool FLUIDITY, bool SRC>
+template <int BW, bool FLUIDITY, bool SRC, bool MOIST>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params P) {
+  // SRC: temperature is advected (sphere source or moist atmosphere); MOIST: vapor too, then the
+  // saturation adjustment of (vapor, cloud water = density, temperature).
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float dtiles[BPW][A3];
   __shared__ float ttiles[SRC ? BPW : 1][SRC ? A3 : 1];
+  __shared__ float vtiles[MOIST ? BPW : 1][MOIST ? A3 : 1];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
+  const bool sphere = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
@@ -602,6 +612,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
     const bool active = c.valid && g.lev[c.b] != FREE;
     float* dt_ = dtiles[c.sub];
     float* tt = SRC ? ttiles[c.sub] : nullptr;
+    float* vt = MOIST ? vtiles[c.sub] : nullptr;
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     int3 p0 = make_int3(0, 0, 0);
     int lb = 0;
@@ -610,6 +621,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
       p0 = g.pos[c.b];
       stageApron<BW, float>(g, g.den, dt_, c.b, c.t);
       if (SRC) stageApron<BW, float>(g, g.tem, tt, c.b, c.t);
+      if (MOIST) stageApron<BW, float>(g, g.vap, vt, c.b, c.t);
       if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
     }
     __syncthreads();
@@ -618,12 +630,13 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
     if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
       g.tden[cell] = 0.f;
       if (SRC) g.ttem[cell] = 0.f;
+      if (MOIST) g.tvap[cell] = 0.f;
       continue;
     }
     float px, py, pz;
     int x, y, z, level;
     footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
-    float q = 0.f, tq = 0.f;
+    float q = 0.f, tq = 0.f, vq = 0.f;
     TileSample<BW, FLUIDITY> ts;
     if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
       if (!ts.zero) {
@@ -635,6 +648,10 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
           if (SRC) {
             const float tk = temperatureBC(P, tt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
             tq = k == 0 ? tk * ts.w[0] : tq + tk * ts.w[k];
+          }
+          if (MOIST) {
+            const float vk = vaporBC(P, vt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+            vq = k == 0 ? vk * ts.w[0] : vq + vk * ts.w[k];
           }
         }
       }
@@ -651,14 +668,23 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
             const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
             tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
           }
+          if (MOIST) {
+            const float vk = vaporBC(P, g.vap[s.i[k]], cx, cy, cz, s.scale);
+            vq = k == 0 ? vk * s.w[0] : vq + vk * s.w[k];
+          }
         }
       }
     }
     if (SRC) {
       const float sc = (float)(1 << level);
-      if (inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
+      if (sphere && inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
         q = P.src_density;
         tq = sourceTemperature(P, x, y, z, level);
+      }
+      if (MOIST) {
+        tq = tq - (P.dt * P.adiabatic_rate) * g.vel[cell].y;  // expansion cooling of rising air
+        saturationAdjust(P, vq, q, tq);
+        g.tvap[cell] = vq;
       }
       g.ttem[cell] = tq;
     }
@@ -720,8 +746,9 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_seed(DG g, dcg_params P) {
     g.tem[cell] = sourceTemperature(P, x, y, z, level);
   } else {
     g.den[cell] = 0.f;
-    g.tem[cell] = P.t_ambient;
+    g.tem[cell] = ambientT(P, cy);
   }
+  if (P.moist) g.vap[cell] = ambientVapor(P, cy);
 }
 
 }  // namespace dcg

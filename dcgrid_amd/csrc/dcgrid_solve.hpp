@@ -8,9 +8,10 @@
 // FP64 (or FP32 / FP64 throughout), the correction e of A e = R comes from one multigrid
 // V-cycle in FP32 whose grids are the level itself, the coarser levels restricted to the
 // ancestors of the level's cells ("reach"), and below the top level a uniform pyramid of the
-// top level (dense by construction). Smoothing is red-black Gauss-Seidel; transfers are the
-// mean of the 8 cells of a restriction group and injection. Residuals on coarse grids are kept
-// in density units (R / h^2), so restriction is a plain mean.
+// top level (dense by construction). Smoothing is red-black Gauss-Seidel; restriction is the
+// mean of the 8 cells of a restriction group, prolongation the trilinear 27/9/3/1 stencil.
+// Residuals on coarse grids are kept in density units (R / h^2), so restriction is a plain
+// mean and each grid uses its own cell size in the operator.
 #pragma once
 
 #include "dcgrid_fluid.hpp"
@@ -227,18 +228,32 @@ __global__ void __launch_bounds__(256) k_sv_restrict(DG g, int level, int lmin, 
   }
 }
 
-// e += e(parent) on the active cells of `level` (injection).
+// e += P e(parent level) on the active cells of `level`: the 27/9/3/1 trilinear stencil of
+// k_prolongate over the parent's apron, with any stencil cell that is not an active cell of the
+// parent level (coarser, or outside the region being solved) replaced by the own parent cell.
 template <int BW, class T>
 __global__ void __launch_bounds__(256) k_sv_prolong(DG g, int level, int lmin, T* __restrict__ e,
                                                     const uint8_t* __restrict__ reach) {
-  constexpr int BV = Geo<BW>::BV;
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const uint32_t b = g.offs[level] + i / BV;
     const int t = (int)(i % BV);
     const uint32_t c = b * BV + t;
-    if (reach[c] > lmin || g.par[b] == NFU) continue;
-    e[c] = e[c] + e[parentCellOf<BW>(g, b, t)];
+    const uint32_t ps = g.par[b];
+    if (reach[c] > lmin || ps == NFU) continue;
+    const int3 p0 = g.pos[b];
+    int lx, ly, lz;
+    locPos<BB>((uint32_t)t, lx, ly, lz);
+    uint32_t q[8];
+    parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
+    T v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const bool ok = g.lev[q[k] / BV] == level + 1 && reach[q[k]] <= lmin;
+      v[k] = e[ok ? q[k] : q[0]];
+    }
+    e[c] = e[c] + ((T)27 * v[0] + (T)9 * (v[1] + v[2] + v[3]) + (T)3 * (v[4] + v[5] + v[6]) + v[7]) / (T)64;
   }
 }
 
@@ -287,11 +302,36 @@ __global__ void __launch_bounds__(256) k_sv_restrict_top(DG g, int lmin, const T
   }
 }
 
-// e += e(pyramid parent) on the active top-level cells.
+// Trilinear 27/9/3/1 prolongation from pyramid grid Dc at fine cell (x,y,z) of a grid of size
+// (w,h,d) (edge rows collapse onto their own parent, as k_u_prolongate); inactive coarse cells
+// are replaced by the own parent.
+template <class T>
+__device__ __forceinline__ T uProlong(const UDim& Dc, int x, int y, int z, int w, int h, int d,
+                                      const T* __restrict__ ue, const uint8_t* __restrict__ uact) {
+  const int X = x >> 1, Y = y >> 1, Z = z >> 1;
+  const int sx = (x == 0 || x == w - 1) ? 0 : 2 * (x & 1) - 1;
+  const int sy = (y == 0 || y == h - 1) ? 0 : 2 * (y & 1) - 1;
+  const int sz = (z == 0 || z == d - 1) ? 0 : 2 * (z & 1) - 1;
+  const uint32_t u0 = uAt(Dc, X, Y, Z);
+  const uint32_t q[8] = {u0,
+                         uAt(Dc, X + sx, Y, Z),
+                         uAt(Dc, X, Y + sy, Z),
+                         uAt(Dc, X, Y, Z + sz),
+                         uAt(Dc, X + sx, Y + sy, Z),
+                         uAt(Dc, X + sx, Y, Z + sz),
+                         uAt(Dc, X, Y + sy, Z + sz),
+                         uAt(Dc, X + sx, Y + sy, Z + sz)};
+  T v[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = ue[uact[q[k]] ? q[k] : u0];
+  return ((T)27 * v[0] + (T)9 * (v[1] + v[2] + v[3]) + (T)3 * (v[4] + v[5] + v[6]) + v[7]) / (T)64;
+}
+
+// e += P e(pyramid) on the active top-level cells.
 template <int BW, class T>
 __global__ void __launch_bounds__(256) k_sv_prolong_top(DG g, int lmin, T* __restrict__ e,
                                                         const uint8_t* __restrict__ reach, const T* __restrict__ ue,
-                                                        UDim D1) {
+                                                        const uint8_t* __restrict__ uact, UDim D1) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const int top = g.L - 1;
   const uint32_t n = g.ctl[CTL_LOADS + top] * BV;
@@ -303,7 +343,7 @@ __global__ void __launch_bounds__(256) k_sv_prolong_top(DG g, int lmin, T* __res
     const int3 p0 = g.pos[b];
     int x, y, z;
     locPos<BB>((uint32_t)t, x, y, z);
-    e[c] = e[c] + ue[uAt(D1, (p0.x + x) >> 1, (p0.y + y) >> 1, (p0.z + z) >> 1)];
+    e[c] = e[c] + uProlong<T>(D1, p0.x + x, p0.y + y, p0.z + z, 2 * D1.w, 2 * D1.h, 2 * D1.d, ue, uact);
   }
 }
 
@@ -375,7 +415,7 @@ __global__ void __launch_bounds__(256) k_svu_prolong(UDim D, UDim Dc, T* __restr
   const uint32_t u = D.off + i;
   if (!act[u]) return;
   const int x = (int)(i % D.w), y = (int)((i / D.w) % D.h), z = (int)(i / ((uint32_t)D.w * D.h));
-  e[u] = e[u] + e[uAt(Dc, x >> 1, y >> 1, z >> 1)];
+  e[u] = e[u] + uProlong<T>(Dc, x, y, z, D.w, D.h, D.d, e, act);
 }
 
 // Coarsest pyramid grid: `sweeps` red-black Gauss-Seidel sweeps in one workgroup.

@@ -336,6 +336,7 @@ struct UniformSim : SimBase {
     gz = size[2];
     if (gx <= 0 || gy <= 0 || gz <= 0 || gx != P.gx || gy != P.gy || gz != P.gz)
       return fail(DCG_E_ARG, "size must be positive and equal params gx,gy,gz");
+    if (P.moist) return fail(DCG_E_UNSUPPORTED, "the moist atmosphere is implemented for the DCGrid");
     const int minDim = std::min(gx, std::min(gy, gz));
     int cellSize = 2;
     L = 1;

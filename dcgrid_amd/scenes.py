@@ -8,12 +8,13 @@ The numbered workloads are the five benchmark configurations:
   1  adaptive DCGrid 256^3, 8^3 blocks, 3 levels, 4096-block budget, feature refinement, 200 steps
   2  adaptive DCGrid 512^3, 8^3 blocks, 4 levels, 32768-block budget, 100 steps
   3  uniform 128^3 (differentiable control; forward part)
-  4  terrain-scale DCGrid 1024x1024x256, 8^3 blocks, 4 levels (all dense at its budget)
+  4  terrain cloud scene on a DCGrid 1024x256x1024 (y up), 8^3 blocks, 4 levels, all dense at
+     its budget: fBm terrain, moist atmosphere with condensation, ambient wind
 """
 from dataclasses import dataclass
 
 from ._abi import (Params, PROJECT_LOCAL, PROJECT_MULTILEVEL, SCORE_FEATURES, SCORE_GEOMETRIC,
-                   SOURCE_NONE, SOURCE_SPHERE_BUOYANT)
+                   SOLID_TERRAIN, SOURCE_NONE, SOURCE_SPHERE_BUOYANT)
 
 SEED_BASE = 20261202
 
@@ -81,6 +82,42 @@ def _plume(p, center, radius, noise_vel=0.0):
     return p
 
 
+def terrain_scene(p, scale=1):
+    """Terrain cloud scene (extension): a 10 m grid (times `scale`), 1 s steps, a 5-octave fBm
+    heightfield of 60 cells (600 m) with a 256-cell lattice period, a moist layer of 1 km at
+    90 % relative humidity under a 6.5 K/km lapse rate from 20 C at the ground, saturation
+    adjustment with latent heating, expansion cooling of rising air (9.8 K/km), cloud-water
+    loading and thermal buoyancy, and a 2 m/s ambient wind in a seeded horizontal direction."""
+    import math
+    p.dt = 1.0
+    p.dx = 10.0 * scale
+    p.rdx = 1.0 / p.dx
+    p.velocity_emission_rate = 0.0
+    p.density_emission_rate = 0.0
+    p.emission_radius = 0.0
+    p.source_mode = SOURCE_NONE
+    p.enable_additional_solids = 1
+    p.solid_mode = SOLID_TERRAIN
+    p.terrain_height = 60.0 / scale
+    p.terrain_period = 256.0 / scale
+    p.moist = 1
+    p.t_ambient = 20.0
+    p.lapse_rate = 0.0065 * p.dx          # K per finest cell
+    p.qsat_scale = 3.8e-3                  # kg/kg at 0 C (Magnus form)
+    p.humidity = 0.9
+    p.moist_layer = 100.0 / scale          # 1 km
+    p.cond_rate = 0.5
+    p.latent_heat = 2.5e6 / 1004.0         # L / c_p: K per kg/kg condensed
+    p.adiabatic_rate = 0.0098              # K per m of ascent
+    p.buoy_alpha = 9.81                    # cloud-water loading
+    p.buoy_beta = 9.81 / 293.15            # thermal expansion
+    theta = 2.0 * math.pi * ((SEED_BASE + 4) % 360) / 360.0
+    p.wind[0], p.wind[1], p.wind[2] = 2.0 * math.cos(theta), 0.0, 2.0 * math.sin(theta)
+    p.init_velocity_noise = 0.2
+    p.noise_period = 64.0 / scale
+    return p
+
+
 def config(i, scale=1):
     """Workload of benchmark configuration i. `scale` shrinks the domain for quick tests."""
     if i == 0:
@@ -125,14 +162,16 @@ def config(i, scale=1):
     if i == 4:
         gx, gz, gy = 1024 // scale, 1024 // scale, 256 // scale
         p = reference_params(gx, gy, gz)
-        _plume(p, (gx / 2, gy / 8, gz / 2), 32.0 / scale, noise_vel=0.2)
+        terrain_scene(p, scale)
         p.seed = SEED_BASE + 4
         p.block_width = 8
-        p.max_levels = 4
+        # four levels need the height to be a multiple of 8 * 2^3 cells
+        p.max_levels = 4 if gy % 64 == 0 else 3
         p.score_mode = SCORE_FEATURES
         p.refine_threshold = 1.0
-        full = sum((gx // (8 << l)) * (gy // (8 << l)) * (gz // (8 << l)) for l in range(4))
-        return Workload("dcgrid1024x256x1024_bw8_l4", "dcgrid", (gx, gy, gz), p, 300, full)
+        L = p.max_levels
+        full = sum((gx // (8 << l)) * (gy // (8 << l)) * (gz // (8 << l)) for l in range(L))
+        return Workload("dcgrid1024x256x1024_bw8_l4_terrain", "dcgrid", (gx, gy, gz), p, 300, full)
     raise ValueError(i)
 
 

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define DCG_ABI_VERSION 1
+#define DCG_ABI_VERSION 2
 #define DCG_MAX_LEVELS 16
 
 /* Status codes. */
@@ -57,7 +57,8 @@ enum {
   DCG_FIELD_DIVERGENCE = 5,
   DCG_FIELD_TEMPERATURE = 6,
   DCG_FIELD_VORTICITY = 7,
-  DCG_NUM_FIELDS = 8
+  DCG_FIELD_VAPOR = 8, /* water vapor of the moist atmosphere (extension) */
+  DCG_NUM_FIELDS = 9
 };
 
 /* Projection schedule used by dcg_step. DCG_PROJECT_SOLVE (extension) solves, level by level
@@ -78,6 +79,10 @@ enum {
 
 /* Sources. */
 enum { DCG_SOURCE_NONE = 0, DCG_SOURCE_SPHERE_BUOYANT = 1 };
+
+/* Solid geometry when enable_additional_solids is set: the reference's sphere
+ * (src/sdf.cuh:8-18) or a seeded fBm heightfield (extension). */
+enum { DCG_SOLID_SPHERE = 0, DCG_SOLID_TERRAIN = 1 };
 
 /*
  * Simulation parameters. The first block equals the simulation fields of the reference's
@@ -129,7 +134,22 @@ typedef struct dcg_params {
   int32_t solve_max_cycles; /* refinement cycles per level before giving up (50) */
   int32_t solve_smooth;     /* red-black Gauss-Seidel sweeps before and after each coarse
                                correction (2; at most 10, so <= 20 per cycle and grid) */
-  int32_t reserved[4];
+
+  /* Terrain and moist atmosphere (extension). */
+  int32_t solid_mode;     /* DCG_SOLID_* */
+  float terrain_height;   /* heightfield maximum, finest cells */
+  float terrain_period;   /* lattice period of the first of 5 fBm octaves, finest cells */
+  int32_t moist;          /* 1: vapor field, saturation adjustment with latent heating, ambient
+                             temperature t_ambient - lapse_rate * height, buoyancy from it */
+  float lapse_rate;       /* ambient temperature drop per finest cell of height */
+  float qsat_scale;       /* saturation vapor q_s(T) = qsat_scale * exp(17.27 T / (T + 237.3)) */
+  float humidity;         /* relative humidity of the ambient moist layer (initial and inflow) */
+  float moist_layer;      /* height of the moist layer, finest cells */
+  float cond_rate;        /* fraction of the super- (or under-) saturation converted per step */
+  float latent_heat;      /* temperature change per unit of condensed water */
+  float wind[3];          /* ambient wind, world units per second (initial field, lateral inflow) */
+  float adiabatic_rate;   /* cooling of rising air per world unit of ascent (moist only) */
+  int32_t reserved[3];
 } dcg_params;
 
 /* Outcome of the last DCG_PROJECT_SOLVE projection, per level (0 finest). */

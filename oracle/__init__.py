@@ -50,6 +50,11 @@ def library():
             ("orc_mipmap_idx", [C.c_int32] * 3 + [C.c_uint64] * 4, C.c_uint64),
             ("orc_value_noise", [C.c_float] * 3 + [C.c_uint32], C.c_float),
             ("orc_hash01", [C.c_uint32] + [C.c_int32] * 4, C.c_float),
+            ("orc_exp_det", [C.c_float], C.c_float),
+            ("orc_qsat", [C.POINTER(Params), C.c_float], C.c_float),
+            ("orc_terrain_height", [C.POINTER(Params), C.c_float, C.c_float], C.c_float),
+            ("orc_cell_fluidity", [C.POINTER(Params)] + [C.c_int32] * 4, C.c_float),
+            ("orc_saturation_adjust", [C.POINTER(Params)] + [C.POINTER(C.c_float)] * 3, None),
         ]:
             f = getattr(lib, name)
             f.argtypes = args

@@ -74,6 +74,12 @@ uint64_t orc_mipmap_idx(int32_t x, int32_t y, int32_t z, uint64_t w, uint64_t h,
                         uint64_t scale);                          /* grid_math.cuh:43-52 */
 float orc_value_noise(float x, float y, float z, uint32_t seed);
 float orc_hash01(uint32_t seed, int32_t x, int32_t y, int32_t z, int32_t w);
+/* Moist atmosphere and terrain (extension). */
+float orc_exp_det(float x);
+float orc_qsat(const dcg_params* p, float t);
+float orc_terrain_height(const dcg_params* p, float x, float z);
+float orc_cell_fluidity(const dcg_params* p, int32_t x, int32_t y, int32_t z, int32_t scale);
+void orc_saturation_adjust(const dcg_params* p, float* qv, float* qc, float* t);
 
 #ifdef __cplusplus
 }

@@ -51,8 +51,8 @@ int orc_set_params(orc_sim* s, const dcg_params* p) {
   IMPL(s);
   if (!p) return DCG_E_ARG;
   if (p->gx != I->P.gx || p->gy != I->P.gy || p->gz != I->P.gz || p->block_width != I->P.block_width ||
-      p->max_levels != I->P.max_levels)
-    return I->fail(DCG_E_ARG, "grid size, block width and level cap are fixed at creation");
+      p->max_levels != I->P.max_levels || p->moist != I->P.moist)
+    return I->fail(DCG_E_ARG, "grid size, block width, level cap and moist atmosphere are fixed at creation");
   I->P = *p;
   return DCG_OK;
 }
@@ -193,5 +193,14 @@ uint64_t orc_mipmap_idx(int32_t x, int32_t y, int32_t z, uint64_t w, uint64_t h,
 }
 float orc_value_noise(float x, float y, float z, uint32_t seed) { return orc::valueNoise(x, y, z, seed); }
 float orc_hash01(uint32_t seed, int32_t x, int32_t y, int32_t z, int32_t w) { return orc::hash01(seed, x, y, z, w); }
+float orc_exp_det(float x) { return orc::expDet(x); }
+float orc_qsat(const dcg_params* p, float t) { return orc::qsat(*p, t); }
+float orc_terrain_height(const dcg_params* p, float x, float z) { return orc::terrainHeight(*p, x, z); }
+float orc_cell_fluidity(const dcg_params* p, int32_t x, int32_t y, int32_t z, int32_t scale) {
+  return orc::cellFluidity(*p, x, y, z, scale);
+}
+void orc_saturation_adjust(const dcg_params* p, float* qv, float* qc, float* t) {
+  orc::saturationAdjust<float>(*p, *qv, *qc, *t);
+}
 
 }  // extern "C"

@@ -98,10 +98,31 @@ inline bool inletHit(const dcg_params& P, int x, int y, int z, int scale) {
 inline bool outside(const dcg_params& P, int x, int y, int z, int scale) {
   return x < 0 || y < 0 || z < 0 || x * scale >= P.gx || y * scale >= P.gy || z * scale >= P.gz;
 }
+// Deterministic exp (extension helper, identical float operations on the GPU): Cody-Waite
+// reduction by ln 2 and the degree-7 Taylor polynomial on |r| <= ln2/2, scaled by 2^n.
+inline float expDet(float x) {
+  const float n = floorf(x * 1.44269504f + .5f);
+  const float r = (x - n * 0.693145752f) - n * 1.42860677e-06f;
+  const float p =
+      1.f + r * (1.f + r * (.5f + r * (1.f / 6.f + r * (1.f / 24.f + r * (1.f / 120.f + r * (1.f / 720.f + r * (1.f / 5040.f)))))));
+  return ldexpf(p, (int)n);
+}
+// Moist atmosphere (extension; no counterpart in the reference).
+inline float ambientT(const dcg_params& P, float hy) { return P.moist ? P.t_ambient - P.lapse_rate * hy : P.t_ambient; }
+inline float qsat(const dcg_params& P, float T) { return P.qsat_scale * expDet(17.27f * T / (T + 237.3f)); }
+inline float ambientVapor(const dcg_params& P, float hy) {
+  return hy < P.moist_layer ? P.humidity * qsat(P, ambientT(P, hy)) : 0.f;
+}
+inline float cellHeight(int y, int scale) { return ((float)y + .5f) * (float)scale; }
+
 template <class R>
 inline V3<R> velocityBC(const dcg_params& P, const V3<R>& v, int x, int y, int z, int scale) {
   if (inletHit(P, x, y, z, scale)) return V3<R>{(R)0, (R)P.velocity_emission_rate, (R)0};
-  if (outside(P, x, y, z, scale)) return V3<R>{(R)0, (R)0, (R)0};
+  if (outside(P, x, y, z, scale)) {
+    // lateral boundaries carry the ambient wind (zero unless configured); floor and lid are walls
+    if (y >= 0 && y * scale < P.gy) return V3<R>{(R)P.wind[0], (R)P.wind[1], (R)P.wind[2]};
+    return V3<R>{(R)0, (R)0, (R)0};
+  }
   return v;
 }
 template <class R>
@@ -111,25 +132,47 @@ inline R densityBC(const dcg_params& P, R q, int x, int y, int z, int scale) {
   return q;
 }
 // Temperature has no counterpart in the reference; it behaves like density except that
-// the inlet and outside values are the ambient temperature.
+// the inlet value is the ambient temperature and outside cells hold the ambient profile.
 template <class R>
 inline R temperatureBC(const dcg_params& P, R q, int x, int y, int z, int scale) {
   if (inletHit(P, x, y, z, scale)) return (R)P.t_ambient;
-  if (outside(P, x, y, z, scale)) return (R)P.t_ambient;
+  if (outside(P, x, y, z, scale)) return (R)ambientT(P, cellHeight(y, scale));
   return q;
 }
+template <class R>
+inline R vaporBC(const dcg_params& P, R q, int x, int y, int z, int scale) {
+  if (outside(P, x, y, z, scale)) return (R)ambientVapor(P, cellHeight(y, scale));
+  return q;
+}
+// Saturation adjustment of one cell (condensation / evaporation with latent heating).
+template <class R>
+inline void saturationAdjust(const dcg_params& P, R& qv, R& qc, R& T) {
+  const R ex = qv - (R)qsat(P, (float)T);
+  R c = (R)P.cond_rate * ex;
+  if (c < -qc) c = -qc;
+  qv = qv - c;
+  qc = qc + c;
+  T = T + (R)P.latent_heat * c;
+}
 
-// getCellFluidity, src/utils/sim_utils.cu:11-22, with sphereSDF src/sdf.cuh:8-18.
+// getCellFluidity, src/utils/sim_utils.cu:11-22, with sphereSDF src/sdf.cuh:8-18, or with the
+// terrain heightfield (extension): signed distance = height of the cell centre above the terrain.
+inline float terrainHeight(const dcg_params& P, float x, float z);
 inline float cellFluidity(const dcg_params& P, int x, int y, int z, int scale) {
   const float sqrt3 = 1.73205f;
   if (!P.enable_additional_solids) return 1.f;
   const float px = ((float)x + .5f) * (float)scale;
   const float py = ((float)y + .5f) * (float)scale;
   const float pz = ((float)z + .5f) * (float)scale;
-  const float cx = (float)P.gx * .5f, cy = (float)P.gy * .45f, cz = (float)P.gz * .5f;
-  const float r = 2000.f * P.rdx;
-  const float dx = px - cx, dy = py - cy, dz = pz - cz;
-  const float d = sqrtf(dx * dx + dy * dy + dz * dz) - r;
+  float d;
+  if (P.solid_mode == DCG_SOLID_TERRAIN) {
+    d = py - terrainHeight(P, px, pz);
+  } else {
+    const float cx = (float)P.gx * .5f, cy = (float)P.gy * .45f, cz = (float)P.gz * .5f;
+    const float r = 2000.f * P.rdx;
+    const float dx = px - cx, dy = py - cy, dz = pz - cz;
+    d = sqrtf(dx * dx + dy * dy + dz * dz) - r;
+  }
   const float overlap = clampf(.5f - d / ((float)scale * sqrt3), 0.f, 1.f);
   return 1.f - overlap;
 }
@@ -184,6 +227,17 @@ inline float fbm3(float x, float y, float z, float period, uint32_t seed) {
   }
   return n / 1.75f;
 }
+// Terrain heightfield (finest cells): 5-octave fBm of value noise mapped to [0, terrain_height].
+inline float terrainHeight(const dcg_params& P, float x, float z) {
+  float n = 0.f, amp = 1.f, freq = 1.f / P.terrain_period;
+  const uint32_t seed = P.seed ^ 0x3c6ef372u;
+  for (int o = 0; o < 5; o++) {
+    n = n + amp * valueNoise(x * freq, 0.f, z * freq, seed + 7919u * (uint32_t)o);
+    amp = amp * .5f;
+    freq = freq * 2.f;
+  }
+  return P.terrain_height * (.5f + .5f * (n / 1.9375f));
+}
 // Initial velocity of the seeded scene at a cell centre (finest-cell units).
 inline void initVelocity(const dcg_params& P, float cx, float cy, float cz, float out[3]) {
   for (int c = 0; c < 3; c++) {
@@ -194,6 +248,7 @@ inline void initVelocity(const dcg_params& P, float cx, float cy, float cz, floa
     const uint32_t s = P.seed ^ (0x51ed270bu * (uint32_t)(c + 1));
     out[c] = P.init_velocity_noise * fbm3(cx, cy, cz, P.noise_period, s);
   }
+  for (int c = 0; c < 3; c++) out[c] = out[c] + P.wind[c];
 }
 // Sphere source membership of a cell centre (finest-cell units).
 inline bool inSource(const dcg_params& P, float cx, float cy, float cz) {

@@ -46,6 +46,7 @@ struct DCGrid : Sim {
   std::vector<uint8_t> lev, flg;
   std::vector<size_t> par, chi, cidx, freeIdx;
   std::vector<R> den, tem, flu, dive, pre, t_pre, t_den, t_tem;
+  std::vector<R> vap, t_vap;  // water vapor (moist atmosphere only)
   std::vector<V3<R>> vel, t_vel, vort;
   std::vector<float> sbs, bs;  // subblock scores (8B + 1 pad), block scores (B)
   std::vector<size_t> touched;
@@ -132,6 +133,10 @@ struct DCGrid : Sim {
     vel.assign(C, V3<R>{0, 0, 0});
     t_vel.assign(C, V3<R>{0, 0, 0});
     vort.assign(C, V3<R>{0, 0, 0});
+    if (P.moist) {
+      vap.assign(C, 0);
+      t_vap.assign(C, 0);
+    }
     sbs.assign(8 * B + 1, -FLT_MAX);
     bs.assign(B, -FLT_MAX);
     return reset();
@@ -613,8 +618,10 @@ struct DCGrid : Sim {
     out[7] = pa[id + i + j + k];
   }
   // k_dcgrid_propagate_values, dcgrid_adaptation.cu:92-143
+  // temperature is advected and drives buoyancy: sphere source or moist atmosphere
+  bool thermo() const { return P.source_mode == DCG_SOURCE_SPHERE_BUOYANT || P.moist; }
   void propagate(int level) {
-    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const bool src = thermo();
     const R w27 = (R)(27.f / 64.f), w9 = (R)(9.f / 64.f), w3 = (R)(3.f / 64.f), w1 = (R)(1.f / 64.f);
 #pragma omp parallel for schedule(static)
     for (long long tl = 0; tl < (long long)touched.size(); tl++) {
@@ -629,6 +636,9 @@ struct DCGrid : Sim {
         if (src)
           tem[c] = w27 * tem[q[0]] + w9 * (tem[q[1]] + tem[q[2]] + tem[q[3]]) +
                    w3 * (tem[q[4]] + tem[q[5]] + tem[q[6]]) + w1 * tem[q[7]];
+        if (src && P.moist)
+          vap[c] = w27 * vap[q[0]] + w9 * (vap[q[1]] + vap[q[2]] + vap[q[3]]) +
+                   w3 * (vap[q[4]] + vap[q[5]] + vap[q[6]]) + w1 * vap[q[7]];
         const V3<R> a = mul(vel[q[0]], w27);
         const V3<R> bb = mul(add(add(vel[q[1]], vel[q[2]]), vel[q[3]]), w9);
         const V3<R> cc = mul(add(add(vel[q[4]], vel[q[5]]), vel[q[6]]), w3);
@@ -765,7 +775,7 @@ struct DCGrid : Sim {
   // advectVelocity(): k_dcgrid_advect_velocity + copy + accumulateVelocity
   // (fluid_simulation_dcgrid.cu:263-268, dcgrid_fluid.cu:112-127)
   int advectVelocity() override {
-    const bool buoy = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const bool buoy = thermo();
     const R dt = (R)P.dt;
 #pragma omp parallel for schedule(static)
     for (long long bl = 0; bl < (long long)B; bl++) {
@@ -783,7 +793,10 @@ struct DCGrid : Sim {
         sample(x, y, z, s);
         V3<R> nv = sampleVelocity(s);
         if (buoy) {
-          const R bf = dt * ((R)P.buoy_beta * (tem[c] - (R)P.t_ambient) - (R)P.buoy_alpha * den[c]);
+          int cx, cy, cz;
+          cellPos(c, cx, cy, cz);
+          const R ta = (R)ambientT(P, cellHeight(cy, 1 << lev[b]));
+          const R bf = dt * ((R)P.buoy_beta * (tem[c] - ta) - (R)P.buoy_alpha * den[c]);
           nv.y = nv.y + bf;
         }
         t_vel[c] = nv;
@@ -807,7 +820,7 @@ struct DCGrid : Sim {
   // (fluid_simulation_dcgrid.cu:313-318, dcgrid_fluid.cu:129-144); temperature and the sphere
   // source are extensions.
   int advectDensity() override {
-    const bool src = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    const bool src = thermo(), moist = P.moist != 0, sphere = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
 #pragma omp parallel for schedule(static)
     for (long long bl = 0; bl < (long long)B; bl++) {
       const size_t b = (size_t)bl;
@@ -819,6 +832,7 @@ struct DCGrid : Sim {
         if (!isLeaf(c)) {
           t_den[c] = 0;
           if (src) t_tem[c] = 0;
+          if (moist) t_vap[c] = 0;
           continue;
         }
         R x, y, z;
@@ -832,9 +846,15 @@ struct DCGrid : Sim {
           cellPos(c, cx, cy, cz);
           const float fcx = ((float)cx + .5f) * (float)scale, fcy = ((float)cy + .5f) * (float)scale,
                       fcz = ((float)cz + .5f) * (float)scale;
-          if (inSource(P, fcx, fcy, fcz)) {
+          if (sphere && inSource(P, fcx, fcy, fcz)) {
             q = (R)P.src_density;
             t = (R)sourceTemperature(P, cx, cy, cz, level);
+          }
+          if (moist) {
+            R v = sampleScalar(s, vap, [&](R a, int xx, int yy, int zz, int sc) { return vaporBC(P, a, xx, yy, zz, sc); });
+            t = t - ((R)P.dt * (R)P.adiabatic_rate) * vel[c].y;  // expansion cooling of rising air
+            saturationAdjust(P, v, q, t);
+            t_vap[c] = v;
           }
           t_tem[c] = t;
         }
@@ -846,6 +866,10 @@ struct DCGrid : Sim {
     if (src) {
       copyAllocated(t_tem, tem);
       accumulate(tem, (R)0);
+    }
+    if (moist) {
+      copyAllocated(t_vap, vap);
+      accumulate(vap, (R)0);
     }
     return DCG_OK;
   }
@@ -1131,13 +1155,15 @@ struct DCGrid : Sim {
           tem[c] = (R)sourceTemperature(P, x, y, z, level);
         } else {
           den[c] = 0;
-          tem[c] = (R)P.t_ambient;
+          tem[c] = (R)ambientT(P, cy);
         }
+        if (P.moist) vap[c] = (R)ambientVapor(P, cy);
       }
     }
     accumulate(vel, V3<R>{0, 0, 0});
     accumulate(den, (R)0);
     accumulate(tem, (R)0);
+    if (P.moist) accumulate(vap, (R)0);
     return DCG_OK;
   }
 
@@ -1171,11 +1197,13 @@ struct DCGrid : Sim {
   }
   int fieldSize(int f, uint64_t* n) override {
     if (f < 0 || f >= DCG_NUM_FIELDS) return fail(DCG_E_ARG, "unknown field");
+    if (f == DCG_FIELD_VAPOR && !P.moist) return fail(DCG_E_ARG, "no vapor field (moist atmosphere off)");
     *n = (f == DCG_FIELD_VELOCITY || f == DCG_FIELD_VORTICITY) ? 3 * C : C;
     return DCG_OK;
   }
   std::vector<R>* scalar(int f) {
     switch (f) {
+      case DCG_FIELD_VAPOR: return &vap;
       case DCG_FIELD_DENSITY: return &den;
       case DCG_FIELD_TEMPERATURE: return &tem;
       case DCG_FIELD_PRESSURE: return &pre;

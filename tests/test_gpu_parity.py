@@ -66,6 +66,14 @@ def dc_workload(name):
     if name == "cfg2_128":
         w = scenes.config(2, scale=4)
         return w.kind, w.size, w.params, w.max_num_blocks, True
+    if name == "cfg4_128":  # terrain + moist atmosphere + wind, all levels dense
+        w = scenes.config(4, scale=8)
+        return w.kind, w.size, w.params, w.max_num_blocks, True
+    if name == "cfg4_128_sparse":  # the same scene with sparse finer levels (moves, refinement)
+        w = scenes.config(4, scale=8)
+        w.params.score_mode = scenes.SCORE_GEOMETRIC
+        w.params.refine_threshold = 1e-4
+        return w.kind, w.size, w.params, 16 + 128 + 256, True
     raise KeyError(name)
 
 
@@ -89,7 +97,8 @@ def assert_tables_equal(g, c):
         assert ig.levels_list(k) == ic.levels_list(k)
 
 
-DC_CASES = ["ref64_bw4", "ref64_bw4_solids", "ref64_bw8", "cfg1_64", "cfg1_128", "cfg2_128"]
+DC_CASES = ["ref64_bw4", "ref64_bw4_solids", "ref64_bw8", "cfg1_64", "cfg1_128", "cfg2_128", "cfg4_128",
+            "cfg4_128_sparse"]
 
 
 @pytest.mark.parametrize("name", DC_CASES)
@@ -104,7 +113,7 @@ def test_dcgrid_steps_bitwise_vs_fp32_oracle(name):
             if stage == "adapt_topology":
                 assert_tables_equal(g, c)
         m = allocated_cells(c)
-        for f in FIELDS_DC:
+        for f in FIELDS_DC + (("vapor",) if g.params.moist else ()):
             assert bitwise(g.export_field(f), c.export_field(f), m), (name, step, f)
         assert g.debug_stats() == c.debug_stats()
 

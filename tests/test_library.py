@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol():
 
 def test_abi_version_and_defaults():
     lib = dcgrid_amd.library()
-    assert lib.dcg_abi_version() == 1
+    assert lib.dcg_abi_version() == 2
     p = dcgrid_amd.default_params()
     # src/data/sim_params.cpp:6-17; rdx = 1/dx (src/main.cpp:17)
     assert (p.gx, p.gy, p.gz) == (128, 128, 128)
