@@ -24,6 +24,7 @@ static unsigned fillGrid(size_t n) {
 
 SimBase::~SimBase() {
   if (stream) hipStreamSynchronize(stream);
+  dropGraph();
   for (auto& p : pending) {
     hipEventDestroy(p.a);
     hipEventDestroy(p.b);
@@ -31,6 +32,46 @@ SimBase::~SimBase() {
   for (auto e : eventPool) hipEventDestroy(e);
   for (void* p : allocations) hipFree(p);
   if (stream) hipStreamDestroy(stream);
+}
+
+int SimBase::steps(int n) {
+  const bool useGraph = graphOn && graphCapable() && timingMask == 0 && P.project_mode != DCG_PROJECT_SOLVE && n >= 2;
+  int i = 0;
+  if (useGraph) {
+    std::vector<char> key;
+    graphKeyBytes(key);
+    if (!graphExec || key != graphKey) {
+      dropGraph();
+      DCG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      int e = stepOnce();
+      if (!e) e = stepOnce();
+      hipGraph_t captured = nullptr;
+      const hipError_t ce = hipStreamEndCapture(stream, &captured);
+      if (e) {
+        if (captured) hipGraphDestroy(captured);
+        return e;
+      }
+      if (ce != hipSuccess) return fail(DCG_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      graph = captured;
+      DCG_HIP(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+      std::vector<char> after;
+      graphKeyBytes(after);
+      if (after != key) {  // the captured pair must leave the recorded state as it found it
+        dropGraph();
+        return fail(DCG_E_HIP, "step capture changed the recorded state");
+      }
+      graphKey = key;
+    }
+    for (; i + 1 < n; i += 2) {
+      DCG_HIP(hipGraphLaunch(graphExec, stream));
+      countHost(2);
+    }
+  }
+  for (; i < n; i++) {
+    DCG_TRY(stepOnce());
+    countHost(1);
+  }
+  return DCG_OK;
 }
 
 int SimBase::createStream() {
@@ -137,7 +178,7 @@ void dcg_default_params(dcg_params* p) {
   p->solve_precision = DCG_SOLVE_MIXED;
   p->solve_tol = 1e-8f;
   p->solve_max_cycles = 50;
-  p->solve_smooth = 2;
+  p->solve_smooth = 3;
 }
 
 int dcg_create_uniform(const int32_t size[3], const dcg_params* p, dcg_sim** out) {
@@ -223,14 +264,12 @@ int dcg_advect_density(dcg_sim* s) {
 int dcg_step(dcg_sim* s, int32_t n) {
   IMPL(s);
   if (n < 0) return I->fail(DCG_E_ARG, "negative step count");
-  for (int i = 0; i < n; i++) {
-    int e;
-    if ((e = I->advectVelocity())) return e;
-    if ((e = I->adaptTopology())) return e;
-    if ((e = I->countStep())) return e;
-    if ((e = I->projectByMode())) return e;
-    if ((e = I->advectDensity())) return e;
-  }
+  return I->steps(n);
+}
+int dcg_set_graph(dcg_sim* s, int32_t enable) {
+  IMPL(s);
+  I->graphOn = enable != 0;
+  if (!I->graphOn) I->dropGraph();
   return DCG_OK;
 }
 int dcg_debug_stats(dcg_sim* s, double* out) {

@@ -58,8 +58,10 @@ struct SimBase {
   virtual int exportField(int field, float* dst, uint64_t n) = 0;
   virtual int importField(int field, const float* src, uint64_t n) = 0;
   virtual int exportTables(dcg_tables* t) = 0;
-  // Active cells of one step, accumulated by dcg_step after adaptTopology.
+  // Active cells of one step, accumulated by dcg_step after adaptTopology: on the device
+  // (countStep, part of the captured step) or on the host (countHost, per executed step).
   virtual int countStep() = 0;
+  virtual void countHost(uint64_t steps) {}
   virtual int cellSteps(uint64_t* out) = 0;
   virtual int runStage(int stage, int arg) = 0;
   // Projection with the level systems solved to a tolerance (extension).
@@ -87,6 +89,32 @@ struct SimBase {
                            float* otem) {
     return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
   }
+
+  // Step replay through a HIP graph: two consecutive steps are captured once (the host-side
+  // pointer swaps of the advected fields return to their start after two steps) and replayed
+  // while the state the launches were recorded with (graphKeyBytes) is unchanged.
+  bool graphOn = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  std::vector<char> graphKey;
+  virtual bool graphCapable() const { return false; }
+  virtual void graphKeyBytes(std::vector<char>& out) const { out.clear(); }
+  void dropGraph() {
+    if (graphExec) hipGraphExecDestroy(graphExec);
+    if (graph) hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+    graphKey.clear();
+  }
+  int stepOnce() {
+    int e;
+    if ((e = advectVelocity())) return e;
+    if ((e = adaptTopology())) return e;
+    if ((e = countStep())) return e;
+    if ((e = projectByMode())) return e;
+    return advectDensity();
+  }
+  int steps(int n);
 
   int fail(int code, const std::string& m) {
     err = m;

@@ -378,6 +378,14 @@ struct DCGridSim : SimBase {
     batchK = -1;
     return endTimed(DCG_K_JACOBI, &batch, 2 * pairs);
   }
+  bool graphCapable() const override { return true; }
+  void graphKeyBytes(std::vector<char>& out) const override {
+    out.clear();
+    auto put = [&](const void* p, size_t n) { out.insert(out.end(), (const char*)p, (const char*)p + n); };
+    put(&g, sizeof(g));
+    put(&P, sizeof(P));
+    put(&customFluidity, sizeof(customFluidity));
+  }
   int countStep() override {
     DCG_LAUNCH(DCG_K_OTHER, "k_count_step", (k_count_step<BW><<<1, 64, 0, stream>>>(g, cellStepCounter)));
     return DCG_OK;
@@ -463,6 +471,15 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&uactBuf, off + 1));
     return DCG_OK;
   }
+  // Ghost factor of a Dirichlet face on the grid m levels coarser than the solved level: the
+  // finest grid puts the zero half a fine cell beyond the region's face, i.e. at
+  // d = (1/2 + 2^-(m+1)) coarse cells from the boundary cell's centre; the linear ghost through
+  // it is (1 - 1/d) e_c.
+  static float ghostFactor(int m) {
+    if (m <= 0) return 0.f;
+    const double d = 0.5 + std::ldexp(1.0, -(m + 1));
+    return (float)(1.0 - 1.0 / d);
+  }
   float levelH2(int level) const {
     const float s = (float)(1 << level);
     return s * s * P.dx * P.dx;  // alpha of k_dcgrid_jacobi, dcgrid_multigrid_solver.cu:11
@@ -474,17 +491,18 @@ struct DCGridSim : SimBase {
     for (int s = 0; s < sweeps; s++)
       for (int color = 0; color < 2; color++)
         DCG_LAUNCH(DCG_K_OTHER, "k_sv_rbgs",
-                   (k_sv_rbgs<BW, T><<<cellGrid(level), 256, 0, stream>>>(g, level, lmin, color, e, rho, reachBuf, levelH2(level))));
+                   (k_sv_rbgs<BW, T><<<cellGrid(level), 256, 0, stream>>>(g, level, lmin, color, e, rho, reachBuf, levelH2(level), ghostFactor(level - lmin))));
     return DCG_OK;
   }
   template <class T>
-  int smoothU(int j, int sweeps) {
+  int smoothU(int j, int lmin, int sweeps) {
     const UDim& D = pyr[j];
     const unsigned n = blocksFor((size_t)D.w * D.h * D.d, 256);
     const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
+    const float gm = ghostFactor(L + j - lmin);
     for (int k = 0; k < sweeps; k++)
       for (int color = 0; color < 2; color++)
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_rbgs", (k_svu_rbgs<T><<<n, 256, 0, stream>>>(D, color, (T*)ueBuf, (const T*)urhoBuf, uactBuf, h2)));
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_rbgs", (k_svu_rbgs<T><<<n, 256, 0, stream>>>(D, color, (T*)ueBuf, (const T*)urhoBuf, uactBuf, h2, gm)));
     return DCG_OK;
   }
   // One V-cycle for the correction of level lmin (right-hand side in rho on its cells).
@@ -499,31 +517,32 @@ struct DCGridSim : SimBase {
       DCG_TRY(smooth<T>(k, lmin, nu));
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, T><<<cellGrid(k + 1), 256, 0, stream>>>(g, k + 1, e, rho)));
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
-                 (k_sv_restrict<BW, T><<<cellGrid(k), 256, 0, stream>>>(g, k, lmin, e, rho, reachBuf, levelH2(k))));
+                 (k_sv_restrict<BW, T><<<cellGrid(k), 256, 0, stream>>>(g, k, lmin, e, rho, reachBuf, levelH2(k), ghostFactor(k - lmin))));
     }
     DCG_TRY(smooth<T>(top, lmin, nu));
     if (!pyr.empty()) {
       T* ue = (T*)ueBuf;
       T* ur = (T*)urhoBuf;
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict_top",
-                 (k_sv_restrict_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, rho, reachBuf, ue, ur, uactBuf, pyr[0], levelH2(top))));
+                 (k_sv_restrict_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, rho, reachBuf, ue, ur, uactBuf, pyr[0], levelH2(top), ghostFactor(top - lmin))));
       const int J = (int)pyr.size();
       for (int j = 0; j + 1 < J; j++) {
-        DCG_TRY(smoothU<T>(j, nu));
+        DCG_TRY(smoothU<T>(j, lmin, nu));
         const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
         const UDim& Dc = pyr[j + 1];
         DCG_LAUNCH(DCG_K_OTHER, "k_svu_restrict",
-                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(pyr[j], Dc, ue, ur, uactBuf, h2)));
+                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(pyr[j], Dc, ue, ur, uactBuf, h2, ghostFactor(L + j - lmin))));
       }
       {
         const float s = (float)(1 << (L + J - 1)), h2 = s * s * P.dx * P.dx;
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_bottom", (k_svu_bottom<T><<<1, 1024, 0, stream>>>(pyr[J - 1], ue, ur, uactBuf, h2, 40)));
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_bottom",
+                   (k_svu_bottom<T><<<1, 1024, 0, stream>>>(pyr[J - 1], ue, ur, uactBuf, h2, ghostFactor(L + J - 1 - lmin), 40)));
       }
       for (int j = J - 2; j >= 0; j--) {
         const UDim& D = pyr[j];
         DCG_LAUNCH(DCG_K_OTHER, "k_svu_prolong",
                    (k_svu_prolong<T><<<blocksFor((size_t)D.w * D.h * D.d, 256), 256, 0, stream>>>(D, pyr[j + 1], ue, uactBuf)));
-        DCG_TRY(smoothU<T>(j, nu));
+        DCG_TRY(smoothU<T>(j, lmin, nu));
       }
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_top",
                  (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, uactBuf, pyr[0])));

@@ -157,21 +157,26 @@ __global__ void __launch_bounds__(256) k_sv_zero(DG g, int level, T* __restrict_
 }
 
 // (A e)_c / h^2 pieces of an active cell: sum of the coupled neighbours and the diagonal.
-// Neighbours on coarser levels and inactive ones hold a zero correction.
+// Neighbours on coarser levels and inactive ones are the Dirichlet boundary of the region being
+// solved. On the level itself they hold a zero correction; on coarser grids the zero lies
+// between the boundary cell and that neighbour (where the finest grid puts it), which a ghost
+// value gm * e_c (gm <= 0, from the grid's depth) reproduces.
 template <int BW, class T>
 __device__ __forceinline__ void svStencil(const DG& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
-                                          const uint8_t* __restrict__ reach, T& sum, int& diag) {
+                                          const uint8_t* __restrict__ reach, T gm, T& sum, T& diag) {
   const uint32_t c = b * Geo<BW>::BV + t;
   sum = (T)0;
-  diag = 6;
+  diag = (T)6;
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     bool co;
     const uint32_t nb = faceNb<BW>(g, b, t, d, co);
     if (nb == c) {
-      diag--;
+      diag -= (T)1;
     } else if (!co && reach[nb] <= lmin) {
       sum += e[nb];
+    } else {
+      diag -= gm;
     }
   }
 }
@@ -180,7 +185,7 @@ __device__ __forceinline__ void svStencil(const DG& g, uint32_t b, int t, int lm
 template <int BW, class T>
 __global__ void __launch_bounds__(256) k_sv_rbgs(DG g, int level, int lmin, int color, T* __restrict__ e,
                                                  const T* __restrict__ rho, const uint8_t* __restrict__ reach,
-                                                 float h2) {
+                                                 float h2, float gm) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -192,10 +197,9 @@ __global__ void __launch_bounds__(256) k_sv_rbgs(DG g, int level, int lmin, int 
     int x, y, z;
     locPos<BB>((uint32_t)t, x, y, z);
     if (((p0.x + x + p0.y + y + p0.z + z) & 1) != color) continue;
-    T sum;
-    int diag;
-    svStencil<BW, T>(g, b, t, lmin, e, reach, sum, diag);
-    e[c] = (sum + (T)h2 * rho[c]) / (T)diag;
+    T sum, diag;
+    svStencil<BW, T>(g, b, t, lmin, e, reach, (T)gm, sum, diag);
+    e[c] = (sum + (T)h2 * rho[c]) / diag;
   }
 }
 
@@ -205,7 +209,7 @@ __global__ void __launch_bounds__(256) k_sv_rbgs(DG g, int level, int lmin, int 
 template <int BW, class T>
 __global__ void __launch_bounds__(256) k_sv_restrict(DG g, int level, int lmin, T* __restrict__ e,
                                                      T* __restrict__ rho, const uint8_t* __restrict__ reach,
-                                                     float h2) {
+                                                     float h2, float gm) {
   constexpr int BV = Geo<BW>::BV, GPB = BV / 8;
   const uint32_t groups = g.ctl[CTL_LOADS + level] * GPB;
   const T inv = (T)1 / (T)h2;
@@ -217,10 +221,9 @@ __global__ void __launch_bounds__(256) k_sv_restrict(DG g, int level, int lmin, 
     for (int j = 0; j < 8; j++) {
       const uint32_t c = b * BV + t0 + j;
       if (reach[c] > lmin) continue;
-      T sum;
-      int diag;
-      svStencil<BW, T>(g, b, t0 + j, lmin, e, reach, sum, diag);
-      acc += rho[c] - ((T)diag * e[c] - sum) * inv;
+      T sum, diag;
+      svStencil<BW, T>(g, b, t0 + j, lmin, e, reach, (T)gm, sum, diag);
+      acc += rho[c] - (diag * e[c] - sum) * inv;
     }
     const uint32_t pc = parentCellOf<BW>(g, b, t0);
     rho[pc] = acc * (T).125;
@@ -273,7 +276,7 @@ __global__ void __launch_bounds__(256) k_sv_restrict_top(DG g, int lmin, const T
                                                          const T* __restrict__ rho,
                                                          const uint8_t* __restrict__ reach, T* __restrict__ ue,
                                                          T* __restrict__ urho, uint8_t* __restrict__ uact, UDim D1,
-                                                         float h2) {
+                                                         float h2, float gm) {
   constexpr int BV = Geo<BW>::BV, GPB = BV / 8, BB = Geo<BW>::BB;
   const int top = g.L - 1;
   const uint32_t groups = g.ctl[CTL_LOADS + top] * GPB;
@@ -287,10 +290,9 @@ __global__ void __launch_bounds__(256) k_sv_restrict_top(DG g, int lmin, const T
       const uint32_t c = b * BV + t0 + j;
       if (reach[c] > lmin) continue;
       any = true;
-      T sum;
-      int diag;
-      svStencil<BW, T>(g, b, t0 + j, lmin, e, reach, sum, diag);
-      acc += rho[c] - ((T)diag * e[c] - sum) * inv;
+      T sum, diag;
+      svStencil<BW, T>(g, b, t0 + j, lmin, e, reach, (T)gm, sum, diag);
+      acc += rho[c] - (diag * e[c] - sum) * inv;
     }
     const int3 p0 = g.pos[b];
     int x, y, z;
@@ -347,27 +349,31 @@ __global__ void __launch_bounds__(256) k_sv_prolong_top(DG g, int lmin, T* __res
   }
 }
 
-// Pyramid stencil: clamped (Neumann) at the domain walls, zero correction at inactive cells.
+// Pyramid stencil: clamped (Neumann) at the domain walls; inactive cells are the Dirichlet
+// boundary of the solved region (ghost gm * e_c, as in svStencil).
 template <class T>
 __device__ __forceinline__ void uStencil(const UDim& D, int x, int y, int z, const T* __restrict__ e,
-                                         const uint8_t* __restrict__ act, T& sum, int& diag) {
+                                         const uint8_t* __restrict__ act, T gm, T& sum, T& diag) {
   sum = (T)0;
-  diag = 6;
+  diag = (T)6;
   const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     if (nx[d] < 0 || ny[d] < 0 || nz[d] < 0 || nx[d] >= D.w || ny[d] >= D.h || nz[d] >= D.d) {
-      diag--;
+      diag -= (T)1;
       continue;
     }
     const uint32_t u = uAt(D, nx[d], ny[d], nz[d]);
-    if (act[u]) sum += e[u];
+    if (act[u])
+      sum += e[u];
+    else
+      diag -= gm;
   }
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_svu_rbgs(UDim D, int color, T* __restrict__ e, const T* __restrict__ rho,
-                                                  const uint8_t* __restrict__ act, float h2) {
+                                                  const uint8_t* __restrict__ act, float h2, float gm) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -375,16 +381,15 @@ __global__ void __launch_bounds__(256) k_svu_rbgs(UDim D, int color, T* __restri
   if (((x + y + z) & 1) != color) return;
   const uint32_t u = D.off + i;
   if (!act[u]) return;
-  T sum;
-  int diag;
-  uStencil<T>(D, x, y, z, e, act, sum, diag);
-  e[u] = (sum + (T)h2 * rho[u]) / (T)diag;
+  T sum, diag;
+  uStencil<T>(D, x, y, z, e, act, (T)gm, sum, diag);
+  e[u] = (sum + (T)h2 * rho[u]) / diag;
 }
 
 // Residual of pyramid grid D restricted into the next grid Dc (thread per coarse cell).
 template <class T>
 __global__ void __launch_bounds__(256) k_svu_restrict(UDim D, UDim Dc, T* __restrict__ e, T* __restrict__ rho,
-                                                      uint8_t* __restrict__ act, float h2) {
+                                                      uint8_t* __restrict__ act, float h2, float gm) {
   const uint32_t n = (uint32_t)Dc.w * Dc.h * Dc.d;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -397,10 +402,9 @@ __global__ void __launch_bounds__(256) k_svu_restrict(UDim D, UDim Dc, T* __rest
     const uint32_t u = uAt(D, x, y, z);
     if (!act[u]) continue;
     any = true;
-    T sum;
-    int diag;
-    uStencil<T>(D, x, y, z, e, act, sum, diag);
-    acc += rho[u] - ((T)diag * e[u] - sum) * inv;
+    T sum, diag;
+    uStencil<T>(D, x, y, z, e, act, (T)gm, sum, diag);
+    acc += rho[u] - (diag * e[u] - sum) * inv;
   }
   rho[Dc.off + i] = acc * (T).125;
   e[Dc.off + i] = (T)0;
@@ -421,7 +425,7 @@ __global__ void __launch_bounds__(256) k_svu_prolong(UDim D, UDim Dc, T* __restr
 // Coarsest pyramid grid: `sweeps` red-black Gauss-Seidel sweeps in one workgroup.
 template <class T>
 __global__ void __launch_bounds__(1024) k_svu_bottom(UDim D, T* __restrict__ e, const T* __restrict__ rho,
-                                                     const uint8_t* __restrict__ act, float h2, int sweeps) {
+                                                     const uint8_t* __restrict__ act, float h2, float gm, int sweeps) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
   for (int s = 0; s < 2 * sweeps; s++) {
     const int color = s & 1;
@@ -430,10 +434,9 @@ __global__ void __launch_bounds__(1024) k_svu_bottom(UDim D, T* __restrict__ e, 
       if (((x + y + z) & 1) != color) continue;
       const uint32_t u = D.off + i;
       if (!act[u]) continue;
-      T sum;
-      int diag;
-      uStencil<T>(D, x, y, z, e, act, sum, diag);
-      e[u] = (sum + (T)h2 * rho[u]) / (T)diag;
+      T sum, diag;
+      uStencil<T>(D, x, y, z, e, act, (T)gm, sum, diag);
+      e[u] = (sum + (T)h2 * rho[u]) / diag;
     }
     __syncthreads();
   }

@@ -451,9 +451,15 @@ struct UniformSim : SimBase {
     batchK = -1;
     return endTimed(DCG_K_JACOBI, &batch, 2 * pairs);
   }
-  int countStep() override {
-    cellStepCount += N;
-    return DCG_OK;
+  int countStep() override { return DCG_OK; }
+  void countHost(uint64_t steps) override { cellStepCount += steps * N; }
+  bool graphCapable() const override { return true; }
+  void graphKeyBytes(std::vector<char>& out) const override {
+    out.clear();
+    auto put = [&](const void* p, size_t n) { out.insert(out.end(), (const char*)p, (const char*)p + n); };
+    put(&g, sizeof(g));
+    put(&P, sizeof(P));
+    put(&customFluidity, sizeof(customFluidity));
   }
   int cellSteps(uint64_t* out) override {
     *out = cellStepCount;
