@@ -250,7 +250,11 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing pass")
+    ap.add_argument("--timing-steps", type=int, default=20,
+                    help="steps of the separate pass that times every Jacobi launch with HIP events")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel of the timed steps eagerly (no HIP graph replay)")
     ap.add_argument("--solve", action="store_true",
                     help="time the projection solve to 1e-8 (mixed, FP32, FP64) instead of the step")
     ap.add_argument("--copy-probe", action="store_true",
@@ -280,9 +284,10 @@ def main():
     sim.synchronize()
     info0 = sim.info()
     act0 = int(info0.active_cells)
-    if not args.no_timing:
-        sim.set_timing(["jacobi"])
-    log(f"[bench] timing {args.steps} steps (active cells {act0})")
+    # timed steps: whole-step throughput with the steps replayed in pairs from one HIP graph
+    # (per-kernel event timing would force eager launches, so it runs in a pass of its own)
+    sim.set_graph(not args.eager)
+    log(f"[bench] timing {args.steps} steps (active cells {act0}, graph {not args.eager})")
     cs0 = sim.cell_steps()
     sim.synchronize()
     t0 = time.perf_counter()
@@ -297,16 +302,25 @@ def main():
     value = cell_steps / elapsed
 
     roof = None
-    if not args.no_timing:
+    if not args.no_timing and args.timing_steps > 0:
+        # roofline pass: the following steps with every Jacobi launch bracketed by HIP events on
+        # the simulation's stream
+        sim.set_graph(False)
+        sim.set_timing(["jacobi"])
+        cs1 = sim.cell_steps()
+        sim.step(args.timing_steps)
+        sim.synchronize()
+        tcs = sim.cell_steps() - cs1
+        info2 = sim.info()
         tm = sim.timing()
         j_ms, j_n = tm["jacobi"]
         p = w.params
         if p.project_top_pairs == p.project_level_pairs:
             # every active cell is swept 2*pairs times per step: exact over the varying topology
-            jb = 2 * p.project_level_pairs * 12 * cell_steps
+            jb = 2 * p.project_level_pairs * 12 * tcs
             jl = j_n
         else:
-            jb, jl = jacobi_bytes(info1, p)
+            jb, jl = jacobi_bytes(info2, p)
         if j_n > 0 and j_ms > 0:
             per_launch_bytes = jb / jl
             avg_ms = j_ms / j_n
@@ -321,6 +335,7 @@ def main():
                 "traffic": measured_traffic(w.name, "k_jacobi"),
                 "avg_launch_us": round(avg_ms * 1e3, 3),
                 "launches": int(j_n),
+                "timed_steps": args.timing_steps,
                 "algorithmic_bytes_per_launch": int(per_launch_bytes),
             }
     sb = step_bytes(info1, w.params)
@@ -348,6 +363,7 @@ def main():
             "block_loads": info1.levels_list("block_loads"),
             "step_algorithmic_bytes": sb,
             "step_algorithmic_GBps": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
+            "launch": "eager" if args.eager else "hip graph (pairs of steps)",
         },
         "roofline": roof,
         "cpu_baseline": None,

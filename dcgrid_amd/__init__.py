@@ -26,6 +26,14 @@ def library():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        try:
+            # One HIP runtime per process. torch ships its own libamdhip64 under the same soname
+            # (libamdhip64.so.7): loaded first, it also satisfies this library's dependency, so
+            # torch tensors and this library share one runtime. Loaded after this library,
+            # torch would map a second runtime, which then fails to initialise the device.
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         fns = bind(lib, "dcg_")
         lib.dcg_default_params.argtypes = [C.POINTER(Params)]
@@ -36,6 +44,8 @@ def library():
         lib.dcg_synchronize.restype = C.c_int
         lib.dcg_set_timing.argtypes = [C.c_void_p, C.c_int32]
         lib.dcg_set_timing.restype = C.c_int
+        lib.dcg_set_graph.argtypes = [C.c_void_p, C.c_int32]
+        lib.dcg_set_graph.restype = C.c_int
         lib.dcg_timing_get.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         lib.dcg_timing_get.restype = C.c_int
         lib.dcg_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
@@ -85,6 +95,10 @@ class Simulation(Handle):
         for k in kernels:
             mask |= 1 << K_NAMES.index(k)
         self._check(_lib.dcg_set_timing(self._h, mask), "set_timing")
+
+    def set_graph(self, enable=True):
+        """Replay step() in pairs of steps through a captured HIP graph (off by default)."""
+        self._check(_lib.dcg_set_graph(self._h, 1 if enable else 0), "set_graph")
 
     def timing(self):
         """{kernel name: (total ms, launches)} accumulated since timing was enabled."""

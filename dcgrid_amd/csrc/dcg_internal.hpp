@@ -93,7 +93,7 @@ struct SimBase {
   // Step replay through a HIP graph: two consecutive steps are captured once (the host-side
   // pointer swaps of the advected fields return to their start after two steps) and replayed
   // while the state the launches were recorded with (graphKeyBytes) is unchanged.
-  bool graphOn = true;
+  bool graphOn = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   std::vector<char> graphKey;

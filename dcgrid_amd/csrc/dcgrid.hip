@@ -510,7 +510,7 @@ struct DCGridSim : SimBase {
   int vcycle(int lmin) {
     T* e = (T*)eBuf;
     T* rho = (T*)rhoBuf;
-    const int nu = std::max(1, std::min(P.solve_smooth, 10));
+    const int nu = P.solve_smooth > 0 ? std::min(P.solve_smooth, 10) : 3;
     const int top = L - 1;
     DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, T><<<cellGrid(lmin), 256, 0, stream>>>(g, lmin, e, e)));
     for (int k = lmin; k < top; k++) {
@@ -572,13 +572,13 @@ struct DCGridSim : SimBase {
     const float alpha = levelH2(l);
     const bool singular = hostLoads[l] == maxL[l] && maxL[l] == fullL[l];
     const double N = (double)hostLoads[l] * G::BV;
-    double r0 = -1, rr = 0, prev = 0;
+    double r0 = -1, rr = 0, prev = 0, shift = 0;
     int cycles = 0, stall = 0;
-    const double tol = (double)P.solve_tol;
-    const int maxCycles = std::max(1, P.solve_max_cycles);
+    const double tol = P.solve_tol > 0 ? (double)P.solve_tol : 1e-8;
+    const int maxCycles = P.solve_max_cycles > 0 ? P.solve_max_cycles : 50;
     for (;;) {
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_residual",
-                 (k_sv_residual<BW, TX, T><<<kSolveWG, 256, 0, stream>>>(g, l, x, rho, partBuf, alpha, alpha)));
+                 (k_sv_residual<BW, TX, T><<<kSolveWG, 256, 0, stream>>>(g, l, x, rho, partBuf, alpha, alpha, shift)));
       double s[4];
       DCG_TRY(readPartials(4, s));
       double m = 0;
@@ -598,8 +598,12 @@ struct DCGridSim : SimBase {
         stall = 0;
       }
       prev = rr;
-      if (singular)
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_shift", (k_sv_shift<BW, T><<<cellGrid(l), 256, 0, stream>>>(g, l, rho, m / (double)alpha)));
+      // the mean of R (the right-hand side's inconsistency) stays the same from cycle to cycle:
+      // the residual kernel subtracts the last one, this corrects what is left
+      if (singular && m != shift) {
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_shift", (k_sv_shift<BW, T><<<cellGrid(l), 256, 0, stream>>>(g, l, rho, (m - shift) / (double)alpha)));
+        shift = m;
+      }
       DCG_TRY(vcycle<T>(l));
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_update", (k_sv_update<BW, TX, T><<<cellGrid(l), 256, 0, stream>>>(g, l, x, (const T*)eBuf, reachBuf)));
       cycles++;

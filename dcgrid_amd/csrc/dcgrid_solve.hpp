@@ -71,13 +71,15 @@ __global__ void __launch_bounds__(256) k_sv_reach(DG g, int level, uint8_t* __re
   }
 }
 
-// Fine-level residual R = sum(apron x) - 6 x - alpha div (precision TX), stored as rho = R / h^2
-// (precision T); per-workgroup partial sums of R, R^2 and of the zero-guess residual R0 and
-// R0^2 go to part[4 * blockIdx.x ...].
+// Fine-level residual R = sum(apron x) - 6 x - alpha div (precision TX), stored as
+// rho = (R - shift) / h^2 (precision T); per-workgroup partial sums of R, R^2 and of the
+// zero-guess residual R0 and R0^2 go to part[4 * blockIdx.x ...]. A singular level passes the
+// mean of R as `shift`, so the zero-mean part is formed before the rounding to T: rounding R
+// first would leave an error of T's epsilon times that mean, which does not shrink with R.
 template <int BW, class TX, class T>
 __global__ void __launch_bounds__(256) k_sv_residual(DG g, int level, const TX* __restrict__ x,
                                                      T* __restrict__ rho, double* __restrict__ part, float alpha,
-                                                     float h2) {
+                                                     float h2, double shift) {
   constexpr int BV = Geo<BW>::BV;
   __shared__ double red[4][256];
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(256) k_sv_residual(DG g, int level, const TX* 
     const TX ad = (TX)alpha * (TX)g.div[c];
     const TX R = sum - (TX)6 * x[c] - ad;
     const TX R0 = ghosts - ad;
-    rho[c] = (T)(R / (TX)h2);
+    rho[c] = (T)((R - (TX)shift) / (TX)h2);
     s[0] += (double)R;
     s[1] += (double)R * (double)R;
     s[2] += (double)R0;

@@ -130,10 +130,10 @@ typedef struct dcg_params {
 
   /* DCG_PROJECT_SOLVE (extension, DCGrid). */
   int32_t solve_precision;  /* DCG_SOLVE_* */
-  float solve_tol;          /* per-level relative residual |r| / |r(0)| to reach (1e-8) */
-  int32_t solve_max_cycles; /* refinement cycles per level before giving up (50) */
+  float solve_tol;          /* per-level relative residual |r| / |r(0)| to reach (1e-8; <= 0: 1e-8) */
+  int32_t solve_max_cycles; /* refinement cycles per level before giving up (50; <= 0: 50) */
   int32_t solve_smooth;     /* red-black Gauss-Seidel sweeps before and after each coarse
-                               correction (3; at most 10, so <= 20 per cycle and grid) */
+                               correction (3; <= 0: 3; at most 10) */
 
   /* Terrain and moist atmosphere (extension). */
   int32_t solid_mode;     /* DCG_SOLID_* */
@@ -232,10 +232,10 @@ int dcg_advect_velocity(dcg_sim* s);
 int dcg_project(dcg_sim* s);
 int dcg_project_local(dcg_sim* s);
 int dcg_advect_density(dcg_sim* s);
-/* dcg_step replays pairs of steps through a HIP graph captured on first use (and again after
- * anything the recorded launches depend on changed: parameters, a single-stage call that
- * swapped field buffers). Per-kernel timing and DCG_PROJECT_SOLVE run eagerly. enable = 0 turns
- * replay off. Extension; no counterpart in the reference. */
+/* enable != 0: dcg_step replays pairs of steps through a HIP graph captured on first use (and
+ * again after anything the recorded launches depend on changed: parameters, a single-stage call
+ * that swapped field buffers). Per-kernel timing and DCG_PROJECT_SOLVE run eagerly. Off by
+ * default. Extension; no counterpart in the reference. */
 int dcg_set_graph(dcg_sim* s, int32_t enable);
 /* Extension: the projection with the level systems solved to solve_tol (DCG_PROJECT_SOLVE),
  * and the report of the last such solve (synchronises). DCGrid only. */
