@@ -31,8 +31,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+STORAGE_BYTES = {0: 4, 1: 2, 2: 2}  # dcg_params.storage -> bytes per stored scalar
+STORAGE_NAMES = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
 def jacobi_bytes(info, params):
-    """Algorithmic bytes and launch count of the Jacobi half-sweeps of one project() call."""
+    """Algorithmic bytes and launch count of the Jacobi half-sweeps of one project() call: each
+    half-sweep reads p and div and writes p (three stored scalars per cell)."""
+    S = STORAGE_BYTES[params.storage]
     bv = int(info.block_width) ** 3
     L = int(info.levels)
     total_bytes = 0
@@ -40,16 +46,18 @@ def jacobi_bytes(info, params):
     for level in range(L):
         pairs = params.project_top_pairs if level == L - 1 else params.project_level_pairs
         cells = int(info.block_loads[level]) * bv
-        total_bytes += 2 * pairs * cells * 12
+        total_bytes += 2 * pairs * cells * 3 * S
         launches += 2 * pairs
     return total_bytes, launches
 
 
 def step_bytes(info, params):
-    """Algorithmic HBM bytes of one step (per-stage compulsory field traffic, FP32, no solids)."""
+    """Algorithmic HBM bytes of one step (per-stage compulsory field traffic, no solids); S is the
+    stored scalar size (4 for FP32 fields, 2 for FP16/BF16), V = 3 S."""
     bv = int(info.block_width) ** 3
     L = int(info.levels)
-    V, S = 12, 4
+    S = STORAGE_BYTES[params.storage]
+    V = 3 * S
     src = params.source_mode != 0
     k = 2 if src else 1  # advected scalars
     act = sum(int(info.block_loads[l]) for l in range(L)) * bv
@@ -57,10 +65,10 @@ def step_bytes(info, params):
     jb, _ = jacobi_bytes(info, params)
     b = act * (2 * V + (2 * S if src else 0))          # advect u (+ buoyancy)
     b += below * 9 * V // 8 * 2                         # restrict u twice
-    b += act * (V + 4) + below * 9 * 4 // 8             # divergence + restrict div
+    b += act * (V + S) + below * 9 * S // 8             # divergence + restrict div
     b += jb                                             # Jacobi sweeps
-    b += sum(int(info.block_loads[l]) for l in range(L - 1)) * bv * 4 * 9 // 8  # prolongate
-    b += act * (2 * V + 4)                              # gradient subtraction
+    b += sum(int(info.block_loads[l]) for l in range(L - 1)) * bv * S * 9 // 8  # prolongate
+    b += act * (2 * V + S)                              # gradient subtraction
     b += act * (V + 2 * k * S) + below * 9 * k * S // 8  # advect scalars + restrict
     return b
 
@@ -253,6 +261,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing pass")
     ap.add_argument("--timing-steps", type=int, default=20,
                     help="steps of the separate pass that times every Jacobi launch with HIP events")
+    ap.add_argument("--storage", choices=sorted(STORAGE_NAMES), default="fp32",
+                    help="field storage format in HBM (FP32 arithmetic either way)")
     ap.add_argument("--graph", action="store_true",
                     help="replay the timed steps in pairs from one captured HIP graph (measured no "
                          "faster than eager launches on MI355X for this step)")
@@ -269,6 +279,10 @@ def main():
     import dcgrid_amd
     from dcgrid_amd import scenes
     w = scenes.config(args.config)
+    w.params.storage = STORAGE_NAMES[args.storage]
+    if args.storage != "fp32" and w.kind != "dcgrid":
+        print(json.dumps({"error": "16-bit field storage is implemented for the DCGrid configurations"}))
+        sys.exit(2)
     if args.config == 3:
         print(json.dumps(adjoint_bench(w, args)), flush=True)
         return
@@ -317,7 +331,7 @@ def main():
         p = w.params
         if p.project_top_pairs == p.project_level_pairs:
             # every active cell is swept 2*pairs times per step: exact over the varying topology
-            jb = 2 * p.project_level_pairs * 12 * tcs
+            jb = 2 * p.project_level_pairs * 3 * STORAGE_BYTES[p.storage] * tcs
             jl = j_n
         else:
             jb, jl = jacobi_bytes(info2, p)
@@ -364,6 +378,7 @@ def main():
             "step_algorithmic_bytes": sb,
             "step_algorithmic_GBps": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
             "launch": "hip graph (pairs of steps)" if args.graph else "eager",
+            "field_storage": args.storage,
         },
         "roofline": roof,
         "cpu_baseline": None,

@@ -53,6 +53,7 @@ SOLVE_MIXED, SOLVE_FP32, SOLVE_FP64 = 0, 1, 2
 SCORE_GEOMETRIC, SCORE_FEATURES = 0, 1
 SOURCE_NONE, SOURCE_SPHERE_BUOYANT = 0, 1
 SOLID_SPHERE, SOLID_TERRAIN = 0, 1
+STORAGE_FP32, STORAGE_FP16, STORAGE_BF16 = 0, 1, 2
 
 STAGE_DIVERGENCE = 0
 STAGE_RESTRICT = 1
@@ -118,7 +119,8 @@ class Params(C.Structure):
         ("latent_heat", C.c_float),
         ("wind", C.c_float * 3),
         ("adiabatic_rate", C.c_float),
-        ("reserved", C.c_int32 * 3),
+        ("storage", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
     def copy(self):

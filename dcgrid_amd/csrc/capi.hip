@@ -183,6 +183,10 @@ void dcg_default_params(dcg_params* p) {
 
 int dcg_create_uniform(const int32_t size[3], const dcg_params* p, dcg_sim** out) {
   if (!size || !p || !out) return DCG_E_ARG;
+  if (p->storage != DCG_STORAGE_FP32) {
+    g_create_error = "16-bit field storage is implemented for the DCGrid";
+    return DCG_E_UNSUPPORTED;
+  }
   int st = 0;
   SimBase* s = dcg::makeUniform(size, *p, &st, &g_create_error);
   if (!s) return st;
@@ -213,8 +217,9 @@ int dcg_set_params(dcg_sim* s, const dcg_params* p) {
   IMPL(s);
   if (!p) return DCG_E_ARG;
   if (p->gx != I->P.gx || p->gy != I->P.gy || p->gz != I->P.gz || p->block_width != I->P.block_width ||
-      p->max_levels != I->P.max_levels || p->moist != I->P.moist)
-    return I->fail(DCG_E_ARG, "grid size, block width, level cap and moist atmosphere are fixed at creation");
+      p->max_levels != I->P.max_levels || p->moist != I->P.moist || p->storage != I->P.storage)
+    return I->fail(DCG_E_ARG,
+                   "grid size, block width, level cap, moist atmosphere and field storage are fixed at creation");
   I->P = *p;
   return DCG_OK;
 }

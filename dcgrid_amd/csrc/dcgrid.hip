@@ -5,6 +5,7 @@
 // on them are launched over the static per-level capacity and exit early.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "dcg_internal.hpp"
 #include "dcgrid_fluid.hpp"
@@ -24,7 +25,69 @@ static inline unsigned slotGrid(size_t blocks) {
   return n == 0 ? 1u : (n < cap ? n : cap);
 }
 
-template <int BW>
+// Host conversions of the 16-bit storage formats (dcg_export_field / dcg_import_field): exact
+// widening, and rounding to nearest even as the device stores do.
+static float widenF16(uint16_t h) {
+  const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  uint32_t u;
+  if (e == 0x1F) {
+    u = sign | 0x7F800000u | (m << 13);
+  } else if (e != 0) {
+    u = sign | ((e + 112u) << 23) | (m << 13);
+  } else {
+    float f = (float)m * 5.9604644775390625e-8f;  // m * 2^-24, exact
+    std::memcpy(&u, &f, 4);
+    u |= sign;
+  }
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+static uint16_t narrowF16(float f) {
+  uint32_t a;
+  std::memcpy(&a, &f, 4);
+  const uint16_t sign = (uint16_t)((a >> 16) & 0x8000u);
+  a &= 0x7FFFFFFFu;
+  if (a >= 0x47800000u) return sign | (a > 0x7F800000u ? 0x7E00u : 0x7C00u);  // overflow, inf, NaN
+  if (a < 0x38800000u) {  // subnormal result: adding 0.5 rounds to the 2^-24 grid (nearest even)
+    float t;
+    std::memcpy(&t, &a, 4);
+    t += 0.5f;
+    uint32_t tb;
+    std::memcpy(&tb, &t, 4);
+    return sign | (uint16_t)(tb - 0x3F000000u);
+  }
+  a += 0xC8000FFFu + ((a >> 13) & 1u);  // exponent rebias (15 - 127) << 23, round half to even
+  return sign | (uint16_t)(a >> 13);
+}
+static float widenBF16(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+static uint16_t narrowBF16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+template <class SF>
+struct StorageHost;
+template <>
+struct StorageHost<StoreF16> {
+  static float widen(uint16_t h) { return widenF16(h); }
+  static uint16_t narrow(float f) { return narrowF16(f); }
+};
+template <>
+struct StorageHost<StoreBF16> {
+  static float widen(uint16_t h) { return widenBF16(h); }
+  static uint16_t narrow(float f) { return narrowBF16(f); }
+};
+
+// SF: storage type of the fields in HBM (float, StoreF16 or StoreBF16).
+template <int BW, class SF>
 struct DCGridSim : SimBase {
   using G = Geo<BW>;
   int gx = 0, gy = 0, gz = 0, L = 0, S = 0;
@@ -32,7 +95,7 @@ struct DCGridSim : SimBase {
   size_t C = 0;
   uint32_t maxL[DCG_MAX_LEVELS] = {}, fullL[DCG_MAX_LEVELS] = {}, offs[DCG_MAX_LEVELS] = {};
   uint32_t hsize[DCG_MAX_LEVELS] = {}, hoff[DCG_MAX_LEVELS] = {};
-  DG g{};
+  DGT<SF> g{};
   float* sbs = nullptr;   // 8B + 1 (last entry stays -FLT_MAX)
   float* bs = nullptr;    // B
   float* feat = nullptr;  // 8B
@@ -158,6 +221,12 @@ struct DCGridSim : SimBase {
     return reset();
   }
 
+  // Zero bits are +0 in every storage format; element sizes are multiples of 2 bytes and C is
+  // even, so the byte count is a multiple of 4.
+  template <class T>
+  int zeroField(T* p, size_t n) {
+    return fill32(p, 0, n * sizeof(T) / 4);
+  }
   // reset(), fluid_simulation_dcgrid.cu:212-261
   int reset() override {
     DCG_TRY(fill32(g.hkey, 0xFFFFFFFFu, H));
@@ -168,15 +237,15 @@ struct DCGridSim : SimBase {
     DCG_TRY(fill32(g.par, NFU, B));
     DCG_TRY(fill32(g.chi, NFU, (size_t)8 * B));
     DCG_TRY(fill32(g.nd, NFU, (size_t)4 * 27 * B));
-    DCG_TRY(fill32(g.den, 0, C));
-    DCG_TRY(fill32(g.tem, 0, C));
-    DCG_TRY(fill32(g.vel, 0, 3 * C));
-    DCG_TRY(fill32(g.flu, 0, C));
-    DCG_TRY(fill32(g.div, 0, C));
-    DCG_TRY(fill32(g.pre, 0, C));
-    DCG_TRY(fill32(g.tpre, 0, C));
-    DCG_TRY(fill32(g.vort, 0, 3 * C));
-    if (g.vap) DCG_TRY(fill32(g.vap, 0, C));
+    DCG_TRY(zeroField(g.den, C));
+    DCG_TRY(zeroField(g.tem, C));
+    DCG_TRY(zeroField(g.vel, C));
+    DCG_TRY(zeroField(g.flu, C));
+    DCG_TRY(zeroField(g.div, C));
+    DCG_TRY(zeroField(g.pre, C));
+    DCG_TRY(zeroField(g.tpre, C));
+    DCG_TRY(zeroField(g.vort, C));
+    if (g.vap) DCG_TRY(zeroField(g.vap, C));
     DCG_TRY(fill32(g.ctl, 0, CTL_WORDS));
     DCG_TRY(fill32(cellStepCounter, 0, 2));
     uint32_t loads[DCG_MAX_LEVELS] = {};
@@ -647,7 +716,14 @@ struct DCGridSim : SimBase {
     rep.levels = L;
     rep.converged = 1;
     if (P.solve_precision == DCG_SOLVE_FP32) {
-      DCG_TRY((solveAll<float, float>(g.pre)));
+      if constexpr (std::is_same<SF, float>::value) {
+        DCG_TRY((solveAll<float, float>(g.pre)));
+      } else {  // FP32 iterate in the FP64 buffer, rounded to the field storage at the end
+        float* x32 = (float*)xd;
+        DCG_TRY((solveAll<float, float>(x32)));
+        for (int l = 0; l < L; l++)
+          DCG_LAUNCH(DCG_K_OTHER, "k_sv_store", (k_sv_store<BW, float><<<cellGrid(l), 256, 0, stream>>>(g, l, x32)));
+      }
     } else {
       if (P.solve_precision == DCG_SOLVE_FP64)
         DCG_TRY((solveAll<double, double>(xd)));
@@ -724,32 +800,48 @@ struct DCGridSim : SimBase {
     *n = (f == DCG_FIELD_VELOCITY || f == DCG_FIELD_VORTICITY) ? 3 * C : C;
     return DCG_OK;
   }
-  float* fieldPtr(int f) {
+  void* fieldPtr(int f) {
     switch (f) {
       case DCG_FIELD_DENSITY: return g.den;
-      case DCG_FIELD_VELOCITY: return (float*)g.vel;
+      case DCG_FIELD_VELOCITY: return g.vel;
       case DCG_FIELD_FLUIDITY: return g.flu;
       case DCG_FIELD_PRESSURE: return g.pre;
       case DCG_FIELD_T_PRESSURE: return g.tpre;
       case DCG_FIELD_DIVERGENCE: return g.div;
       case DCG_FIELD_TEMPERATURE: return g.tem;
-      case DCG_FIELD_VORTICITY: return (float*)g.vort;
+      case DCG_FIELD_VORTICITY: return g.vort;
       case DCG_FIELD_VAPOR: return g.vap;
       default: return nullptr;
     }
   }
+  // Fields are exchanged as FP32: 16-bit storage is widened exactly on export and rounded to
+  // nearest even on import.
   int exportField(int f, float* dst, uint64_t n) override {
     uint64_t need;
     DCG_TRY(fieldSize(f, &need));
     if (n != need) return fail(DCG_E_ARG, "size mismatch");
-    DCG_HIP(hipMemcpyAsync(dst, fieldPtr(f), n * sizeof(float), hipMemcpyDeviceToHost, stream));
-    return sync();
+    if constexpr (std::is_same<SF, float>::value) {
+      DCG_HIP(hipMemcpyAsync(dst, fieldPtr(f), n * sizeof(float), hipMemcpyDeviceToHost, stream));
+      return sync();
+    } else {
+      std::vector<uint16_t> h(n);
+      DCG_HIP(hipMemcpyAsync(h.data(), fieldPtr(f), n * 2, hipMemcpyDeviceToHost, stream));
+      DCG_TRY(sync());
+      for (uint64_t i = 0; i < n; i++) dst[i] = StorageHost<SF>::widen(h[i]);
+      return DCG_OK;
+    }
   }
   int importField(int f, const float* src, uint64_t n) override {
     uint64_t need;
     DCG_TRY(fieldSize(f, &need));
     if (n != need) return fail(DCG_E_ARG, "size mismatch");
-    DCG_HIP(hipMemcpyAsync(fieldPtr(f), src, n * sizeof(float), hipMemcpyHostToDevice, stream));
+    if constexpr (std::is_same<SF, float>::value) {
+      DCG_HIP(hipMemcpyAsync(fieldPtr(f), src, n * sizeof(float), hipMemcpyHostToDevice, stream));
+    } else {
+      std::vector<uint16_t> h(n);
+      for (uint64_t i = 0; i < n; i++) h[i] = StorageHost<SF>::narrow(src[i]);
+      DCG_HIP(hipMemcpyAsync(fieldPtr(f), h.data(), n * 2, hipMemcpyHostToDevice, stream));
+    }
     if (f == DCG_FIELD_FLUIDITY) customFluidity = true;
     return sync();
   }
@@ -825,22 +917,32 @@ struct DCGridSim : SimBase {
   }
 };
 
+template <int BW, class SF>
+static SimBase* makeSim(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p, int* st) {
+  auto* d = new DCGridSim<BW, SF>();
+  *st = d->create(size, maxBlocks, p);
+  return d;
+}
+template <int BW>
+static SimBase* makeBW(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p, int* st) {
+  if (p.storage == DCG_STORAGE_FP16) return makeSim<BW, StoreF16>(size, maxBlocks, p, st);
+  if (p.storage == DCG_STORAGE_BF16) return makeSim<BW, StoreBF16>(size, maxBlocks, p, st);
+  return makeSim<BW, float>(size, maxBlocks, p, st);
+}
+
 SimBase* makeDCGrid(const int32_t size[3], uint64_t maxBlocks, const dcg_params& p, int* status, std::string* err) {
-  SimBase* s;
-  int st;
-  if (p.block_width == 4) {
-    auto* d = new DCGridSim<4>();
-    st = d->create(size, maxBlocks, p);
-    s = d;
-  } else if (p.block_width == 8) {
-    auto* d = new DCGridSim<8>();
-    st = d->create(size, maxBlocks, p);
-    s = d;
-  } else {
+  if (p.block_width != 4 && p.block_width != 8) {
     *status = DCG_E_UNSUPPORTED;
     *err = "block_width must be 4 or 8";
     return nullptr;
   }
+  if (p.storage != DCG_STORAGE_FP32 && p.storage != DCG_STORAGE_FP16 && p.storage != DCG_STORAGE_BF16) {
+    *status = DCG_E_ARG;
+    *err = "unknown field storage format";
+    return nullptr;
+  }
+  int st = DCG_OK;
+  SimBase* s = p.block_width == 4 ? makeBW<4>(size, maxBlocks, p, &st) : makeBW<8>(size, maxBlocks, p, &st);
   *status = st;
   if (st != DCG_OK) {
     *err = s->err;

@@ -29,7 +29,49 @@ struct Geo {
   static constexpr int FACE = 6 * BW * BW;      // face cells of a block's apron
 };
 
-struct DG {
+// Storage formats of the fields in HBM (dcg_params.storage). A field element converts to FP32
+// when read and rounds to nearest even when written (v_cvt_f16_f32 / v_cvt_pk_bf16_f32); all
+// arithmetic is FP32, so a kernel computes exactly what it computes on FP32 fields from the
+// widened values and only its stores differ.
+//
+// The value is first pinned as an FP32 register value: otherwise the backend folds a product
+// feeding the conversion into v_fma_mixlo_f16 (the exact product rounded once to FP16), which
+// differs from rounding the FP32 product (a value the FP32 computation defines) to FP16.
+__device__ __forceinline__ float pinF32(float f) {
+  asm("" : "+v"(f));
+  return f;
+}
+struct StoreF16 {
+  uint16_t bits;
+  StoreF16() = default;
+  __device__ __forceinline__ StoreF16(float f) : bits(__builtin_bit_cast(uint16_t, (_Float16)pinF32(f))) {}
+  __device__ __forceinline__ operator float() const { return (float)__builtin_bit_cast(_Float16, bits); }
+};
+struct StoreBF16 {
+  uint16_t bits;
+  StoreBF16() = default;
+  __device__ __forceinline__ StoreBF16(float f) : bits(__builtin_bit_cast(uint16_t, (__bf16)pinF32(f))) {}
+  __device__ __forceinline__ operator float() const { return __uint_as_float((uint32_t)bits << 16); }
+};
+// Velocity-like fields: three stored components, 6 bytes per cell in the 16-bit formats.
+template <class S>
+struct StoreV3 {
+  S x, y, z;
+  StoreV3() = default;
+  __device__ __forceinline__ StoreV3(float3 v) : x(v.x), y(v.y), z(v.z) {}
+  __device__ __forceinline__ operator float3() const { return make_float3(x, y, z); }
+};
+template <class S>
+struct StoreTraits {
+  using V = StoreV3<S>;
+};
+template <>
+struct StoreTraits<float> {
+  using V = float3;
+};
+
+// Tables of the block grid (everything but the fields).
+struct DGBase {
   int gx, gy, gz, L, S;
   uint32_t B;
   uint32_t offs[DCG_MAX_LEVELS], maxL[DCG_MAX_LEVELS], fullL[DCG_MAX_LEVELS];
@@ -44,20 +86,28 @@ struct DG {
   uint32_t* hkey;
   uint32_t* hval;
   uint32_t* ctl;  // device control words, see CTL_* below
-  float* den;
-  float* tem;
-  float* flu;
-  float* div;
-  float* pre;
-  float* tpre;
-  float* tden;
-  float* ttem;
-  float* vap;   // water vapor (moist atmosphere; allocated when enabled at creation)
-  float* tvap;
-  float3* vel;
-  float3* tvel;
-  float3* vort;
 };
+
+// The grid with its fields stored as SF (float, StoreF16 or StoreBF16).
+template <class SF>
+struct DGT : DGBase {
+  using Scalar = SF;
+  using Vec = typename StoreTraits<SF>::V;
+  SF* den;
+  SF* tem;
+  SF* flu;
+  SF* div;
+  SF* pre;
+  SF* tpre;
+  SF* tden;
+  SF* ttem;
+  SF* vap;  // water vapor (moist atmosphere; allocated when enabled at creation)
+  SF* tvap;
+  Vec* vel;
+  Vec* tvel;
+  Vec* vort;
+};
+using DG = DGT<float>;
 
 // Control words (uint32 each) kept on the device so the step never waits for the host.
 enum : int {
@@ -92,7 +142,7 @@ __device__ __forceinline__ int dirIndex(int dx, int dy, int dz) { return (dx + 1
 // per-level hash tables upward from l0; here the walk goes down the child links from the first
 // dense level, which reach exactly the allocated blocks.
 template <int BW>
-__device__ __forceinline__ uint32_t findBlock(const DG& g, int qx, int qy, int qz, int l0) {
+__device__ __forceinline__ uint32_t findBlock(const DGBase& g, int qx, int qy, int qz, int l0) {
   constexpr int BB = Geo<BW>::BB;
   int lv = l0 > g.S ? l0 : g.S;
   int sh = lv - l0;
@@ -126,7 +176,7 @@ __device__ __forceinline__ int4 makeNd(uint32_t nb, uint32_t shift, int ox, int 
 // Global cell index of apron entry (ai,aj,ak) in [0,BW+1]^3 of block b: the value the
 // reference's cellIndices table holds for that entry.
 template <int BW>
-__device__ __forceinline__ uint32_t apronCell(const DG& g, uint32_t b, int ai, int aj, int ak) {
+__device__ __forceinline__ uint32_t apronCell(const DGBase& g, uint32_t b, int ai, int aj, int ak) {
   constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
   const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
   const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);

@@ -41,13 +41,13 @@ __device__ __forceinline__ BlockCtx<BW> blockCtx(uint32_t firstBlock, uint32_t c
 // of all levels in level order (level < 0), BPW blocks per workgroup and iteration, with the
 // counts read from the device control words: no workgroup is spent on free blocks and the host
 // needs no block counts.
-__device__ __forceinline__ uint32_t slotTotal(const DG& g, int level) {
+__device__ __forceinline__ uint32_t slotTotal(const DGBase& g, int level) {
   if (level >= 0) return g.ctl[CTL_LOADS + level];
   uint32_t n = 0;
   for (int l = 0; l < g.L; l++) n += g.ctl[CTL_LOADS + l];
   return n;
 }
-__device__ __forceinline__ uint32_t slotBlock(const DG& g, int level, uint32_t s) {
+__device__ __forceinline__ uint32_t slotBlock(const DGBase& g, int level, uint32_t s) {
   if (level >= 0) return g.offs[level] + s;
   for (int l = 0; l < g.L; l++) {
     const uint32_t n = g.ctl[CTL_LOADS + l];
@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t slotStride() {
   return gridDim.x * Geo<BW>::BPW;
 }
 template <int BW>
-__device__ __forceinline__ BlockCtx<BW> slotCtx(const DG& g, int level, uint32_t base, uint32_t total) {
+__device__ __forceinline__ BlockCtx<BW> slotCtx(const DGBase& g, int level, uint32_t base, uint32_t total) {
   constexpr int BV = Geo<BW>::BV;
   BlockCtx<BW> c;
   c.sub = threadIdx.x / BV;
@@ -84,9 +84,9 @@ __device__ __forceinline__ int apronIndexOfLocal(int t) {
   return ((x + 1) * A + (y + 1)) * A + (z + 1);
 }
 
-// Stage one scalar field of a block (interior + 6 face layers) into tile[A^3].
-template <int BW>
-__device__ __forceinline__ void stageScalar(const DG& g, const float* __restrict__ f, float* tile, uint32_t b, int t) {
+// Stage one scalar field of a block (interior + 6 face layers) into tile[A^3] (widened to FP32).
+template <int BW, class SF>
+__device__ __forceinline__ void stageScalar(const DGBase& g, const SF* __restrict__ f, float* tile, uint32_t b, int t) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE;
   tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
   for (int fc = t; fc < FACE; fc += BV) {
@@ -97,8 +97,8 @@ __device__ __forceinline__ void stageScalar(const DG& g, const float* __restrict
 }
 
 // Fluidity of the six apron neighbours of local cell t (1 when the field is known to be 1).
-template <int BW, bool FLUIDITY>
-__device__ __forceinline__ void neighbourFluidity(const DG& g, uint32_t b, int t, float w[6]) {
+template <int BW, bool FLUIDITY, class G>
+__device__ __forceinline__ void neighbourFluidity(const G& g, uint32_t b, int t, float w[6]) {
   if (!FLUIDITY) {
 #pragma unroll
     for (int k = 0; k < 6; k++) w[k] = 1.f;
@@ -120,9 +120,9 @@ __device__ __forceinline__ void neighbourFluidity(const DG& g, uint32_t b, int t
 // k_dcgrid_jacobi / k_dcgrid_jacobi_inv (dcgrid_multigrid_solver.cu:5-41):
 // out = (sum of 6 apron neighbours of `in` - (scale*dx)^2 * div) / 6 on every cell of every
 // block of the level. Neighbours on coarser levels are read from the same array `in`.
-template <int BW>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(DG g, int level, const float* __restrict__ in,
-                                                         float* __restrict__ out, float alpha) {
+template <int BW, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(G g, int level, const typename G::Scalar* __restrict__ in,
+                                                         typename G::Scalar* __restrict__ out, float alpha) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float tiles[BPW][A3];
   const uint32_t total = slotTotal(g, level);
@@ -144,8 +144,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(DG g, int level, const f
 // ------------------------------------------------------------------ divergence
 // k_dcgrid_calc_divergence (dcgrid_fluid.cu:174-230): zero p, p~ and div on all cells; stage the
 // velocity with boundary conditions on the apron faces; central divergence on leaf cells.
-template <int BW, bool FLUIDITY>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(DG g, dcg_params P) {
+template <int BW, bool FLUIDITY, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
                 BB = Geo<BW>::BB, SBB = Geo<BW>::SBB;
   __shared__ float3 tiles[BPW][A3];
@@ -188,8 +188,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(DG g, dcg_params P) 
 
 // ------------------------------------------------------------------ gradient subtraction
 // k_dcgrid_apply_pressure (dcgrid_fluid.cu:232-259), leaf cells only.
-template <int BW, bool FLUIDITY>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(DG g, dcg_params P) {
+template <int BW, bool FLUIDITY, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, SBB = Geo<BW>::SBB;
   __shared__ float tiles[BPW][A3];
   const uint32_t total = slotTotal(g, -1);
@@ -222,8 +222,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(DG g, dcg_params
 // ------------------------------------------------------------------ vorticity
 // k_dcgrid_calc_vorticity (dcgrid_fluid.cu:146-172): curl of fluidity-weighted velocity on the
 // apron, every cell of every allocated block (no boundary conditions).
-template <int BW, bool FLUIDITY>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(DG g, dcg_params P) {
+template <int BW, bool FLUIDITY, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE;
   __shared__ float3 tiles[BPW][A3];
   const uint32_t total = slotTotal(g, -1);
@@ -272,9 +272,22 @@ __device__ __forceinline__ float addv(float a, float b) { return a + b; }
 __device__ __forceinline__ float3 addv(float3 a, float3 b) { return f3add(a, b); }
 __device__ __forceinline__ float mulv(float a, float s) { return a * s; }
 __device__ __forceinline__ float3 mulv(float3 a, float s) { return f3mul(a, s); }
+// FP32 arithmetic type of a stored element type.
+template <class T>
+struct Wide {
+  using type = float;
+};
+template <>
+struct Wide<float3> {
+  using type = float3;
+};
+template <class S>
+struct Wide<StoreV3<S>> {
+  using type = float3;
+};
 
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_restrict(DG g, int level, T* __restrict__ ch) {
+__global__ void __launch_bounds__(256) k_restrict(DGBase g, int level, T* __restrict__ ch) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, GPB = BV / 8;
   const uint32_t groups = g.ctl[CTL_LOADS + level] * GPB;  // allocated prefix of the level slice
   for (uint32_t gi = blockIdx.x * 256 + threadIdx.x; gi < groups; gi += gridDim.x * 256) {
@@ -283,10 +296,11 @@ __global__ void __launch_bounds__(256) k_restrict(DG g, int level, T* __restrict
     if (ps == NFU) continue;
     const uint32_t grp = gi % GPB;
     const uint32_t parentCell = (ps / 8) * BV + ((ps % 8) << SBB) + grp;
+    using W = typename Wide<T>::type;
     const T* src = ch + (size_t)b * BV + 8 * grp;
-    T acc = zeroOf<T>();
+    W acc = zeroOf<W>();
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc = addv(acc, src[i]);
+    for (int i = 0; i < 8; i++) acc = addv(acc, (W)src[i]);
     ch[parentCell] = mulv(acc, .125f);
   }
 }
@@ -296,7 +310,7 @@ __global__ void __launch_bounds__(256) k_restrict(DG g, int level, T* __restrict
 // (dcgrid_adaptation.cu:104-126, dcgrid_multigrid_solver.cu:51-63); order 000,001,010,100,011,
 // 101,110,111 of the reference.
 template <int BW>
-__device__ __forceinline__ void parentStencil(const DG& g, uint32_t ps, int x, int y, int z, uint32_t q[8]) {
+__device__ __forceinline__ void parentStencil(const DGBase& g, uint32_t ps, int x, int y, int z, uint32_t q[8]) {
   const uint32_t pb = ps / 8;
   const int ai = 1 + (x / 2) % BW, aj = 1 + (y / 2) % BW, ak = 1 + (z / 2) % BW;
   const int si = x % 2 ? 1 : -1, sj = y % 2 ? 1 : -1, sk = z % 2 ? 1 : -1;
@@ -311,8 +325,8 @@ __device__ __forceinline__ void parentStencil(const DG& g, uint32_t ps, int x, i
 }
 
 // k_dcgrid_prolongate (dcgrid_multigrid_solver.cu:43-76)
-template <int BW>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(DG g, int level) {
+template <int BW, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(G g, int level) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t total = slotTotal(g, level);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
@@ -325,7 +339,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(DG g, int level) {
     locPos<BB>((uint32_t)c.t, lx, ly, lz);
     uint32_t q[8];
     parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
-    const float* p = g.pre;
+    const auto* p = g.pre;
     g.pre[c.b * BV + c.t] = (27.f * p[q[0]] + 9.f * (p[q[1]] + p[q[2]] + p[q[3]]) +
                              3.f * (p[q[4]] + p[q[5]] + p[q[6]]) + p[q[7]]) /
                             64.f;
@@ -334,8 +348,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(DG g, int level) {
 
 // k_dcgrid_propagate_values (dcgrid_adaptation.cu:92-143) for the blocks of `level` touched by
 // this adaptation: the moved blocks (list) and the new children (a contiguous index range).
-template <int BW, bool SRC>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_propagate(DG g, dcg_params P, int level,
+template <int BW, bool SRC, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_propagate(G g, dcg_params P, int level,
                                                             const uint32_t* __restrict__ moved) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW;
   const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
@@ -354,19 +368,19 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_propagate(DG g, dcg_params P, i
     parentStencil<BW>(g, ps, x, y, z, q);
     const float w27 = 27.f / 64.f, w9 = 9.f / 64.f, w3 = 3.f / 64.f, w1 = 1.f / 64.f;
     const uint32_t cell = b * BV + t;
-    const float* d = g.den;
+    const auto* d = g.den;
     g.den[cell] = w27 * d[q[0]] + w9 * (d[q[1]] + d[q[2]] + d[q[3]]) + w3 * (d[q[4]] + d[q[5]] + d[q[6]]) + w1 * d[q[7]];
     if (SRC) {
-      const float* tt = g.tem;
+      const auto* tt = g.tem;
       g.tem[cell] =
           w27 * tt[q[0]] + w9 * (tt[q[1]] + tt[q[2]] + tt[q[3]]) + w3 * (tt[q[4]] + tt[q[5]] + tt[q[6]]) + w1 * tt[q[7]];
       if (P.moist) {
-        const float* vv = g.vap;
+        const auto* vv = g.vap;
         g.vap[cell] =
             w27 * vv[q[0]] + w9 * (vv[q[1]] + vv[q[2]] + vv[q[3]]) + w3 * (vv[q[4]] + vv[q[5]] + vv[q[6]]) + w1 * vv[q[7]];
       }
     }
-    const float3* v = g.vel;
+    const auto* v = g.vel;
     const float3 a0 = f3mul(v[q[0]], w27);
     const float3 a1 = f3mul(f3add(f3add(v[q[1]], v[q[2]]), v[q[3]]), w9);
     const float3 a2 = f3mul(f3add(f3add(v[q[4]], v[q[5]]), v[q[6]]), w3);
@@ -385,7 +399,8 @@ struct CrossSample {
   float w[8];
   int x0, y0, z0, scale;
   bool zero;
-  __device__ __forceinline__ void init(const DG& g, float px, float py, float pz) {
+  template <class G>
+  __device__ __forceinline__ void init(const G& g, float px, float py, float pz) {
     const int ix = clampi((int)floorf(px), 0, g.gx - 1);
     const int iy = clampi((int)floorf(py), 0, g.gy - 1);
     const int iz = clampi((int)floorf(pz), 0, g.gz - 1);
@@ -434,7 +449,7 @@ struct CrossSample {
 
 // Backtraced foot point of a leaf cell (dcgrid_fluid.cu:115-122).
 template <int BW>
-__device__ __forceinline__ void footPoint(const DG& g, const dcg_params& P, uint32_t b, int t, float3 v, float& px,
+__device__ __forceinline__ void footPoint(const DGBase& g, const dcg_params& P, uint32_t b, int t, float3 v, float& px,
                                           float& py, float& pz, int& x, int& y, int& z, int& level) {
   constexpr int BB = Geo<BW>::BB;
   const int3 p0 = g.pos[b];
@@ -461,7 +476,7 @@ struct TileSample {
   float w[8];
   int x0, y0, z0, scale;
   bool zero;
-  __device__ __forceinline__ bool init(const DG& g, float px, float py, float pz, uint32_t b, int3 p0, int lb,
+  __device__ __forceinline__ bool init(const DGBase& g, float px, float py, float pz, uint32_t b, int3 p0, int lb,
                                        const float* fluTile) {
     constexpr int A = Geo<BW>::A, BB = Geo<BW>::BB;
     const int ix = clampi((int)floorf(px), 0, g.gx - 1);
@@ -517,8 +532,8 @@ struct TileSample {
 
 // Stage a block's values over its whole apron ((BW+2)^3 entries, edges and corners included):
 // tile[a] = field[cell of apron entry a], raw (boundary conditions are applied per sample).
-template <int BW, class T>
-__device__ __forceinline__ void stageApron(const DG& g, const T* __restrict__ f, T* tile, uint32_t b, int t) {
+template <int BW, class T, class SF>
+__device__ __forceinline__ void stageApron(const DGBase& g, const SF* __restrict__ f, T* tile, uint32_t b, int t) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3;
   tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
   for (int a = t; a < A3; a += BV) {
@@ -531,8 +546,8 @@ __device__ __forceinline__ void stageApron(const DG& g, const T* __restrict__ f,
 // k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension). The block's
 // velocity (and fluidity) with the full apron are staged in LDS; samples whose foot point stays
 // in the block read the tile, the others gather through the neighbour tables.
-template <int BW, bool FLUIDITY, bool BUOY>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_params P) {
+template <int BW, bool FLUIDITY, bool BUOY, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float3 vtiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
@@ -595,8 +610,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(DG g, dcg_param
 
 // k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
 // with the same LDS fast path as the velocity.
-template <int BW, bool FLUIDITY, bool SRC, bool MOIST>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params P) {
+template <int BW, bool FLUIDITY, bool SRC, bool MOIST, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params P) {
   // SRC: temperature is advected (sphere source or moist atmosphere); MOIST: vapor too, then the
   // saturation adjustment of (vapor, cloud water = density, temperature).
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
@@ -696,8 +711,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(DG g, dcg_params
 // k_dcgrid_debug_stats (dcgrid_structure.cu:224-251): per block, sum over leaf cells in apron
 // [i][j][k] order of scale*|div - (sum p_n - 6p)*rdx^2/scale^2|. One thread per block keeps the
 // reference's summation order.
-template <int BW>
-__global__ void __launch_bounds__(64) k_residual(DG g, dcg_params P, float* __restrict__ stats) {
+template <int BW, class G>
+__global__ void __launch_bounds__(64) k_residual(G g, dcg_params P, float* __restrict__ stats) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
   const uint32_t b = blockIdx.x * 64 + threadIdx.x;
   if (b >= g.B) return;
@@ -714,7 +729,7 @@ __global__ void __launch_bounds__(64) k_residual(DG g, dcg_params P, float* __re
       for (int k = 1; k <= BW; k++) {
         const uint32_t cidx = apronCell<BW>(g, b, i, j, k);
         if (g.chi[cidx >> SBB] != NFU) continue;
-        const float* p = g.pre;
+        const auto* p = g.pre;
         const float pc = p[cidx];
         const float pl = p[apronCell<BW>(g, b, i - 1, j, k)], pr = p[apronCell<BW>(g, b, i + 1, j, k)];
         const float pd = p[apronCell<BW>(g, b, i, j - 1, k)], pu = p[apronCell<BW>(g, b, i, j + 1, k)];
@@ -727,8 +742,8 @@ __global__ void __launch_bounds__(64) k_residual(DG g, dcg_params P, float* __re
 }
 
 // ------------------------------------------------------------------ seeded scene (extension)
-template <int BW>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_seed(DG g, dcg_params P) {
+template <int BW, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_seed(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const BlockCtx<BW> c = blockCtx<BW>(0, g.B);
   if (!c.valid || g.lev[c.b] == FREE) return;

@@ -21,7 +21,7 @@ namespace dcg {
 // Face neighbour (dir 0..5: x-, x+, y-, y+, z-, z+) of local cell t of block b, as the apron
 // table holds it; `coarser` tells whether it lies on a coarser level.
 template <int BW>
-__device__ __forceinline__ uint32_t faceNb(const DG& g, uint32_t b, int t, int dir, bool& coarser) {
+__device__ __forceinline__ uint32_t faceNb(const DGBase& g, uint32_t b, int t, int dir, bool& coarser) {
   constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
   int x, y, z;
   locPos<BB>((uint32_t)t, x, y, z);
@@ -46,7 +46,7 @@ __device__ __forceinline__ uint32_t faceNb(const DG& g, uint32_t b, int t, int d
 
 // Parent cell (level + 1) of cell t of block b: the restriction-group mapping of accumulate<T>.
 template <int BW>
-__device__ __forceinline__ uint32_t parentCellOf(const DG& g, uint32_t b, int t) {
+__device__ __forceinline__ uint32_t parentCellOf(const DGBase& g, uint32_t b, int t) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
   const uint32_t ps = g.par[b];
   return (ps / 8) * BV + ((ps % 8) << SBB) + (uint32_t)(t >> 3);
@@ -55,7 +55,7 @@ __device__ __forceinline__ uint32_t parentCellOf(const DG& g, uint32_t b, int t)
 // reach[c] = finest level with cells inside cell c (its own level for a leaf cell), for the
 // cells of `level`; levels are processed finest first.
 template <int BW>
-__global__ void __launch_bounds__(256) k_sv_reach(DG g, int level, uint8_t* __restrict__ reach) {
+__global__ void __launch_bounds__(256) k_sv_reach(DGBase g, int level, uint8_t* __restrict__ reach) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -76,8 +76,8 @@ __global__ void __launch_bounds__(256) k_sv_reach(DG g, int level, uint8_t* __re
 // zero-guess residual R0 and R0^2 go to part[4 * blockIdx.x ...]. A singular level passes the
 // mean of R as `shift`, so the zero-mean part is formed before the rounding to T: rounding R
 // first would leave an error of T's epsilon times that mean, which does not shrink with R.
-template <int BW, class TX, class T>
-__global__ void __launch_bounds__(256) k_sv_residual(DG g, int level, const TX* __restrict__ x,
+template <int BW, class TX, class T, class G>
+__global__ void __launch_bounds__(256) k_sv_residual(G g, int level, const TX* __restrict__ x,
                                                      T* __restrict__ rho, double* __restrict__ part, float alpha,
                                                      float h2, double shift) {
   constexpr int BV = Geo<BW>::BV;
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_sv_residual(DG g, int level, const TX* 
 
 // rho -= shift on the cells of a level (zero-mean right-hand side of a singular level).
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_shift(DG g, int level, T* __restrict__ v, double shift) {
+__global__ void __launch_bounds__(256) k_sv_shift(DGBase g, int level, T* __restrict__ v, double shift) {
   constexpr int BV = Geo<BW>::BV;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_sv_shift(DG g, int level, T* __restrict
 
 // Sum of v over the cells of a level (per-workgroup partials).
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_sum(DG g, int level, const T* __restrict__ v, double* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_sv_sum(DGBase g, int level, const T* __restrict__ v, double* __restrict__ part) {
   constexpr int BV = Geo<BW>::BV;
   __shared__ double red[256];
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_sv_sum(DG g, int level, const T* __rest
 
 // e = 0, rho = 0 on the cells of a level.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_zero(DG g, int level, T* __restrict__ e, T* __restrict__ rho) {
+__global__ void __launch_bounds__(256) k_sv_zero(DGBase g, int level, T* __restrict__ e, T* __restrict__ rho) {
   constexpr int BV = Geo<BW>::BV;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(256) k_sv_zero(DG g, int level, T* __restrict_
 // between the boundary cell and that neighbour (where the finest grid puts it), which a ghost
 // value gm * e_c (gm <= 0, from the grid's depth) reproduces.
 template <int BW, class T>
-__device__ __forceinline__ void svStencil(const DG& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
+__device__ __forceinline__ void svStencil(const DGBase& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
                                           const uint8_t* __restrict__ reach, T gm, T& sum, T& diag) {
   const uint32_t c = b * Geo<BW>::BV + t;
   sum = (T)0;
@@ -185,7 +185,7 @@ __device__ __forceinline__ void svStencil(const DG& g, uint32_t b, int t, int lm
 
 // One red-black Gauss-Seidel half-sweep on the active cells of `level` with parity `color`.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_rbgs(DG g, int level, int lmin, int color, T* __restrict__ e,
+__global__ void __launch_bounds__(256) k_sv_rbgs(DGBase g, int level, int lmin, int color, T* __restrict__ e,
                                                  const T* __restrict__ rho, const uint8_t* __restrict__ reach,
                                                  float h2, float gm) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) k_sv_rbgs(DG g, int level, int lmin, int 
 // cells (level + 1): the mean over each restriction group (inactive cells contribute 0). The
 // parents' correction starts at 0.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_restrict(DG g, int level, int lmin, T* __restrict__ e,
+__global__ void __launch_bounds__(256) k_sv_restrict(DGBase g, int level, int lmin, T* __restrict__ e,
                                                      T* __restrict__ rho, const uint8_t* __restrict__ reach,
                                                      float h2, float gm) {
   constexpr int BV = Geo<BW>::BV, GPB = BV / 8;
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) k_sv_restrict(DG g, int level, int lmin, 
 // k_prolongate over the parent's apron, with any stencil cell that is not an active cell of the
 // parent level (coarser, or outside the region being solved) replaced by the own parent cell.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_prolong(DG g, int level, int lmin, T* __restrict__ e,
+__global__ void __launch_bounds__(256) k_sv_prolong(DGBase g, int level, int lmin, T* __restrict__ e,
                                                     const uint8_t* __restrict__ reach) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
@@ -274,7 +274,7 @@ __device__ __forceinline__ uint32_t uAt(const UDim& D, int x, int y, int z) {
 // Top-level residual restricted into the first pyramid grid (dims of the top level / 2): the
 // 8 cells of a restriction group form a 2x2x2 cube.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_restrict_top(DG g, int lmin, const T* __restrict__ e,
+__global__ void __launch_bounds__(256) k_sv_restrict_top(DGBase g, int lmin, const T* __restrict__ e,
                                                          const T* __restrict__ rho,
                                                          const uint8_t* __restrict__ reach, T* __restrict__ ue,
                                                          T* __restrict__ urho, uint8_t* __restrict__ uact, UDim D1,
@@ -333,7 +333,7 @@ __device__ __forceinline__ T uProlong(const UDim& Dc, int x, int y, int z, int w
 
 // e += P e(pyramid) on the active top-level cells.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_prolong_top(DG g, int lmin, T* __restrict__ e,
+__global__ void __launch_bounds__(256) k_sv_prolong_top(DGBase g, int lmin, T* __restrict__ e,
                                                         const uint8_t* __restrict__ reach, const T* __restrict__ ue,
                                                         const uint8_t* __restrict__ uact, UDim D1) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(1024) k_svu_bottom(UDim D, T* __restrict__ e, 
 
 // x += e on the cells of a level (the refinement update; FP64 in the mixed solve).
 template <int BW, class TX, class T>
-__global__ void __launch_bounds__(256) k_sv_update(DG g, int level, TX* __restrict__ x, const T* __restrict__ e,
+__global__ void __launch_bounds__(256) k_sv_update(DGBase g, int level, TX* __restrict__ x, const T* __restrict__ e,
                                                    const uint8_t* __restrict__ reach) {
   constexpr int BV = Geo<BW>::BV;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
@@ -458,7 +458,7 @@ __global__ void __launch_bounds__(256) k_sv_update(DG g, int level, TX* __restri
 
 // Initial guess of a level from its parent level: the 27/9/3/1 stencil of k_prolongate on x.
 template <int BW, class TX>
-__global__ void __launch_bounds__(256) k_sv_prolong_x(DG g, int level, TX* __restrict__ x) {
+__global__ void __launch_bounds__(256) k_sv_prolong_x(DGBase g, int level, TX* __restrict__ x) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -478,8 +478,8 @@ __global__ void __launch_bounds__(256) k_sv_prolong_x(DG g, int level, TX* __res
 }
 
 // pressure = (float) x on the cells of a level.
-template <int BW, class TX>
-__global__ void __launch_bounds__(256) k_sv_store(DG g, int level, const TX* __restrict__ x) {
+template <int BW, class TX, class G>
+__global__ void __launch_bounds__(256) k_sv_store(G g, int level, const TX* __restrict__ x) {
   constexpr int BV = Geo<BW>::BV;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {

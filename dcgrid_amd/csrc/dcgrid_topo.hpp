@@ -22,8 +22,8 @@ namespace dcg {
 // k_dcgrid_activate_level (dcgrid_structure.cu:104-178) for one dense level: position, level,
 // parent subblock, children on a dense finer level, neighbour table (same-level neighbours, the
 // block itself for out-of-domain directions), zero fields and fluidity.
-template <int BW>
-__global__ void __launch_bounds__(256) k_activate_dense(DG g, dcg_params P, int level) {
+template <int BW, class G>
+__global__ void __launch_bounds__(256) k_activate_dense(G g, dcg_params P, int level) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t n = (uint32_t)(g.rx[level] * g.ry[level] * g.rz[level]);
   const uint32_t k = blockIdx.x * 256 + threadIdx.x;
@@ -68,14 +68,14 @@ __global__ void __launch_bounds__(256) k_activate_dense(DG g, dcg_params P, int 
 }
 
 // ------------------------------------------------------------------ scores
-__device__ __forceinline__ bool levelBlocked(const DG& g, int level) {
+__device__ __forceinline__ bool levelBlocked(const DGBase& g, int level) {
   return (level == 0 && g.ctl[CTL_LOADS + 0] == g.fullL[0]) ||
          (level > 0 && g.ctl[CTL_LOADS + level - 1] == g.fullL[level - 1]);
 }
 
 // k_dcgrid_calc_subblock_scores, geometric score (dcgrid_adaptation.cu:10-34).
 template <int BW>
-__global__ void __launch_bounds__(256) k_sub_scores_geo(DG g, float* __restrict__ sbs) {
+__global__ void __launch_bounds__(256) k_sub_scores_geo(DGBase g, float* __restrict__ sbs) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= 8 * g.B) return;
   const uint32_t b = s / 8;
@@ -101,8 +101,8 @@ __global__ void __launch_bounds__(256) k_sub_scores_geo(DG g, float* __restrict_
 // with the apron faces in LDS; the subblock maximum is an LDS reduction. Computed once per
 // adaptation: the fields and the (pre-refresh) apron do not change between the score pass before
 // the moves and the one before refinement, so both passes see the same feature values.
-template <int BW>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(DG g, dcg_params P, float* __restrict__ feat) {
+template <int BW, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(G g, dcg_params P, float* __restrict__ feat) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
                 SBB = Geo<BW>::SBB;
   __shared__ float tiles[BPW][A3];
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(DG g, dcg_params P
 
 // Subblock scores from the feature values: -FLT_MAX where the reference's conditions exclude the
 // subblock (dcgrid_adaptation.cu:18-24).
-__global__ void __launch_bounds__(256) k_sub_scores_feat(DG g, const float* __restrict__ feat, float* __restrict__ sbs) {
+__global__ void __launch_bounds__(256) k_sub_scores_feat(DGBase g, const float* __restrict__ feat, float* __restrict__ sbs) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= 8 * g.B) return;
   const int level = g.lev[s / 8];
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_sub_scores_feat(DG g, const float* __re
 
 // k_dcgrid_accumulate_subblock_scores (dcgrid_adaptation.cu:42-63), including the reference's
 // ninth term s[8] (subblock 0 of the next block).
-__global__ void __launch_bounds__(256) k_block_scores(DG g, const float* __restrict__ sbs, float* __restrict__ bs) {
+__global__ void __launch_bounds__(256) k_block_scores(DGBase g, const float* __restrict__ sbs, float* __restrict__ bs) {
   const uint32_t b = blockIdx.x * 256 + threadIdx.x;
   if (b >= g.B) return;
   float r = -FLT_MAX;
@@ -221,7 +221,7 @@ __device__ __forceinline__ uint32_t wgExclusiveScan(uint32_t v, uint32_t* sh /*[
 // MODE 1: destination subblocks (subblocks of level+1, k = min(lim+1, n)).
 // Writes the k winners unsorted as packed keys (key32 << 32 | position) to cand.
 template <int MODE>
-__global__ void __launch_bounds__(1024) k_select(DG g, int level, const float* __restrict__ scores,
+__global__ void __launch_bounds__(1024) k_select(DGBase g, int level, const float* __restrict__ scores,
                                                  unsigned long long* __restrict__ cand) {
   __shared__ uint32_t hist[2048];
   __shared__ uint32_t sh[17];
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(1024) k_select(DG g, int level, const float* _
 
 // Exact position of every selected candidate (keys are unique): out[rank] = base + position.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_rank(DG g, int level, const unsigned long long* __restrict__ cand,
+__global__ void __launch_bounds__(256) k_rank(DGBase g, int level, const unsigned long long* __restrict__ cand,
                                               uint32_t* __restrict__ out) {
   __shared__ unsigned long long tile[1024];
   const uint32_t k = g.ctl[CTL_CAND];
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(256) k_rank(DG g, int level, const unsigned lo
 // failing position. Marks of blocks containing destination subblocks (dest[1..M], the
 // reference's "next" destination) are applied to the level above, which the next level's
 // selection reads.
-__global__ void __launch_bounds__(1024) k_match(DG g, int level, const uint32_t* __restrict__ mc,
+__global__ void __launch_bounds__(1024) k_match(DGBase g, int level, const uint32_t* __restrict__ mc,
                                                 const uint32_t* __restrict__ dc, float* __restrict__ bs,
                                                 const float* __restrict__ sbs) {
   __shared__ uint32_t sFail;
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(1024) k_match(DG g, int level, const uint32_t*
 // applied coarse to fine so a block moved under a parent that moved takes the parent's new
 // position.
 template <int BW>
-__global__ void __launch_bounds__(256) k_move(DG g, int level, const uint32_t* __restrict__ mc,
+__global__ void __launch_bounds__(256) k_move(DGBase g, int level, const uint32_t* __restrict__ mc,
                                               const uint32_t* __restrict__ dc) {
   const uint32_t m = blockIdx.x * 256 + threadIdx.x;
   if (m >= g.ctl[CTL_MOVES + level]) return;
@@ -388,14 +388,14 @@ __global__ void __launch_bounds__(256) k_move(DG g, int level, const uint32_t* _
 
 // Adds the active cells of this step (all cells of allocated blocks) to a 64-bit counter.
 template <int BW>
-__global__ void __launch_bounds__(64) k_count_step(DG g, unsigned long long* __restrict__ counter) {
+__global__ void __launch_bounds__(64) k_count_step(DGBase g, unsigned long long* __restrict__ counter) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long s = 0;
   for (int l = 0; l < g.L; l++) s += (unsigned long long)g.ctl[CTL_LOADS + l] * Geo<BW>::BV;
   *counter += s;
 }
 
-__global__ void __launch_bounds__(64) k_total_moves(DG g) {
+__global__ void __launch_bounds__(64) k_total_moves(DGBase g) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t s = 0;
   for (int l = 0; l < g.L - 1; l++) s += g.ctl[CTL_MOVES + l];
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(64) k_total_moves(DG g) {
 // Open addressing per level, probe stride 127 (dcgrid_utils.cuh:98-140). Slot placement depends
 // on the order of concurrent insertions; the content is a deterministic map.
 template <int BW>
-__device__ __forceinline__ void hashInsert(const DG& g, int level, int3 p, uint32_t blockIndex) {
+__device__ __forceinline__ void hashInsert(const DGBase& g, int level, int3 p, uint32_t blockIndex) {
   const uint32_t key = gridHash<BW>(p.x, p.y, p.z);
   const uint32_t o = g.hoff[level], size = g.hsize[level];
   uint32_t slot = key % size;
@@ -423,7 +423,7 @@ __device__ __forceinline__ void hashInsert(const DG& g, int level, int3 p, uint3
 }
 // cudaMemset of the tables before the refill (fluid_simulation_dcgrid.cu:327-328), only when
 // blocks were moved.
-__global__ void __launch_bounds__(256) k_hash_clear(DG g, uint32_t H) {
+__global__ void __launch_bounds__(256) k_hash_clear(DGBase g, uint32_t H) {
   if (g.ctl[CTL_TOTAL_MOVES] == 0) return;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < H; i += gridDim.x * 256) {
     g.hkey[i] = 0xFFFFFFFFu;
@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(256) k_hash_clear(DG g, uint32_t H) {
 }
 // k_dcgrid_refill_hash_table (dcgrid_adaptation.cu:180-201)
 template <int BW>
-__global__ void __launch_bounds__(256) k_hash_refill(DG g) {
+__global__ void __launch_bounds__(256) k_hash_refill(DGBase g) {
   if (g.ctl[CTL_TOTAL_MOVES] == 0) return;
   const uint32_t b = blockIdx.x * 256 + threadIdx.x;
   if (b >= g.B || g.lev[b] == FREE) return;
@@ -444,7 +444,7 @@ __global__ void __launch_bounds__(256) k_hash_refill(DG g) {
 // 469): candidates are subblocks of level l with score > threshold in index order; when there are
 // more than the limit, the largest indices are kept. Writes the selected list (ascending) and the
 // count; children are allocated by rank.
-__global__ void __launch_bounds__(1024) k_refine_select(DG g, dcg_params P, const float* __restrict__ sbs,
+__global__ void __launch_bounds__(1024) k_refine_select(DGBase g, dcg_params P, const float* __restrict__ sbs,
                                                         uint32_t* __restrict__ list) {
   __shared__ uint32_t sh[17];
   __shared__ uint32_t sTotal;
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(1024) k_refine_select(DG g, dcg_params P, cons
 // k_dcgrid_refine_subblocks (dcgrid_adaptation.cu:145-178) with allocation by rank: the r-th
 // selected subblock of level l gets block offs[l-1] + loads[l-1] + r.
 template <int BW>
-__global__ void __launch_bounds__(256) k_refine_apply(DG g, const uint32_t* __restrict__ list) {
+__global__ void __launch_bounds__(256) k_refine_apply(DGBase g, const uint32_t* __restrict__ list) {
   const int l = blockIdx.y + 1;
   const uint32_t r = blockIdx.x * 256 + threadIdx.x;
   const uint32_t take = g.ctl[CTL_REFINES + l];
@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(256) k_refine_apply(DG g, const uint32_t* __re
 // point to the block itself); other blocks follow a neighbour that moved, or step into the child
 // of a refined coarser neighbour.
 template <int BW>
-__global__ void __launch_bounds__(256) k_refresh(DG g) {
+__global__ void __launch_bounds__(256) k_refresh(DGBase g) {
   constexpr int BB = Geo<BW>::BB;
   const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
   const uint32_t b = gid / 27, d = gid % 27;

@@ -84,6 +84,11 @@ enum { DCG_SOURCE_NONE = 0, DCG_SOURCE_SPHERE_BUOYANT = 1 };
  * (src/sdf.cuh:8-18) or a seeded fBm heightfield (extension). */
 enum { DCG_SOLID_SPHERE = 0, DCG_SOLID_TERRAIN = 1 };
 
+/* Storage format of the DCGrid fields in HBM (extension): every field value is rounded to nearest
+ * even when a kernel stores it and widened to FP32 when loaded; arithmetic stays FP32. Fields are
+ * exported and imported as FP32 (exact widening, rounding on import). */
+enum { DCG_STORAGE_FP32 = 0, DCG_STORAGE_FP16 = 1, DCG_STORAGE_BF16 = 2 };
+
 /*
  * Simulation parameters. The first block equals the simulation fields of the reference's
  * SimParams (src/data/sim_params.h:14-30; render fields are out of scope). Everything
@@ -149,7 +154,8 @@ typedef struct dcg_params {
   float latent_heat;      /* temperature change per unit of condensed water */
   float wind[3];          /* ambient wind, world units per second (initial field, lateral inflow) */
   float adiabatic_rate;   /* cooling of rising air per world unit of ascent (moist only) */
-  int32_t reserved[3];
+  int32_t storage;        /* DCG_STORAGE_* of the fields, fixed at creation (DCGrid only) */
+  int32_t reserved[2];
 } dcg_params;
 
 /* Outcome of the last DCG_PROJECT_SOLVE projection, per level (0 finest). */

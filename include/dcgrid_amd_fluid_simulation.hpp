@@ -106,13 +106,15 @@ class FluidSimulation {
     dcg_params p = detail::globalParams();
     dcg_params cur;
     dcg_get_params(sim_, &cur);
-    // grid shape (and the moist atmosphere) are fixed at construction; the rest follows the upload
+    // grid shape, moist atmosphere and field storage are fixed at construction; the rest follows
+    // the upload
     p.gx = cur.gx;
     p.gy = cur.gy;
     p.gz = cur.gz;
     p.block_width = cur.block_width;
     p.max_levels = cur.max_levels;
     p.moist = cur.moist;
+    p.storage = cur.storage;
     detail::check(dcg_set_params(sim_, &p), sim_, "copySimParamsToDevice");
     generation_ = detail::globalGeneration();
   }

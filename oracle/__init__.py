@@ -55,6 +55,7 @@ def library():
             ("orc_terrain_height", [C.POINTER(Params), C.c_float, C.c_float], C.c_float),
             ("orc_cell_fluidity", [C.POINTER(Params)] + [C.c_int32] * 4, C.c_float),
             ("orc_saturation_adjust", [C.POINTER(Params)] + [C.POINTER(C.c_float)] * 3, None),
+            ("orc_round_storage", [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32], None),
         ]:
             f = getattr(lib, name)
             f.argtypes = args

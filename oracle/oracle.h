@@ -80,6 +80,8 @@ float orc_qsat(const dcg_params* p, float t);
 float orc_terrain_height(const dcg_params* p, float x, float z);
 float orc_cell_fluidity(const dcg_params* p, int32_t x, int32_t y, int32_t z, int32_t scale);
 void orc_saturation_adjust(const dcg_params* p, float* qv, float* qc, float* t);
+/* Field storage formats (extension): value after a store in format `storage` (DCG_STORAGE_*). */
+void orc_round_storage(const float* in, float* out, uint64_t n, int32_t storage);
 
 #ifdef __cplusplus
 }

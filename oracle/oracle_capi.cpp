@@ -20,6 +20,10 @@ extern "C" {
 
 int orc_create_uniform(const int32_t size[3], const dcg_params* p, int32_t precision, orc_sim** out) {
   if (!size || !p || !out || (precision != 32 && precision != 64)) return DCG_E_ARG;
+  if (p->storage != DCG_STORAGE_FP32) {
+    g_create_error = "16-bit field storage is implemented for the DCGrid";
+    return DCG_E_UNSUPPORTED;
+  }
   int st = 0;
   orc::Sim* s = orc::makeUniform(size, *p, precision, &st, &g_create_error);
   if (!s) return st;
@@ -30,6 +34,7 @@ int orc_create_uniform(const int32_t size[3], const dcg_params* p, int32_t preci
 int orc_create_dcgrid(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p, int32_t precision,
                       orc_sim** out) {
   if (!size || !p || !out || (precision != 32 && precision != 64)) return DCG_E_ARG;
+  if (p->storage < DCG_STORAGE_FP32 || p->storage > DCG_STORAGE_BF16) return DCG_E_ARG;
   int st = 0;
   orc::Sim* s = orc::makeDCGrid(size, max_num_blocks, *p, precision, &st, &g_create_error);
   if (!s) return st;
@@ -51,8 +56,9 @@ int orc_set_params(orc_sim* s, const dcg_params* p) {
   IMPL(s);
   if (!p) return DCG_E_ARG;
   if (p->gx != I->P.gx || p->gy != I->P.gy || p->gz != I->P.gz || p->block_width != I->P.block_width ||
-      p->max_levels != I->P.max_levels || p->moist != I->P.moist)
-    return I->fail(DCG_E_ARG, "grid size, block width, level cap and moist atmosphere are fixed at creation");
+      p->max_levels != I->P.max_levels || p->moist != I->P.moist || p->storage != I->P.storage)
+    return I->fail(DCG_E_ARG,
+                   "grid size, block width, level cap, moist atmosphere and field storage are fixed at creation");
   I->P = *p;
   return DCG_OK;
 }
@@ -201,6 +207,9 @@ float orc_cell_fluidity(const dcg_params* p, int32_t x, int32_t y, int32_t z, in
 }
 void orc_saturation_adjust(const dcg_params* p, float* qv, float* qc, float* t) {
   orc::saturationAdjust<float>(*p, *qv, *qc, *t);
+}
+void orc_round_storage(const float* in, float* out, uint64_t n, int32_t storage) {
+  for (uint64_t i = 0; i < n; i++) out[i] = orc::roundStorage(in[i], storage);
 }
 
 }  // extern "C"

@@ -260,6 +260,26 @@ inline float sourceTemperature(const dcg_params& P, int x, int y, int z, int lev
   return P.src_temperature + P.src_noise * (2.f * u - 1.f);
 }
 
+// ---------------------------------------------------------------- field storage formats
+// Value of v after a store into a field of storage format `storage` (dcg_params.storage, an
+// extension): IEEE binary16 or bfloat16, round to nearest even, read back exactly as float.
+// Restated from the formats' definitions: the quantum of v is 2^(E - mantissa bits) with E the
+// exponent of v clamped to the format's minimum normal exponent (subnormals share that quantum);
+// std::nearbyint rounds v / quantum to nearest even (default rounding mode) and both scalings
+// by a power of two are exact. binary16 overflows to infinity above 65504 after rounding.
+inline float roundStorage(float v, int storage) {
+  if (storage == DCG_STORAGE_FP32 || v == 0.f || !std::isfinite(v)) return v;
+  int e;
+  std::frexp(v, &e);  // |v| = m 2^e, m in [0.5, 1)
+  const int mant = storage == DCG_STORAGE_FP16 ? 10 : 7;
+  const int emin = storage == DCG_STORAGE_FP16 ? -14 : -126;
+  const int E = e - 1 < emin ? emin : e - 1;
+  const float q = std::ldexp(1.f, E - mant);
+  const float r = std::nearbyint(v / q) * q;
+  if (storage == DCG_STORAGE_FP16 && std::fabs(r) > 65504.f) return std::copysign(INFINITY, v);
+  return r;
+}
+
 struct Sim {
   dcg_params P;
   std::string err;

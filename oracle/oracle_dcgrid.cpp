@@ -52,6 +52,9 @@ struct DCGrid : Sim {
   std::vector<size_t> touched;
 
   int kind() const override { return DCG_KIND_DCGRID; }
+  // A value stored to a field (dcg_params.storage; FP64 values round through FP32).
+  R st(R v) const { return P.storage ? (R)roundStorage((float)v, P.storage) : v; }
+  V3<R> st(const V3<R>& v) const { return P.storage ? V3<R>{st(v.x), st(v.y), st(v.z)} : v; }
 
   // ---------------------------------------------------------------- planner
   // FluidSimulationDCGrid ctor, fluid_simulation_dcgrid.cu:9-140.
@@ -340,9 +343,9 @@ struct DCGrid : Sim {
               for (int k = 1; k <= BW; k++) {
                 const size_t c = ap(b, i, j, k);
                 den[c] = 0;
-                tem[c] = (R)P.t_ambient;
+                tem[c] = st((R)P.t_ambient);
                 vel[c] = V3<R>{0, 0, 0};
-                flu[c] = (R)cellFluidity(P, x + i - 1, y + j - 1, z + k - 1, scale);
+                flu[c] = st((R)cellFluidity(P, x + i - 1, y + j - 1, z + k - 1, scale));
               }
         }
   }
@@ -368,10 +371,10 @@ struct DCGrid : Sim {
         const R wb = flu[ci[a - 1]], wf = flu[ci[a + 1]];
         const V3<R> vl = vel[ci[a - A2]], vr = vel[ci[a + A2]], vd = vel[ci[a - A]], vu = vel[ci[a + A]];
         const V3<R> vb = vel[ci[a - 1]], vf = vel[ci[a + 1]];
-        vort[c] = mul(V3<R>{(wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y),
-                            (wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z),
-                            (wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)},
-                      alpha);
+        vort[c] = st(mul(V3<R>{(wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y),
+                               (wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z),
+                               (wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)},
+                         alpha));
       }
     }
   }
@@ -631,22 +634,22 @@ struct DCGrid : Sim {
         const size_t c = b * BV + l;
         size_t q[8];
         parentStencil(c, q);
-        den[c] = w27 * den[q[0]] + w9 * (den[q[1]] + den[q[2]] + den[q[3]]) +
-                 w3 * (den[q[4]] + den[q[5]] + den[q[6]]) + w1 * den[q[7]];
+        den[c] = st(w27 * den[q[0]] + w9 * (den[q[1]] + den[q[2]] + den[q[3]]) +
+                    w3 * (den[q[4]] + den[q[5]] + den[q[6]]) + w1 * den[q[7]]);
         if (src)
-          tem[c] = w27 * tem[q[0]] + w9 * (tem[q[1]] + tem[q[2]] + tem[q[3]]) +
-                   w3 * (tem[q[4]] + tem[q[5]] + tem[q[6]]) + w1 * tem[q[7]];
+          tem[c] = st(w27 * tem[q[0]] + w9 * (tem[q[1]] + tem[q[2]] + tem[q[3]]) +
+                      w3 * (tem[q[4]] + tem[q[5]] + tem[q[6]]) + w1 * tem[q[7]]);
         if (src && P.moist)
-          vap[c] = w27 * vap[q[0]] + w9 * (vap[q[1]] + vap[q[2]] + vap[q[3]]) +
-                   w3 * (vap[q[4]] + vap[q[5]] + vap[q[6]]) + w1 * vap[q[7]];
+          vap[c] = st(w27 * vap[q[0]] + w9 * (vap[q[1]] + vap[q[2]] + vap[q[3]]) +
+                      w3 * (vap[q[4]] + vap[q[5]] + vap[q[6]]) + w1 * vap[q[7]]);
         const V3<R> a = mul(vel[q[0]], w27);
         const V3<R> bb = mul(add(add(vel[q[1]], vel[q[2]]), vel[q[3]]), w9);
         const V3<R> cc = mul(add(add(vel[q[4]], vel[q[5]]), vel[q[6]]), w3);
         const V3<R> dd = mul(vel[q[7]], w1);
-        vel[c] = add(add(add(a, bb), cc), dd);
+        vel[c] = st(add(add(add(a, bb), cc), dd));
         int x, y, z;
         cellPos(c, x, y, z);
-        flu[c] = (R)cellFluidity(P, x, y, z, 1 << level);
+        flu[c] = st((R)cellFluidity(P, x, y, z, 1 << level));
       }
     }
   }
@@ -679,7 +682,7 @@ struct DCGrid : Sim {
         T acc = zero;
         const size_t c0 = b * BV + 8 * g;
         for (int i = 0; i < 8; i++) acc = addT(acc, ch[c0 + i]);
-        ch[parentCell] = mulT(acc, (R).125f);
+        ch[parentCell] = st(mulT(acc, (R).125f));
       }
     }
   }
@@ -799,7 +802,7 @@ struct DCGrid : Sim {
           const R bf = dt * ((R)P.buoy_beta * (tem[c] - ta) - (R)P.buoy_alpha * den[c]);
           nv.y = nv.y + bf;
         }
-        t_vel[c] = nv;
+        t_vel[c] = st(nv);
       }
     }
     copyAllocated(t_vel, vel);
@@ -854,11 +857,11 @@ struct DCGrid : Sim {
             R v = sampleScalar(s, vap, [&](R a, int xx, int yy, int zz, int sc) { return vaporBC(P, a, xx, yy, zz, sc); });
             t = t - ((R)P.dt * (R)P.adiabatic_rate) * vel[c].y;  // expansion cooling of rising air
             saturationAdjust(P, v, q, t);
-            t_vap[c] = v;
+            t_vap[c] = st(v);
           }
-          t_tem[c] = t;
+          t_tem[c] = st(t);
         }
-        t_den[c] = q;
+        t_den[c] = st(q);
       }
     }
     copyAllocated(t_den, den);
@@ -882,7 +885,7 @@ struct DCGrid : Sim {
       const int scale = 1 << lev[b];
       const size_t* ci = &cidx[b * A3];
       // staged velocity: interior raw, border with BCs
-      std::vector<V3<R>> st(A3, V3<R>{0, 0, 0});
+      std::vector<V3<R>> stg(A3, V3<R>{0, 0, 0});
       for (int l = 0; l < BV; l++) {
         const size_t c = b * BV + l;
         pre[c] = 0;
@@ -892,13 +895,13 @@ struct DCGrid : Sim {
         cellPos(c, x, y, z);
         const int a = apronOf(c);
         const int ax = a / A2, ay = (a / A) % A, az = a % A;
-        st[a] = vel[c];
-        if (ax == 1) st[a - A2] = velocityBC(P, vel[ci[a - A2]], x - 1, y, z, scale);
-        if (ay == 1) st[a - A] = velocityBC(P, vel[ci[a - A]], x, y - 1, z, scale);
-        if (az == 1) st[a - 1] = velocityBC(P, vel[ci[a - 1]], x, y, z - 1, scale);
-        if (ax == BW) st[a + A2] = velocityBC(P, vel[ci[a + A2]], x + 1, y, z, scale);
-        if (ay == BW) st[a + A] = velocityBC(P, vel[ci[a + A]], x, y + 1, z, scale);
-        if (az == BW) st[a + 1] = velocityBC(P, vel[ci[a + 1]], x, y, z + 1, scale);
+        stg[a] = vel[c];
+        if (ax == 1) stg[a - A2] = velocityBC(P, vel[ci[a - A2]], x - 1, y, z, scale);
+        if (ay == 1) stg[a - A] = velocityBC(P, vel[ci[a - A]], x, y - 1, z, scale);
+        if (az == 1) stg[a - 1] = velocityBC(P, vel[ci[a - 1]], x, y, z - 1, scale);
+        if (ax == BW) stg[a + A2] = velocityBC(P, vel[ci[a + A2]], x + 1, y, z, scale);
+        if (ay == BW) stg[a + A] = velocityBC(P, vel[ci[a + A]], x, y + 1, z, scale);
+        if (az == BW) stg[a + 1] = velocityBC(P, vel[ci[a + 1]], x, y, z + 1, scale);
       }
       const R alpha = (R).5f * (R)P.rdx / (R)scale;
       for (int l = 0; l < BV; l++) {
@@ -907,8 +910,8 @@ struct DCGrid : Sim {
         const int a = apronOf(c);
         const R wl = flu[ci[a - A2]], wr = flu[ci[a + A2]], wd = flu[ci[a - A]], wu = flu[ci[a + A]];
         const R wb = flu[ci[a - 1]], wf = flu[ci[a + 1]];
-        const V3<R>&vl = st[a - A2], &vr = st[a + A2], &vd = st[a - A], &vu = st[a + A], &vb = st[a - 1], &vf = st[a + 1];
-        dive[c] = alpha * (wr * vr.x - wl * vl.x + wu * vu.y - wd * vd.y + wf * vf.z - wb * vb.z);
+        const V3<R>&vl = stg[a - A2], &vr = stg[a + A2], &vd = stg[a - A], &vu = stg[a + A], &vb = stg[a - 1], &vf = stg[a + 1];
+        dive[c] = st(alpha * (wr * vr.x - wl * vl.x + wu * vu.y - wd * vd.y + wf * vf.z - wb * vb.z));
       }
     }
   }
@@ -924,9 +927,9 @@ struct DCGrid : Sim {
       for (int l = 0; l < BV; l++) {
         const size_t c = b * BV + l;
         const int a = apronOf(c);
-        out[c] = (in[ci[a - A2]] + in[ci[a + A2]] + in[ci[a - A]] + in[ci[a + A]] + in[ci[a - 1]] + in[ci[a + 1]] -
-                  alpha * dive[c]) /
-                 (R)6;
+        out[c] = st((in[ci[a - A2]] + in[ci[a + A2]] + in[ci[a - A]] + in[ci[a + A]] + in[ci[a - 1]] + in[ci[a + 1]] -
+                     alpha * dive[c]) /
+                    (R)6);
       }
     }
   }
@@ -940,9 +943,9 @@ struct DCGrid : Sim {
         const size_t c = b * BV + l;
         size_t q[8];
         parentStencil(c, q);
-        pre[c] = ((R)27 * pre[q[0]] + (R)9 * (pre[q[1]] + pre[q[2]] + pre[q[3]]) +
-                  (R)3 * (pre[q[4]] + pre[q[5]] + pre[q[6]]) + pre[q[7]]) /
-                 (R)64;
+        pre[c] = st(((R)27 * pre[q[0]] + (R)9 * (pre[q[1]] + pre[q[2]] + pre[q[3]]) +
+                     (R)3 * (pre[q[4]] + pre[q[5]] + pre[q[6]]) + pre[q[7]]) /
+                    (R)64);
       }
     }
   }
@@ -964,6 +967,7 @@ struct DCGrid : Sim {
         v.x = v.x - alpha * (flu[ci[a + A2]] * (pre[ci[a + A2]] - p) + flu[ci[a - A2]] * (p - pre[ci[a - A2]]));
         v.y = v.y - alpha * (flu[ci[a + A]] * (pre[ci[a + A]] - p) + flu[ci[a - A]] * (p - pre[ci[a - A]]));
         v.z = v.z - alpha * (flu[ci[a + 1]] * (pre[ci[a + 1]] - p) + flu[ci[a - 1]] * (p - pre[ci[a - 1]]));
+        v = st(v);
       }
     }
   }
@@ -1094,6 +1098,8 @@ struct DCGrid : Sim {
       if (level < L - 1) prolongate(level);  // initial guess from the solved coarser level
       solveLevel(level);
     }
+    // the solved pressure is stored once, at the end
+    for (size_t c = 0; c < C; c++) pre[c] = st(pre[c]);
     applyPressure();
     accumulate(vel, V3<R>{0, 0, 0});
     return DCG_OK;
@@ -1149,15 +1155,15 @@ struct DCGrid : Sim {
                     cz = ((float)z + .5f) * (float)scale;
         float u[3];
         initVelocity(P, cx, cy, cz, u);
-        vel[c] = V3<R>{(R)u[0], (R)u[1], (R)u[2]};
+        vel[c] = st(V3<R>{(R)u[0], (R)u[1], (R)u[2]});
         if (src && inSource(P, cx, cy, cz)) {
-          den[c] = (R)P.src_density;
-          tem[c] = (R)sourceTemperature(P, x, y, z, level);
+          den[c] = st((R)P.src_density);
+          tem[c] = st((R)sourceTemperature(P, x, y, z, level));
         } else {
           den[c] = 0;
-          tem[c] = (R)ambientT(P, cy);
+          tem[c] = st((R)ambientT(P, cy));
         }
-        if (P.moist) vap[c] = (R)ambientVapor(P, cy);
+        if (P.moist) vap[c] = st((R)ambientVapor(P, cy));
       }
     }
     accumulate(vel, V3<R>{0, 0, 0});
@@ -1236,11 +1242,11 @@ struct DCGrid : Sim {
     if (n != need) return fail(DCG_E_ARG, "size mismatch");
     if (f == DCG_FIELD_VELOCITY || f == DCG_FIELD_VORTICITY) {
       std::vector<V3<R>>& v = f == DCG_FIELD_VELOCITY ? vel : vort;
-      for (size_t i = 0; i < C; i++) v[i] = V3<R>{(R)src[3 * i], (R)src[3 * i + 1], (R)src[3 * i + 2]};
+      for (size_t i = 0; i < C; i++) v[i] = st(V3<R>{(R)src[3 * i], (R)src[3 * i + 1], (R)src[3 * i + 2]});
       return DCG_OK;
     }
     std::vector<R>* v = scalar(f);
-    for (size_t i = 0; i < n; i++) (*v)[i] = (R)src[i];
+    for (size_t i = 0; i < n; i++) (*v)[i] = st((R)src[i]);
     return DCG_OK;
   }
   int runStage(int stage, int arg) override {
