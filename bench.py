@@ -144,7 +144,8 @@ def cpu_baseline(w, warm_steps, budget_s):
 def adjoint_bench(w, args):
     """Differentiable control workload (uniform 128^3, 64 steps, checkpoints every 8): time of
     the plain forward rollout, of the recorded forward (with checkpoints) and of the backward
-    pass (segment recomputation + adjoint kernels); ratio = backward / plain forward."""
+    pass (segment recomputation + adjoint kernels); value = backward / plain forward, and the
+    adjoint steps alone (backward minus the recorded forward) / plain forward beside it."""
     import torch
     import dcgrid_amd
     from dcgrid_amd import scenes
@@ -199,6 +200,10 @@ def adjoint_bench(w, args):
         "forward_ms": round(1e3 * t_fwd, 3),
         "recorded_forward_ms": round(1e3 * t_tape, 3),
         "backward_ms": round(1e3 * t_bwd, 3),
+        # the backward pass re-runs every checkpoint segment forward (about the recorded
+        # forward's cost); what remains is the adjoint steps themselves
+        "adjoint_steps_ms": round(1e3 * (t_bwd - t_tape), 3),
+        "adjoint_step_ratio": round((t_bwd - t_tape) / t_fwd, 3),
         "forward_cell_steps_per_s": round(N * steps / t_fwd, 1),
         "grad_norm_velocity": float(torch.linalg.vector_norm(ov)),
     }

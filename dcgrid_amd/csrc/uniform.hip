@@ -35,13 +35,24 @@ struct UDims {
 __device__ __forceinline__ size_t uidx(const UGrid& g, int x, int y, int z) {
   return ((size_t)z * g.gy + y) * g.gx + x;
 }
+// Coordinates of linear index i of a grid w x h x ... (x fastest) in 32-bit arithmetic: grids
+// have fewer than 2^32 cells (checked at creation), and a 64-bit division is a long software
+// sequence on the GPU.
+__device__ __forceinline__ void ucoord(size_t i, int w, int h, int& x, int& y, int& z) {
+  const uint32_t ii = (uint32_t)i, uw = (uint32_t)w, uh = (uint32_t)h;
+  const uint32_t q = ii / uw;
+  x = (int)(ii - q * uw);
+  y = (int)(q % uh);
+  z = (int)(q / uh);
+}
 
 // k_uniform_set_solidity_ratio, uniformgrid_structure.cu:23-31
 __global__ void __launch_bounds__(256) k_u_fluidity(dcg_params P, UGrid g, UDims L, int scale) {
   const size_t n = (size_t)L.w * L.h * L.d;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+  int x, y, z;
+  ucoord(i, L.w, L.h, x, y, z);
   g.flu[L.off + i] = cellFluidity(P, x, y, z, scale);
 }
 
@@ -106,7 +117,8 @@ __global__ void __launch_bounds__(256) k_u_advect_velocity(dcg_params P, UGrid g
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(i, g.gx, g.gy, x, y, z);
   const float3 v = g.vel[i];
   const float px = ((float)x + .5f) - v.x * P.dt * P.rdx;
   const float py = ((float)y + .5f) - v.y * P.dt * P.rdx;
@@ -135,7 +147,8 @@ __global__ void __launch_bounds__(256) k_u_advect_density(dcg_params P, UGrid g)
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(i, g.gx, g.gy, x, y, z);
   const float3 v = g.vel[i];
   const float px = ((float)x + .5f) - v.x * P.dt * P.rdx;
   const float py = ((float)y + .5f) - v.y * P.dt * P.rdx;
@@ -171,7 +184,8 @@ __global__ void __launch_bounds__(256) k_u_divergence(dcg_params P, UGrid g) {
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(i, g.gx, g.gy, x, y, z);
   const size_t pl = (size_t)g.gx * g.gy;
   const size_t il = x > 0 ? i - 1 : i, ir = x < g.gx - 1 ? i + 1 : i;
   const size_t id = y > 0 ? i - g.gx : i, iu = y < g.gy - 1 ? i + g.gx : i;
@@ -191,30 +205,32 @@ __global__ void __launch_bounds__(256) k_u_divergence(dcg_params P, UGrid g) {
 }
 
 // k_uniform_restrict, uniformgrid_fluid.cu:134-160
-__global__ void __launch_bounds__(256) k_u_restrict(UGrid g, UDims L, UDims Lc) {
-  const size_t n = (size_t)L.w * L.h * L.d;
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+__device__ __forceinline__ void uRestrictCell(float* __restrict__ pre, float* __restrict__ tpre, float* __restrict__ div,
+                                              const UDims& L, const UDims& Lc, size_t i) {
+  int x, y, z;
+  ucoord(i, L.w, L.h, x, y, z);
   const size_t w = Lc.w, h = Lc.h;
   const size_t i000 = Lc.off + ((size_t)(2 * z) * h + 2 * y) * w + 2 * x;
   const size_t i001 = i000 + 1, i010 = i000 + w, i011 = i010 + 1;
   const size_t i100 = i000 + w * h, i101 = i100 + 1, i110 = i100 + w, i111 = i110 + 1;
   const size_t o = L.off + i;
-  g.pre[o] = 0.f;
-  g.tpre[o] = 0.f;
-  g.div[o] = .125f * (g.div[i000] + g.div[i001] + g.div[i010] + g.div[i011] + g.div[i100] + g.div[i101] +
-                      g.div[i110] + g.div[i111]);
+  pre[o] = 0.f;
+  tpre[o] = 0.f;
+  div[o] = .125f * (div[i000] + div[i001] + div[i010] + div[i011] + div[i100] + div[i101] + div[i110] + div[i111]);
+}
+__global__ void __launch_bounds__(256) k_u_restrict(UGrid g, UDims L, UDims Lc) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uRestrictCell(g.pre, g.tpre, g.div, L, Lc, i);
 }
 
 // calcPressure / k_uniform_jacobi(_inv), uniformgrid_fluid.cu:162-204. The workgroup covers a
 // 64x4 tile of an x-y plane so the y neighbours are in the same L1 lines.
-__global__ void __launch_bounds__(256) k_u_jacobi(const float* __restrict__ in, float* __restrict__ out,
-                                                  const float* __restrict__ div, UDims L, float alpha) {
-  const size_t n = (size_t)L.w * L.h * L.d;
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+__device__ __forceinline__ void uJacobiCell(const float* __restrict__ in, float* __restrict__ out,
+                                            const float* __restrict__ div, const UDims& L, float alpha, size_t i) {
+  int x, y, z;
+  ucoord(i, L.w, L.h, x, y, z);
   const size_t pl = (size_t)L.w * L.h;
   const size_t il = x > 0 ? i - 1 : i, ir = x < L.w - 1 ? i + 1 : i;
   const size_t id = y > 0 ? i - L.w : i, iu = y < L.h - 1 ? i + L.w : i;
@@ -222,13 +238,18 @@ __global__ void __launch_bounds__(256) k_u_jacobi(const float* __restrict__ in, 
   const float* p = in + L.off;
   out[L.off + i] = (p[il] + p[ir] + p[id] + p[iu] + p[ib] + p[iF] - alpha * div[L.off + i]) / 6.f;
 }
-
-// k_uniform_prolongate, uniformgrid_fluid.cu:206-237
-__global__ void __launch_bounds__(256) k_u_prolongate(float* __restrict__ pre, UDims L, UDims Lp) {
+__global__ void __launch_bounds__(256) k_u_jacobi(const float* __restrict__ in, float* __restrict__ out,
+                                                  const float* __restrict__ div, UDims L, float alpha) {
   const size_t n = (size_t)L.w * L.h * L.d;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+  uJacobiCell(in, out, div, L, alpha, i);
+}
+
+// k_uniform_prolongate, uniformgrid_fluid.cu:206-237
+__device__ __forceinline__ void uProlongCell(float* __restrict__ pre, const UDims& L, const UDims& Lp, size_t i) {
+  int x, y, z;
+  ucoord(i, L.w, L.h, x, y, z);
   const int w = L.w, h = L.h, d = L.d;
   const size_t i000 = Lp.off + ((size_t)(z / 2) * Lp.h + (y / 2)) * Lp.w + (x / 2);
   const int sx = (x == 0 || x == w - 1) ? 0 : 2 * (x % 2) - 1;
@@ -245,6 +266,75 @@ __global__ void __launch_bounds__(256) k_u_prolongate(float* __restrict__ pre, U
                     3.f * (pre[i011] + pre[i101] + pre[i110]) + pre[i111]) /
                    64.f;
 }
+__global__ void __launch_bounds__(256) k_u_prolongate(float* __restrict__ pre, UDims L, UDims Lp) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uProlongCell(pre, L, Lp, i);
+}
+
+// The levels of the pyramid small enough for one workgroup (<= kCoarseCells cells each),
+// finest first: their part of project() (restriction into them, the top level's sweeps, and
+// prolongation and sweeps back down to the first of them) in one launch. Their pressure,
+// temporary pressure and divergence live in LDS for the whole sequence (at most
+// kCoarseCells * 8/7 cells), phase after phase with a workgroup barrier between; each cell is
+// computed by the same per-cell code as the per-level kernels, and the results are written back.
+// Saves a launch per restriction, half-sweep and prolongation on levels whose work is far below
+// one launch's latency.
+constexpr size_t kCoarseCells = 4096;
+constexpr int kCoarseLds = 4096 + 512 + 64 + 8 + 1 + 1;  // >= sum of a geometric pyramid from 4096 cells
+struct ULevels {
+  UDims d[DCG_MAX_LEVELS];
+};
+__device__ __forceinline__ size_t ucells(const UDims& D) { return (size_t)D.w * D.h * D.d; }
+// Dimensions of levels Ls.. with offsets relative to level Ls (their LDS layout).
+__device__ __forceinline__ void coarseLocal(const ULevels& D, int L, int Ls, ULevels& S, size_t& total) {
+  total = 0;
+  for (int l = Ls; l < L; l++) {
+    S.d[l] = D.d[l];
+    S.d[l].off = D.d[l].off - D.d[Ls].off;
+    total = S.d[l].off + ucells(D.d[l]);
+  }
+}
+__global__ void __launch_bounds__(1024) k_u_coarse(UGrid g, ULevels D, int L, int Ls, int topPairs, int levelPairs,
+                                                   float dx) {
+  __shared__ float sp[kCoarseLds], st[kCoarseLds], sd[kCoarseLds];
+  ULevels S;
+  size_t total;
+  coarseLocal(D, L, Ls, S, total);
+  const size_t base = D.d[Ls].off;
+  for (size_t i = threadIdx.x; i < total; i += blockDim.x) {
+    sp[i] = g.pre[base + i];
+    st[i] = g.tpre[base + i];
+    sd[i] = g.div[base + i];
+  }
+  __syncthreads();
+  for (int l = Ls + 1; l < L; l++) {
+    for (size_t i = threadIdx.x; i < ucells(S.d[l]); i += blockDim.x) uRestrictCell(sp, st, sd, S.d[l], S.d[l - 1], i);
+    __syncthreads();
+  }
+  auto sweeps = [&](int l, int pairs) {
+    const int scale = 1 << l;
+    const float alpha = dx * dx * (float)scale * (float)scale;  // as jacobiPairs
+    for (int p = 0; p < pairs; p++) {
+      for (size_t i = threadIdx.x; i < ucells(S.d[l]); i += blockDim.x) uJacobiCell(sp, st, sd, S.d[l], alpha, i);
+      __syncthreads();
+      for (size_t i = threadIdx.x; i < ucells(S.d[l]); i += blockDim.x) uJacobiCell(st, sp, sd, S.d[l], alpha, i);
+      __syncthreads();
+    }
+  };
+  sweeps(L - 1, topPairs);
+  for (int l = L - 2; l >= Ls; l--) {
+    for (size_t i = threadIdx.x; i < ucells(S.d[l]); i += blockDim.x) uProlongCell(sp, S.d[l], S.d[l + 1], i);
+    __syncthreads();
+    sweeps(l, levelPairs);
+  }
+  for (size_t i = threadIdx.x; i < total; i += blockDim.x) {
+    g.pre[base + i] = sp[i];
+    g.tpre[base + i] = st[i];
+    g.div[base + i] = sd[i];
+  }
+}
 
 // k_uniform_apply_pressure, uniformgrid_fluid.cu:239-260
 template <bool FLUIDITY>
@@ -252,7 +342,8 @@ __global__ void __launch_bounds__(256) k_u_apply_pressure(dcg_params P, UGrid g)
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(i, g.gx, g.gy, x, y, z);
   const size_t pl = (size_t)g.gx * g.gy;
   const size_t il = x > 0 ? i - 1 : i, ir = x < g.gx - 1 ? i + 1 : i;
   const size_t id = y > 0 ? i - g.gx : i, iu = y < g.gy - 1 ? i + g.gx : i;
@@ -283,7 +374,8 @@ __global__ void __launch_bounds__(256) k_u_seed(dcg_params P, UGrid g) {
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(i, g.gx, g.gy, x, y, z);
   const float cx = (float)x + .5f, cy = (float)y + .5f, cz = (float)z + .5f;
   g.vel[i] = initVelocity(P, cx, cy, cz);
   if (P.source_mode == DCG_SOURCE_SPHERE_BUOYANT && inSource(P, cx, cy, cz)) {
@@ -311,6 +403,7 @@ struct UniformSim : SimBase {
   float* stats = nullptr;
   float* hstats = nullptr;
   bool customFluidity = false;
+  std::vector<char> fluKey;  // parameters the fluidity pyramid was last computed with
   uint64_t cellStepCount = 0;
 
   // Differentiable rollout: checkpoints of the state (density, temperature, velocity) every
@@ -345,6 +438,7 @@ struct UniformSim : SimBase {
       cellSize *= 2;
     }
     N = (size_t)gx * gy * gz;
+    if (N >= 0xFFFFFFFFull) return fail(DCG_E_UNSUPPORTED, "the uniform grid is limited to 2^32 - 1 cells");
     MC = 0;
     for (size_t s = 1; gx % s == 0 && gy % s == 0 && gz % s == 0; s *= 2) MC += N / (s * s * s);
     size_t off = 0;
@@ -383,6 +477,8 @@ struct UniformSim : SimBase {
     DCG_TRY(fill32(g.pre, 0, MC));
     DCG_TRY(fill32(g.flu, 0, MC));
     customFluidity = false;
+    fluKey.clear();
+    dropGraph();  // a captured step may rely on the fluidity computed before this reset
     cellStepCount = 0;
     return init();
   }
@@ -396,12 +492,19 @@ struct UniformSim : SimBase {
     DCG_TRY(fill32(g.tem, ta, N));
     return DCG_OK;
   }
-  // adaptTopology(): fluidity per level, fluid_simulation_uniform.cu:143-147
+  // adaptTopology(): fluidity per level, fluid_simulation_uniform.cu:143-147. The pyramid is a
+  // function of the parameters alone, so it is recomputed only when they changed since its last
+  // computation, after reset, or when a fluidity field was imported (which this call overwrites,
+  // as the reference does); otherwise the array already holds exactly these values.
   int adaptTopology() override {
-    for (int l = 0; l < L; l++) {
-      const UDims& D = dims[l];
-      const size_t n = (size_t)D.w * D.h * D.d;
-      DCG_LAUNCH(DCG_K_OTHER, "k_u_fluidity", (k_u_fluidity<<<nblk(n), 256, 0, stream>>>(P, g, D, 1 << l)));
+    const char* pb = (const char*)&P;
+    if (customFluidity || fluKey.size() != sizeof(P) || std::memcmp(fluKey.data(), pb, sizeof(P)) != 0) {
+      for (int l = 0; l < L; l++) {
+        const UDims& D = dims[l];
+        const size_t n = (size_t)D.w * D.h * D.d;
+        DCG_LAUNCH(DCG_K_OTHER, "k_u_fluidity", (k_u_fluidity<<<nblk(n), 256, 0, stream>>>(P, g, D, 1 << l)));
+      }
+      fluKey.assign(pb, pb + sizeof(P));
     }
     customFluidity = false;
     return DCG_OK;
@@ -473,14 +576,40 @@ struct UniformSim : SimBase {
     return DCG_OK;
   }
   // project(), fluid_simulation_uniform.cu:96-124
+  // First level from which every coarser level has at most kCoarseCells cells (L if none).
+  // (levels stored contiguously from Ls on, in total at most kCoarseLds cells: the LDS arrays)
+  int coarseStart() const {
+    int Ls = L;
+    size_t total = 0;
+    while (Ls > 0) {
+      const size_t n = (size_t)dims[Ls - 1].w * dims[Ls - 1].h * dims[Ls - 1].d;
+      if (n > kCoarseCells || total + n > (size_t)kCoarseLds) break;
+      total += n;
+      Ls--;
+    }
+    return Ls;
+  }
+  ULevels levels() const {
+    ULevels u{};
+    for (int l = 0; l < L && l < DCG_MAX_LEVELS; l++) u.d[l] = dims[l];
+    return u;
+  }
+  // project(), fluid_simulation_uniform.cu:96-124; the levels from coarseStart() up run in one
+  // workgroup (k_u_coarse), the others level by level.
   int project() override {
     DCG_TRY(divergence());
-    for (int l = 1; l < L; l++) {
+    const int Ls = coarseStart();
+    for (int l = 1; l < L && l <= Ls; l++) {
       const UDims& D = dims[l];
       DCG_LAUNCH(DCG_K_RESTRICT, "k_u_restrict", (k_u_restrict<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(g, D, dims[l - 1])));
     }
-    DCG_TRY(jacobiPairs(L - 1, P.project_top_pairs));
-    for (int l = L - 2; l >= 0; l--) {
+    if (Ls < L) {
+      DCG_LAUNCH(DCG_K_JACOBI, "k_u_coarse",
+                 (k_u_coarse<<<1, 1024, 0, stream>>>(g, levels(), L, Ls, P.project_top_pairs, P.project_level_pairs, P.dx)));
+    } else {
+      DCG_TRY(jacobiPairs(L - 1, P.project_top_pairs));
+    }
+    for (int l = std::min(Ls, L - 1) - 1; l >= 0; l--) {
       const UDims& D = dims[l];
       DCG_LAUNCH(DCG_K_PROLONGATE, "k_u_prolongate", (k_u_prolongate<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(g.pre, D, dims[l + 1])));
       DCG_TRY(jacobiPairs(l, P.project_level_pairs));
@@ -582,15 +711,19 @@ struct UniformSim : SimBase {
   float* stateDen(float* base, int k) { return base + (size_t)k * 5 * N; }
   float* stateTem(float* base, int k) { return base + (size_t)k * 5 * N + N; }
   float3* stateVel(float* base, int k) { return (float3*)(base + (size_t)k * 5 * N + 2 * N); }
+  // Temperature changes only with the sphere source (it is advected then); otherwise it is the
+  // same in every state and is neither saved nor restored.
   int saveState(float* base, int k) {
     DCG_HIP(hipMemcpyAsync(stateDen(base, k), g.den, N * 4, hipMemcpyDeviceToDevice, stream));
-    DCG_HIP(hipMemcpyAsync(stateTem(base, k), g.tem, N * 4, hipMemcpyDeviceToDevice, stream));
+    if (P.source_mode == DCG_SOURCE_SPHERE_BUOYANT)
+      DCG_HIP(hipMemcpyAsync(stateTem(base, k), g.tem, N * 4, hipMemcpyDeviceToDevice, stream));
     DCG_HIP(hipMemcpyAsync(stateVel(base, k), g.vel, N * 12, hipMemcpyDeviceToDevice, stream));
     return DCG_OK;
   }
   int loadState(float* base, int k) {
     DCG_HIP(hipMemcpyAsync(g.den, stateDen(base, k), N * 4, hipMemcpyDeviceToDevice, stream));
-    DCG_HIP(hipMemcpyAsync(g.tem, stateTem(base, k), N * 4, hipMemcpyDeviceToDevice, stream));
+    if (P.source_mode == DCG_SOURCE_SPHERE_BUOYANT)
+      DCG_HIP(hipMemcpyAsync(g.tem, stateTem(base, k), N * 4, hipMemcpyDeviceToDevice, stream));
     DCG_HIP(hipMemcpyAsync(g.vel, stateVel(base, k), N * 12, hipMemcpyDeviceToDevice, stream));
     return DCG_OK;
   }
@@ -644,7 +777,7 @@ struct UniformSim : SimBase {
     return DCG_OK;
   }
   int window(int slot) {
-    DCG_TRY(fill32(winBuf + slot, 0, 1));
+    DCG_TRY(fill32(winBuf + slot, 1, 1));  // every state's radius is at least 1
     DCG_LAUNCH(DCG_K_OTHER, "k_ua_window", (k_ua_window<<<nblk(N), 256, 0, stream>>>(P, g, stateVel(segBuf, slot), winBuf + slot)));
     return DCG_OK;
   }
@@ -657,7 +790,7 @@ struct UniformSim : SimBase {
     const float3* vel0 = stateVel(segBuf, s);
     const float3* vel1 = stateVel(segBuf, s + 1);
     // advectDensity: lq -> lq0 (and lt -> lt0 with the source), foot terms into lu
-#define UA_ADV_D(F, S) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_advect_density", (k_ua_advect_density<F, S><<<nblk(N), 256, 0, stream>>>(P, g, den0, tem0, vel1, lq, lt, lq0, lt0, lu, winBuf + s + 1, adjErr)))
+#define UA_ADV_D(F, S) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_advect_density", (k_ua_advect_density<F, S><<<adjTiles(g.gx, g.gy, g.gz), 256, 0, stream>>>(P, g, den0, tem0, vel1, lq, lt, lq0, lt0, lu, winBuf + s + 1, adjErr)))
     if (fl && src) UA_ADV_D(true, true);
     else if (fl) UA_ADV_D(true, false);
     else if (src) UA_ADV_D(false, true);
@@ -683,13 +816,18 @@ struct UniformSim : SimBase {
     if (P.project_mode == DCG_PROJECT_LOCAL) {
       DCG_TRY(pairsT(0, P.project_local_pairs));
     } else {
-      for (int l = 0; l <= L - 2; l++) {
+      const int Ls = coarseStart();  // levels from Ls up: one workgroup (k_ua_coarse)
+      for (int l = 0; l <= L - 2 && l < Ls; l++) {
         DCG_TRY(pairsT(l, P.project_level_pairs));
         const UDims& C = dims[l + 1];
-        DCG_LAUNCH(DCG_K_PROLONGATE, "k_ua_prolongate_T", (k_ua_prolongate_T<<<nblk((size_t)C.w * C.h * C.d), 256, 0, stream>>>(lp, dims[l], C)));
+        DCG_LAUNCH(DCG_K_PROLONGATE, "k_ua_prolongate_T", (k_ua_prolongate_T<<<prolTTiles(C), 256, 0, stream>>>(lp, dims[l], C)));
       }
-      DCG_TRY(pairsT(L - 1, P.project_top_pairs));
-      for (int l = L - 1; l >= 1; l--) {
+      if (Ls < L)
+        DCG_LAUNCH(DCG_K_JACOBI, "k_ua_coarse",
+                   (k_ua_coarse<<<1, 1024, 0, stream>>>(lp, ltp, ld, levels(), L, Ls, P.project_top_pairs, P.project_level_pairs, P.dx)));
+      else
+        DCG_TRY(pairsT(L - 1, P.project_top_pairs));
+      for (int l = std::min(Ls, L - 1); l >= 1; l--) {
         const UDims& D = dims[l - 1];
         DCG_LAUNCH(DCG_K_RESTRICT, "k_ua_restrict_T", (k_ua_restrict_T<<<nblk((size_t)D.w * D.h * D.d), 256, 0, stream>>>(ld, D, dims[l])));
       }
@@ -700,7 +838,7 @@ struct UniformSim : SimBase {
       DCG_LAUNCH(DCG_K_DIVERGENCE, "k_ua_div_T", (k_ua_div_T<false><<<nblk(N), 256, 0, stream>>>(P, g, ld, lu)));
     // advectVelocity: lu -> lu0, buoyancy into lq0 / lt0
     float* ltOut = src ? lt0 : lt;  // without the source temperature is not advected
-#define UA_ADV_V(F, B) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_ua_advect_velocity", (k_ua_advect_velocity<F, B><<<nblk(N), 256, 0, stream>>>(P, g, vel0, lu, lu0, lq0, ltOut, winBuf + s, adjErr)))
+#define UA_ADV_V(F, B) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_ua_advect_velocity", (k_ua_advect_velocity<F, B><<<adjTiles(g.gx, g.gy, g.gz), 256, 0, stream>>>(P, g, vel0, lu, lu0, lq0, ltOut, winBuf + s, adjErr)))
     if (fl && src) UA_ADV_V(true, true);
     else if (fl) UA_ADV_V(true, false);
     else if (src) UA_ADV_V(false, true);

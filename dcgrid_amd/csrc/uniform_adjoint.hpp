@@ -28,7 +28,8 @@ __device__ __forceinline__ void ufoot(const dcg_params& P, int x, int y, int z, 
 __device__ __forceinline__ int ubase(float p) { return (int)floorf(p - .5f); }
 
 // Largest |offset| (and |offset+1|) between a cell and the base corner of its foot; one int
-// per state, reduced with atomicMax on a global word.
+// per state, reduced with atomicMax on a global word (skipped when the word already holds at
+// least the workgroup's maximum: thousands of workgroups otherwise serialise on one address).
 __global__ void __launch_bounds__(256) k_ua_window(dcg_params P, UGrid g, const float3* __restrict__ vel,
                                                    int* __restrict__ out) {
   __shared__ int red[256];
@@ -36,7 +37,8 @@ __global__ void __launch_bounds__(256) k_ua_window(dcg_params P, UGrid g, const 
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   int r = 0;
   if (i < n) {
-    const int x = (int)(i % g.gx), y = (int)((i / g.gx) % g.gy), z = (int)(i / ((size_t)g.gx * g.gy));
+    int x, y, z;
+  ucoord(i, g.gx, g.gy, x, y, z);
     float px, py, pz;
     ufoot(P, x, y, z, vel[i], px, py, pz);
     const int o[3] = {ubase(px) - x, ubase(py) - y, ubase(pz) - z};
@@ -49,7 +51,7 @@ __global__ void __launch_bounds__(256) k_ua_window(dcg_params P, UGrid g, const 
     if (threadIdx.x < s) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicMax(out, red[0]);
+  if (threadIdx.x == 0 && red[0] > *(volatile const int*)out) atomicMax(out, red[0]);
 }
 
 // Unnormalised corner weights f_c t_c of a foot and d t_c / d d_a.
@@ -125,6 +127,70 @@ __device__ __forceinline__ int windowRadius(const int* __restrict__ win, int* __
   return R;
 }
 
+// Tiled transposed gather. A workgroup owns a tile of ADJ_TX x ADJ_TY x ADJ_TZ destination
+// cells. When every foot of the state stays within one cell of its own (window radius 1, the
+// case for CFL < 1), the sources that can reach the tile are the tile grown by one cell: each is
+// sampled once per workgroup into LDS (weights, base-corner offset, its lambda) and every
+// destination then walks its 27 candidate sources in the same order as the untiled gather, so
+// the sums are the same floats. Larger windows take the untiled per-destination gather.
+constexpr int ADJ_TX = 8, ADJ_TY = 8, ADJ_TZ = 4;  // 256 destinations per workgroup
+constexpr int ADJ_SX = ADJ_TX + 2, ADJ_SY = ADJ_TY + 2, ADJ_SZ = ADJ_TZ + 2;
+constexpr int ADJ_SRC = ADJ_SX * ADJ_SY * ADJ_SZ;  // sources of a tile at window radius 1
+static_assert(ADJ_TX * ADJ_TY * ADJ_TZ == 256, "one destination per thread");
+
+__host__ __device__ inline unsigned adjTiles(int gx, int gy, int gz) {
+  return (unsigned)(((gx + ADJ_TX - 1) / ADJ_TX) * ((gy + ADJ_TY - 1) / ADJ_TY) * ((gz + ADJ_TZ - 1) / ADJ_TZ));
+}
+// Destination cell of this thread and the tile origin.
+__device__ __forceinline__ void adjTileCell(const UGrid& g, int& X0, int& Y0, int& Z0, int& x, int& y, int& z) {
+  const int ntx = (g.gx + ADJ_TX - 1) / ADJ_TX, nty = (g.gy + ADJ_TY - 1) / ADJ_TY;
+  const int b = (int)blockIdx.x;
+  X0 = (b % ntx) * ADJ_TX;
+  Y0 = ((b / ntx) % nty) * ADJ_TY;
+  Z0 = (b / (ntx * nty)) * ADJ_TZ;
+  const int t = (int)threadIdx.x;
+  x = X0 + t % ADJ_TX;
+  y = Y0 + (t / ADJ_TX) % ADJ_TY;
+  z = Z0 + t / (ADJ_TX * ADJ_TY);
+}
+// Base-corner offset (ubase(p) - source coordinate, in [-2, 1] at radius 1) packed per axis.
+__device__ __forceinline__ int adjPack(int ox, int oy, int oz) { return (ox + 8) | ((oy + 8) << 4) | ((oz + 8) << 8); }
+// Corner of the source's sample that is destination (x,y,z), or -1.
+__device__ __forceinline__ int adjCorner(int packed, int xi, int yi, int zi, int x, int y, int z) {
+  const int bx = x - (xi + (packed & 15) - 8), by = y - (yi + ((packed >> 4) & 15) - 8),
+            bz = z - (zi + ((packed >> 8) & 15) - 8);
+  if ((unsigned)bx > 1u || (unsigned)by > 1u || (unsigned)bz > 1u) return -1;
+  return (bx << 2) | (by << 1) | bz;
+}
+// Region index of source (xi,yi,zi) in the tile grown by one cell.
+__device__ __forceinline__ int adjSlot(int X0, int Y0, int Z0, int xi, int yi, int zi) {
+  return ((zi - Z0 + 1) * ADJ_SY + (yi - Y0 + 1)) * ADJ_SX + (xi - X0 + 1);
+}
+// Samples source k of the grown tile under velocity `vel`: normalised weights (0 for a zero
+// sample, a masked source or a cell outside the domain) and the packed base offset.
+template <bool FLUIDITY, bool MASK>
+__device__ __forceinline__ bool adjSampleSource(const dcg_params& P, const UGrid& g, const float3* __restrict__ vel,
+                                                int X0, int Y0, int Z0, int k, float w[8], int& packed, size_t& i) {
+  const int xi = X0 - 1 + k % ADJ_SX, yi = Y0 - 1 + (k / ADJ_SX) % ADJ_SY, zi = Z0 - 1 + k / (ADJ_SX * ADJ_SY);
+#pragma unroll
+  for (int c = 0; c < 8; c++) w[c] = 0.f;
+  packed = adjPack(0, 0, 0);
+  i = 0;
+  if (!uinDomain(g, xi, yi, zi)) return false;
+  i = uidx(g, xi, yi, zi);
+  if (MASK && inSource(P, (float)xi + .5f, (float)yi + .5f, (float)zi + .5f)) return true;
+  float px, py, pz;
+  ufoot(P, xi, yi, zi, vel[i], px, py, pz);
+  packed = adjPack(ubase(px) - xi, ubase(py) - yi, ubase(pz) - zi);
+  USample s;
+  usample<FLUIDITY>(g, px, py, pz, s);
+  if (!s.zero) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) w[c] = s.w[c];
+  }
+  return true;
+}
+
 // Adjoint of advectDensity under velocity vel1 = u_{k+1}: lq0 = A^T lq (and lt0 = A^T lt with
 // the source), sources masked inside the emitting sphere; lu += foot terms (den0/tem0 = state k).
 template <bool FLUIDITY, bool SRC>
@@ -135,24 +201,56 @@ __global__ void __launch_bounds__(256) k_ua_advect_density(dcg_params P, UGrid g
                                                            float* __restrict__ lq0, float* __restrict__ lt0,
                                                            float3* __restrict__ lu, const int* __restrict__ win,
                                                            int* __restrict__ errFlag) {
-  const size_t n = (size_t)g.gx * g.gy * g.gz;
-  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  __shared__ float sw[8][ADJ_SRC];
+  __shared__ int sb[ADJ_SRC];
+  __shared__ float sq[ADJ_SRC];
+  __shared__ float st[SRC ? ADJ_SRC : 1];
   const int R = windowRadius(win, errFlag);
-  if (j >= n) return;
-  const int x = (int)(j % g.gx), y = (int)((j / g.gx) % g.gy), z = (int)(j / ((size_t)g.gx * g.gy));
-  // transposed gather
+  int X0, Y0, Z0, x, y, z;
+  adjTileCell(g, X0, Y0, Z0, x, y, z);
+  const bool inside = x < g.gx && y < g.gy && z < g.gz;
   float aq = 0.f, at = 0.f;
-  for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
-    for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
-      for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
-        const size_t i = uidx(g, xi, yi, zi);
-        if (SRC && inSource(P, (float)xi + .5f, (float)yi + .5f, (float)zi + .5f)) continue;
-        const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel1[i], x, y, z);
-        if (w != 0.f) {
-          aq += w * lq[i];
-          if (SRC) at += w * lt[i];
+  if (R <= 1) {  // uniform over the launch
+    for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
+      float w[8];
+      int packed;
+      size_t i;
+      const bool dom = adjSampleSource<FLUIDITY, SRC>(P, g, vel1, X0, Y0, Z0, k, w, packed, i);
+#pragma unroll
+      for (int c = 0; c < 8; c++) sw[c][k] = w[c];
+      sb[k] = packed;
+      sq[k] = dom ? lq[i] : 0.f;
+      if (SRC) st[k] = dom ? lt[i] : 0.f;
+    }
+    __syncthreads();
+    if (!inside) return;
+    for (int zi = max(z - 1, 0); zi <= min(z + 1, g.gz - 1); zi++)
+      for (int yi = max(y - 1, 0); yi <= min(y + 1, g.gy - 1); yi++)
+        for (int xi = max(x - 1, 0); xi <= min(x + 1, g.gx - 1); xi++) {
+          const int k = adjSlot(X0, Y0, Z0, xi, yi, zi);
+          const int c = adjCorner(sb[k], xi, yi, zi, x, y, z);
+          if (c < 0) continue;
+          const float w = sw[c][k];
+          if (w != 0.f) {
+            aq += w * sq[k];
+            if (SRC) at += w * st[k];
+          }
         }
-      }
+  } else {
+    if (!inside) return;
+    for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
+      for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
+        for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
+          const size_t i = uidx(g, xi, yi, zi);
+          if (SRC && inSource(P, (float)xi + .5f, (float)yi + .5f, (float)zi + .5f)) continue;
+          const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel1[i], x, y, z);
+          if (w != 0.f) {
+            aq += w * lq[i];
+            if (SRC) at += w * lt[i];
+          }
+        }
+  }
+  const size_t j = uidx(g, x, y, z);
   lq0[j] = aq;
   if (SRC) lt0[j] = at;
   // foot term of cell j
@@ -182,19 +280,50 @@ __global__ void __launch_bounds__(256) k_ua_advect_velocity(dcg_params P, UGrid 
                                                             float3* __restrict__ lu0, float* __restrict__ lq0,
                                                             float* __restrict__ lt0, const int* __restrict__ win,
                                                             int* __restrict__ errFlag) {
-  const size_t n = (size_t)g.gx * g.gy * g.gz;
-  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  __shared__ float sw[8][ADJ_SRC];
+  __shared__ int sb[ADJ_SRC];
+  __shared__ float sl[3][ADJ_SRC];
   const int R = windowRadius(win, errFlag);
-  if (j >= n) return;
-  const int x = (int)(j % g.gx), y = (int)((j / g.gx) % g.gy), z = (int)(j / ((size_t)g.gx * g.gy));
+  int X0, Y0, Z0, x, y, z;
+  adjTileCell(g, X0, Y0, Z0, x, y, z);
+  const bool inside = x < g.gx && y < g.gy && z < g.gz;
   float3 acc = make_float3(0.f, 0.f, 0.f);
-  for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
-    for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
-      for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
-        const size_t i = uidx(g, xi, yi, zi);
-        const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel0[i], x, y, z);
-        if (w != 0.f) acc = f3add(acc, f3mul(lus[i], w));
-      }
+  if (R <= 1) {  // uniform over the launch
+    for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
+      float w[8];
+      int packed;
+      size_t i;
+      const bool dom = adjSampleSource<FLUIDITY, false>(P, g, vel0, X0, Y0, Z0, k, w, packed, i);
+#pragma unroll
+      for (int c = 0; c < 8; c++) sw[c][k] = w[c];
+      sb[k] = packed;
+      const float3 l = dom ? lus[i] : make_float3(0.f, 0.f, 0.f);
+      sl[0][k] = l.x;
+      sl[1][k] = l.y;
+      sl[2][k] = l.z;
+    }
+    __syncthreads();
+    if (!inside) return;
+    for (int zi = max(z - 1, 0); zi <= min(z + 1, g.gz - 1); zi++)
+      for (int yi = max(y - 1, 0); yi <= min(y + 1, g.gy - 1); yi++)
+        for (int xi = max(x - 1, 0); xi <= min(x + 1, g.gx - 1); xi++) {
+          const int k = adjSlot(X0, Y0, Z0, xi, yi, zi);
+          const int c = adjCorner(sb[k], xi, yi, zi, x, y, z);
+          if (c < 0) continue;
+          const float w = sw[c][k];
+          if (w != 0.f) acc = f3add(acc, f3mul(make_float3(sl[0][k], sl[1][k], sl[2][k]), w));
+        }
+  } else {
+    if (!inside) return;
+    for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
+      for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
+        for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
+          const size_t i = uidx(g, xi, yi, zi);
+          const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel0[i], x, y, z);
+          if (w != 0.f) acc = f3add(acc, f3mul(lus[i], w));
+        }
+  }
+  const size_t j = uidx(g, x, y, z);
   const float3 lam = lus[j];
   float px, py, pz;
   ufoot(P, x, y, z, vel0[j], px, py, pz);
@@ -227,7 +356,8 @@ __global__ void __launch_bounds__(256) k_ua_grad_T(dcg_params P, UGrid g, const 
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
-  const int x = (int)(j % g.gx), y = (int)((j / g.gx) % g.gy), z = (int)(j / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(j, g.gx, g.gy, x, y, z);
   const size_t pl = (size_t)g.gx * g.gy;
   const float a = .5f * P.rdx;
   const float fj = FLUIDITY ? g.flu[j] : 1.f;
@@ -259,12 +389,10 @@ __global__ void __launch_bounds__(256) k_ua_grad_T(dcg_params P, UGrid g, const 
 
 // One reversed Jacobi half-sweep on a level: out = S in (clamped 7-point sum / 6, symmetric),
 // ld += -alpha/6 * in.
-__global__ void __launch_bounds__(256) k_ua_jacobi(const float* __restrict__ in, float* __restrict__ out,
-                                                   float* __restrict__ ld, UDims L, float alpha) {
-  const size_t n = (size_t)L.w * L.h * L.d;
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
+__device__ __forceinline__ void uaJacobiCell(const float* __restrict__ in, float* __restrict__ out,
+                                             float* __restrict__ ld, const UDims& L, float alpha, size_t i) {
+  int x, y, z;
+  ucoord(i, L.w, L.h, x, y, z);
   const size_t pl = (size_t)L.w * L.h;
   const size_t il = x > 0 ? i - 1 : i, ir = x < L.w - 1 ? i + 1 : i;
   const size_t id = y > 0 ? i - L.w : i, iu = y < L.h - 1 ? i + L.w : i;
@@ -272,6 +400,13 @@ __global__ void __launch_bounds__(256) k_ua_jacobi(const float* __restrict__ in,
   const float* p = in + L.off;
   out[L.off + i] = (p[il] + p[ir] + p[id] + p[iu] + p[ib] + p[iF]) / 6.f;
   ld[L.off + i] = ld[L.off + i] - alpha * (p[i] / 6.f);
+}
+__global__ void __launch_bounds__(256) k_ua_jacobi(const float* __restrict__ in, float* __restrict__ out,
+                                                   float* __restrict__ ld, UDims L, float alpha) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uaJacobiCell(in, out, ld, L, alpha, i);
 }
 
 // Weights with which fine coordinate x (of a level of width w) prolongates from coarse X:
@@ -284,11 +419,90 @@ __device__ __forceinline__ float prolW(int x, int w, int X) {
   return r;
 }
 // Prolongation transposed: lambda of the coarse level (Lc) from lambda of the fine level (L).
+// The trilinear weights are separable, so a workgroup stages the fine region of a tile of
+// PT_X x PT_Y x PT_Z coarse cells in LDS and contracts it one axis at a time (x, then y, then z):
+// about 1.3 global reads per fine cell instead of 8.
+constexpr int PT_X = 8, PT_Y = 8, PT_Z = 4;
+constexpr int PF_X = 2 * PT_X + 2, PF_Y = 2 * PT_Y + 2, PF_Z = 2 * PT_Z + 2;
+static_assert(PT_X * PT_Y * PT_Z == 256, "the z pass maps one coarse cell of the tile to each thread");
+__host__ __device__ inline unsigned prolTTiles(const UDims& Lc) {
+  return (unsigned)(((Lc.w + PT_X - 1) / PT_X) * ((Lc.h + PT_Y - 1) / PT_Y) * ((Lc.d + PT_Z - 1) / PT_Z));
+}
 __global__ void __launch_bounds__(256) k_ua_prolongate_T(float* __restrict__ lp, UDims L, UDims Lc) {
-  const size_t n = (size_t)Lc.w * Lc.h * Lc.d;
-  const size_t I = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (I >= n) return;
-  const int X = (int)(I % Lc.w), Y = (int)((I / Lc.w) % Lc.h), Z = (int)(I / ((size_t)Lc.w * Lc.h));
+  __shared__ float fine[PF_Z * PF_Y * PF_X];
+  __shared__ float ax[PF_Z * PF_Y * PT_X];
+  __shared__ float ay[PF_Z * PT_Y * PT_X];
+  const int ntx = (Lc.w + PT_X - 1) / PT_X, nty = (Lc.h + PT_Y - 1) / PT_Y;
+  const int b = (int)blockIdx.x;
+  const int X0 = (b % ntx) * PT_X, Y0 = ((b / ntx) % nty) * PT_Y, Z0 = (b / (ntx * nty)) * PT_Z;
+  const int fx0 = 2 * X0 - 1, fy0 = 2 * Y0 - 1, fz0 = 2 * Z0 - 1;
+  for (int k = threadIdx.x; k < PF_Z * PF_Y * PF_X; k += 256) {
+    const int x = fx0 + k % PF_X, y = fy0 + (k / PF_X) % PF_Y, z = fz0 + k / (PF_X * PF_Y);
+    const bool in = x >= 0 && y >= 0 && z >= 0 && x < L.w && y < L.h && z < L.d;
+    fine[k] = in ? lp[L.off + ((size_t)z * L.h + y) * L.w + x] : 0.f;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < PF_Z * PF_Y * PT_X; k += 256) {  // x: fine x -> coarse X
+    const int cx = k % PT_X, fy = (k / PT_X) % PF_Y, fz = k / (PT_X * PF_Y);
+    const int X = X0 + cx;
+    float s = 0.f;
+    for (int d = 0; d < 4; d++) {
+      const int x = 2 * X - 1 + d;
+      if (x < 0 || x >= L.w) continue;
+      const float w = prolW(x, L.w, X);
+      if (w != 0.f) s += w * fine[(fz * PF_Y + fy) * PF_X + (x - fx0)];
+    }
+    ax[k] = s;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < PF_Z * PT_Y * PT_X; k += 256) {  // y
+    const int cx = k % PT_X, cy = (k / PT_X) % PT_Y, fz = k / (PT_X * PT_Y);
+    const int Y = Y0 + cy;
+    float s = 0.f;
+    for (int d = 0; d < 4; d++) {
+      const int y = 2 * Y - 1 + d;
+      if (y < 0 || y >= L.h) continue;
+      const float w = prolW(y, L.h, Y);
+      if (w != 0.f) s += w * ax[(fz * PF_Y + (y - fy0)) * PT_X + cx];
+    }
+    ay[k] = s;
+  }
+  __syncthreads();
+  {  // z, one coarse cell per thread
+    const int k = threadIdx.x;
+    const int cx = k % PT_X, cy = (k / PT_X) % PT_Y, cz = k / (PT_X * PT_Y);
+    const int X = X0 + cx, Y = Y0 + cy, Z = Z0 + cz;
+    if (X >= Lc.w || Y >= Lc.h || Z >= Lc.d) return;
+    float s = 0.f;
+    for (int d = 0; d < 4; d++) {
+      const int z = 2 * Z - 1 + d;
+      if (z < 0 || z >= L.d) continue;
+      const float w = prolW(z, L.d, Z);
+      if (w != 0.f) s += w * ay[((z - fz0) * PT_Y + cy) * PT_X + cx];
+    }
+    lp[Lc.off + ((size_t)Z * Lc.h + Y) * Lc.w + X] = s;
+  }
+}
+
+// Restriction transposed: every child cell of level L gains 1/8 of its parent's lambda (Lp).
+__device__ __forceinline__ void uaRestrictTCell(float* __restrict__ ld, const UDims& L, const UDims& Lp, size_t i) {
+  int x, y, z;
+  ucoord(i, L.w, L.h, x, y, z);
+  const size_t par = Lp.off + ((size_t)(z / 2) * Lp.h + (y / 2)) * Lp.w + (x / 2);
+  ld[L.off + i] = ld[L.off + i] + .125f * ld[par];
+}
+__global__ void __launch_bounds__(256) k_ua_restrict_T(float* __restrict__ ld, UDims L, UDims Lp) {
+  const size_t n = (size_t)L.w * L.h * L.d;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uaRestrictTCell(ld, L, Lp, i);
+}
+
+// Prolongation transposed for one coarse cell I of level Lc (direct 4x4x4 gather; used on the
+// small levels handled by k_ua_coarse).
+__device__ __forceinline__ void uaProlongTCell(float* __restrict__ lp, const UDims& L, const UDims& Lc, size_t I) {
+  int X, Y, Z;
+  ucoord(I, Lc.w, Lc.h, X, Y, Z);
   float s = 0.f;
   for (int z = max(2 * Z - 1, 0); z <= min(2 * Z + 2, L.d - 1); z++) {
     const float wz = prolW(z, L.d, Z);
@@ -306,14 +520,48 @@ __global__ void __launch_bounds__(256) k_ua_prolongate_T(float* __restrict__ lp,
   lp[Lc.off + I] = s;
 }
 
-// Restriction transposed: every child cell of level L gains 1/8 of its parent's lambda (Lp).
-__global__ void __launch_bounds__(256) k_ua_restrict_T(float* __restrict__ ld, UDims L, UDims Lp) {
-  const size_t n = (size_t)L.w * L.h * L.d;
-  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % L.w), y = (int)((i / L.w) % L.h), z = (int)(i / ((size_t)L.w * L.h));
-  const size_t par = Lp.off + ((size_t)(z / 2) * Lp.h + (y / 2)) * Lp.w + (x / 2);
-  ld[L.off + i] = ld[L.off + i] + .125f * ld[par];
+// The adjoint of k_u_coarse's part of project() on the small levels (from Ls up), in one
+// workgroup with the levels' lambdas in LDS: reversed sweeps and transposed prolongation going
+// up, the top level's reversed sweeps, transposed restriction coming down to Ls.
+__global__ void __launch_bounds__(1024) k_ua_coarse(float* __restrict__ lp, float* __restrict__ ltp,
+                                                    float* __restrict__ ld, ULevels D, int L, int Ls, int topPairs,
+                                                    int levelPairs, float dx) {
+  __shared__ float sp[kCoarseLds], st[kCoarseLds], sd[kCoarseLds];
+  ULevels S;
+  size_t total;
+  coarseLocal(D, L, Ls, S, total);
+  const size_t base = D.d[Ls].off;
+  for (size_t i = threadIdx.x; i < total; i += blockDim.x) {
+    sp[i] = lp[base + i];
+    st[i] = ltp[base + i];
+    sd[i] = ld[base + i];
+  }
+  __syncthreads();
+  auto sweepsT = [&](int l, int pairs) {
+    const int scale = 1 << l;
+    const float alpha = dx * dx * (float)scale * (float)scale;
+    for (int p = 0; p < pairs; p++) {
+      for (size_t i = threadIdx.x; i < ucells(S.d[l]); i += blockDim.x) uaJacobiCell(sp, st, sd, S.d[l], alpha, i);
+      __syncthreads();
+      for (size_t i = threadIdx.x; i < ucells(S.d[l]); i += blockDim.x) uaJacobiCell(st, sp, sd, S.d[l], alpha, i);
+      __syncthreads();
+    }
+  };
+  for (int l = Ls; l <= L - 2; l++) {
+    sweepsT(l, levelPairs);
+    for (size_t I = threadIdx.x; I < ucells(S.d[l + 1]); I += blockDim.x) uaProlongTCell(sp, S.d[l], S.d[l + 1], I);
+    __syncthreads();
+  }
+  sweepsT(L - 1, topPairs);
+  for (int l = L - 1; l >= Ls + 1; l--) {
+    for (size_t i = threadIdx.x; i < ucells(S.d[l - 1]); i += blockDim.x) uaRestrictTCell(sd, S.d[l - 1], S.d[l], i);
+    __syncthreads();
+  }
+  for (size_t i = threadIdx.x; i < total; i += blockDim.x) {
+    lp[base + i] = sp[i];
+    ltp[base + i] = st[i];
+    ld[base + i] = sd[i];
+  }
 }
 
 // D^T: lu += (d div / d u)^T ld on level 0.
@@ -323,7 +571,8 @@ __global__ void __launch_bounds__(256) k_ua_div_T(dcg_params P, UGrid g, const f
   const size_t n = (size_t)g.gx * g.gy * g.gz;
   const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= n) return;
-  const int x = (int)(j % g.gx), y = (int)((j / g.gx) % g.gy), z = (int)(j / ((size_t)g.gx * g.gy));
+  int x, y, z;
+  ucoord(j, g.gx, g.gy, x, y, z);
   const size_t pl = (size_t)g.gx * g.gy;
   const float af = (.5f * P.rdx) * (FLUIDITY ? g.flu[j] : 1.f);
   float3 v = lu[j];
