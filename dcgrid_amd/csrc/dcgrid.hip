@@ -388,13 +388,24 @@ struct DCGridSim : SimBase {
     DCG_TRY(divergenceKernel());
     return restrictAll(g.div);
   }
+  // One Jacobi half-sweep of a level (fwd: p -> p~, else p~ -> p); `two`: two blocks per
+  // sub-group and iteration; dense levels (l >= S) address their faces without descriptors.
+  int jacobiLaunch(int level, bool fwd, bool two, unsigned n, float alpha) {
+    auto* in = fwd ? g.pre : g.tpre;
+    auto* out = fwd ? g.tpre : g.pre;
+    const bool dense = level >= S;
+#define JAC(NB, D) DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW, NB, D><<<n, G::WG, 0, stream>>>(g, level, in, out, alpha)))
+    if (two && dense) JAC(2, true);
+    else if (two) JAC(2, false);
+    else if (dense) JAC(1, true);
+    else JAC(1, false);
+#undef JAC
+    return DCG_OK;
+  }
   int jacobiOne(int level, bool fwd) {
     const int s = 1 << level;
     const float alpha = (float)(s * s) * P.dx * P.dx;
-    const unsigned n = slotGrid<BW>(maxL[level]);
-    DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi",
-               (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, fwd ? g.pre : g.tpre, fwd ? g.tpre : g.pre, alpha)));
-    return DCG_OK;
+    return jacobiLaunch(level, fwd, false, slotGrid<BW>(maxL[level]), alpha);
   }
   int prolongate(int level) {
     DCG_LAUNCH(DCG_K_PROLONGATE, "k_prolongate",
@@ -440,10 +451,12 @@ struct DCGridSim : SimBase {
     Pending batch;  // the run of back-to-back sweeps of one level is timed as one batch
     DCG_TRY(beginTimed(DCG_K_JACOBI, &batch));
     batchK = DCG_K_JACOBI;
-    for (int i = 0; i < pairs; i++) {
-      DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, g.pre, g.tpre, alpha)));
-      DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW><<<n, G::WG, 0, stream>>>(g, level, g.tpre, g.pre, alpha)));
-    }
+    // two blocks per sub-group and iteration once the level spans more than two rounds of the
+    // grid (the large levels, bound by reads in flight)
+    const bool two = (size_t)maxL[level] > 2 * (size_t)n * G::BPW;
+    for (int i = 0; i < pairs; i++)
+      for (int h = 0; h < 2; h++)
+        DCG_TRY(jacobiLaunch(level, h == 0, two, n, alpha));
     batchK = -1;
     return endTimed(DCG_K_JACOBI, &batch, 2 * pairs);
   }

@@ -195,6 +195,23 @@ __device__ __forceinline__ uint32_t apronCell(const DGBase& g, uint32_t b, int a
   return ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> sh, (aj - 1 - d.z) >> sh, (ak - 1 - d.w) >> sh);
 }
 
+// Apron cell of a block on a dense level (l >= S). Every block of such a level is allocated at
+// the dense index (bx*ry + by)*rz + bz and never moves, so its neighbour descriptors are fixed:
+// the same-level neighbour at the dense index, or the clamped own cell outside the domain (the
+// form k_activate_dense writes). The cell follows from the block origin p0 alone, without the
+// dependent descriptor load of apronCell (same cell).
+template <int BW>
+__device__ __forceinline__ uint32_t denseApronCell(const DGBase& g, int l, uint32_t b, int3 p0, int ai, int aj, int ak) {
+  constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
+  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
+  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
+  const int nx = (p0.x >> BB) + dx, ny = (p0.y >> BB) + dy, nz = (p0.z >> BB) + dz;
+  if (nx < 0 || ny < 0 || nz < 0 || nx >= g.rx[l] || ny >= g.ry[l] || nz >= g.rz[l])
+    return b * BV + loc<BB>(clampi(ai, 1, BW) - 1, clampi(aj, 1, BW) - 1, clampi(ak, 1, BW) - 1);
+  const uint32_t nb = g.offs[l] + (uint32_t)((nx * g.ry[l] + ny) * g.rz[l] + nz);
+  return nb * BV + loc<BB>(ai - 1 - BW * dx, aj - 1 - BW * dy, ak - 1 - BW * dz);
+}
 // Apron coordinates of face cell f in [0, 6*BW*BW): faces x-,x+,y-,y+,z-,z+.
 template <int BW>
 __device__ __forceinline__ void faceCell(int f, int& ai, int& aj, int& ak) {

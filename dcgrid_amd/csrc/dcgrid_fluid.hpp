@@ -95,6 +95,19 @@ __device__ __forceinline__ void stageScalar(const DGBase& g, const SF* __restric
     tile[(ai * A + aj) * A + ak] = f[apronCell<BW>(g, b, ai, aj, ak)];
   }
 }
+// stageScalar on a dense level (l >= S): the faces come from denseApronCell.
+template <int BW, class SF>
+__device__ __forceinline__ void stageScalarDense(const DGBase& g, int l, const SF* __restrict__ f, float* tile, uint32_t b,
+                                                 int t) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE;
+  tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
+  const int3 p0 = g.pos[b];
+  for (int fc = t; fc < FACE; fc += BV) {
+    int ai, aj, ak;
+    faceCell<BW>(fc, ai, aj, ak);
+    tile[(ai * A + aj) * A + ak] = f[denseApronCell<BW>(g, l, b, p0, ai, aj, ak)];
+  }
+}
 
 // Fluidity of the six apron neighbours of local cell t (1 when the field is known to be 1).
 template <int BW, bool FLUIDITY, class G>
@@ -120,24 +133,44 @@ __device__ __forceinline__ void neighbourFluidity(const G& g, uint32_t b, int t,
 // k_dcgrid_jacobi / k_dcgrid_jacobi_inv (dcgrid_multigrid_solver.cu:5-41):
 // out = (sum of 6 apron neighbours of `in` - (scale*dx)^2 * div) / 6 on every cell of every
 // block of the level. Neighbours on coarser levels are read from the same array `in`.
-template <int BW, class G>
+// NB blocks per sub-group and iteration: their loads are issued together, so a thread keeps NB
+// blocks' worth of reads in flight (the kernel is bound by latency times bytes in flight on the
+// large levels, where every workgroup walks many blocks).
+// DENSE: the level is dense (l >= S), its faces come from denseApronCell (no descriptor loads).
+template <int BW, int NB, bool DENSE, class G>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(G g, int level, const typename G::Scalar* __restrict__ in,
                                                          typename G::Scalar* __restrict__ out, float alpha) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
-  __shared__ float tiles[BPW][A3];
+  __shared__ float tiles[NB][BPW][A3];
   const uint32_t total = slotTotal(g, level);
-  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
-    __syncthreads();  // the previous block's tile is no longer read
-    const BlockCtx<BW> c = slotCtx<BW>(g, level, base, total);
-    float* tile = tiles[c.sub];
-    if (c.valid) stageScalar<BW>(g, in, tile, c.b, c.t);
+  const uint32_t stride = slotStride<BW>();
+  for (uint32_t base = firstSlot<BW>(); base < total; base += NB * stride) {
+    __syncthreads();  // the previous blocks' tiles are no longer read
+    BlockCtx<BW> c[NB];
+    float dv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      c[k] = slotCtx<BW>(g, level, base + k * stride, total);
+      dv[k] = c[k].valid ? (float)g.div[c[k].b * BV + c[k].t] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      if (!c[k].valid) continue;
+      if (DENSE)
+        stageScalarDense<BW>(g, level, in, tiles[k][c[k].sub], c[k].b, c[k].t);
+      else
+        stageScalar<BW>(g, in, tiles[k][c[k].sub], c[k].b, c[k].t);
+    }
     __syncthreads();
-    if (!c.valid) continue;
-    const int a = apronIndexOfLocal<BW>(c.t);
-    const uint32_t cell = c.b * BV + c.t;
-    out[cell] = (tile[a - A * A] + tile[a + A * A] + tile[a - A] + tile[a + A] + tile[a - 1] + tile[a + 1] -
-                 alpha * g.div[cell]) /
-                6.f;
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      if (!c[k].valid) continue;
+      const float* tile = tiles[k][c[k].sub];
+      const int a = apronIndexOfLocal<BW>(c[k].t);
+      out[c[k].b * BV + c[k].t] =
+          (tile[a - A * A] + tile[a + A * A] + tile[a - A] + tile[a + A] + tile[a - 1] + tile[a + 1] - alpha * dv[k]) /
+          6.f;
+    }
   }
 }
 
@@ -530,16 +563,17 @@ struct TileSample {
   }
 };
 
-// Stage a block's values over its whole apron ((BW+2)^3 entries, edges and corners included):
-// tile[a] = field[cell of apron entry a], raw (boundary conditions are applied per sample).
-template <int BW, class T, class SF>
-__device__ __forceinline__ void stageApron(const DGBase& g, const SF* __restrict__ f, T* tile, uint32_t b, int t) {
+// Visits the whole apron of block b ((BW+2)^3 entries, edges and corners included) for thread t:
+// op(a, cell) for the thread's interior cell and for its share of the border entries, with the
+// cell of each entry resolved once (apronCell) however many fields are staged from it.
+template <int BW, class Op>
+__device__ __forceinline__ void forApron(const DGBase& g, uint32_t b, int t, Op op) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3;
-  tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
+  op(apronIndexOfLocal<BW>(t), b * BV + t);
   for (int a = t; a < A3; a += BV) {
     const int ai = a / (A * A), aj = (a / A) % A, ak = a % A;
     if (ai > 0 && ai <= BW && aj > 0 && aj <= BW && ak > 0 && ak <= BW) continue;
-    tile[a] = f[apronCell<BW>(g, b, ai, aj, ak)];
+    op(a, apronCell<BW>(g, b, ai, aj, ak));
   }
 }
 
@@ -563,8 +597,10 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(G g, dcg_params
     if (active) {
       lb = g.lev[c.b];
       p0 = g.pos[c.b];
-      stageApron<BW, float3>(g, g.vel, vt, c.b, c.t);
-      if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
+      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t cell) {
+        vt[a] = g.vel[cell];
+        if (FLUIDITY) ft[a] = g.flu[cell];
+      });
     }
     __syncthreads();
     if (!active) continue;
@@ -634,10 +670,12 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params 
     if (active) {
       lb = g.lev[c.b];
       p0 = g.pos[c.b];
-      stageApron<BW, float>(g, g.den, dt_, c.b, c.t);
-      if (SRC) stageApron<BW, float>(g, g.tem, tt, c.b, c.t);
-      if (MOIST) stageApron<BW, float>(g, g.vap, vt, c.b, c.t);
-      if (FLUIDITY) stageApron<BW, float>(g, g.flu, ft, c.b, c.t);
+      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t cell) {
+        dt_[a] = g.den[cell];
+        if (SRC) tt[a] = g.tem[cell];
+        if (MOIST) vt[a] = g.vap[cell];
+        if (FLUIDITY) ft[a] = g.flu[cell];
+      });
     }
     __syncthreads();
     if (!active) continue;

@@ -69,6 +69,9 @@ def dc_workload(name):
     if name == "cfg4_128":  # terrain + moist atmosphere + wind, all levels dense
         w = scenes.config(4, scale=8)
         return w.kind, w.size, w.params, w.max_num_blocks, True
+    if name == "cfg4_256":  # level 0 has 8192 dense blocks: the multi-block-per-iteration sweeps
+        w = scenes.config(4, scale=4)
+        return w.kind, w.size, w.params, w.max_num_blocks, True
     if name == "cfg4_128_sparse":  # the same scene with sparse finer levels (moves, refinement)
         w = scenes.config(4, scale=8)
         w.params.score_mode = scenes.SCORE_GEOMETRIC
@@ -99,6 +102,23 @@ def assert_tables_equal(g, c):
 
 DC_CASES = ["ref64_bw4", "ref64_bw4_solids", "ref64_bw8", "cfg1_64", "cfg1_128", "cfg2_128", "cfg4_128",
             "cfg4_128_sparse"]
+
+
+def test_dcgrid_large_level_steps_bitwise():
+    """A level with more blocks than two rounds of the launch grid (config 4 at 256x64x256: 8192
+    dense blocks at level 0) takes the Jacobi variant with two blocks per iteration: two steps
+    bit-exact against the FP32 oracle."""
+    g, c = pair("cfg4_256")
+    for step in range(2):
+        say(f"[case] cfg4_256 step {step}")
+        g.step(1)
+        c.step(1)
+        assert_tables_equal(g, c)
+        m = allocated_cells(c)
+        for f in FIELDS_DC + ("vapor",):
+            assert bitwise(g.export_field(f), c.export_field(f), m), ("cfg4_256", step, f)
+    g.close()
+    c.close()
 
 
 @pytest.mark.parametrize("name", DC_CASES)
