@@ -388,15 +388,17 @@ struct DCGridSim : SimBase {
     DCG_TRY(divergenceKernel());
     return restrictAll(g.div);
   }
-  // One Jacobi half-sweep of a level (fwd: p -> p~, else p~ -> p); `two`: two blocks per
-  // sub-group and iteration; dense levels (l >= S) address their faces without descriptors.
-  int jacobiLaunch(int level, bool fwd, bool two, unsigned n, float alpha) {
+  // One Jacobi half-sweep of a level (fwd: p -> p~, else p~ -> p); nb: blocks per sub-group
+  // and iteration (1, 2 or 4); dense levels (l >= S) address their faces without descriptors.
+  int jacobiLaunch(int level, bool fwd, int nb, unsigned n, float alpha) {
     auto* in = fwd ? g.pre : g.tpre;
     auto* out = fwd ? g.tpre : g.pre;
     const bool dense = level >= S;
 #define JAC(NB, D) DCG_LAUNCH(DCG_K_JACOBI, "k_jacobi", (k_jacobi<BW, NB, D><<<n, G::WG, 0, stream>>>(g, level, in, out, alpha)))
-    if (two && dense) JAC(2, true);
-    else if (two) JAC(2, false);
+    if (nb == 4 && dense) JAC(4, true);
+    else if (nb == 4) JAC(4, false);
+    else if (nb == 2 && dense) JAC(2, true);
+    else if (nb == 2) JAC(2, false);
     else if (dense) JAC(1, true);
     else JAC(1, false);
 #undef JAC
@@ -405,7 +407,7 @@ struct DCGridSim : SimBase {
   int jacobiOne(int level, bool fwd) {
     const int s = 1 << level;
     const float alpha = (float)(s * s) * P.dx * P.dx;
-    return jacobiLaunch(level, fwd, false, slotGrid<BW>(maxL[level]), alpha);
+    return jacobiLaunch(level, fwd, 1, slotGrid<BW>(maxL[level]), alpha);
   }
   int prolongate(int level) {
     DCG_LAUNCH(DCG_K_PROLONGATE, "k_prolongate",
@@ -451,12 +453,13 @@ struct DCGridSim : SimBase {
     Pending batch;  // the run of back-to-back sweeps of one level is timed as one batch
     DCG_TRY(beginTimed(DCG_K_JACOBI, &batch));
     batchK = DCG_K_JACOBI;
-    // two blocks per sub-group and iteration once the level spans more than two rounds of the
-    // grid (the large levels, bound by reads in flight)
-    const bool two = (size_t)maxL[level] > 2 * (size_t)n * G::BPW;
+    // more blocks per sub-group and iteration once the level spans several rounds of the grid
+    // (the large levels, bound by reads in flight)
+    const size_t rounds = (size_t)maxL[level] / ((size_t)n * G::BPW);
+    const int nb = rounds >= 16 ? 4 : rounds >= 2 ? 2 : 1;
     for (int i = 0; i < pairs; i++)
       for (int h = 0; h < 2; h++)
-        DCG_TRY(jacobiLaunch(level, h == 0, two, n, alpha));
+        DCG_TRY(jacobiLaunch(level, h == 0, nb, n, alpha));
     batchK = -1;
     return endTimed(DCG_K_JACOBI, &batch, 2 * pairs);
   }
