@@ -21,6 +21,7 @@ struct BlockCtx {
   uint32_t b;  // global block index (may be >= B for tail threads)
   int t;       // local cell (Morton) index
   int sub;     // sub-group (block slot) inside the workgroup
+  int level;   // level of block b (slot walks only)
   bool valid;
 };
 template <int BW>
@@ -47,13 +48,20 @@ __device__ __forceinline__ uint32_t slotTotal(const DGBase& g, int level) {
   for (int l = 0; l < g.L; l++) n += g.ctl[CTL_LOADS + l];
   return n;
 }
-__device__ __forceinline__ uint32_t slotBlock(const DGBase& g, int level, uint32_t s) {
+// Block of slot s, and its level (the level whose prefix holds the slot: every block of a
+// walk is allocated, so its level needs no table read).
+__device__ __forceinline__ uint32_t slotBlock(const DGBase& g, int level, uint32_t s, int& blockLevel) {
+  blockLevel = level;
   if (level >= 0) return g.offs[level] + s;
   for (int l = 0; l < g.L; l++) {
     const uint32_t n = g.ctl[CTL_LOADS + l];
-    if (s < n) return g.offs[l] + s;
+    if (s < n) {
+      blockLevel = l;
+      return g.offs[l] + s;
+    }
     s -= n;
   }
+  blockLevel = 0;
   return 0;
 }
 template <int BW>
@@ -72,7 +80,8 @@ __device__ __forceinline__ BlockCtx<BW> slotCtx(const DGBase& g, int level, uint
   c.t = threadIdx.x % BV;
   const uint32_t slot = base + c.sub;
   c.valid = slot < total;
-  c.b = c.valid ? slotBlock(g, level, slot) : 0;
+  c.level = 0;
+  c.b = c.valid ? slotBlock(g, level, slot, c.level) : 0;
   return c;
 }
 
@@ -109,24 +118,22 @@ __device__ __forceinline__ void stageScalarDense(const DGBase& g, int l, const S
   }
 }
 
-// Fluidity of the six apron neighbours of local cell t (1 when the field is known to be 1).
-template <int BW, bool FLUIDITY, class G>
-__device__ __forceinline__ void neighbourFluidity(const G& g, uint32_t b, int t, float w[6]) {
+// Fluidity of the six apron neighbours of tile entry a, from the block's staged fluidity tile
+// (1 when the field is known to be 1).
+template <int BW, bool FLUIDITY>
+__device__ __forceinline__ void tileFluidity(const float* ft, int a, float w[6]) {
+  constexpr int A = Geo<BW>::A;
   if (!FLUIDITY) {
 #pragma unroll
     for (int k = 0; k < 6; k++) w[k] = 1.f;
     return;
   }
-  constexpr int BB = Geo<BW>::BB;
-  int x, y, z;
-  locPos<BB>((uint32_t)t, x, y, z);
-  const int ai = x + 1, aj = y + 1, ak = z + 1;
-  w[0] = g.flu[apronCell<BW>(g, b, ai - 1, aj, ak)];  // l
-  w[1] = g.flu[apronCell<BW>(g, b, ai + 1, aj, ak)];  // r
-  w[2] = g.flu[apronCell<BW>(g, b, ai, aj - 1, ak)];  // d
-  w[3] = g.flu[apronCell<BW>(g, b, ai, aj + 1, ak)];  // u
-  w[4] = g.flu[apronCell<BW>(g, b, ai, aj, ak - 1)];  // b
-  w[5] = g.flu[apronCell<BW>(g, b, ai, aj, ak + 1)];  // f
+  w[0] = ft[a - A * A];  // l
+  w[1] = ft[a + A * A];  // r
+  w[2] = ft[a - A];      // d
+  w[3] = ft[a + A];      // u
+  w[4] = ft[a - 1];      // b
+  w[5] = ft[a + 1];      // f
 }
 
 // ------------------------------------------------------------------ Jacobi half-sweep
@@ -180,42 +187,45 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(G g, int level, const ty
 template <int BW, bool FLUIDITY, class G>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
-                BB = Geo<BW>::BB, SBB = Geo<BW>::SBB;
+                SBB = Geo<BW>::SBB;
   __shared__ float3 tiles[BPW][A3];
+  __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
-    const bool active = c.valid && g.lev[c.b] != FREE;
     float3* tile = tiles[c.sub];
-    int3 p0 = make_int3(0, 0, 0);
-    int lb = 0, scale = 1;
+    float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+    const int scale = 1 << c.level;
     const uint32_t cell = c.b * BV + c.t;
-    if (active) {
-      lb = g.lev[c.b];
-      scale = 1 << lb;
-      p0 = g.pos[c.b];
+    bool leaf = false;
+    if (c.valid) {
+      const int3 p0 = g.pos[c.b];
+      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
       g.pre[cell] = 0.f;
       g.tpre[cell] = 0.f;
       g.div[cell] = 0.f;
-      tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
+      const int ac = apronIndexOfLocal<BW>(c.t);
+      tile[ac] = g.vel[cell];
+      if (FLUIDITY) ft[ac] = g.flu[cell];
       for (int fc = c.t; fc < FACE; fc += BV) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        const float3 v = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
-        tile[(ai * A + aj) * A + ak] = velocityBC(P, v, p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
+        const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
+        const int a = (ai * A + aj) * A + ak;
+        tile[a] = velocityBC(P, (float3)g.vel[src], p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
+        if (FLUIDITY) ft[a] = g.flu[src];
       }
     }
     __syncthreads();
-    if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) continue;
+    if (!leaf) continue;
     const float alpha = .5f * P.rdx / (float)scale;
-    float w[6];
-    neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
     const int a = apronIndexOfLocal<BW>(c.t);
+    float w[6];
+    tileFluidity<BW, FLUIDITY>(ft, a, w);
     const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
                  vf = tile[a + 1];
     g.div[cell] = alpha * (w[1] * vr.x - w[0] * vl.x + w[3] * vu.y - w[2] * vd.y + w[5] * vf.z - w[4] * vb.z);
-    (void)BB;
   }
 }
 
@@ -223,28 +233,41 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(G g, dcg_params P) {
 // k_dcgrid_apply_pressure (dcgrid_fluid.cu:232-259), leaf cells only.
 template <int BW, bool FLUIDITY, class G>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(G g, dcg_params P) {
-  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, SBB = Geo<BW>::SBB;
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
+                SBB = Geo<BW>::SBB;
   __shared__ float tiles[BPW][A3];
+  __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
-    const bool active = c.valid && g.lev[c.b] != FREE;
     float* tile = tiles[c.sub];
-    int lb = 0;
-    if (active) {
-      lb = g.lev[c.b];
-      stageScalar<BW>(g, g.pre, tile, c.b, c.t);
+    float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+    const uint32_t cell = c.b * BV + c.t;
+    bool leaf = false;
+    float3 v = make_float3(0.f, 0.f, 0.f);
+    if (c.valid) {
+      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
+      if (leaf) v = g.vel[cell];  // only this thread reads and writes its cell's velocity
+      const int ac = apronIndexOfLocal<BW>(c.t);
+      tile[ac] = g.pre[cell];
+      if (FLUIDITY) ft[ac] = g.flu[cell];
+      for (int fc = c.t; fc < FACE; fc += BV) {
+        int ai, aj, ak;
+        faceCell<BW>(fc, ai, aj, ak);
+        const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
+        const int a = (ai * A + aj) * A + ak;
+        tile[a] = g.pre[src];
+        if (FLUIDITY) ft[a] = g.flu[src];
+      }
     }
     __syncthreads();
-    if (!active || g.chi[8 * c.b + (c.t >> SBB)] != NFU) continue;
-    const float alpha = .5f * P.rdx / (float)(1 << lb);
-    float w[6];
-    neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
+    if (!leaf) continue;
+    const float alpha = .5f * P.rdx / (float)(1 << c.level);
     const int a = apronIndexOfLocal<BW>(c.t);
-    const uint32_t cell = c.b * BV + c.t;
+    float w[6];
+    tileFluidity<BW, FLUIDITY>(ft, a, w);
     const float p = tile[a];
-    float3 v = g.vel[cell];
     v.x = v.x - alpha * (w[1] * (tile[a + A * A] - p) + w[0] * (p - tile[a - A * A]));
     v.y = v.y - alpha * (w[3] * (tile[a + A] - p) + w[2] * (p - tile[a - A]));
     v.z = v.z - alpha * (w[5] * (tile[a + 1] - p) + w[4] * (p - tile[a - 1]));
@@ -259,29 +282,33 @@ template <int BW, bool FLUIDITY, class G>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE;
   __shared__ float3 tiles[BPW][A3];
+  __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
-    const bool active = c.valid && g.lev[c.b] != FREE;
     float3* tile = tiles[c.sub];
-    int lb = 0;
+    float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     const uint32_t cell = c.b * BV + c.t;
-    if (active) {
-      lb = g.lev[c.b];
-      tile[apronIndexOfLocal<BW>(c.t)] = g.vel[cell];
+    if (c.valid) {
+      const int ac = apronIndexOfLocal<BW>(c.t);
+      tile[ac] = g.vel[cell];
+      if (FLUIDITY) ft[ac] = g.flu[cell];
       for (int fc = c.t; fc < FACE; fc += BV) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        tile[(ai * A + aj) * A + ak] = g.vel[apronCell<BW>(g, c.b, ai, aj, ak)];
+        const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
+        const int a = (ai * A + aj) * A + ak;
+        tile[a] = g.vel[src];
+        if (FLUIDITY) ft[a] = g.flu[src];
       }
     }
     __syncthreads();
-    if (!active) continue;
-    const float alpha = .5f * P.rdx / (float)(1 << lb);
-    float w[6];
-    neighbourFluidity<BW, FLUIDITY>(g, c.b, c.t, w);
+    if (!c.valid) continue;
+    const float alpha = .5f * P.rdx / (float)(1 << c.level);
     const int a = apronIndexOfLocal<BW>(c.t);
+    float w[6];
+    tileFluidity<BW, FLUIDITY>(ft, a, w);
     const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
                  vf = tile[a + 1];
     const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
@@ -482,11 +509,9 @@ struct CrossSample {
 
 // Backtraced foot point of a leaf cell (dcgrid_fluid.cu:115-122).
 template <int BW>
-__device__ __forceinline__ void footPoint(const DGBase& g, const dcg_params& P, uint32_t b, int t, float3 v, float& px,
-                                          float& py, float& pz, int& x, int& y, int& z, int& level) {
+__device__ __forceinline__ void footPoint(const dcg_params& P, int3 p0, int level, int t, float3 v, float& px, float& py,
+                                          float& pz, int& x, int& y, int& z) {
   constexpr int BB = Geo<BW>::BB;
-  const int3 p0 = g.pos[b];
-  level = g.lev[b];
   const float scale = (float)(1 << level);
   const float alpha = P.dt * P.rdx;
   int lx, ly, lz;
@@ -502,15 +527,15 @@ __device__ __forceinline__ void footPoint(const DGBase& g, const dcg_params& P, 
 // Same sample as CrossSample when the foot point's finest block is the thread's own block b:
 // corners and fluidity come from the block's LDS tile (interior + full apron, raw values).
 // Returns false (and the caller takes the global path) when the foot point lies in another
-// block or in a refined subblock of b.
+// block or in a refined subblock of b (chi8: b's eight child links, staged in LDS).
 template <int BW, bool FLUIDITY>
 struct TileSample {
   int a0;  // tile index of corner 000
   float w[8];
   int x0, y0, z0, scale;
   bool zero;
-  __device__ __forceinline__ bool init(const DGBase& g, float px, float py, float pz, uint32_t b, int3 p0, int lb,
-                                       const float* fluTile) {
+  __device__ __forceinline__ bool init(const DGBase& g, float px, float py, float pz, const uint32_t* chi8, int3 p0,
+                                       int lb, const float* fluTile) {
     constexpr int A = Geo<BW>::A, BB = Geo<BW>::BB;
     const int ix = clampi((int)floorf(px), 0, g.gx - 1);
     const int iy = clampi((int)floorf(py), 0, g.gy - 1);
@@ -518,7 +543,7 @@ struct TileSample {
     const int qx = (ix >> lb) - p0.x, qy = (iy >> lb) - p0.y, qz = (iz >> lb) - p0.z;
     if ((unsigned)qx >= (unsigned)BW || (unsigned)qy >= (unsigned)BW || (unsigned)qz >= (unsigned)BW) return false;
     const int t = (((qx >> (BB - 1)) & 1) << 2) | (((qy >> (BB - 1)) & 1) << 1) | ((qz >> (BB - 1)) & 1);
-    if (g.chi[8 * b + t] != NFU) return false;
+    if (chi8[t] != NFU) return false;
     scale = 1 << lb;
     const float scaleInv = 1.f / (float)scale;
     const float x = px * scaleInv - .5f, y = py * scaleInv - .5f, z = pz * scaleInv - .5f;
@@ -585,36 +610,44 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(G g, dcg_params
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   __shared__ float3 vtiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
+  __shared__ uint32_t chis[BPW][8];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
-    const bool active = c.valid && g.lev[c.b] != FREE;
     float3* vt = vtiles[c.sub];
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+    uint32_t* chi8 = chis[c.sub];
+    const uint32_t cell = c.b * BV + c.t;
     int3 p0 = make_int3(0, 0, 0);
-    int lb = 0;
-    if (active) {
-      lb = g.lev[c.b];
+    bool leaf = false;
+    float tc = 0.f, dc = 0.f;
+    if (c.valid) {
       p0 = g.pos[c.b];
-      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t cell) {
-        vt[a] = g.vel[cell];
-        if (FLUIDITY) ft[a] = g.flu[cell];
+      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
+      if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
+      if (BUOY && leaf) {
+        tc = g.tem[cell];
+        dc = g.den[cell];
+      }
+      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t src) {
+        vt[a] = g.vel[src];
+        if (FLUIDITY) ft[a] = g.flu[src];
       });
     }
     __syncthreads();
-    if (!active) continue;
-    const uint32_t cell = c.b * BV + c.t;
-    if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
+    if (!c.valid) continue;
+    if (!leaf) {
       g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
       continue;
     }
+    const int level = c.level;
     float px, py, pz;
-    int x, y, z, level;
-    footPoint<BW>(g, P, c.b, c.t, vt[apronIndexOfLocal<BW>(c.t)], px, py, pz, x, y, z, level);
+    int x, y, z;
+    footPoint<BW>(P, p0, level, c.t, vt[apronIndexOfLocal<BW>(c.t)], px, py, pz, x, y, z);
     float3 acc = make_float3(0.f, 0.f, 0.f);
     TileSample<BW, FLUIDITY> ts;
-    if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
+    if (ts.init(g, px, py, pz, chi8, p0, level, ft)) {
       if (!ts.zero) {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -637,7 +670,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(G g, dcg_params
     }
     if (BUOY) {
       const float ta = ambientT(P, cellHeight(y, 1 << level));
-      const float bf = P.dt * (P.buoy_beta * (g.tem[cell] - ta) - P.buoy_alpha * g.den[cell]);
+      const float bf = P.dt * (P.buoy_beta * (tc - ta) - P.buoy_alpha * dc);
       acc.y = acc.y + bf;
     }
     g.tvel[cell] = acc;
@@ -655,43 +688,48 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params 
   __shared__ float ttiles[SRC ? BPW : 1][SRC ? A3 : 1];
   __shared__ float vtiles[MOIST ? BPW : 1][MOIST ? A3 : 1];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
+  __shared__ uint32_t chis[BPW][8];
   const bool sphere = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
-    const bool active = c.valid && g.lev[c.b] != FREE;
     float* dt_ = dtiles[c.sub];
     float* tt = SRC ? ttiles[c.sub] : nullptr;
     float* vt = MOIST ? vtiles[c.sub] : nullptr;
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
+    uint32_t* chi8 = chis[c.sub];
+    const uint32_t cell = c.b * BV + c.t;
     int3 p0 = make_int3(0, 0, 0);
-    int lb = 0;
-    if (active) {
-      lb = g.lev[c.b];
+    bool leaf = false;
+    float3 u = make_float3(0.f, 0.f, 0.f);
+    if (c.valid) {
       p0 = g.pos[c.b];
-      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t cell) {
-        dt_[a] = g.den[cell];
-        if (SRC) tt[a] = g.tem[cell];
-        if (MOIST) vt[a] = g.vap[cell];
-        if (FLUIDITY) ft[a] = g.flu[cell];
+      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
+      if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
+      if (leaf) u = g.vel[cell];
+      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t src) {
+        dt_[a] = g.den[src];
+        if (SRC) tt[a] = g.tem[src];
+        if (MOIST) vt[a] = g.vap[src];
+        if (FLUIDITY) ft[a] = g.flu[src];
       });
     }
     __syncthreads();
-    if (!active) continue;
-    const uint32_t cell = c.b * BV + c.t;
-    if (g.chi[8 * c.b + (c.t >> SBB)] != NFU) {
+    if (!c.valid) continue;
+    if (!leaf) {
       g.tden[cell] = 0.f;
       if (SRC) g.ttem[cell] = 0.f;
       if (MOIST) g.tvap[cell] = 0.f;
       continue;
     }
+    const int level = c.level;
     float px, py, pz;
-    int x, y, z, level;
-    footPoint<BW>(g, P, c.b, c.t, g.vel[cell], px, py, pz, x, y, z, level);
+    int x, y, z;
+    footPoint<BW>(P, p0, level, c.t, u, px, py, pz, x, y, z);
     float q = 0.f, tq = 0.f, vq = 0.f;
     TileSample<BW, FLUIDITY> ts;
-    if (ts.init(g, px, py, pz, c.b, p0, lb, ft)) {
+    if (ts.init(g, px, py, pz, chi8, p0, level, ft)) {
       if (!ts.zero) {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -735,7 +773,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params 
         tq = sourceTemperature(P, x, y, z, level);
       }
       if (MOIST) {
-        tq = tq - (P.dt * P.adiabatic_rate) * g.vel[cell].y;  // expansion cooling of rising air
+        tq = tq - (P.dt * P.adiabatic_rate) * u.y;  // expansion cooling of rising air
         saturationAdjust(P, vq, q, tq);
         g.tvap[cell] = vq;
       }

@@ -114,10 +114,12 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(G g, dcg_params P,
     const int sub = c.sub, t = c.t;
     const uint32_t b = c.b;
     const bool active = c.valid;
-    const int level = active ? (int)g.lev[b] : 0;
+    const int level = c.level;
     float* tile = tiles[sub];
     if (t < 8) smax[sub][t] = 0u;
+    float3 w = make_float3(0.f, 0.f, 0.f);
     if (active) {
+      w = g.vort[b * BV + t];
       tile[apronIndexOfLocal<BW>(t)] = g.den[b * BV + t];
       for (int fc = t; fc < FACE; fc += BV) {
         int ai, aj, ak;
@@ -134,7 +136,6 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(G g, dcg_params P,
       const float gyv = alpha * (tile[a + A] - tile[a - A]);
       const float gzv = alpha * (tile[a + 1] - tile[a - 1]);
       const float gm = __builtin_sqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-      const float3 w = g.vort[b * BV + t];
       const float wm = __builtin_sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
       const float fg = gm / P.grad_threshold, fw = wm / P.vort_threshold;
       cs = fg > fw ? fg : fw;
