@@ -280,12 +280,11 @@ struct DCGridSim : SimBase {
   int adaptTopology() override {
     const unsigned nWG = slotGrid<BW>(B);
     DCG_TRY(fill32(g.flg, 0, B));
-    if (P.score_mode == DCG_SCORE_FEATURES) {
+    if (P.score_mode == DCG_SCORE_FEATURES) {  // vorticity and subblock features in one pass
       if (fluidityOn())
-        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
+        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_features", (k_features<BW, true><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
       else
-        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_vorticity", (k_vorticity<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
-      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_sub_features", (k_sub_features<BW><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
+        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_features", (k_features<BW, false><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
     }
     // moveBlocks (:348-437)
     DCG_TRY(subScores());

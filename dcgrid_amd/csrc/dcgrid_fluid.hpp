@@ -384,25 +384,45 @@ __device__ __forceinline__ void parentStencil(const DGBase& g, uint32_t ps, int 
   q[7] = apronCell<BW>(g, pb, ai + si, aj + sj, ak + sk);
 }
 
-// k_dcgrid_prolongate (dcgrid_multigrid_solver.cu:43-76)
+// k_dcgrid_prolongate (dcgrid_multigrid_solver.cu:43-76). The parent cells a child block reads
+// are the (BW/2 + 2)^3 region of the parent's apron around the child's octant: the workgroup
+// stages that region once (one apron lookup per entry) and every cell takes its eight
+// parentStencil values from LDS (the same cells in the same order).
 template <int BW, class G>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_prolongate(G g, int level) {
-  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW, H = BW / 2 + 2;
+  __shared__ float tiles[BPW][H * H * H];
   const uint32_t total = slotTotal(g, level);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    __syncthreads();  // the previous block's region is no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, level, base, total);
-    if (!c.valid) continue;
-    const uint32_t ps = g.par[c.b];
+    const uint32_t ps = c.valid ? g.par[c.b] : NFU;
+    float* tile = tiles[c.sub];
+    int3 p0 = make_int3(0, 0, 0);
+    int ox = 0, oy = 0, oz = 0;  // parent apron coordinates of the region's first entry
+    if (ps != NFU) {
+      p0 = g.pos[c.b];
+      ox = (p0.x / 2) % BW;
+      oy = (p0.y / 2) % BW;
+      oz = (p0.z / 2) % BW;
+      if (c.t < H * H * H) {
+        const int i = c.t / (H * H), j = (c.t / H) % H, k = c.t % H;
+        tile[c.t] = g.pre[apronCell<BW>(g, ps / 8, ox + i, oy + j, oz + k)];
+      }
+    }
+    __syncthreads();
     if (ps == NFU) continue;
-    const int3 p0 = g.pos[c.b];
     int lx, ly, lz;
     locPos<BB>((uint32_t)c.t, lx, ly, lz);
-    uint32_t q[8];
-    parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
-    const auto* p = g.pre;
-    g.pre[c.b * BV + c.t] = (27.f * p[q[0]] + 9.f * (p[q[1]] + p[q[2]] + p[q[3]]) +
-                             3.f * (p[q[4]] + p[q[5]] + p[q[6]]) + p[q[7]]) /
-                            64.f;
+    const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
+    // parentStencil's apron coordinates, relative to the staged region
+    const int ti = 1 + (x / 2) % BW - ox, tj = 1 + (y / 2) % BW - oy, tk = 1 + (z / 2) % BW - oz;
+    const int si = x % 2 ? 1 : -1, sj = y % 2 ? 1 : -1, sk = z % 2 ? 1 : -1;
+    auto at = [&](int i, int j, int k) { return tile[(i * H + j) * H + k]; };
+    const float q0 = at(ti, tj, tk), q1 = at(ti, tj, tk + sk), q2 = at(ti, tj + sj, tk), q3 = at(ti + si, tj, tk);
+    const float q4 = at(ti, tj + sj, tk + sk), q5 = at(ti + si, tj, tk + sk), q6 = at(ti + si, tj + sj, tk);
+    const float q7 = at(ti + si, tj + sj, tk + sk);
+    g.pre[c.b * BV + c.t] = (27.f * q0 + 9.f * (q1 + q2 + q3) + 3.f * (q4 + q5 + q6) + q7) / 64.f;
   }
 }
 

@@ -97,53 +97,73 @@ __global__ void __launch_bounds__(256) k_sub_scores_geo(DGBase g, float* __restr
 }
 
 // Feature score (extension): per cell max(|grad density| / grad_threshold, |vorticity| /
-// vort_threshold); a subblock's feature value is the maximum over its cells. Density is staged
-// with the apron faces in LDS; the subblock maximum is an LDS reduction. Computed once per
+// vort_threshold); a subblock's feature value is the maximum over its cells. One pass computes
+// the vorticity field (k_vorticity's values, stored as k_vorticity stores them) and the
+// features: velocity, fluidity and density are staged with the apron faces in LDS, the magnitude
+// uses the stored vorticity, and the subblock maximum is an LDS reduction. Computed once per
 // adaptation: the fields and the (pre-refresh) apron do not change between the score pass before
 // the moves and the one before refinement, so both passes see the same feature values.
-template <int BW, class G>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_sub_features(G g, dcg_params P, float* __restrict__ feat) {
+template <int BW, bool FLUIDITY, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_features(G g, dcg_params P, float* __restrict__ feat) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
                 SBB = Geo<BW>::SBB;
-  __shared__ float tiles[BPW][A3];
+  __shared__ float3 vtiles[BPW][A3];
+  __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
+  __shared__ float dtiles[BPW][A3];
   __shared__ unsigned int smax[BPW][8];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
-    __syncthreads();  // the previous block's tile and maxima are no longer read
+    __syncthreads();  // the previous block's tiles and maxima are no longer read
     const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
     const int sub = c.sub, t = c.t;
-    const uint32_t b = c.b;
-    const bool active = c.valid;
-    const int level = c.level;
-    float* tile = tiles[sub];
+    const uint32_t b = c.b, cell = b * BV + t;
+    float3* vt = vtiles[sub];
+    float* ft = FLUIDITY ? ftiles[sub] : nullptr;
+    float* dt_ = dtiles[sub];
     if (t < 8) smax[sub][t] = 0u;
-    float3 w = make_float3(0.f, 0.f, 0.f);
-    if (active) {
-      w = g.vort[b * BV + t];
-      tile[apronIndexOfLocal<BW>(t)] = g.den[b * BV + t];
+    if (c.valid) {
+      const int ac = apronIndexOfLocal<BW>(t);
+      vt[ac] = g.vel[cell];
+      if (FLUIDITY) ft[ac] = g.flu[cell];
+      dt_[ac] = g.den[cell];
       for (int fc = t; fc < FACE; fc += BV) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        tile[(ai * A + aj) * A + ak] = g.den[apronCell<BW>(g, b, ai, aj, ak)];
+        const uint32_t src = apronCell<BW>(g, b, ai, aj, ak);
+        const int a = (ai * A + aj) * A + ak;
+        vt[a] = g.vel[src];
+        if (FLUIDITY) ft[a] = g.flu[src];
+        dt_[a] = g.den[src];
       }
     }
     __syncthreads();
     float cs = 0.f;
-    if (active) {
-      const float alpha = .5f * P.rdx / (float)(1 << level);
+    if (c.valid) {
+      const float alpha = .5f * P.rdx / (float)(1 << c.level);
       const int a = apronIndexOfLocal<BW>(t);
-      const float gxv = alpha * (tile[a + A * A] - tile[a - A * A]);
-      const float gyv = alpha * (tile[a + A] - tile[a - A]);
-      const float gzv = alpha * (tile[a + 1] - tile[a - 1]);
+      // vorticity (k_vorticity)
+      float w[6];
+      tileFluidity<BW, FLUIDITY>(ft, a, w);
+      const float3 vl = vt[a - A * A], vr = vt[a + A * A], vd = vt[a - A], vu = vt[a + A], vb = vt[a - 1],
+                   vf = vt[a + 1];
+      const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
+      g.vort[cell] = make_float3(alpha * ((wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y)),
+                                 alpha * ((wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z)),
+                                 alpha * ((wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)));
+      const float3 om = g.vort[cell];  // the stored value (rounded as stored)
+      // density gradient
+      const float gxv = alpha * (dt_[a + A * A] - dt_[a - A * A]);
+      const float gyv = alpha * (dt_[a + A] - dt_[a - A]);
+      const float gzv = alpha * (dt_[a + 1] - dt_[a - 1]);
       const float gm = __builtin_sqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-      const float wm = __builtin_sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+      const float wm = __builtin_sqrtf(om.x * om.x + om.y * om.y + om.z * om.z);
       const float fg = gm / P.grad_threshold, fw = wm / P.vort_threshold;
       cs = fg > fw ? fg : fw;
     }
     // scores are >= 0, so their bit patterns order like their values
     atomicMax(&smax[sub][t >> SBB], __float_as_uint(cs));
     __syncthreads();
-    if (t < 8 && active) feat[8 * b + t] = __uint_as_float(smax[sub][t]);
+    if (t < 8 && c.valid) feat[8 * b + t] = __uint_as_float(smax[sub][t]);
   }
 }
 
