@@ -60,6 +60,12 @@ def library():
         lib.dcg_tape_forward.restype = C.c_int
         lib.dcg_tape_backward.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         lib.dcg_tape_backward.restype = C.c_int
+        lib.dcg_state_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        lib.dcg_state_size.restype = C.c_int
+        for name in ("dcg_save_state", "dcg_load_state"):
+            f = getattr(lib, name)
+            f.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+            f.restype = C.c_int
         _lib, _fns = lib, fns
     return _lib
 
@@ -109,6 +115,39 @@ class Simulation(Handle):
             self._check(_lib.dcg_timing_get(self._h, k, C.byref(ms), C.byref(n)), "timing_get")
             out[name] = (ms.value, n.value)
         return out
+
+    # -- versioned state (the DCGSTATE format, dcgrid_amd.state)
+    def state_size(self):
+        n = C.c_uint64()
+        self._check(_lib.dcg_state_size(self._h, C.byref(n)), "state_size")
+        return n.value
+
+    def save_state(self, path):
+        """Write the simulation's state to `path` (memory-mapped, no second host copy)."""
+        import numpy as np
+        n = self.state_size()
+        mm = np.memmap(path, dtype=np.uint8, mode="w+", shape=(n,))
+        try:
+            self._check(_lib.dcg_save_state(self._h, C.c_void_p(mm.ctypes.data), n), "save_state")
+            mm.flush()
+        finally:
+            del mm
+        return n
+
+    def load_state(self, path):
+        """Restore a state written by save_state from a grid created with the same shape."""
+        import numpy as np
+        mm = np.memmap(path, dtype=np.uint8, mode="r")
+        try:
+            self._check(_lib.dcg_load_state(self._h, C.c_void_p(mm.ctypes.data), mm.size), "load_state")
+        finally:
+            del mm
+        self.params = self.get_params()
+
+    def get_params(self):
+        p = Params()
+        self._check(_lib.dcg_get_params(self._h, C.byref(p)), "get_params")
+        return p
 
     # -- device-pointer access and the differentiable rollout (uniform grid)
     def export_field_device(self, field, ptr, nfloats):

@@ -335,6 +335,136 @@ int dcg_export_tables(dcg_sim* s, dcg_tables* t) {
   if (!t) return DCG_E_ARG;
   return I->exportTables(t);
 }
+// ------------------------------------------------------------------ state format
+namespace {
+struct StateHeader {
+  char magic[8];
+  uint32_t version, headerBytes;
+  int32_t kind, storage, size[3], blockWidth, levels;
+  uint32_t flags, paramsBytes, sectionCount;
+  uint64_t maxNumBlocks, numCells;
+};
+static_assert(sizeof(StateHeader) == 72, "state header layout");
+struct StateEntry {
+  uint32_t id, elem;
+  uint64_t offset, bytes;
+};
+static_assert(sizeof(StateEntry) == 24, "state entry layout");
+const char kStateMagic[8] = {'D', 'C', 'G', 'S', 'T', 'A', 'T', 'E'};
+uint64_t align256(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+// Header, section entries and total size of the state of I as it is now.
+int stateLayout(dcg::SimBase* I, StateHeader& h, std::vector<StateEntry>& ents,
+                std::vector<dcg::SimBase::StateSection>& secs, uint64_t& total) {
+  dcg_info inf;
+  DCG_TRY(I->info(&inf));
+  I->stateSections(secs);
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kStateMagic, 8);
+  h.version = DCG_STATE_VERSION;
+  h.kind = inf.kind;
+  h.storage = I->P.storage;
+  for (int a = 0; a < 3; a++) h.size[a] = inf.size[a];
+  h.blockWidth = inf.block_width;
+  h.levels = inf.levels;
+  h.flags = I->stateFlags();
+  h.paramsBytes = (uint32_t)sizeof(dcg_params);
+  h.sectionCount = (uint32_t)secs.size();
+  h.maxNumBlocks = inf.max_num_blocks;
+  h.numCells = inf.num_cells;
+  h.headerBytes = (uint32_t)(sizeof(StateHeader) + sizeof(dcg_params) + secs.size() * sizeof(StateEntry));
+  ents.clear();
+  uint64_t off = align256(h.headerBytes);
+  for (const auto& s : secs) {
+    ents.push_back({s.id, s.elem, off, s.bytes});
+    off = align256(off + s.bytes);
+  }
+  total = off;
+  return DCG_OK;
+}
+}  // namespace
+
+int dcg_state_size(dcg_sim* s, uint64_t* bytes) {
+  IMPL(s);
+  if (!bytes) return DCG_E_ARG;
+  StateHeader h;
+  std::vector<StateEntry> ents;
+  std::vector<SimBase::StateSection> secs;
+  return stateLayout(I, h, ents, secs, *bytes);
+}
+
+int dcg_save_state(dcg_sim* s, void* dst, uint64_t bytes) {
+  IMPL(s);
+  if (!dst) return DCG_E_ARG;
+  StateHeader h;
+  std::vector<StateEntry> ents;
+  std::vector<SimBase::StateSection> secs;
+  uint64_t total;
+  DCG_TRY(stateLayout(I, h, ents, secs, total));
+  if (bytes < total) return I->fail(DCG_E_ARG, "state buffer smaller than dcg_state_size");
+  char* out = (char*)dst;
+  std::memset(out, 0, align256(h.headerBytes));
+  std::memcpy(out, &h, sizeof(h));
+  std::memcpy(out + sizeof(h), &I->P, sizeof(dcg_params));
+  std::memcpy(out + sizeof(h) + sizeof(dcg_params), ents.data(), ents.size() * sizeof(StateEntry));
+  DCG_TRY(I->sync());  // every launch that writes the state has finished
+  for (size_t i = 0; i < secs.size(); i++) {
+    const uint64_t pad = align256(ents[i].offset + ents[i].bytes) - (ents[i].offset + ents[i].bytes);
+    if (pad) std::memset(out + ents[i].offset + ents[i].bytes, 0, pad);
+    const hipError_t e = hipMemcpyAsync(out + ents[i].offset, secs[i].dev, secs[i].bytes, hipMemcpyDeviceToHost, I->stream);
+    if (e != hipSuccess) return I->fail(DCG_E_HIP, std::string("state copy: ") + hipGetErrorString(e));
+  }
+  return I->sync();
+}
+
+int dcg_load_state(dcg_sim* s, const void* src, uint64_t bytes) {
+  IMPL(s);
+  if (!src) return DCG_E_ARG;
+  const char* in = (const char*)src;
+  StateHeader fh;
+  if (bytes < sizeof(fh)) return I->fail(DCG_E_ARG, "state truncated (header)");
+  std::memcpy(&fh, in, sizeof(fh));
+  if (std::memcmp(fh.magic, kStateMagic, 8) != 0) return I->fail(DCG_E_ARG, "not a DCGSTATE buffer");
+  if (fh.version != DCG_STATE_VERSION)
+    return I->fail(DCG_E_UNSUPPORTED, "state version " + std::to_string(fh.version) + " (this library reads " +
+                                          std::to_string(DCG_STATE_VERSION) + ")");
+  StateHeader h;
+  std::vector<StateEntry> ents;
+  std::vector<SimBase::StateSection> secs;
+  uint64_t total;
+  DCG_TRY(stateLayout(I, h, ents, secs, total));
+  if (fh.paramsBytes != h.paramsBytes || fh.headerBytes != h.headerBytes)
+    return I->fail(DCG_E_ARG, "state header does not match this library's parameter layout");
+  if (fh.kind != h.kind || fh.storage != h.storage || std::memcmp(fh.size, h.size, sizeof(h.size)) != 0 ||
+      fh.blockWidth != h.blockWidth || fh.levels != h.levels || fh.maxNumBlocks != h.maxNumBlocks ||
+      fh.numCells != h.numCells || fh.sectionCount != h.sectionCount)
+    return I->fail(DCG_E_ARG, "state was saved from a grid of another kind, size, block width, level count, "
+                              "budget, storage or moist setting");
+  if (bytes < h.headerBytes) return I->fail(DCG_E_ARG, "state truncated (section table)");
+  std::vector<StateEntry> fe(ents.size());
+  std::memcpy(fe.data(), in + sizeof(fh) + sizeof(dcg_params), fe.size() * sizeof(StateEntry));
+  for (size_t i = 0; i < ents.size(); i++) {
+    if (fe[i].id != ents[i].id || fe[i].elem != ents[i].elem || fe[i].bytes != ents[i].bytes ||
+        fe[i].offset != ents[i].offset)
+      return I->fail(DCG_E_ARG, "state section " + std::to_string(i) + " does not match this grid");
+    if (fe[i].offset + fe[i].bytes > bytes) return I->fail(DCG_E_ARG, "state truncated (section data)");
+  }
+  dcg_params p;
+  std::memcpy(&p, in + sizeof(fh), sizeof(p));
+  if (p.gx != I->P.gx || p.gy != I->P.gy || p.gz != I->P.gz || p.block_width != I->P.block_width ||
+      p.max_levels != I->P.max_levels || p.moist != I->P.moist || p.storage != I->P.storage)
+    return I->fail(DCG_E_ARG, "state parameters differ in what is fixed at creation");
+  DCG_TRY(I->sync());  // no launch still reads or writes the arrays being replaced
+  for (size_t i = 0; i < secs.size(); i++) {
+    const hipError_t e = hipMemcpyAsync(secs[i].dev, in + fe[i].offset, secs[i].bytes, hipMemcpyHostToDevice, I->stream);
+    if (e != hipSuccess) return I->fail(DCG_E_HIP, std::string("state copy: ") + hipGetErrorString(e));
+  }
+  DCG_TRY(I->sync());
+  I->P = p;
+  I->afterStateLoad(fh.flags);
+  return DCG_OK;
+}
+
 int dcg_run_stage(dcg_sim* s, int32_t stage, int32_t arg) {
   IMPL(s);
   return I->runStage(stage, arg);

@@ -90,6 +90,19 @@ struct SimBase {
     return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
   }
 
+  // Persistent state (dcg_save_state / dcg_load_state): the device arrays every later step
+  // depends on, one section each, plus host flags.
+  struct StateSection {
+    uint32_t id;    // DCG_STATE_*
+    uint32_t elem;  // bytes per element
+    void* dev;      // device array
+    uint64_t bytes;
+  };
+  virtual void stateSections(std::vector<StateSection>& out) = 0;
+  virtual uint32_t stateFlags() const = 0;
+  // After a state was copied in: host flags restored, caches derived from the old state dropped.
+  virtual void afterStateLoad(uint32_t flags) = 0;
+
   // Step replay through a HIP graph: two consecutive steps are captured once (the host-side
   // pointer swaps of the advected fields return to their start after two steps) and replayed
   // while the state the launches were recorded with (graphKeyBytes) is unchanged.

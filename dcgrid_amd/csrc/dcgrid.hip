@@ -829,6 +829,32 @@ struct DCGridSim : SimBase {
       default: return nullptr;
     }
   }
+  // State: the block tables verbatim (neighbour descriptors and hash included: the descriptors'
+  // out-of-domain clamp form depends on how each block was placed), the control words and the
+  // exported fields in their storage format.
+  void stateSections(std::vector<StateSection>& out) override {
+    const uint32_t es = (uint32_t)sizeof(SF);
+    out = {{DCG_STATE_CONTROL, 4, g.ctl, (uint64_t)CTL_WORDS * 4},
+           {DCG_STATE_POSITIONS, 4, g.pos, (uint64_t)B * 12},
+           {DCG_STATE_LEVELS, 1, g.lev, (uint64_t)B},
+           {DCG_STATE_FLAGS, 4, g.flg, (uint64_t)B * 4},
+           {DCG_STATE_PARENTS, 4, g.par, (uint64_t)B * 4},
+           {DCG_STATE_CHILDREN, 4, g.chi, (uint64_t)B * 32},
+           {DCG_STATE_NEIGHBOURS, 4, g.nd, (uint64_t)B * 27 * 16},
+           {DCG_STATE_HASH_KEYS, 4, g.hkey, (uint64_t)H * 4},
+           {DCG_STATE_HASH_VALUES, 4, g.hval, (uint64_t)H * 4}};
+    for (int f = 0; f < DCG_NUM_FIELDS; f++) {
+      if (!fieldPtr(f)) continue;
+      const uint64_t comps = (f == DCG_FIELD_VELOCITY || f == DCG_FIELD_VORTICITY) ? 3 : 1;
+      out.push_back({(uint32_t)(DCG_STATE_FIELD + f), es, fieldPtr(f), comps * C * es});
+    }
+  }
+  uint32_t stateFlags() const override { return customFluidity ? 1u : 0u; }
+  void afterStateLoad(uint32_t flags) override {
+    customFluidity = (flags & 1u) != 0;
+    dropGraph();
+  }
+
   // Fields are exchanged as FP32: 16-bit storage is widened exactly on export and rounded to
   // nearest even on import.
   int exportField(int f, float* dst, uint64_t n) override {

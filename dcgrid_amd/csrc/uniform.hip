@@ -663,6 +663,22 @@ struct UniformSim : SimBase {
       default: return fail(DCG_E_ARG, "field not available on the uniform grid");
     }
   }
+  // State: the exported fields (pressure, fluidity and divergence with their mip pyramids).
+  void stateSections(std::vector<StateSection>& out) override {
+    out.clear();
+    for (int f = 0; f < DCG_NUM_FIELDS; f++) {
+      uint64_t n;
+      if (!fieldPtr(f) || fieldSize(f, &n) != DCG_OK) continue;
+      out.push_back({(uint32_t)(DCG_STATE_FIELD + f), 4, fieldPtr(f), n * 4});
+    }
+  }
+  uint32_t stateFlags() const override { return customFluidity ? 1u : 0u; }
+  void afterStateLoad(uint32_t flags) override {
+    customFluidity = (flags & 1u) != 0;
+    fluKey.clear();  // the pyramid is recomputed from the loaded parameters at the next adaptation
+    tapeValid = false;
+    dropGraph();
+  }
   float* fieldPtr(int f) {
     switch (f) {
       case DCG_FIELD_DENSITY: return g.den;

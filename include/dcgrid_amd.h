@@ -307,6 +307,47 @@ int dcg_tape_backward(dcg_sim* s, const float* g_density, const float* g_velocit
 /* Block tables in reference semantics (DCGrid only). */
 int dcg_export_tables(dcg_sim* s, dcg_tables* t);
 
+/* Versioned state of a simulation (extension: the reference keeps its state in device memory
+ * only, src/dcgrid/dcgrid.h:5-43 and src/uniformgrid/uniformgrid.h:31-46). A state holds
+ * everything later steps depend on, so a run resumed from it continues bit for bit as the run it
+ * was saved from: the parameters, the DCGrid block tables (with the neighbour descriptors and the
+ * hash verbatim) and device control words (block loads, move limits), and the fields in their
+ * storage format. Counters (dcg_cell_steps), timing and the graph are not part of it.
+ *
+ * Layout (little endian, version DCG_STATE_VERSION):
+ *   0   char[8]  "DCGSTATE"
+ *   8   uint32   version
+ *   12  uint32   header bytes = 72 + params bytes + 24 * sections
+ *   16  int32    kind (DCG_KIND_*), storage (DCG_STORAGE_*), size[3], block_width, levels
+ *               (the dcg_info values of the grid)
+ *   44  uint32   flags (bit 0: the fluidity field was imported)
+ *   48  uint32   params bytes (= sizeof(dcg_params)), 52 uint32 section count
+ *   56  uint64   max_num_blocks, 64 uint64 num_cells (dcg_info)
+ *   72  dcg_params
+ *   then one 24-byte entry per section: uint32 id (DCG_STATE_*), uint32 element bytes,
+ *   uint64 offset from the start, uint64 bytes; section data starts at 256-byte aligned offsets.
+ * dcg_state_size gives the bytes of the grid's state; dcg_save_state writes it into a
+ * caller-owned host buffer of at least that size; dcg_load_state restores a state saved from a
+ * grid of the same kind, size, block width, levels, budget, storage and moist setting (else
+ * DCG_E_ARG, before anything changes; an unknown version is DCG_E_UNSUPPORTED), parameters
+ * included. Both synchronise. */
+#define DCG_STATE_VERSION 1
+enum {
+  DCG_STATE_CONTROL = 1,       /* DCGrid: device control words, uint32 (loads, move limits, ...) */
+  DCG_STATE_POSITIONS = 2,     /* DCGrid: block origin int32 x, y, z per block */
+  DCG_STATE_LEVELS = 3,        /* DCGrid: uint8 per block, 0xFF free */
+  DCG_STATE_FLAGS = 4,         /* DCGrid: uint32 per block */
+  DCG_STATE_PARENTS = 5,       /* DCGrid: uint32 per block, 8*parent + subblock, 0xFFFFFFFF none */
+  DCG_STATE_CHILDREN = 6,      /* DCGrid: uint32 per subblock (8 per block), 0xFFFFFFFF none */
+  DCG_STATE_NEIGHBOURS = 7,    /* DCGrid: 27 int32x4 neighbour descriptors per block */
+  DCG_STATE_HASH_KEYS = 8,     /* DCGrid: uint32 per hash slot */
+  DCG_STATE_HASH_VALUES = 9,   /* DCGrid: uint32 per hash slot */
+  DCG_STATE_FIELD = 16         /* + DCG_FIELD_*: the field in its storage format */
+};
+int dcg_state_size(dcg_sim* s, uint64_t* bytes);
+int dcg_save_state(dcg_sim* s, void* dst, uint64_t bytes);
+int dcg_load_state(dcg_sim* s, const void* src, uint64_t bytes);
+
 /* Per-kernel timing with HIP events recorded on the handle's stream around every launch of the
  * kernel classes selected by `mask` (bit k = kernel id k below; 0 disables). Setting the mask
  * clears the accumulated totals; dcg_timing_get synchronises and returns the total duration and
