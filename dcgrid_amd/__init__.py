@@ -10,6 +10,7 @@ import os
 from ._abi import (  # noqa: F401
     DCG_OK, DCGError, FIELDS, Info, Params, PROJECT_LOCAL, PROJECT_MULTILEVEL, SCORE_FEATURES,
     SCORE_GEOMETRIC, SOURCE_NONE, SOURCE_SPHERE_BUOYANT, K_NAMES, Handle, bind, hash_map,
+    RENDER_FLUID, RENDER_DENSITY, RENDER_VELOCITY, RENDER_FLUIDITY, RENDER_RESOLUTION, RenderParams,
 )
 from . import scenes  # noqa: F401
 
@@ -60,6 +61,12 @@ def library():
         lib.dcg_tape_forward.restype = C.c_int
         lib.dcg_tape_backward.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         lib.dcg_tape_backward.restype = C.c_int
+        lib.dcg_default_render_params.argtypes = [C.c_void_p, C.POINTER(RenderParams)]
+        lib.dcg_default_render_params.restype = None
+        for name in ("dcg_render", "dcg_render_host"):
+            f = getattr(lib, name)
+            f.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]
+            f.restype = C.c_int
         lib.dcg_state_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         lib.dcg_state_size.restype = C.c_int
         for name in ("dcg_save_state", "dcg_load_state"):
@@ -115,6 +122,25 @@ class Simulation(Handle):
             self._check(_lib.dcg_timing_get(self._h, k, C.byref(ms), C.byref(n)), "timing_get")
             out[name] = (ms.value, n.value)
         return out
+
+    # -- rendering (FluidSimulation::render)
+    def default_render_params(self):
+        """The reference's render defaults and a camera in front of this grid's domain."""
+        r = RenderParams()
+        _lib.dcg_default_render_params(self._h, C.byref(r))
+        return r
+
+    def render(self, r=None):
+        """Ray-marched image as a (height, width, 4) float32 RGBA array."""
+        import numpy as np
+        r = r or self.default_render_params()
+        out = np.empty((r.height, r.width, 4), dtype=np.float32)
+        self._check(_lib.dcg_render_host(self._h, C.byref(r), C.c_void_p(out.ctypes.data)), "render")
+        return out
+
+    def render_device(self, r, ptr):
+        """Render into caller-owned device memory at `ptr` (4 * width * height floats)."""
+        self._check(_lib.dcg_render(self._h, C.byref(r), C.c_void_p(ptr)), "render")
 
     # -- versioned state (the DCGSTATE format, dcgrid_amd.state)
     def state_size(self):

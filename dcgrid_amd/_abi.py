@@ -193,6 +193,28 @@ class Tables(C.Structure):
     ]
 
 
+RENDER_FLUID, RENDER_DENSITY, RENDER_VELOCITY, RENDER_FLUIDITY, RENDER_RESOLUTION = range(5)
+
+
+class RenderParams(C.Structure):
+    """dcg_render_params: the render() arguments and SimParams render fields
+    (src/fluid_simulation.h:19, src/data/sim_params.h:32-44)."""
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("eye_origin", C.c_float * 3), ("eye_dir", C.c_float * 3), ("fov", C.c_float),
+        ("sun_dir", C.c_float * 3),
+        ("channel", C.c_int32), ("solids", C.c_int32), ("shadows", C.c_int32), ("precise", C.c_int32),
+        ("aa_samples", C.c_float), ("ambient", C.c_float),
+        ("background_color", C.c_float * 3), ("floor_color", C.c_float * 3), ("smoke_color", C.c_float * 3),
+        ("scene_color", C.c_float * 3),
+    ]
+
+    def copy(self):
+        r = RenderParams()
+        C.pointer(r)[0] = self
+        return r
+
+
 class DCGError(RuntimeError):
     def __init__(self, code, where, msg):
         super().__init__(f"{where} failed: {ERROR_NAMES.get(code, code)}: {msg}")

@@ -1,5 +1,6 @@
 // capi.hip — extern "C" entry points of libdcgrid_amd (declared in include/dcgrid_amd.h) and
 // the shared host machinery of SimBase (stream, fills, timing).
+#include <cmath>
 #include <cstring>
 
 #include "dcg_internal.hpp"
@@ -335,6 +336,86 @@ int dcg_export_tables(dcg_sim* s, dcg_tables* t) {
   if (!t) return DCG_E_ARG;
   return I->exportTables(t);
 }
+// ------------------------------------------------------------------ rendering
+void dcg_default_render_params(const dcg_sim* s, dcg_render_params* r) {
+  if (!r) return;
+  std::memset(r, 0, sizeof(*r));
+  // SimParams::defaultParams render fields, src/data/sim_params.cpp:19-32 (render_channel as
+  // set for the fluid view, src/simulation.cpp:67-68)
+  r->solids = 1;
+  r->shadows = 1;
+  r->precise = 1;
+  r->channel = DCG_RENDER_FLUID;
+  r->aa_samples = 1.f;
+  r->ambient = .3f;
+  const float floor_c[3] = {178.f / 255.f, 158.f / 255.f, 135.f / 255.f};
+  const float scene_c[3] = {107.f / 255.f, 163.f / 255.f, 204.f / 255.f};
+  for (int a = 0; a < 3; a++) {
+    r->floor_color[a] = floor_c[a];
+    r->smoke_color[a] = .9f;
+    r->scene_color[a] = scene_c[a];
+  }
+  // a camera (the reference takes it from its engine): 256 x 256, 45 degree field of view, from
+  // in front of the domain's z = 0 face looking along +z slightly downwards, sun high
+  r->width = r->height = 256;
+  r->fov = .70710678f;
+  const dcg_params* P = s && s->impl ? &s->impl->P : nullptr;
+  const float gx = P ? (float)P->gx : 128.f, gy = P ? (float)P->gy : 128.f, gz = P ? (float)P->gz : 128.f;
+  const float dx = P ? P->dx : 1.f;
+  r->eye_origin[0] = .5f * gx * dx;
+  r->eye_origin[1] = .6f * gy * dx;
+  r->eye_origin[2] = -.9f * gz * dx;
+  r->eye_dir[1] = -.2f;
+  r->eye_dir[2] = .9797959f;
+  r->sun_dir[0] = .3f;
+  r->sun_dir[1] = .9055385f;
+  r->sun_dir[2] = -.3f;
+}
+
+// Inputs for which every loop of the ray marcher ends: bounded samples per pixel, finite
+// vectors, and eye positions within 1e6 cells (a march distance then stays below 2^22 cells, so
+// its 0.5-cell steps always advance the float distance).
+static int checkRender(SimBase* I, const dcg_render_params* r) {
+  if (r->width <= 0 || r->height <= 0 || r->width > 16384 || r->height > 16384)
+    return I->fail(DCG_E_ARG, "render resolution must be 1..16384");
+  if (!(r->aa_samples >= 1.f && r->aa_samples <= 8.f)) return I->fail(DCG_E_ARG, "aa_samples must be in [1, 8]");
+  if (r->channel < DCG_RENDER_FLUID || r->channel > DCG_RENDER_RESOLUTION) return I->fail(DCG_E_ARG, "unknown channel");
+  const float* v[] = {r->eye_origin, r->eye_dir, r->sun_dir, r->background_color, r->floor_color, r->smoke_color,
+                      r->scene_color};
+  for (const float* a : v)
+    for (int k = 0; k < 3; k++)
+      if (!std::isfinite(a[k])) return I->fail(DCG_E_ARG, "render vectors must be finite");
+  if (!std::isfinite(r->fov) || !std::isfinite(r->ambient)) return I->fail(DCG_E_ARG, "fov and ambient must be finite");
+  for (int k = 0; k < 3; k++)
+    if (!(std::fabs(r->eye_origin[k] * I->P.rdx) <= 1e6f)) return I->fail(DCG_E_ARG, "eye farther than 1e6 cells");
+  return DCG_OK;
+}
+
+int dcg_render(dcg_sim* s, const dcg_render_params* r, float* rgba_device) {
+  IMPL(s);
+  if (!r || !rgba_device) return DCG_E_ARG;
+  DCG_TRY(checkRender(I, r));
+  return I->render(*r, rgba_device);
+}
+
+int dcg_render_host(dcg_sim* s, const dcg_render_params* r, float* rgba_host) {
+  IMPL(s);
+  if (!r || !rgba_host) return DCG_E_ARG;
+  DCG_TRY(checkRender(I, r));
+  const size_t bytes = (size_t)r->width * r->height * 4 * sizeof(float);
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, bytes);
+  if (e != hipSuccess) return I->fail(DCG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  int st = I->render(*r, (float*)d);
+  if (!st) {
+    e = hipMemcpyAsync(rgba_host, d, bytes, hipMemcpyDeviceToHost, I->stream);
+    if (e != hipSuccess) st = I->fail(DCG_E_HIP, std::string("render copy: ") + hipGetErrorString(e));
+  }
+  const int sst = I->sync();
+  (void)hipFree(d);
+  return st ? st : sst;
+}
+
 // ------------------------------------------------------------------ state format
 namespace {
 struct StateHeader {

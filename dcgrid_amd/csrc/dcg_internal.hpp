@@ -90,6 +90,9 @@ struct SimBase {
     return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
   }
 
+  // Ray-marched image of the fields into caller-owned device memory (float4 per pixel).
+  virtual int render(const dcg_render_params& r, float* rgbaDevice) = 0;
+
   // Persistent state (dcg_save_state / dcg_load_state): the device arrays every later step
   // depends on, one section each, plus host flags.
   struct StateSection {

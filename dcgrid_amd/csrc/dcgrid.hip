@@ -11,8 +11,46 @@
 #include "dcgrid_fluid.hpp"
 #include "dcgrid_solve.hpp"
 #include "dcgrid_topo.hpp"
+#include "render.hpp"
 
 namespace dcg {
+
+// Ray-march samples of the DCGrid (src/dcgrid/dcgrid_rendering.cu:6-64): the finest block
+// containing the sample's cell; coarse = that block's cell at (cell mod BW) in finest
+// coordinates (the reference's indexing, also for coarser blocks); precise = trilinear over the
+// block's apron around the position in the block's level.
+template <int BW, class G>
+struct DCMarchSampler {
+  G g;
+  __device__ void sample(const dcg_params& P, int precise, float3 pos, MarchAux& a) const {
+    constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+    const int ix = clampi((int)floorf(pos.x), 0, g.gx - 1);
+    const int iy = clampi((int)floorf(pos.y), 0, g.gy - 1);
+    const int iz = clampi((int)floorf(pos.z), 0, g.gz - 1);
+    a.ipos = make_int3(ix, iy, iz);
+    const uint32_t b = findBlock<BW>(g, ix, iy, iz, 0);
+    if (!precise) {
+      const uint32_t c = b * BV + loc<BB>(ix % BW, iy % BW, iz % BW);
+#pragma unroll
+      for (int k = 0; k < 8; k++) a.cell[k] = c;
+      a.density = (float)g.den[c];
+      return;
+    }
+    const float scaleInv = 1.f / (float)(1 << g.lev[b]);
+    const float x = pos.x * scaleInv - .5f, y = pos.y * scaleInv - .5f, z = pos.z * scaleInv - .5f;
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    a.frac = make_float3(x - xf, y - yf, z - zf);
+    const int3 bp = g.pos[b];
+    const int ai = clampi((int)(xf + 1.f - (float)bp.x), 0, BW);
+    const int aj = clampi((int)(yf + 1.f - (float)bp.y), 0, BW);
+    const int ak = clampi((int)(zf + 1.f - (float)bp.z), 0, BW);
+#pragma unroll
+    for (int k = 0; k < 8; k++) a.cell[k] = apronCell<BW>(g, b, ai + ((k >> 2) & 1), aj + ((k >> 1) & 1), ak + (k & 1));
+    a.density = marchInterp(g.den, a);
+  }
+  __device__ float3 velocity(const MarchAux& a) const { return marchInterp3(g.vel, a); }
+  __device__ float fluidity(const MarchAux& a) const { return marchInterp(g.flu, a); }
+};
 
 static inline int iDivUpH(int a, int b) { return (a % b != 0) ? (a / b + 1) : (a / b); }
 static inline unsigned blocksFor(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
@@ -829,6 +867,13 @@ struct DCGridSim : SimBase {
       default: return nullptr;
     }
   }
+  int render(const dcg_render_params& r, float* rgba) override {
+    const dim3 grid((unsigned)((r.width + 15) / 16), (unsigned)((r.height + 15) / 16));
+    DCMarchSampler<BW, DGT<SF>> smp{g};
+    DCG_LAUNCH(DCG_K_OTHER, "k_render", (k_render<<<grid, 256, 0, stream>>>(smp, P, r, (float4*)rgba)));
+    return DCG_OK;
+  }
+
   // State: the block tables verbatim (neighbour descriptors and hash included: the descriptors'
   // out-of-domain clamp form depends on how each block was placed), the control words and the
   // exported fields in their storage format.

@@ -10,6 +10,7 @@
 
 #include "dcg_device.hpp"
 #include "dcg_internal.hpp"
+#include "render.hpp"
 
 namespace dcg {
 
@@ -27,6 +28,16 @@ struct UGrid {
   float3* tvel;
 };
 
+// Ray-march samples of the uniform grid (src/uniformgrid/uniformgrid_rendering.cu:5-47):
+// coarse = the cell of the truncated position, precise = trilinear with the reference's
+// edge handling (the lower corner clamped, the upper one repeated at the far faces).
+struct UMarchSampler {
+  UGrid g;
+  __device__ void sample(const dcg_params& P, int precise, float3 pos, MarchAux& a) const;
+  __device__ float3 velocity(const MarchAux& a) const { return marchInterp3(g.vel, a); }
+  __device__ float fluidity(const MarchAux& a) const { return marchInterp(g.flu, a); }
+};
+
 struct UDims {
   int w, h, d;
   size_t off;
@@ -35,6 +46,36 @@ struct UDims {
 __device__ __forceinline__ size_t uidx(const UGrid& g, int x, int y, int z) {
   return ((size_t)z * g.gy + y) * g.gx + x;
 }
+__device__ void UMarchSampler::sample(const dcg_params& P, int precise, float3 pos, MarchAux& a) const {
+  if (!precise) {
+    const int ix = clampi((int)pos.x, 0, g.gx - 1), iy = clampi((int)pos.y, 0, g.gy - 1);
+    const int iz = clampi((int)pos.z, 0, g.gz - 1);
+    a.ipos = make_int3(ix, iy, iz);
+    const uint32_t c = (uint32_t)uidx(g, ix, iy, iz);
+#pragma unroll
+    for (int k = 0; k < 8; k++) a.cell[k] = c;
+    a.density = g.den[c];
+    return;
+  }
+  const float x = pos.x - .5f, y = pos.y - .5f, z = pos.z - .5f;
+  const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+  a.ipos = make_int3((int)xf, (int)yf, (int)zf);
+  a.frac = make_float3(x - xf, y - yf, z - zf);
+  const int x0 = clampi(a.ipos.x, 0, g.gx - 1), y0 = clampi(a.ipos.y, 0, g.gy - 1), z0 = clampi(a.ipos.z, 0, g.gz - 1);
+  const uint32_t xs = a.ipos.x < g.gx - 1 ? 1u : 0u;
+  const uint32_t ys = a.ipos.y < g.gy - 1 ? (uint32_t)g.gx : 0u;
+  const uint32_t zs = a.ipos.z < g.gz - 1 ? (uint32_t)g.gx * (uint32_t)g.gy : 0u;
+  a.cell[0] = (uint32_t)uidx(g, x0, y0, z0);
+  a.cell[1] = a.cell[0] + zs;
+  a.cell[2] = a.cell[0] + ys;
+  a.cell[3] = a.cell[1] + ys;
+  a.cell[4] = a.cell[0] + xs;
+  a.cell[5] = a.cell[4] + zs;
+  a.cell[6] = a.cell[4] + ys;
+  a.cell[7] = a.cell[6] + zs;
+  a.density = marchInterp(g.den, a);
+}
+
 // Coordinates of linear index i of a grid w x h x ... (x fastest) in 32-bit arithmetic: grids
 // have fewer than 2^32 cells (checked at creation), and a 64-bit division is a long software
 // sequence on the GPU.
@@ -663,6 +704,13 @@ struct UniformSim : SimBase {
       default: return fail(DCG_E_ARG, "field not available on the uniform grid");
     }
   }
+  int render(const dcg_render_params& r, float* rgba) override {
+    const dim3 grid((unsigned)((r.width + 15) / 16), (unsigned)((r.height + 15) / 16));
+    UMarchSampler smp{g};
+    DCG_LAUNCH(DCG_K_OTHER, "k_render", (k_render<<<grid, 256, 0, stream>>>(smp, P, r, (float4*)rgba)));
+    return DCG_OK;
+  }
+
   // State: the exported fields (pressure, fluidity and divergence with their mip pyramids).
   void stateSections(std::vector<StateSection>& out) override {
     out.clear();

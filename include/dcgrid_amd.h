@@ -91,7 +91,7 @@ enum { DCG_STORAGE_FP32 = 0, DCG_STORAGE_FP16 = 1, DCG_STORAGE_BF16 = 2 };
 
 /*
  * Simulation parameters. The first block equals the simulation fields of the reference's
- * SimParams (src/data/sim_params.h:14-30; render fields are out of scope). Everything
+ * SimParams (src/data/sim_params.h:14-30; its render fields are in dcg_render_params). Everything
  * after `enable_additional_solids` is a knob the reference hard-codes, or an extension.
  */
 typedef struct dcg_params {
@@ -306,6 +306,41 @@ int dcg_tape_backward(dcg_sim* s, const float* g_density, const float* g_velocit
 
 /* Block tables in reference semantics (DCGrid only). */
 int dcg_export_tables(dcg_sim* s, dcg_tables* t);
+
+/* Rendering (FluidSimulation::render, src/fluid_simulation.h:19; k_dcgrid_render
+ * src/dcgrid/dcgrid_rendering.cu:66-92, k_uniform_render src/uniformgrid/uniformgrid_rendering.cu,
+ * ray marcher src/raymarching.cuh:63-275). The channels are RenderChannel
+ * (src/data/sim_params.h:4-12); the render fields of SimParams (sim_params.h:32-44) and the
+ * arguments of render() form dcg_render_params. The image is written as float4 RGBA per pixel,
+ * pixel (x, y) at float offset 4 * (y * width + x) (the reference's surf2Dwrite at column x,
+ * row y). dcg_render writes caller-owned device memory on the handle's stream;
+ * dcg_render_host copies the image into caller-owned host memory and synchronises. */
+enum {
+  DCG_RENDER_FLUID = 0,      /* ray-marched smoke with shadows, floor and solids */
+  DCG_RENDER_DENSITY = 1,    /* debug slice z = gz/2: density */
+  DCG_RENDER_VELOCITY = 2,   /* debug slice: |velocity| */
+  DCG_RENDER_FLUIDITY = 3,   /* debug slice: fluidity */
+  DCG_RENDER_RESOLUTION = 4  /* debug slice: checkerboard of 4-cell squares */
+};
+typedef struct dcg_render_params {
+  int32_t width, height;     /* resolution (render()'s int2) */
+  float eye_origin[3];       /* world units */
+  float eye_dir[3];          /* the view is rotated about x by (eye_dir.y, eye_dir.z) */
+  float fov;
+  float sun_dir[3];          /* unit vector towards the sun */
+  int32_t channel;           /* DCG_RENDER_* */
+  int32_t solids;            /* render_solids: floor and the solid scene */
+  int32_t shadows;           /* render_shadows */
+  int32_t precise;           /* render_precise: trilinear over the finest block's apron */
+  float aa_samples;          /* samples per pixel side */
+  float ambient;
+  float background_color[3], floor_color[3], smoke_color[3], scene_color[3];
+} dcg_render_params;
+/* SimParams::defaultParams render fields (src/data/sim_params.cpp:19-32), a 256 x 256 view
+ * from the middle of the domain's -z side, sun overhead-slanted. */
+void dcg_default_render_params(const dcg_sim* s, dcg_render_params* r);
+int dcg_render(dcg_sim* s, const dcg_render_params* r, float* rgba_device);
+int dcg_render_host(dcg_sim* s, const dcg_render_params* r, float* rgba_host);
 
 /* Versioned state of a simulation (extension: the reference keeps its state in device memory
  * only, src/dcgrid/dcgrid.h:5-43 and src/uniformgrid/uniformgrid.h:31-46). A state holds

@@ -11,7 +11,7 @@ and pinned by tests/test_oracle_*.py.
 import ctypes as C
 import os
 
-from dcgrid_amd._abi import Handle, Info, Params, bind
+from dcgrid_amd._abi import Handle, Info, Params, RenderParams, bind
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
@@ -40,6 +40,8 @@ def library():
         lib.orc_tape_backward.restype = C.c_int
         lib.orc_lookup.argtypes = [C.c_void_p] + [C.c_int32] * 4
         lib.orc_lookup.restype = C.c_int64
+        lib.orc_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]
+        lib.orc_render.restype = C.c_int
         lib.orc_plan.argtypes = [C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Params), C.POINTER(Info)]
         lib.orc_plan.restype = C.c_int
         for name, args, res in [
@@ -92,6 +94,13 @@ class OracleSim(Handle):
         self._extra = (C.c_int32(precision),)
         self.precision = precision
         super().__init__(kind, size, params, max_num_blocks)
+
+    def render(self, r):
+        """The reference's ray marcher on the CPU: (height, width, 4) float32 RGBA."""
+        import numpy as np
+        out = np.empty((r.height, r.width, 4), dtype=np.float32)
+        self._check(_lib.orc_render(self._h, C.byref(r), C.c_void_p(out.ctypes.data)), "render")
+        return out
 
     def lookup(self, x, y, z, level=0):
         """getBlockIndexDeep on the oracle's hash tables (-1 when nothing contains the cell)."""

@@ -38,6 +38,8 @@ int orc_project_solve(orc_sim* s);
 int orc_solve_report(orc_sim* s, dcg_solve_info* out);
 int orc_step(orc_sim* s, int32_t n);
 int orc_debug_stats(orc_sim* s, double* out);
+/* Ray-marched image (src/raymarching.cuh), RGBA float per pixel, row-major. */
+int orc_render(orc_sim* s, const dcg_render_params* r, float* out);
 int orc_seed_scene(orc_sim* s);
 int orc_info_get(orc_sim* s, dcg_info* out);
 int orc_field_size(orc_sim* s, int32_t field, uint64_t* nfloats);

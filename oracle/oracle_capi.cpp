@@ -111,6 +111,12 @@ int orc_step(orc_sim* s, int32_t n) {
   }
   return DCG_OK;
 }
+int orc_render(orc_sim* s, const dcg_render_params* r, float* out) {
+  IMPL(s);
+  if (!r || !out || r->width <= 0 || r->height <= 0 || !(r->aa_samples >= 1.f && r->aa_samples <= 8.f))
+    return DCG_E_ARG;
+  return I->render(*r, out);
+}
 int orc_debug_stats(orc_sim* s, double* out) {
   IMPL(s);
   if (!out) return DCG_E_ARG;

@@ -300,6 +300,8 @@ struct Sim {
   virtual int importField(int field, const double* src, uint64_t n) = 0;
   virtual int exportTables(dcg_tables* t) = 0;
   virtual int runStage(int stage, int arg) = 0;
+  // Ray-marched image (FluidSimulation::render), RGBA float per pixel, row-major.
+  virtual int render(const dcg_render_params& r, float* out) { return fail(DCG_E_UNSUPPORTED, "render"); }
   // getBlockIndexDeep (DCGrid only): finest allocated block at level >= `level` containing the
   // level-local cell (x,y,z); -1 when not applicable.
   virtual int64_t lookup(int x, int y, int z, int level) { return -1; }

@@ -24,6 +24,7 @@
 #include <numeric>
 
 #include "oracle_common.hpp"
+#include "oracle_render.hpp"
 
 namespace orc {
 
@@ -1110,6 +1111,49 @@ struct DCGrid : Sim {
   }
 
   // k_dcgrid_debug_stats + host sum, dcgrid_structure.cu:224-251, fluid_simulation_dcgrid.cu:517-528
+  // Samplers of src/dcgrid/dcgrid_rendering.cu:6-64 over the reference-semantics tables.
+  struct MarchSampler {
+    const DCGrid* d;
+    void sample(const dcg_params& P, int precise, F3 pos, Aux& a) const {
+      a.ipos[0] = clampi((int)floorf(pos.x), 0, d->gx - 1);
+      a.ipos[1] = clampi((int)floorf(pos.y), 0, d->gy - 1);
+      a.ipos[2] = clampi((int)floorf(pos.z), 0, d->gz - 1);
+      int x = a.ipos[0], y = a.ipos[1], z = a.ipos[2], level = 0;
+      const size_t b = d->getBlockIndexDeep(x, y, z, level);
+      if (!precise) {
+        const size_t c = (size_t)d->BV * b + localIndex(a.ipos[0] % d->BW, a.ipos[1] % d->BW, a.ipos[2] % d->BW, d->BB);
+        for (int k = 0; k < 8; k++) a.cell[k] = c;
+        a.density = (float)d->den[c];
+        return;
+      }
+      const float scaleInv = 1.f / (float)(1 << level);
+      const float fx = pos.x * scaleInv - .5f, fy = pos.y * scaleInv - .5f, fz = pos.z * scaleInv - .5f;
+      const float xf = floorf(fx), yf = floorf(fy), zf = floorf(fz);
+      a.frac[0] = fx - xf;
+      a.frac[1] = fy - yf;
+      a.frac[2] = fz - zf;
+      const int i = clampi((int)(xf + 1.f - (float)d->px(b)), 0, d->BW);
+      const int j = clampi((int)(yf + 1.f - (float)d->py(b)), 0, d->BW);
+      const int k = clampi((int)(zf + 1.f - (float)d->pz(b)), 0, d->BW);
+      const size_t* ap = &d->cidx[b * d->A3 + (size_t)d->A2 * i + (size_t)d->A * j + k];
+      a.cell[0] = ap[0];
+      a.cell[1] = ap[1];
+      a.cell[2] = ap[d->A];
+      a.cell[3] = ap[d->A + 1];
+      a.cell[4] = ap[d->A2];
+      a.cell[5] = ap[d->A2 + 1];
+      a.cell[6] = ap[d->A2 + d->A];
+      a.cell[7] = ap[d->A2 + d->A + 1];
+      a.density = interp(d->den, a);
+    }
+    F3 velocity(const Aux& a) const { return interp3(d->vel, a); }
+    float fluidity(const Aux& a) const { return interp(d->flu, a); }
+  };
+  int render(const dcg_render_params& r, float* out) override {
+    renderImage(MarchSampler{this}, P, r, out);
+    return DCG_OK;
+  }
+
   int debugStats(double* out) override {
     std::vector<R> stats(B, (R)0);
 #pragma omp parallel for schedule(static)

@@ -8,6 +8,7 @@
 #include <algorithm>
 
 #include "oracle_common.hpp"
+#include "oracle_render.hpp"
 
 namespace orc {
 
@@ -652,6 +653,51 @@ struct Uniform : Sim {
   }
 
   // debugStats(): k_uniform_debug_stats + host sum (uniform.cu:160-175; structure.cu:33-42)
+  // Samplers of src/uniformgrid/uniformgrid_rendering.cu:5-47.
+  struct MarchSampler {
+    const Uniform* u;
+    size_t idx(int x, int y, int z) const { return ((size_t)z * u->gy + y) * u->gx + x; }
+    void sample(const dcg_params& P, int precise, F3 pos, Aux& a) const {
+      if (!precise) {
+        a.ipos[0] = clampi((int)pos.x, 0, u->gx - 1);
+        a.ipos[1] = clampi((int)pos.y, 0, u->gy - 1);
+        a.ipos[2] = clampi((int)pos.z, 0, u->gz - 1);
+        const size_t c = idx(a.ipos[0], a.ipos[1], a.ipos[2]);
+        for (int k = 0; k < 8; k++) a.cell[k] = c;
+        a.density = (float)u->den[c];
+        return;
+      }
+      const float x = pos.x - .5f, y = pos.y - .5f, z = pos.z - .5f;
+      const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+      a.ipos[0] = (int)xf;
+      a.ipos[1] = (int)yf;
+      a.ipos[2] = (int)zf;
+      a.frac[0] = x - xf;
+      a.frac[1] = y - yf;
+      a.frac[2] = z - zf;
+      const int x0 = clampi(a.ipos[0], 0, u->gx - 1), y0 = clampi(a.ipos[1], 0, u->gy - 1);
+      const int z0 = clampi(a.ipos[2], 0, u->gz - 1);
+      const size_t xs = a.ipos[0] < u->gx - 1 ? 1 : 0;
+      const size_t ys = a.ipos[1] < u->gy - 1 ? (size_t)u->gx : 0;
+      const size_t zs = a.ipos[2] < u->gz - 1 ? (size_t)u->gx * u->gy : 0;
+      a.cell[0] = idx(x0, y0, z0);
+      a.cell[1] = a.cell[0] + zs;
+      a.cell[2] = a.cell[0] + ys;
+      a.cell[3] = a.cell[1] + ys;
+      a.cell[4] = a.cell[0] + xs;
+      a.cell[5] = a.cell[4] + zs;
+      a.cell[6] = a.cell[4] + ys;
+      a.cell[7] = a.cell[6] + zs;
+      a.density = interp(u->den, a);
+    }
+    F3 velocity(const Aux& a) const { return interp3(u->vel, a); }
+    float fluidity(const Aux& a) const { return interp(u->flu, a); }
+  };
+  int render(const dcg_render_params& r, float* out) override {
+    renderImage(MarchSampler{this}, P, r, out);
+    return DCG_OK;
+  }
+
   int debugStats(double* out) override {
     const size_t numBins = N / 256;
     std::vector<R> stats(numBins, (R)0);

@@ -46,8 +46,8 @@ def test_abi_version_and_defaults():
 
 def test_struct_layout_matches_header(tmp_path):
     """The ctypes mirror has the C compiler's layout of every public struct field."""
-    from dcgrid_amd._abi import Info, Params, Tables
-    structs = {"dcg_params": Params, "dcg_info": Info, "dcg_tables": Tables}
+    from dcgrid_amd._abi import Info, Params, RenderParams, Tables
+    structs = {"dcg_params": Params, "dcg_info": Info, "dcg_tables": Tables, "dcg_render_params": RenderParams}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dcgrid_amd.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
