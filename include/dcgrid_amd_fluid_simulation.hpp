@@ -5,7 +5,8 @@
 // advectVelocity/project/projectLocal/advectDensity/debugStats and protected size/numCells),
 // src/dcgrid/fluid_simulation_dcgrid.h:7 (FluidSimulationDCGrid(size, maxNumBlocks)),
 // src/uniformgrid/fluid_simulation_uniform.h:7 (FluidSimulationUniform(size)) and
-// src/utils/sim_utils.h:6 (copySimParamsToDevice). render() is out of scope.
+// src/utils/sim_utils.h:6 (copySimParamsToDevice) and render(), which writes float4 RGBA into a
+// device buffer instead of a cudaSurfaceObject_t.
 //
 // Differences a caller sees:
 //  * errors throw dcgrid_amd::Error instead of printing and calling exit()
@@ -71,6 +72,30 @@ class FluidSimulation {
     detail::check(dcg_debug_stats(sim_, &v), sim_, "debugStats");
     std::printf(kind_ == DCG_KIND_DCGRID ? "Residual: %g\n" : "Smoke: %g\n", v);
     return v;
+  }
+  // render(dest, resolution, eyeOrigin, eyeDir, fov, sunDir) (src/fluid_simulation.h:19): dest is
+  // device memory of 4 * resolution.x * resolution.y floats; I2/F3 are any types with int x, y /
+  // float x, y, z members (int2, float3). Render settings other than the arguments (channel,
+  // shadows, solids, colours) come from dcg_default_render_params unless `settings` is given.
+  template <class I2, class F3>
+  void render(float* dest, const I2& resolution, const F3& eyeOrigin, const F3& eyeDir, float fov, const F3& sunDir,
+              const dcg_render_params* settings = nullptr) {
+    dcg_render_params r;
+    if (settings)
+      r = *settings;
+    else
+      dcg_default_render_params(sim_, &r);
+    r.width = resolution.x;
+    r.height = resolution.y;
+    const float o[3] = {eyeOrigin.x, eyeOrigin.y, eyeOrigin.z}, d[3] = {eyeDir.x, eyeDir.y, eyeDir.z},
+                sd[3] = {sunDir.x, sunDir.y, sunDir.z};
+    for (int a = 0; a < 3; a++) {
+      r.eye_origin[a] = o[a];
+      r.eye_dir[a] = d[a];
+      r.sun_dir[a] = sd[a];
+    }
+    r.fov = fov;
+    detail::check(dcg_render(sim_, &r, dest), sim_, "render");
   }
   // Handle for everything else in the C ABI (export/import, tables, timing).
   dcg_sim* handle() const { return sim_; }

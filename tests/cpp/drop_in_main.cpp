@@ -14,6 +14,19 @@
 struct int3 {
   int x, y, z;
 };
+struct int2 {
+  int x, y;
+};
+struct float3 {
+  float x, y, z;
+};
+
+// The reference's render(dest, resolution, eyeOrigin, eyeDir, fov, sunDir) call shape
+// (src/simulation.cpp:130-136) compiles against the drop-in (dest: device RGBA buffer).
+void renderFrame(dcgrid_amd::FluidSimulation& sim, float* rgbaDevice) {
+  sim.render(rgbaDevice, int2{64, 48}, float3{5000.f, 6000.f, -9000.f}, float3{0.f, -.2f, .98f}, .7f,
+             float3{.3f, .9f, -.3f});
+}
 
 int main(int argc, char** argv) {
   if (argc != 9) {
