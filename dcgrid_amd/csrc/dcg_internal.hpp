@@ -61,6 +61,12 @@ struct SimBase {
   // Active cells of one step, accumulated by dcg_step after adaptTopology: on the device
   // (countStep, part of the captured step) or on the host (countHost, per executed step).
   virtual int countStep() = 0;
+  // adaptTopology() and the active-cell count of a dcg_step step (a back-end may count inside
+  // its adaptation launches).
+  virtual int adaptTopologyCounted() {
+    const int e = adaptTopology();
+    return e ? e : countStep();
+  }
   virtual void countHost(uint64_t steps) {}
   virtual int cellSteps(uint64_t* out) = 0;
   virtual int runStage(int stage, int arg) = 0;
@@ -125,8 +131,7 @@ struct SimBase {
   int stepOnce() {
     int e;
     if ((e = advectVelocity())) return e;
-    if ((e = adaptTopology())) return e;
-    if ((e = countStep())) return e;
+    if ((e = adaptTopologyCounted())) return e;
     if ((e = projectByMode())) return e;
     return advectDensity();
   }

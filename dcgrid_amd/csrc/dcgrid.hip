@@ -137,7 +137,8 @@ struct DCGridSim : SimBase {
   float* sbs = nullptr;   // 8B + 1 (last entry stays -FLT_MAX)
   float* bs = nullptr;    // B
   float* feat = nullptr;  // 8B
-  unsigned long long* cand = nullptr;
+  unsigned long long* cand = nullptr;   // selected move candidates of a level
+  unsigned long long* cand1 = nullptr;  // selected destination subblocks of a level
   uint32_t* mcBuf = nullptr;  // sorted move candidates, per-level region at offs[l]
   uint32_t* dcBuf = nullptr;  // sorted destination subblocks, region at offs[l] + l
   uint32_t* refList = nullptr;
@@ -247,6 +248,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&bs, B));
     DCG_TRY(alloc(&feat, (size_t)8 * B));
     DCG_TRY(alloc(&cand, (size_t)8 * B + 16));
+    DCG_TRY(alloc(&cand1, (size_t)8 * B + 16));
     DCG_TRY(alloc(&mcBuf, (size_t)B + 16));
     DCG_TRY(alloc(&dcBuf, (size_t)B + 16));
     DCG_TRY(alloc(&refList, (size_t)8 * B));
@@ -314,36 +316,37 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
 
-  // adaptTopology(), fluid_simulation_dcgrid.cu:320-346
-  int adaptTopology() override {
+  // adaptTopology(), fluid_simulation_dcgrid.cu:320-346. counter: the step's active cells are
+  // added to it (dcg_step), or null.
+  int adaptTopology() override { return adapt(nullptr); }
+  int adaptTopologyCounted() override { return adapt(cellStepCounter); }
+  int adapt(unsigned long long* counter) {
     const unsigned nWG = slotGrid<BW>(B);
-    DCG_TRY(fill32(g.flg, 0, B));
     if (P.score_mode == DCG_SCORE_FEATURES) {  // vorticity and subblock features in one pass
       if (fluidityOn())
         DCG_LAUNCH(DCG_K_TOPOLOGY, "k_features", (k_features<BW, true><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
       else
         DCG_LAUNCH(DCG_K_TOPOLOGY, "k_features", (k_features<BW, false><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_scores", (k_scores<BW, true><<<blocksFor(B, 256), 256, 0, stream>>>(g, feat, sbs, bs)));
+    } else {
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_scores", (k_scores<BW, false><<<blocksFor(B, 256), 256, 0, stream>>>(g, feat, sbs, bs)));
     }
-    // moveBlocks (:348-437)
-    DCG_TRY(subScores());
-    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_block_scores", (k_block_scores<<<blocksFor(B, 256), 256, 0, stream>>>(g, sbs, bs)));
+    // moveBlocks (:348-437): levels in order, each level's match marks blocks the next reads
     for (int l = 0; l < L - 1; l++) {
       uint32_t* mc = mcBuf + offs[l];
       uint32_t* dc = dcBuf + offs[l] + l;
       const unsigned rk = blocksFor((size_t)maxL[l] + 1, 256);
-      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_select<0>", (k_select<0><<<1, 1024, 0, stream>>>(g, l, bs, cand)));
-      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_rank<0>", (k_rank<0><<<rk, 256, 0, stream>>>(g, l, cand, mc)));
-      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_select<1>", (k_select<1><<<1, 1024, 0, stream>>>(g, l, sbs, cand)));
-      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_rank<1>", (k_rank<1><<<rk, 256, 0, stream>>>(g, l, cand, dc)));
-      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_match", (k_match<<<1, 1024, 0, stream>>>(g, l, mc, dc, bs, sbs)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_select_pair", (k_select_pair<<<2, 1024, 0, stream>>>(g, l, bs, sbs, cand, cand1)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_rank_pair", (k_rank_pair<<<dim3(rk, 2), 256, 0, stream>>>(g, l, cand, cand1, mc, dc)));
+      DCG_LAUNCH(DCG_K_TOPOLOGY, "k_match",
+                 (k_match<<<1, 1024, 0, stream>>>(g, l, l == L - 2 ? 1 : 0, mc, dc, bs, sbs)));
     }
+    // moves coarse to fine; the first launch also clears the hash for the refill (:326-332)
     for (int l = L - 2; l >= 0; l--)
       DCG_LAUNCH(DCG_K_TOPOLOGY, "k_move",
-                 (k_move<BW><<<blocksFor(maxL[l], 256), 256, 0, stream>>>(g, l, mcBuf + offs[l], dcBuf + offs[l] + l)));
-    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_total_moves", (k_total_moves<<<1, 64, 0, stream>>>(g)));
-    // hash rebuild after moves (:326-332)
-    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_hash_clear", (k_hash_clear<<<blocksFor(H, 256) > 2048 ? 2048 : blocksFor(H, 256), 256, 0, stream>>>(g, H)));
-    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_hash_refill", (k_hash_refill<BW><<<blocksFor(B, 256), 256, 0, stream>>>(g)));
+                 (k_move<BW><<<blocksFor(maxL[l], 256), 256, 0, stream>>>(g, l, mcBuf + offs[l], dcBuf + offs[l] + l,
+                                                                          l == L - 2 ? H : 0u)));
+    if (L > 1) DCG_LAUNCH(DCG_K_TOPOLOGY, "k_hash_refill", (k_hash_refill<BW><<<blocksFor(B, 256), 256, 0, stream>>>(g)));
     // refineSubblocks (:439-483)
     if (L > 1) {
       DCG_TRY(subScores());
@@ -354,7 +357,7 @@ struct DCGridSim : SimBase {
                  (k_refine_apply<BW><<<dim3(blocksFor(mxc, 256), L - 1), 256, 0, stream>>>(g, refList)));
     }
     // apron refresh and value propagation (:336-345)
-    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refresh", (k_refresh<BW><<<blocksFor((size_t)27 * B, 256), 256, 0, stream>>>(g)));
+    DCG_LAUNCH(DCG_K_TOPOLOGY, "k_refresh", (k_refresh<BW><<<blocksFor((size_t)27 * B, 256), 256, 0, stream>>>(g, counter)));
     const bool src = thermo();
     for (int l = L - 2; l >= 0; l--) {
       const unsigned n = std::min(blocksFor(maxL[l], G::BPW), 256u);  // grid-stride over the touched blocks
@@ -402,14 +405,14 @@ struct DCGridSim : SimBase {
     else ADV_D(false, false, false);
 #undef ADV_D
     std::swap(g.den, g.tden);
-    DCG_TRY(restrictAll(g.den));
-    if (src) {
-      std::swap(g.tem, g.ttem);
-      DCG_TRY(restrictAll(g.tem));
-    }
-    if (moist) {
-      std::swap(g.vap, g.tvap);
-      DCG_TRY(restrictAll(g.vap));
+    if (src) std::swap(g.tem, g.ttem);
+    if (moist) std::swap(g.vap, g.tvap);
+    // the advected scalars restricted together, one launch per level
+    for (int l = 0; l < L - 1; l++) {
+      const size_t groups = (size_t)maxL[l] * (G::BV / 8);
+      const unsigned n = blocksFor(groups, 256) < 2048 ? blocksFor(groups, 256) : 2048;
+      DCG_LAUNCH(DCG_K_RESTRICT, "k_restrict3",
+                 (k_restrict3<BW, SF><<<n, 256, 0, stream>>>(g, l, g.den, src ? g.tem : nullptr, moist ? g.vap : nullptr)));
     }
     return DCG_OK;
   }
@@ -508,10 +511,8 @@ struct DCGridSim : SimBase {
     put(&P, sizeof(P));
     put(&customFluidity, sizeof(customFluidity));
   }
-  int countStep() override {
-    DCG_LAUNCH(DCG_K_OTHER, "k_count_step", (k_count_step<BW><<<1, 64, 0, stream>>>(g, cellStepCounter)));
-    return DCG_OK;
-  }
+  // The step's active cells are counted by the refresh launch of adaptTopologyCounted().
+  int countStep() override { return DCG_OK; }
   int cellSteps(uint64_t* out) override {
     unsigned long long v = 0;
     DCG_HIP(hipMemcpyAsync(&v, cellStepCounter, sizeof(v), hipMemcpyDeviceToHost, stream));

@@ -117,8 +117,7 @@ enum : int {
   CTL_MOVES = CTL_LIM + 16,               // [16] matched moves per level
   CTL_REFINES = CTL_MOVES + 16,           // [16] refinements of subblocks of level l (children at l-1)
   CTL_LOADS_BEFORE = CTL_REFINES + 16,    // [16] loads before refinement
-  CTL_CAND = CTL_LOADS_BEFORE + 16,       // candidate count of the current selection
-  CTL_TOTAL_MOVES = CTL_CAND + 1,
+  CTL_TOTAL_MOVES = CTL_LOADS_BEFORE + 16,  // moves of this adaptation, all levels
   CTL_ERROR = CTL_TOTAL_MOVES + 1,        // sticky error bits
   CTL_WORDS = CTL_ERROR + 1
 };

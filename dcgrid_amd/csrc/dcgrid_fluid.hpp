@@ -365,6 +365,33 @@ __global__ void __launch_bounds__(256) k_restrict(DGBase g, int level, T* __rest
   }
 }
 
+// Restriction of one level of up to three fields at once (f1, f2 may be null): the same
+// groups as k_restrict, each field's sum in k_restrict's order.
+template <int BW, class T>
+__global__ void __launch_bounds__(256) k_restrict3(DGBase g, int level, T* __restrict__ f0, T* __restrict__ f1,
+                                                   T* __restrict__ f2) {
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, GPB = BV / 8;
+  const uint32_t groups = g.ctl[CTL_LOADS + level] * GPB;
+  for (uint32_t gi = blockIdx.x * 256 + threadIdx.x; gi < groups; gi += gridDim.x * 256) {
+    const uint32_t b = g.offs[level] + gi / GPB;
+    const uint32_t ps = g.par[b];
+    if (ps == NFU) continue;
+    const uint32_t grp = gi % GPB;
+    const uint32_t parentCell = (ps / 8) * BV + ((ps % 8) << SBB) + grp;
+    const size_t c0 = (size_t)b * BV + 8 * grp;
+    T* fs[3] = {f0, f1, f2};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+      if (!fs[f]) continue;
+      using W = typename Wide<T>::type;
+      W acc = zeroOf<W>();
+#pragma unroll
+      for (int i = 0; i < 8; i++) acc = addv(acc, (W)fs[f][c0 + i]);
+      fs[f][parentCell] = mulv(acc, .125f);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ parent stencil
 // Cells of the parent's apron around child cell (x,y,z) used by propagate and prolongate
 // (dcgrid_adaptation.cu:104-126, dcgrid_multigrid_solver.cu:51-63); order 000,001,010,100,011,

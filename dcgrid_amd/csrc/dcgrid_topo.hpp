@@ -73,17 +73,12 @@ __device__ __forceinline__ bool levelBlocked(const DGBase& g, int level) {
          (level > 0 && g.ctl[CTL_LOADS + level - 1] == g.fullL[level - 1]);
 }
 
-// k_dcgrid_calc_subblock_scores, geometric score (dcgrid_adaptation.cu:10-34).
+// k_dcgrid_calc_subblock_scores, geometric score (dcgrid_adaptation.cu:10-34), of subblock s.
 template <int BW>
-__global__ void __launch_bounds__(256) k_sub_scores_geo(DGBase g, float* __restrict__ sbs) {
-  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= 8 * g.B) return;
+__device__ __forceinline__ float subScoreGeo(const DGBase& g, uint32_t s) {
   const uint32_t b = s / 8;
   const int level = g.lev[b];
-  if (g.chi[s] != NFU || level == FREE || levelBlocked(g, level)) {
-    sbs[s] = -FLT_MAX;
-    return;
-  }
+  if (g.chi[s] != NFU || level == FREE || levelBlocked(g, level)) return -FLT_MAX;
   const float hb = (float)(BW / 2), qb = (float)(BW / 4);
   const float cx = .5f * (float)g.gx, cy = .45f * (float)g.gy, cz = .5f * (float)g.gz;
   const int3 p = g.pos[b];
@@ -93,7 +88,13 @@ __global__ void __launch_bounds__(256) k_sub_scores_geo(DGBase g, float* __restr
   const float qz = sc * (((float)p.z + hb * (float)(s & 1)) + qb);
   const float dx = qx - cx, dy = qy - cy, dz = qz - cz;
   const float d = __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
-  sbs[s] = d < .2f * (float)g.gx ? 0.f : (float)g.gx / d;
+  return d < .2f * (float)g.gx ? 0.f : (float)g.gx / d;
+}
+template <int BW>
+__global__ void __launch_bounds__(256) k_sub_scores_geo(DGBase g, float* __restrict__ sbs) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= 8 * g.B) return;
+  sbs[s] = subScoreGeo<BW>(g, s);
 }
 
 // Feature score (extension): per cell max(|grad density| / grad_threshold, |vorticity| /
@@ -169,25 +170,39 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_features(G g, dcg_params P, flo
 
 // Subblock scores from the feature values: -FLT_MAX where the reference's conditions exclude the
 // subblock (dcgrid_adaptation.cu:18-24).
+__device__ __forceinline__ float subScoreFeat(const DGBase& g, const float* __restrict__ feat, uint32_t s) {
+  const int level = g.lev[s / 8];
+  return g.chi[s] != NFU || level == FREE || levelBlocked(g, level) ? -FLT_MAX : feat[s];
+}
 __global__ void __launch_bounds__(256) k_sub_scores_feat(DGBase g, const float* __restrict__ feat, float* __restrict__ sbs) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
   if (s >= 8 * g.B) return;
-  const int level = g.lev[s / 8];
-  if (g.chi[s] != NFU || level == FREE || levelBlocked(g, level))
-    sbs[s] = -FLT_MAX;
-  else
-    sbs[s] = feat[s];
+  sbs[s] = subScoreFeat(g, feat, s);
 }
 
-// k_dcgrid_accumulate_subblock_scores (dcgrid_adaptation.cu:42-63), including the reference's
-// ninth term s[8] (subblock 0 of the next block).
-__global__ void __launch_bounds__(256) k_block_scores(DGBase g, const float* __restrict__ sbs, float* __restrict__ bs) {
+// First score pass of an adaptation: subblock scores (k_dcgrid_calc_subblock_scores,
+// dcgrid_adaptation.cu:10-34, or the feature values) and block scores
+// (k_dcgrid_accumulate_subblock_scores, :42-63) in one launch, one thread per block. The
+// reference's ninth term s[8] is subblock 0 of the next block (the -FLT_MAX pad after the last
+// block): the thread recomputes that value, which the next block's thread stores. The thread also
+// clears its block's flags for this adaptation, and thread 0 the move total.
+template <int BW, bool FEAT>
+__global__ void __launch_bounds__(256) k_scores(DGBase g, const float* __restrict__ feat, float* __restrict__ sbs,
+                                                float* __restrict__ bs) {
   const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+  if (b == 0) g.ctl[CTL_TOTAL_MOVES] = 0;
   if (b >= g.B) return;
+  g.flg[b] = 0;
+  float s[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const uint32_t sb = 8 * b + k;
+    s[k] = sb >= 8 * g.B ? -FLT_MAX : (FEAT ? subScoreFeat(g, feat, sb) : subScoreGeo<BW>(g, sb));
+    if (k < 8) sbs[sb] = s[k];
+  }
   float r = -FLT_MAX;
   const int level = g.lev[b];
   if (level != FREE && !levelBlocked(g, level)) {
-    const float* s = sbs + 8 * b;
     if (s[0] > 0.f && s[1] > 0.f && s[2] > 0.f && s[3] > 0.f && s[4] > 0.f && s[5] > 0.f && s[6] > 0.f && s[7] > 0.f)
       r = .125f * (s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8]);
   }
@@ -237,38 +252,45 @@ __device__ __forceinline__ uint32_t wgExclusiveScan(uint32_t v, uint32_t* sh /*[
   return r;
 }
 
+// Move limit of `level` for this adaptation (fluid_simulation_dcgrid.cu:377-385).
+__device__ __forceinline__ uint32_t moveLim(const DGBase& g, int level) {
+  const uint32_t d0 = g.maxL[level], d1 = 8 * g.maxL[level + 1];
+  const uint32_t load = g.ctl[CTL_LOADS + level];
+  uint32_t lim = min(min(d0, d1), min(load, g.fullL[level] - load));
+  const uint32_t ml = g.ctl[CTL_MOVE_LIMIT + level];
+  if (ml > 0) lim = min(lim, ml);
+  return lim;
+}
+// Candidates selected per mode: lim move candidates (MODE 0), min(lim + 1, n) destination
+// subblocks (MODE 1; none when lim is 0).
+template <int MODE>
+__device__ __forceinline__ uint32_t selectCount(const DGBase& g, int level, uint32_t lim) {
+  const uint32_t d1 = 8 * g.maxL[level + 1];
+  return MODE == 0 ? lim : (lim == 0 ? 0 : min(lim + 1, d1));
+}
+
 // Single-workgroup selection of the k smallest (key32, index) pairs among n candidates.
-// MODE 0: move candidates (blocks of `level`, k = lim, computed here).
+// MODE 0: move candidates (blocks of `level`, k = lim).
 // MODE 1: destination subblocks (subblocks of level+1, k = min(lim+1, n)).
 // Writes the k winners unsorted as packed keys (key32 << 32 | position) to cand.
+struct SelectShared {
+  uint32_t hist[2048];
+  uint32_t sh[17];
+  uint32_t sPrefix, sRem, sCount;
+};
 template <int MODE>
-__global__ void __launch_bounds__(1024) k_select(DGBase g, int level, const float* __restrict__ scores,
-                                                 unsigned long long* __restrict__ cand) {
-  __shared__ uint32_t hist[2048];
-  __shared__ uint32_t sh[17];
-  __shared__ uint32_t sK, sPrefix, sRem, sCount;
+__device__ void selectBody(const DGBase& g, int level, const float* __restrict__ scores,
+                           unsigned long long* __restrict__ cand, SelectShared& S) {
+  uint32_t* hist = S.hist;
+  uint32_t* sh = S.sh;
   const uint32_t d0 = g.maxL[level], d1 = 8 * g.maxL[level + 1];
   const uint32_t n = MODE == 0 ? d0 : d1;
   const uint32_t base = MODE == 0 ? g.offs[level] : 8 * g.offs[level + 1];
-  if (threadIdx.x == 0) {
-    uint32_t k;
-    if (MODE == 0) {
-      const uint32_t load = g.ctl[CTL_LOADS + level];
-      uint32_t lim = min(min(d0, d1), min(load, g.fullL[level] - load));
-      const uint32_t ml = g.ctl[CTL_MOVE_LIMIT + level];
-      if (ml > 0) lim = min(lim, ml);
-      g.ctl[CTL_LIM + level] = lim;
-      k = lim;
-    } else {
-      const uint32_t lim = g.ctl[CTL_LIM + level];
-      k = lim == 0 ? 0 : min(lim + 1, d1);
-    }
-    sK = k;
-    sCount = 0;
-    g.ctl[CTL_CAND] = k;
-  }
+  const uint32_t lim = moveLim(g, level);  // the same value in both workgroups of a pair
+  if (MODE == 0 && threadIdx.x == 0) g.ctl[CTL_LIM + level] = lim;
+  if (threadIdx.x == 0) S.sCount = 0;
   __syncthreads();
-  const uint32_t k = sK;
+  const uint32_t k = selectCount<MODE>(g, level, lim);
   if (k == 0) return;
   const uint32_t T = blockDim.x;
   auto key32 = [&](uint32_t i) -> uint32_t {
@@ -304,16 +326,16 @@ __global__ void __launch_bounds__(1024) k_select(DGBase g, int level, const floa
     const uint32_t a1 = 2 * threadIdx.x + 1 < nb ? hist[2 * threadIdx.x + 1] : 0;
     const uint32_t ex = wgExclusiveScan(a0 + a1, sh, nullptr);
     if (rem > ex && rem <= ex + a0) {
-      sPrefix = 2 * threadIdx.x;
-      sRem = rem - ex;
+      S.sPrefix = 2 * threadIdx.x;
+      S.sRem = rem - ex;
     } else if (rem > ex + a0 && rem <= ex + a0 + a1) {
-      sPrefix = 2 * threadIdx.x + 1;
-      sRem = rem - ex - a0;
+      S.sPrefix = 2 * threadIdx.x + 1;
+      S.sRem = rem - ex - a0;
     }
     __syncthreads();
-    prefix |= sPrefix << shift;
+    prefix |= S.sPrefix << shift;
     mask |= (nb - 1) << shift;
-    rem = sRem;
+    rem = S.sRem;
     __syncthreads();
   }
   // prefix = key32 of the k-th smallest; take all smaller keys and the first `rem` ties.
@@ -329,19 +351,35 @@ __global__ void __launch_bounds__(1024) k_select(DGBase g, int level, const floa
       seen++;
     }
     if (take) {
-      const uint32_t slot = atomicAdd(&sCount, 1u);
+      const uint32_t slot = atomicAdd(&S.sCount, 1u);
       cand[slot] = ((unsigned long long)kk << 32) | i;
     }
   }
 }
+// Both selections of one level in one launch: workgroup 0 the move candidates (block scores),
+// workgroup 1 the destination subblocks (subblock scores); they read only scores and loads.
+__global__ void __launch_bounds__(1024) k_select_pair(DGBase g, int level, const float* __restrict__ bs,
+                                                      const float* __restrict__ sbs, unsigned long long* __restrict__ cand0,
+                                                      unsigned long long* __restrict__ cand1) {
+  __shared__ SelectShared S;
+  if (blockIdx.x == 0)
+    selectBody<0>(g, level, bs, cand0, S);
+  else
+    selectBody<1>(g, level, sbs, cand1, S);
+}
 
 // Exact position of every selected candidate (keys are unique): out[rank] = base + position.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_rank(DGBase g, int level, const unsigned long long* __restrict__ cand,
-                                              uint32_t* __restrict__ out) {
+// blockIdx.y selects the list: 0 move candidates -> mc, 1 destination subblocks -> dc.
+__global__ void __launch_bounds__(256) k_rank_pair(DGBase g, int level, const unsigned long long* __restrict__ cand0,
+                                                   const unsigned long long* __restrict__ cand1,
+                                                   uint32_t* __restrict__ mc, uint32_t* __restrict__ dc) {
   __shared__ unsigned long long tile[1024];
-  const uint32_t k = g.ctl[CTL_CAND];
-  const uint32_t base = MODE == 0 ? g.offs[level] : 8 * g.offs[level + 1];
+  const int mode = blockIdx.y;
+  const uint32_t lim = g.ctl[CTL_LIM + level];
+  const uint32_t k = mode == 0 ? selectCount<0>(g, level, lim) : selectCount<1>(g, level, lim);
+  const unsigned long long* cand = mode == 0 ? cand0 : cand1;
+  uint32_t* out = mode == 0 ? mc : dc;
+  const uint32_t base = mode == 0 ? g.offs[level] : 8 * g.offs[level + 1];
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (blockIdx.x * 256 >= k) return;
   const unsigned long long mine = i < k ? cand[i] : ~0ull;
@@ -360,14 +398,22 @@ __global__ void __launch_bounds__(256) k_rank(DGBase g, int level, const unsigne
 // in m (block scores ascending, subblock scores descending), so the matched count is the first
 // failing position. Marks of blocks containing destination subblocks (dest[1..M], the
 // reference's "next" destination) are applied to the level above, which the next level's
-// selection reads.
-__global__ void __launch_bounds__(1024) k_match(DGBase g, int level, const uint32_t* __restrict__ mc,
+// selection reads. The last level's match also sums the moves of all levels.
+__device__ __forceinline__ void totalMoves(const DGBase& g, int level, uint32_t M) {
+  uint32_t s = M;
+  for (int l = 0; l < level; l++) s += g.ctl[CTL_MOVES + l];
+  g.ctl[CTL_TOTAL_MOVES] = s;
+}
+__global__ void __launch_bounds__(1024) k_match(DGBase g, int level, int last, const uint32_t* __restrict__ mc,
                                                 const uint32_t* __restrict__ dc, float* __restrict__ bs,
                                                 const float* __restrict__ sbs) {
   __shared__ uint32_t sFail;
   const uint32_t lim = g.ctl[CTL_LIM + level];
   if (lim == 0) {
-    if (threadIdx.x == 0) g.ctl[CTL_MOVES + level] = 0;
+    if (threadIdx.x == 0) {
+      g.ctl[CTL_MOVES + level] = 0;
+      if (last) totalMoves(g, level, 0);
+    }
     return;
   }
   if (threadIdx.x == 0) sFail = lim;
@@ -382,6 +428,7 @@ __global__ void __launch_bounds__(1024) k_match(DGBase g, int level, const uint3
   if (threadIdx.x == 0) {
     g.ctl[CTL_MOVES + level] = M;
     g.ctl[CTL_MOVE_LIMIT + level] = (uint32_t)((float)M * 1.2f);
+    if (last) totalMoves(g, level, M);
   }
   for (uint32_t j = 1 + threadIdx.x; j <= M; j += blockDim.x)
     if (j < d1) bs[dc[j] / 8] = -FLT_MAX;
@@ -390,9 +437,17 @@ __global__ void __launch_bounds__(1024) k_match(DGBase g, int level, const uint3
 // k_dcgrid_move_blocks (dcgrid_adaptation.cu:65-90) for the moves of one level; levels are
 // applied coarse to fine so a block moved under a parent that moved takes the parent's new
 // position.
+// The first move launch (hashSlots > 0) also clears the hash tables for the refill when any
+// block moves (the cudaMemset of fluid_simulation_dcgrid.cu:327-328): nothing reads the hash
+// between the selection and the refill.
 template <int BW>
 __global__ void __launch_bounds__(256) k_move(DGBase g, int level, const uint32_t* __restrict__ mc,
-                                              const uint32_t* __restrict__ dc) {
+                                              const uint32_t* __restrict__ dc, uint32_t hashSlots) {
+  if (hashSlots && g.ctl[CTL_TOTAL_MOVES] != 0)
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < hashSlots; i += gridDim.x * 256) {
+      g.hkey[i] = 0xFFFFFFFFu;
+      g.hval[i] = NFU;
+    }
   const uint32_t m = blockIdx.x * 256 + threadIdx.x;
   if (m >= g.ctl[CTL_MOVES + level]) return;
   const uint32_t b = mc[m], next = dc[m];
@@ -405,22 +460,6 @@ __global__ void __launch_bounds__(256) k_move(DGBase g, int level, const uint32_
                        2 * pp.z + BW * (int)(next % 2));
   atomicOr(&g.flg[b], (uint32_t)MOVED);
   atomicOr(&g.flg[pb], (uint32_t)REFINED);
-}
-
-// Adds the active cells of this step (all cells of allocated blocks) to a 64-bit counter.
-template <int BW>
-__global__ void __launch_bounds__(64) k_count_step(DGBase g, unsigned long long* __restrict__ counter) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  unsigned long long s = 0;
-  for (int l = 0; l < g.L; l++) s += (unsigned long long)g.ctl[CTL_LOADS + l] * Geo<BW>::BV;
-  *counter += s;
-}
-
-__global__ void __launch_bounds__(64) k_total_moves(DGBase g) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t s = 0;
-  for (int l = 0; l < g.L - 1; l++) s += g.ctl[CTL_MOVES + l];
-  g.ctl[CTL_TOTAL_MOVES] = s;
 }
 
 // ------------------------------------------------------------------ hash tables
@@ -441,15 +480,6 @@ __device__ __forceinline__ void hashInsert(const DGBase& g, int level, int3 p, u
     slot = (slot + 127) % size;
   } while (slot != slot0);
   atomicOr(&g.ctl[CTL_ERROR], ERR_HASH_FULL);
-}
-// cudaMemset of the tables before the refill (fluid_simulation_dcgrid.cu:327-328), only when
-// blocks were moved.
-__global__ void __launch_bounds__(256) k_hash_clear(DGBase g, uint32_t H) {
-  if (g.ctl[CTL_TOTAL_MOVES] == 0) return;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < H; i += gridDim.x * 256) {
-    g.hkey[i] = 0xFFFFFFFFu;
-    g.hval[i] = NFU;
-  }
 }
 // k_dcgrid_refill_hash_table (dcgrid_adaptation.cu:180-201)
 template <int BW>
@@ -531,9 +561,16 @@ __global__ void __launch_bounds__(256) k_refine_apply(DGBase g, const uint32_t* 
 // point to the block itself); other blocks follow a neighbour that moved, or step into the child
 // of a refined coarser neighbour.
 template <int BW>
-__global__ void __launch_bounds__(256) k_refresh(DGBase g) {
+// counter (may be null): the active cells of this step (all cells of allocated blocks, loads
+// final after the refinement) are added to it.
+__global__ void __launch_bounds__(256) k_refresh(DGBase g, unsigned long long* __restrict__ counter) {
   constexpr int BB = Geo<BW>::BB;
   const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+  if (counter && gid == 0) {
+    unsigned long long cells = 0;
+    for (int l = 0; l < g.L; l++) cells += (unsigned long long)g.ctl[CTL_LOADS + l] * Geo<BW>::BV;
+    *counter += cells;
+  }
   const uint32_t b = gid / 27, d = gid % 27;
   if (b >= g.B) return;
   const int level = g.lev[b];
