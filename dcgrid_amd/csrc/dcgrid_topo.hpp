@@ -278,6 +278,63 @@ struct SelectShared {
   uint32_t sh[17];
   uint32_t sPrefix, sRem, sCount;
 };
+// The radix select of selectBody with the thread's keys (a segment of at most SELECT_REG_KEYS
+// candidates) loaded once into registers: the same passes, ties and winners.
+constexpr uint32_t SELECT_REG_KEYS = 8;
+template <class KeyFn>
+__device__ __forceinline__ void selectInRegisters(uint32_t lo, uint32_t hi, uint32_t k, KeyFn key32,
+                                                  unsigned long long* __restrict__ cand, SelectShared& S) {
+  const uint32_t T = blockDim.x;
+  uint32_t kr[SELECT_REG_KEYS];
+#pragma unroll
+  for (uint32_t j = 0; j < SELECT_REG_KEYS; j++) kr[j] = lo + j < hi ? key32(lo + j) : 0u;
+  uint32_t prefix = 0, mask = 0, rem = k;
+  for (int pass = 0; pass < 3; pass++) {
+    const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
+    const uint32_t nb = pass == 2 ? 1024u : 2048u;
+    for (uint32_t j = threadIdx.x; j < 2048; j += T) S.hist[j] = 0;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < SELECT_REG_KEYS; j++)
+      if (lo + j < hi && (kr[j] & mask) == prefix) atomicAdd(&S.hist[(kr[j] >> shift) & (nb - 1)], 1u);
+    __syncthreads();
+    const uint32_t a0 = 2 * threadIdx.x < nb ? S.hist[2 * threadIdx.x] : 0;
+    const uint32_t a1 = 2 * threadIdx.x + 1 < nb ? S.hist[2 * threadIdx.x + 1] : 0;
+    const uint32_t ex = wgExclusiveScan(a0 + a1, S.sh, nullptr);
+    if (rem > ex && rem <= ex + a0) {
+      S.sPrefix = 2 * threadIdx.x;
+      S.sRem = rem - ex;
+    } else if (rem > ex + a0 && rem <= ex + a0 + a1) {
+      S.sPrefix = 2 * threadIdx.x + 1;
+      S.sRem = rem - ex - a0;
+    }
+    __syncthreads();
+    prefix |= S.sPrefix << shift;
+    mask |= (nb - 1) << shift;
+    rem = S.sRem;
+    __syncthreads();
+  }
+  uint32_t ties = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < SELECT_REG_KEYS; j++) ties += lo + j < hi && kr[j] == prefix;
+  const uint32_t tieBase = wgExclusiveScan(ties, S.sh, nullptr);
+  uint32_t seen = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < SELECT_REG_KEYS; j++) {
+    if (lo + j >= hi) continue;
+    const uint32_t kk = kr[j];
+    bool take = kk < prefix;
+    if (kk == prefix) {
+      take = tieBase + seen < rem;
+      seen++;
+    }
+    if (take) {
+      const uint32_t slot = atomicAdd(&S.sCount, 1u);
+      cand[slot] = ((unsigned long long)kk << 32) | (lo + j);
+    }
+  }
+}
+
 template <int MODE>
 __device__ void selectBody(const DGBase& g, int level, const float* __restrict__ scores,
                            unsigned long long* __restrict__ cand, SelectShared& S) {
@@ -310,6 +367,10 @@ __device__ void selectBody(const DGBase& g, int level, const float* __restrict__
   }
   const uint32_t seg = (nA + T - 1) / T;
   const uint32_t lo = min(threadIdx.x * seg, nA), hi = min(lo + seg, nA);
+  if (seg <= SELECT_REG_KEYS) {
+    selectInRegisters(lo, hi, k, key32, cand, S);
+    return;
+  }
   // Radix select on key32: 11 + 11 + 10 bits.
   uint32_t prefix = 0, mask = 0, rem = k;
   for (int pass = 0; pass < 3; pass++) {
