@@ -496,7 +496,7 @@ struct DCGridSim : SimBase {
     // more blocks per sub-group and iteration once the level spans several rounds of the grid
     // (the large levels, bound by reads in flight)
     const size_t rounds = (size_t)maxL[level] / ((size_t)n * G::BPW);
-    const int nb = rounds >= 16 ? 4 : rounds >= 2 ? 2 : 1;
+    const int nb = rounds >= 16 ? 4 : rounds >= 2 ? 2 : 1;  // four from 4 rounds measured slower
     for (int i = 0; i < pairs; i++)
       for (int h = 0; h < 2; h++)
         DCG_TRY(jacobiLaunch(level, h == 0, nb, n, alpha));

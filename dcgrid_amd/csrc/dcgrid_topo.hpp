@@ -278,16 +278,36 @@ struct SelectShared {
   uint32_t sh[17];
   uint32_t sPrefix, sRem, sCount;
 };
-// The radix select of selectBody with the thread's keys (a segment of at most SELECT_REG_KEYS
-// candidates) loaded once into registers: the same passes, ties and winners.
+// Adds 1 to hist[bin] for every active lane of the wavefront. Candidate keys are often equal
+// (the -FLT_MAX scores of refined or free subblocks, zero scores): the lanes sharing the first
+// two distinct bins of the wavefront add with one LDS atomic each, the others one atomic per
+// lane, so a dominant value does not serialise. Called by all lanes of the wavefront.
+__device__ __forceinline__ void histAddWave(uint32_t* hist, uint32_t bin, bool active) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(active);
+#pragma unroll
+  for (int round = 0; round < 2; round++) {
+    if (!pending) return;
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lb = __shfl(bin, leader, 64);
+    const unsigned long long same = __ballot(active && bin == lb) & pending;
+    if (lane == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(same));
+    pending &= ~same;
+  }
+  if ((pending >> lane) & 1ull) atomicAdd(&hist[bin], 1u);
+}
+
+// The radix select of selectBody with the thread's keys (a segment of at most KEYS candidates)
+// loaded once into registers: the same passes, ties and winners. Short segments have few equal
+// keys per wavefront, so the histogram takes plain LDS atomics here.
 constexpr uint32_t SELECT_REG_KEYS = 8;
-template <class KeyFn>
+template <uint32_t KEYS, class KeyFn>
 __device__ __forceinline__ void selectInRegisters(uint32_t lo, uint32_t hi, uint32_t k, KeyFn key32,
                                                   unsigned long long* __restrict__ cand, SelectShared& S) {
   const uint32_t T = blockDim.x;
-  uint32_t kr[SELECT_REG_KEYS];
+  uint32_t kr[KEYS];
 #pragma unroll
-  for (uint32_t j = 0; j < SELECT_REG_KEYS; j++) kr[j] = lo + j < hi ? key32(lo + j) : 0u;
+  for (uint32_t j = 0; j < KEYS; j++) kr[j] = lo + j < hi ? key32(lo + j) : 0u;
   uint32_t prefix = 0, mask = 0, rem = k;
   for (int pass = 0; pass < 3; pass++) {
     const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
@@ -295,7 +315,7 @@ __device__ __forceinline__ void selectInRegisters(uint32_t lo, uint32_t hi, uint
     for (uint32_t j = threadIdx.x; j < 2048; j += T) S.hist[j] = 0;
     __syncthreads();
 #pragma unroll
-    for (uint32_t j = 0; j < SELECT_REG_KEYS; j++)
+    for (uint32_t j = 0; j < KEYS; j++)
       if (lo + j < hi && (kr[j] & mask) == prefix) atomicAdd(&S.hist[(kr[j] >> shift) & (nb - 1)], 1u);
     __syncthreads();
     const uint32_t a0 = 2 * threadIdx.x < nb ? S.hist[2 * threadIdx.x] : 0;
@@ -316,11 +336,11 @@ __device__ __forceinline__ void selectInRegisters(uint32_t lo, uint32_t hi, uint
   }
   uint32_t ties = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < SELECT_REG_KEYS; j++) ties += lo + j < hi && kr[j] == prefix;
+  for (uint32_t j = 0; j < KEYS; j++) ties += lo + j < hi && kr[j] == prefix;
   const uint32_t tieBase = wgExclusiveScan(ties, S.sh, nullptr);
   uint32_t seen = 0;
 #pragma unroll
-  for (uint32_t j = 0; j < SELECT_REG_KEYS; j++) {
+  for (uint32_t j = 0; j < KEYS; j++) {
     if (lo + j >= hi) continue;
     const uint32_t kk = kr[j];
     bool take = kk < prefix;
@@ -368,7 +388,7 @@ __device__ void selectBody(const DGBase& g, int level, const float* __restrict__
   const uint32_t seg = (nA + T - 1) / T;
   const uint32_t lo = min(threadIdx.x * seg, nA), hi = min(lo + seg, nA);
   if (seg <= SELECT_REG_KEYS) {
-    selectInRegisters(lo, hi, k, key32, cand, S);
+    selectInRegisters<SELECT_REG_KEYS>(lo, hi, k, key32, cand, S);
     return;
   }
   // Radix select on key32: 11 + 11 + 10 bits.
@@ -378,9 +398,11 @@ __device__ void selectBody(const DGBase& g, int level, const float* __restrict__
     const uint32_t nb = pass == 2 ? 1024u : 2048u;
     for (uint32_t j = threadIdx.x; j < 2048; j += T) hist[j] = 0;
     __syncthreads();
-    for (uint32_t i = lo; i < hi; i++) {
-      const uint32_t kk = key32(i);
-      if ((kk & mask) == prefix) atomicAdd(&hist[(kk >> shift) & (nb - 1)], 1u);
+#pragma unroll 1
+    for (uint32_t j = 0; j < seg; j++) {  // the same trip count in every lane
+      const uint32_t i = lo + j;
+      const uint32_t kk = i < hi ? key32(i) : 0u;
+      histAddWave(hist, (kk >> shift) & (nb - 1), i < hi && (kk & mask) == prefix);
     }
     __syncthreads();
     const uint32_t a0 = 2 * threadIdx.x < nb ? hist[2 * threadIdx.x] : 0;
@@ -568,9 +590,21 @@ __global__ void __launch_bounds__(1024) k_refine_select(DGBase g, dcg_params P, 
   const uint32_t T = blockDim.x;
   const uint32_t seg = (n + T - 1) / T;
   const uint32_t lo = min(threadIdx.x * seg, n), hi = min(lo + seg, n);
-  uint32_t cnt = 0;
-  if (lim > 0)
-    for (uint32_t i = lo; i < hi; i++) cnt += sbs[base + i] > P.refine_threshold;
+  // candidates of a segment of 9..32 as a bit mask, read once (unrolled); shorter segments
+  // take the plain loop (two short passes)
+  constexpr uint32_t REG = 32;
+  const bool inRegs = seg > 8 && seg <= REG;
+  uint32_t bits = 0, cnt = 0;
+  if (lim > 0) {
+    if (inRegs) {
+#pragma unroll
+      for (uint32_t j = 0; j < REG; j++)
+        if (lo + j < hi && sbs[base + lo + j] > P.refine_threshold) bits |= 1u << j;
+      cnt = __popc(bits);
+    } else {
+      for (uint32_t i = lo; i < hi; i++) cnt += sbs[base + i] > P.refine_threshold;
+    }
+  }
   uint32_t total;
   const uint32_t ex = wgExclusiveScan(cnt, sh, &total);
   if (threadIdx.x == 0) sTotal = total;
@@ -584,6 +618,14 @@ __global__ void __launch_bounds__(1024) k_refine_select(DGBase g, dcg_params P, 
   }
   if (lim == 0) return;
   uint32_t r = ex;
+  if (inRegs) {
+    for (uint32_t j = 0; j < REG; j++)
+      if ((bits >> j) & 1u) {
+        if (r >= skip) list[base + (r - skip)] = base + lo + j;
+        r++;
+      }
+    return;
+  }
   for (uint32_t i = lo; i < hi; i++)
     if (sbs[base + i] > P.refine_threshold) {
       if (r >= skip) list[base + (r - skip)] = base + i;
