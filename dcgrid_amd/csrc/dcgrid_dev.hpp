@@ -211,6 +211,15 @@ __device__ __forceinline__ uint32_t denseApronCell(const DGBase& g, int l, uint3
   const uint32_t nb = g.offs[l] + (uint32_t)((nx * g.ry[l] + ny) * g.rz[l] + nz);
   return nb * BV + loc<BB>(ai - 1 - BW * dx, aj - 1 - BW * dy, ak - 1 - BW * dz);
 }
+// Origin of block b of dense level l: k_activate_dense places block offs[l] + k at
+// BW * (k / (ry rz), (k / rz) mod ry, k mod rz), and dense blocks never move, so the origin
+// needs no table read.
+template <int BW>
+__device__ __forceinline__ int3 denseOrigin(const DGBase& g, int l, uint32_t b) {
+  const uint32_t k = b - g.offs[l], rz = (uint32_t)g.rz[l], ry = (uint32_t)g.ry[l];
+  const uint32_t q = k / rz;
+  return make_int3(BW * (int)(q / ry), BW * (int)(q % ry), BW * (int)(k - q * rz));
+}
 // Apron coordinates of face cell f in [0, 6*BW*BW): faces x-,x+,y-,y+,z-,z+.
 template <int BW>
 __device__ __forceinline__ void faceCell(int f, int& ai, int& aj, int& ak) {

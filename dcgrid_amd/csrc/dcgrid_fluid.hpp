@@ -104,13 +104,14 @@ __device__ __forceinline__ void stageScalar(const DGBase& g, const SF* __restric
     tile[(ai * A + aj) * A + ak] = f[apronCell<BW>(g, b, ai, aj, ak)];
   }
 }
-// stageScalar on a dense level (l >= S): the faces come from denseApronCell.
+// stageScalar on a dense level (l >= S): the faces come from denseApronCell, the block origin
+// from the block index.
 template <int BW, class SF>
 __device__ __forceinline__ void stageScalarDense(const DGBase& g, int l, const SF* __restrict__ f, float* tile, uint32_t b,
                                                  int t) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE;
   tile[apronIndexOfLocal<BW>(t)] = f[b * BV + t];
-  const int3 p0 = g.pos[b];
+  const int3 p0 = denseOrigin<BW>(g, l, b);
   for (int fc = t; fc < FACE; fc += BV) {
     int ai, aj, ak;
     faceCell<BW>(fc, ai, aj, ak);
