@@ -278,28 +278,21 @@ struct SelectShared {
   uint32_t sh[17];
   uint32_t sPrefix, sRem, sCount;
 };
-// Adds 1 to hist[bin] for every active lane of the wavefront. Candidate keys are often equal
-// (the -FLT_MAX scores of refined or free subblocks, zero scores): the lanes sharing the first
-// two distinct bins of the wavefront add with one LDS atomic each, the others one atomic per
-// lane, so a dominant value does not serialise. Called by all lanes of the wavefront.
-__device__ __forceinline__ void histAddWave(uint32_t* hist, uint32_t bin, bool active) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long pending = __ballot(active);
-#pragma unroll
-  for (int round = 0; round < 2; round++) {
-    if (!pending) return;
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t lb = __shfl(bin, leader, 64);
-    const unsigned long long same = __ballot(active && bin == lb) & pending;
-    if (lane == leader) atomicAdd(&hist[lb], (uint32_t)__popcll(same));
-    pending &= ~same;
+// Adds 1 to hist[bin] for every active lane of the wavefront. Lanes holding the common key
+// (that of -FLT_MAX scores, shared by every refined or free candidate, often most of them) add
+// with one LDS atomic per wavefront instead of serialising on one bin; the others one atomic
+// each. Called by all lanes of the wavefront.
+__device__ __forceinline__ void histAdd(uint32_t* hist, uint32_t bin, bool active, bool common) {
+  const unsigned long long m = __ballot(active && common);
+  if (active && common) {
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(&hist[bin], (uint32_t)__popcll(m));
+  } else if (active) {
+    atomicAdd(&hist[bin], 1u);
   }
-  if ((pending >> lane) & 1ull) atomicAdd(&hist[bin], 1u);
 }
 
 // The radix select of selectBody with the thread's keys (a segment of at most KEYS candidates)
-// loaded once into registers: the same passes, ties and winners. Short segments have few equal
-// keys per wavefront, so the histogram takes plain LDS atomics here.
+// loaded once into registers: the same passes, ties and winners.
 constexpr uint32_t SELECT_REG_KEYS = 8;
 template <uint32_t KEYS, class KeyFn>
 __device__ __forceinline__ void selectInRegisters(uint32_t lo, uint32_t hi, uint32_t k, KeyFn key32,
@@ -315,7 +308,7 @@ __device__ __forceinline__ void selectInRegisters(uint32_t lo, uint32_t hi, uint
     for (uint32_t j = threadIdx.x; j < 2048; j += T) S.hist[j] = 0;
     __syncthreads();
 #pragma unroll
-    for (uint32_t j = 0; j < KEYS; j++)
+    for (uint32_t j = 0; j < KEYS; j++)  // short segments: few equal keys per wavefront, plain atomics
       if (lo + j < hi && (kr[j] & mask) == prefix) atomicAdd(&S.hist[(kr[j] >> shift) & (nb - 1)], 1u);
     __syncthreads();
     const uint32_t a0 = 2 * threadIdx.x < nb ? S.hist[2 * threadIdx.x] : 0;
@@ -387,10 +380,11 @@ __device__ void selectBody(const DGBase& g, int level, const float* __restrict__
   }
   const uint32_t seg = (nA + T - 1) / T;
   const uint32_t lo = min(threadIdx.x * seg, nA), hi = min(lo + seg, nA);
-  if (seg <= SELECT_REG_KEYS) {
+  if (seg <= SELECT_REG_KEYS) {  // (a larger register segment spills: scratch on every launch)
     selectInRegisters<SELECT_REG_KEYS>(lo, hi, k, key32, cand, S);
     return;
   }
+  const uint32_t common = MODE == 0 ? blockKey(-FLT_MAX) : subKey(-FLT_MAX);
   // Radix select on key32: 11 + 11 + 10 bits.
   uint32_t prefix = 0, mask = 0, rem = k;
   for (int pass = 0; pass < 3; pass++) {
@@ -402,7 +396,7 @@ __device__ void selectBody(const DGBase& g, int level, const float* __restrict__
     for (uint32_t j = 0; j < seg; j++) {  // the same trip count in every lane
       const uint32_t i = lo + j;
       const uint32_t kk = i < hi ? key32(i) : 0u;
-      histAddWave(hist, (kk >> shift) & (nb - 1), i < hi && (kk & mask) == prefix);
+      histAdd(hist, (kk >> shift) & (nb - 1), i < hi && (kk & mask) == prefix, kk == common);
     }
     __syncthreads();
     const uint32_t a0 = 2 * threadIdx.x < nb ? hist[2 * threadIdx.x] : 0;

@@ -340,9 +340,11 @@ __device__ __forceinline__ void coarseLocal(const ULevels& D, int L, int Ls, ULe
 __global__ void __launch_bounds__(1024) k_u_coarse(UGrid g, ULevels D, int L, int Ls, int topPairs, int levelPairs,
                                                    float dx) {
   __shared__ float sp[kCoarseLds], st[kCoarseLds], sd[kCoarseLds];
-  ULevels S;
-  size_t total;
-  coarseLocal(D, L, Ls, S, total);
+  __shared__ ULevels S;  // the level table in LDS: a per-thread copy indexed by level goes to scratch
+  __shared__ size_t sTotal;
+  if (threadIdx.x == 0) coarseLocal(D, L, Ls, S, sTotal);
+  __syncthreads();
+  const size_t total = sTotal;
   const size_t base = D.d[Ls].off;
   for (size_t i = threadIdx.x; i < total; i += blockDim.x) {
     sp[i] = g.pre[base + i];

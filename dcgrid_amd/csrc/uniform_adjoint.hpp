@@ -115,7 +115,13 @@ __device__ __forceinline__ float utransWeight(const dcg_params& P, const UGrid& 
   USample s;
   usample<FLUIDITY>(g, px, py, pz, s);
   if (s.zero) return 0.f;
-  return s.w[(bx << 2) | (by << 1) | bz];
+  // the corner's weight picked with static indices (a data-dependent index into the register
+  // array would place it in scratch, set up on every launch of the kernel)
+  const int c = (bx << 2) | (by << 1) | bz;
+  float w = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; k++) w = k == c ? s.w[k] : w;
+  return w;
 }
 
 __device__ __forceinline__ int windowRadius(const int* __restrict__ win, int* __restrict__ errFlag) {
@@ -527,9 +533,11 @@ __global__ void __launch_bounds__(1024) k_ua_coarse(float* __restrict__ lp, floa
                                                     float* __restrict__ ld, ULevels D, int L, int Ls, int topPairs,
                                                     int levelPairs, float dx) {
   __shared__ float sp[kCoarseLds], st[kCoarseLds], sd[kCoarseLds];
-  ULevels S;
-  size_t total;
-  coarseLocal(D, L, Ls, S, total);
+  __shared__ ULevels S;  // the level table in LDS: a per-thread copy indexed by level goes to scratch
+  __shared__ size_t sTotal;
+  if (threadIdx.x == 0) coarseLocal(D, L, Ls, S, sTotal);
+  __syncthreads();
+  const size_t total = sTotal;
   const size_t base = D.d[Ls].off;
   for (size_t i = threadIdx.x; i < total; i += blockDim.x) {
     sp[i] = lp[base + i];
