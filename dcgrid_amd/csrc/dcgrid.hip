@@ -258,7 +258,24 @@ struct DCGridSim : SimBase {
     uint32_t negBits;
     std::memcpy(&negBits, &negMax, 4);
     DCG_TRY(fill32(sbs, negBits, (size_t)8 * B + 1));
+    DCG_TRY(queryResidentGrids());
     return reset();
+  }
+  // The advection grids are asked of the runtime here, outside any stream capture.
+  int queryResidentGrids() {
+    constexpr int WGA = AdvGeo<BW>::WG;
+    unsigned n = 0;
+    DCG_TRY(residentGrid(k_advect_velocity<BW, true, true, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_velocity<BW, true, false, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_velocity<BW, false, true, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_velocity<BW, false, false, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_density<BW, true, true, true, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_density<BW, false, true, true, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_density<BW, true, true, false, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_density<BW, true, false, false, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_density<BW, false, true, false, DGT<SF>>, WGA, B, n));
+    DCG_TRY(residentGrid(k_advect_density<BW, false, false, false, DGT<SF>>, WGA, B, n));
+    return DCG_OK;
   }
 
   // Zero bits are +0 in every storage format; element sizes are multiples of 2 bytes and C is
@@ -378,11 +395,39 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
 
+  // Workgroups of a grid-stride walk over at most `blocks` blocks that fill the chip once: the
+  // workgroups of `kernel` resident per CU (at its registers and LDS) times the CUs, asked of the
+  // runtime once per kernel.
+  std::vector<std::pair<const void*, unsigned>> residentPerChip;
+  template <class K>
+  int residentGrid(K kernel, int threads, size_t blocks, unsigned& n) {
+    const void* key = (const void*)kernel;
+    unsigned chip = 0;
+    for (const auto& e : residentPerChip)
+      if (e.first == key) chip = e.second;
+    if (chip == 0) {
+      int dev = 0, cus = 0, per = 0;
+      DCG_HIP(hipGetDevice(&dev));
+      DCG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      DCG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0));
+      chip = (unsigned)std::max(per, 1) * (unsigned)std::max(cus, 1);
+      residentPerChip.emplace_back(key, chip);
+    }
+    const unsigned want = blocksFor(blocks, G::BPW);
+    n = want == 0 ? 1u : std::min(want, chip);
+    return DCG_OK;
+  }
+
   // advectVelocity(), fluid_simulation_dcgrid.cu:263-268
   int advectVelocity() override {
-    const unsigned nWG = slotGrid<BW>(B);
     const bool buoy = thermo(), fl = fluidityOn();
-#define ADV_V(F, Bu) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_advect_velocity", (k_advect_velocity<BW, F, Bu><<<nWG, G::WG, 0, stream>>>(g, P)))
+    constexpr int WGA = AdvGeo<BW>::WG;
+#define ADV_V(F, Bu)                                                                                              \
+  do {                                                                                                            \
+    unsigned nWG = 0;                                                                                             \
+    DCG_TRY(residentGrid(k_advect_velocity<BW, F, Bu, DGT<SF>>, WGA, B, nWG));                                          \
+    DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_advect_velocity", (k_advect_velocity<BW, F, Bu><<<nWG, WGA, 0, stream>>>(g, P))); \
+  } while (0)
     if (fl && buoy) ADV_V(true, true);
     else if (fl) ADV_V(true, false);
     else if (buoy) ADV_V(false, true);
@@ -393,9 +438,14 @@ struct DCGridSim : SimBase {
   }
   // advectDensity(), fluid_simulation_dcgrid.cu:313-318
   int advectDensity() override {
-    const unsigned nWG = slotGrid<BW>(B);
     const bool src = thermo(), moist = P.moist != 0, fl = fluidityOn();
-#define ADV_D(F, Sr, M) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_advect_density", (k_advect_density<BW, F, Sr, M><<<nWG, G::WG, 0, stream>>>(g, P)))
+    constexpr int WGA = AdvGeo<BW>::WG;
+#define ADV_D(F, Sr, M)                                                                                           \
+  do {                                                                                                            \
+    unsigned nWG = 0;                                                                                             \
+    DCG_TRY(residentGrid(k_advect_density<BW, F, Sr, M, DGT<SF>>, WGA, B, nWG));                                        \
+    DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_advect_density", (k_advect_density<BW, F, Sr, M><<<nWG, WGA, 0, stream>>>(g, P))); \
+  } while (0)
     if (moist) {
       if (fl) ADV_D(true, true, true);
       else ADV_D(false, true, true);

@@ -636,102 +636,125 @@ struct TileSample {
   }
 };
 
-// Visits the whole apron of block b ((BW+2)^3 entries, edges and corners included) for thread t:
-// op(a, cell) for the thread's interior cell and for its share of the border entries, with the
-// cell of each entry resolved once (apronCell) however many fields are staged from it.
-template <int BW, class Op>
+// Visits the whole apron of block b ((BW+2)^3 entries, edges and corners included) for thread t
+// of the N = BV / CPT threads that handle the block: op(a, cell) for the thread's CPT interior
+// cells (t + j N) and for its share of the border entries, with the cell of each entry resolved
+// once (apronCell) however many fields are staged from it.
+template <int BW, int CPT, class Op>
 __device__ __forceinline__ void forApron(const DGBase& g, uint32_t b, int t, Op op) {
-  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3;
-  op(apronIndexOfLocal<BW>(t), b * BV + t);
-  for (int a = t; a < A3; a += BV) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, N = BV / CPT;
+#pragma unroll
+  for (int j = 0; j < CPT; j++) op(apronIndexOfLocal<BW>(t + j * N), b * BV + t + j * N);
+  for (int a = t; a < A3; a += N) {
     const int ai = a / (A * A), aj = (a / A) % A, ak = a % A;
     if (ai > 0 && ai <= BW && aj > 0 && aj <= BW && ak > 0 && ak <= BW) continue;
     op(a, apronCell<BW>(g, b, ai, aj, ak));
   }
 }
 
+// Advection workgroup shape. The advection kernels hold ~90-100 VGPRs (5 waves per SIMD), so a
+// 512-thread workgroup per BW-8 block leaves two blocks resident per CU, and the kernels are
+// bound by blocks in flight times the staging latency. A BW-8 block is therefore handled by
+// BV / CPT threads with CPT cells each, one cell after the other: more blocks resident at the
+// same register count (two cells per thread: four resident blocks per CU; four cells per thread
+// measured slower). BW 4 keeps four blocks of 64 threads.
+template <int BW>
+struct AdvGeo {
+  static constexpr int CPT = BW == 8 ? 2 : 1;
+  static constexpr int WG = Geo<BW>::WG / CPT;
+};
+
 // k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension). The block's
 // velocity (and fluidity) with the full apron are staged in LDS; samples whose foot point stays
 // in the block read the tile, the others gather through the neighbour tables.
 template <int BW, bool FLUIDITY, bool BUOY, class G>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_velocity(G g, dcg_params P) {
+__global__ void __launch_bounds__(AdvGeo<BW>::WG) k_advect_velocity(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  constexpr int CPT = AdvGeo<BW>::CPT, N = BV / CPT;
   __shared__ float3 vtiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   __shared__ uint32_t chis[BPW][8];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
-    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);  // c.t: the thread's first cell
     float3* vt = vtiles[c.sub];
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     uint32_t* chi8 = chis[c.sub];
-    const uint32_t cell = c.b * BV + c.t;
     int3 p0 = make_int3(0, 0, 0);
-    bool leaf = false;
-    float tc = 0.f, dc = 0.f;
+    float tc = 0.f, dc = 0.f;  // buoyancy inputs of the next cell to advect
     if (c.valid) {
       p0 = g.pos[c.b];
-      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
       if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
-      if (BUOY && leaf) {
-        tc = g.tem[cell];
-        dc = g.den[cell];
+      if (BUOY) {
+        tc = g.tem[c.b * BV + c.t];
+        dc = g.den[c.b * BV + c.t];
       }
-      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t src) {
+      forApron<BW, CPT>(g, c.b, c.t, [&](int a, uint32_t src) {
         vt[a] = g.vel[src];
         if (FLUIDITY) ft[a] = g.flu[src];
       });
     }
     __syncthreads();
     if (!c.valid) continue;
-    if (!leaf) {
-      g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
-      continue;
-    }
     const int level = c.level;
-    float px, py, pz;
-    int x, y, z;
-    footPoint<BW>(P, p0, level, c.t, vt[apronIndexOfLocal<BW>(c.t)], px, py, pz, x, y, z);
-    float3 acc = make_float3(0.f, 0.f, 0.f);
-    TileSample<BW, FLUIDITY> ts;
-    if (ts.init(g, px, py, pz, chi8, p0, level, ft)) {
-      if (!ts.zero) {
+#pragma unroll 1
+    for (int j = 0; j < CPT; j++) {
+      const int t = c.t + j * N;
+      const uint32_t cell = c.b * BV + t;
+      const float tcj = tc, dcj = dc;
+      if (BUOY && j + 1 < CPT) {  // issued before this cell's sample
+        tc = g.tem[cell + N];
+        dc = g.den[cell + N];
+      }
+      if (chi8[t >> SBB] != NFU) {  // not a leaf cell
+        g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
+        continue;
+      }
+      float px, py, pz;
+      int x, y, z;
+      footPoint<BW>(P, p0, level, t, vt[apronIndexOfLocal<BW>(t)], px, py, pz, x, y, z);
+      float3 acc = make_float3(0.f, 0.f, 0.f);
+      TileSample<BW, FLUIDITY> ts;
+      if (ts.init(g, px, py, pz, chi8, p0, level, ft)) {
+        if (!ts.zero) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const float3 vk = velocityBC(P, vt[ts.a0 + ts.off(k)], ts.x0 + ((k >> 2) & 1), ts.y0 + ((k >> 1) & 1),
-                                       ts.z0 + (k & 1), ts.scale);
-          acc = k == 0 ? f3mul(vk, ts.w[0]) : f3add(acc, f3mul(vk, ts.w[k]));
+          for (int k = 0; k < 8; k++) {
+            const float3 vk = velocityBC(P, vt[ts.a0 + ts.off(k)], ts.x0 + ((k >> 2) & 1), ts.y0 + ((k >> 1) & 1),
+                                         ts.z0 + (k & 1), ts.scale);
+            acc = k == 0 ? f3mul(vk, ts.w[0]) : f3add(acc, f3mul(vk, ts.w[k]));
+          }
+        }
+      } else {
+        CrossSample<BW, FLUIDITY> s;
+        s.init(g, px, py, pz);
+        if (!s.zero) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const float3 vk =
+                velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
+            acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
+          }
         }
       }
-    } else {
-      CrossSample<BW, FLUIDITY> s;
-      s.init(g, px, py, pz);
-      if (!s.zero) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const float3 vk =
-              velocityBC(P, g.vel[s.i[k]], s.x0 + ((k >> 2) & 1), s.y0 + ((k >> 1) & 1), s.z0 + (k & 1), s.scale);
-          acc = k == 0 ? f3mul(vk, s.w[0]) : f3add(acc, f3mul(vk, s.w[k]));
-        }
+      if (BUOY) {
+        const float ta = ambientT(P, cellHeight(y, 1 << level));
+        const float bf = P.dt * (P.buoy_beta * (tcj - ta) - P.buoy_alpha * dcj);
+        acc.y = acc.y + bf;
       }
+      g.tvel[cell] = acc;
     }
-    if (BUOY) {
-      const float ta = ambientT(P, cellHeight(y, 1 << level));
-      const float bf = P.dt * (P.buoy_beta * (tc - ta) - P.buoy_alpha * dc);
-      acc.y = acc.y + bf;
-    }
-    g.tvel[cell] = acc;
   }
 }
 
 // k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
 // with the same LDS fast path as the velocity.
 template <int BW, bool FLUIDITY, bool SRC, bool MOIST, class G>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params P) {
+__global__ void __launch_bounds__(AdvGeo<BW>::WG) k_advect_density(G g, dcg_params P) {
   // SRC: temperature is advected (sphere source or moist atmosphere); MOIST: vapor too, then the
   // saturation adjustment of (vapor, cloud water = density, temperature).
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  constexpr int CPT = AdvGeo<BW>::CPT, N = BV / CPT;
   __shared__ float dtiles[BPW][A3];
   __shared__ float ttiles[SRC ? BPW : 1][SRC ? A3 : 1];
   __shared__ float vtiles[MOIST ? BPW : 1][MOIST ? A3 : 1];
@@ -741,22 +764,19 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params 
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
-    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);  // c.t: the thread's first cell
     float* dt_ = dtiles[c.sub];
     float* tt = SRC ? ttiles[c.sub] : nullptr;
     float* vt = MOIST ? vtiles[c.sub] : nullptr;
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     uint32_t* chi8 = chis[c.sub];
-    const uint32_t cell = c.b * BV + c.t;
     int3 p0 = make_int3(0, 0, 0);
-    bool leaf = false;
-    float3 u = make_float3(0.f, 0.f, 0.f);
+    float3 un = make_float3(0.f, 0.f, 0.f);  // velocity of the next cell to advect
     if (c.valid) {
       p0 = g.pos[c.b];
-      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
       if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
-      if (leaf) u = g.vel[cell];
-      forApron<BW>(g, c.b, c.t, [&](int a, uint32_t src) {
+      un = g.vel[c.b * BV + c.t];
+      forApron<BW, CPT>(g, c.b, c.t, [&](int a, uint32_t src) {
         dt_[a] = g.den[src];
         if (SRC) tt[a] = g.tem[src];
         if (MOIST) vt[a] = g.vap[src];
@@ -765,69 +785,76 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_advect_density(G g, dcg_params 
     }
     __syncthreads();
     if (!c.valid) continue;
-    if (!leaf) {
-      g.tden[cell] = 0.f;
-      if (SRC) g.ttem[cell] = 0.f;
-      if (MOIST) g.tvap[cell] = 0.f;
-      continue;
-    }
     const int level = c.level;
-    float px, py, pz;
-    int x, y, z;
-    footPoint<BW>(P, p0, level, c.t, u, px, py, pz, x, y, z);
-    float q = 0.f, tq = 0.f, vq = 0.f;
-    TileSample<BW, FLUIDITY> ts;
-    if (ts.init(g, px, py, pz, chi8, p0, level, ft)) {
-      if (!ts.zero) {
+#pragma unroll 1
+    for (int j = 0; j < CPT; j++) {
+      const int t = c.t + j * N;
+      const uint32_t cell = c.b * BV + t;
+      const float3 u = un;
+      if (j + 1 < CPT) un = g.vel[cell + N];  // issued before this cell's sample
+      if (chi8[t >> SBB] != NFU) {  // not a leaf cell
+        g.tden[cell] = 0.f;
+        if (SRC) g.ttem[cell] = 0.f;
+        if (MOIST) g.tvap[cell] = 0.f;
+        continue;
+      }
+      float px, py, pz;
+      int x, y, z;
+      footPoint<BW>(P, p0, level, t, u, px, py, pz, x, y, z);
+      float q = 0.f, tq = 0.f, vq = 0.f;
+      TileSample<BW, FLUIDITY> ts;
+      if (ts.init(g, px, py, pz, chi8, p0, level, ft)) {
+        if (!ts.zero) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const int cx = ts.x0 + ((k >> 2) & 1), cy = ts.y0 + ((k >> 1) & 1), cz = ts.z0 + (k & 1);
-          const float qk = densityBC(P, dt_[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
-          q = k == 0 ? qk * ts.w[0] : q + qk * ts.w[k];
-          if (SRC) {
-            const float tk = temperatureBC(P, tt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
-            tq = k == 0 ? tk * ts.w[0] : tq + tk * ts.w[k];
+          for (int k = 0; k < 8; k++) {
+            const int cx = ts.x0 + ((k >> 2) & 1), cy = ts.y0 + ((k >> 1) & 1), cz = ts.z0 + (k & 1);
+            const float qk = densityBC(P, dt_[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+            q = k == 0 ? qk * ts.w[0] : q + qk * ts.w[k];
+            if (SRC) {
+              const float tk = temperatureBC(P, tt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+              tq = k == 0 ? tk * ts.w[0] : tq + tk * ts.w[k];
+            }
+            if (MOIST) {
+              const float vk = vaporBC(P, vt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
+              vq = k == 0 ? vk * ts.w[0] : vq + vk * ts.w[k];
+            }
           }
-          if (MOIST) {
-            const float vk = vaporBC(P, vt[ts.a0 + ts.off(k)], cx, cy, cz, ts.scale);
-            vq = k == 0 ? vk * ts.w[0] : vq + vk * ts.w[k];
+        }
+      } else {
+        CrossSample<BW, FLUIDITY> s;
+        s.init(g, px, py, pz);
+        if (!s.zero) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
+            const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
+            q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
+            if (SRC) {
+              const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
+              tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
+            }
+            if (MOIST) {
+              const float vk = vaporBC(P, g.vap[s.i[k]], cx, cy, cz, s.scale);
+              vq = k == 0 ? vk * s.w[0] : vq + vk * s.w[k];
+            }
           }
         }
       }
-    } else {
-      CrossSample<BW, FLUIDITY> s;
-      s.init(g, px, py, pz);
-      if (!s.zero) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const int cx = s.x0 + ((k >> 2) & 1), cy = s.y0 + ((k >> 1) & 1), cz = s.z0 + (k & 1);
-          const float qk = densityBC(P, g.den[s.i[k]], cx, cy, cz, s.scale);
-          q = k == 0 ? qk * s.w[0] : q + qk * s.w[k];
-          if (SRC) {
-            const float tk = temperatureBC(P, g.tem[s.i[k]], cx, cy, cz, s.scale);
-            tq = k == 0 ? tk * s.w[0] : tq + tk * s.w[k];
-          }
-          if (MOIST) {
-            const float vk = vaporBC(P, g.vap[s.i[k]], cx, cy, cz, s.scale);
-            vq = k == 0 ? vk * s.w[0] : vq + vk * s.w[k];
-          }
+      if (SRC) {
+        const float sc = (float)(1 << level);
+        if (sphere && inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
+          q = P.src_density;
+          tq = sourceTemperature(P, x, y, z, level);
         }
+        if (MOIST) {
+          tq = tq - (P.dt * P.adiabatic_rate) * u.y;  // expansion cooling of rising air
+          saturationAdjust(P, vq, q, tq);
+          g.tvap[cell] = vq;
+        }
+        g.ttem[cell] = tq;
       }
+      g.tden[cell] = q;
     }
-    if (SRC) {
-      const float sc = (float)(1 << level);
-      if (sphere && inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
-        q = P.src_density;
-        tq = sourceTemperature(P, x, y, z, level);
-      }
-      if (MOIST) {
-        tq = tq - (P.dt * P.adiabatic_rate) * u.y;  // expansion cooling of rising air
-        saturationAdjust(P, vq, q, tq);
-        g.tvap[cell] = vq;
-      }
-      g.ttem[cell] = tq;
-    }
-    g.tden[cell] = q;
   }
 }
 
