@@ -11,6 +11,8 @@
 // read; face cells are fetched through the block's neighbour table.
 #pragma once
 
+#include <type_traits>
+
 #include "dcgrid_dev.hpp"
 
 namespace dcg {
@@ -663,12 +665,20 @@ struct AdvGeo {
   static constexpr int CPT = BW == 8 ? 2 : 1;
   static constexpr int WG = Geo<BW>::WG / CPT;
 };
+// Minimum waves per SIMD asked of the compiler: 5 (at most 96 VGPRs, five blocks resident per CU)
+// for the BW-8 variants that fit it without spilling; the others keep the default. (SRC is the
+// buoyancy of the velocity kernel, whose BF16 variant would spill.)
+template <int BW, bool FLU, bool SRC, bool MOIST, class G, bool VELOCITY>
+struct AdvWaves {
+  static constexpr bool bf16 = std::is_same<typename G::Scalar, StoreBF16>::value;
+  static constexpr int value = BW == 8 && !MOIST && !(FLU && SRC) && !(VELOCITY && SRC && bf16) ? 5 : 1;
+};
 
 // k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension). The block's
 // velocity (and fluidity) with the full apron are staged in LDS; samples whose foot point stays
 // in the block read the tile, the others gather through the neighbour tables.
 template <int BW, bool FLUIDITY, bool BUOY, class G>
-__global__ void __launch_bounds__(AdvGeo<BW>::WG) k_advect_velocity(G g, dcg_params P) {
+__global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, BUOY, false, G, true>::value)) k_advect_velocity(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   constexpr int CPT = AdvGeo<BW>::CPT, N = BV / CPT;
   __shared__ float3 vtiles[BPW][A3];
@@ -750,7 +760,7 @@ __global__ void __launch_bounds__(AdvGeo<BW>::WG) k_advect_velocity(G g, dcg_par
 // k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
 // with the same LDS fast path as the velocity.
 template <int BW, bool FLUIDITY, bool SRC, bool MOIST, class G>
-__global__ void __launch_bounds__(AdvGeo<BW>::WG) k_advect_density(G g, dcg_params P) {
+__global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, SRC, MOIST, G, false>::value)) k_advect_density(G g, dcg_params P) {
   // SRC: temperature is advected (sphere source or moist atmosphere); MOIST: vapor too, then the
   // saturation adjustment of (vapor, cloud water = density, temperature).
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
@@ -841,6 +851,13 @@ __global__ void __launch_bounds__(AdvGeo<BW>::WG) k_advect_density(G g, dcg_para
         }
       }
       if (SRC) {
+        // the cell's coordinates again, from an opaque copy of t: no live range across the sample
+        int tr = t;
+        asm volatile("" : "+v"(tr));
+        locPos<Geo<BW>::BB>((uint32_t)tr, x, y, z);
+        x |= p0.x;
+        y |= p0.y;
+        z |= p0.z;
         const float sc = (float)(1 << level);
         if (sphere && inSource(P, ((float)x + .5f) * sc, ((float)y + .5f) * sc, ((float)z + .5f) * sc)) {
           q = P.src_density;
