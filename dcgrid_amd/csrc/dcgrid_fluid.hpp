@@ -11,8 +11,6 @@
 // read; face cells are fetched through the block's neighbour table.
 #pragma once
 
-#include <type_traits>
-
 #include "dcgrid_dev.hpp"
 
 namespace dcg {
@@ -665,20 +663,20 @@ struct AdvGeo {
   static constexpr int CPT = BW == 8 ? 2 : 1;
   static constexpr int WG = Geo<BW>::WG / CPT;
 };
-// Minimum waves per SIMD asked of the compiler: 5 (at most 96 VGPRs, five blocks resident per CU)
-// for the BW-8 variants that fit it without spilling; the others keep the default. (SRC is the
-// buoyancy of the velocity kernel, whose BF16 variant would spill.)
-template <int BW, bool FLU, bool SRC, bool MOIST, class G, bool VELOCITY>
+// Minimum waves per SIMD asked of the compiler: 5 for BW 8 (at most 96 VGPRs, five blocks
+// resident per CU). The variants without fluidity-weighted temperature or vapor fit without
+// spilling; the terrain variants (fluidity with buoyancy, moist density) spill a few VGPRs on
+// their cross-block path and still run faster than at four waves.
+template <int BW>
 struct AdvWaves {
-  static constexpr bool bf16 = std::is_same<typename G::Scalar, StoreBF16>::value;
-  static constexpr int value = BW == 8 && !MOIST && !(FLU && SRC) && !(VELOCITY && SRC && bf16) ? 5 : 1;
+  static constexpr int value = BW == 8 ? 5 : 1;
 };
 
 // k_dcgrid_advect_velocity (dcgrid_fluid.cu:112-127) + buoyancy (extension). The block's
 // velocity (and fluidity) with the full apron are staged in LDS; samples whose foot point stays
 // in the block read the tile, the others gather through the neighbour tables.
 template <int BW, bool FLUIDITY, bool BUOY, class G>
-__global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, BUOY, false, G, true>::value)) k_advect_velocity(G g, dcg_params P) {
+__global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW>::value)) k_advect_velocity(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
   constexpr int CPT = AdvGeo<BW>::CPT, N = BV / CPT;
   __shared__ float3 vtiles[BPW][A3];
@@ -692,14 +690,9 @@ __global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, BUOY, 
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     uint32_t* chi8 = chis[c.sub];
     int3 p0 = make_int3(0, 0, 0);
-    float tc = 0.f, dc = 0.f;  // buoyancy inputs of the next cell to advect
     if (c.valid) {
       p0 = g.pos[c.b];
       if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
-      if (BUOY) {
-        tc = g.tem[c.b * BV + c.t];
-        dc = g.den[c.b * BV + c.t];
-      }
       forApron<BW, CPT>(g, c.b, c.t, [&](int a, uint32_t src) {
         vt[a] = g.vel[src];
         if (FLUIDITY) ft[a] = g.flu[src];
@@ -712,14 +705,14 @@ __global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, BUOY, 
     for (int j = 0; j < CPT; j++) {
       const int t = c.t + j * N;
       const uint32_t cell = c.b * BV + t;
-      const float tcj = tc, dcj = dc;
-      if (BUOY && j + 1 < CPT) {  // issued before this cell's sample
-        tc = g.tem[cell + N];
-        dc = g.den[cell + N];
-      }
       if (chi8[t >> SBB] != NFU) {  // not a leaf cell
         g.tvel[cell] = make_float3(0.f, 0.f, 0.f);
         continue;
+      }
+      float tc = 0.f, dc = 0.f;  // buoyancy inputs, issued before the sample and used after it
+      if (BUOY) {
+        tc = g.tem[cell];
+        dc = g.den[cell];
       }
       float px, py, pz;
       int x, y, z;
@@ -749,7 +742,7 @@ __global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, BUOY, 
       }
       if (BUOY) {
         const float ta = ambientT(P, cellHeight(y, 1 << level));
-        const float bf = P.dt * (P.buoy_beta * (tcj - ta) - P.buoy_alpha * dcj);
+        const float bf = P.dt * (P.buoy_beta * (tc - ta) - P.buoy_alpha * dc);
         acc.y = acc.y + bf;
       }
       g.tvel[cell] = acc;
@@ -760,7 +753,7 @@ __global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, BUOY, 
 // k_dcgrid_advect_density (dcgrid_fluid.cu:129-144) + temperature and sphere source (extension),
 // with the same LDS fast path as the velocity.
 template <int BW, bool FLUIDITY, bool SRC, bool MOIST, class G>
-__global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW, FLUIDITY, SRC, MOIST, G, false>::value)) k_advect_density(G g, dcg_params P) {
+__global__ void __launch_bounds__(AdvGeo<BW>::WG, (AdvWaves<BW>::value)) k_advect_density(G g, dcg_params P) {
   // SRC: temperature is advected (sphere source or moist atmosphere); MOIST: vapor too, then the
   // saturation adjustment of (vapor, cloud water = density, temperature).
   constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
