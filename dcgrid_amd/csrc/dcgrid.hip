@@ -261,7 +261,8 @@ struct DCGridSim : SimBase {
     DCG_TRY(queryResidentGrids());
     return reset();
   }
-  // The advection grids are asked of the runtime here, outside any stream capture.
+  // The grids of the occupancy-sized walks are asked of the runtime here, outside any stream
+  // capture.
   int queryResidentGrids() {
     constexpr int WGA = AdvGeo<BW>::WG;
     unsigned n = 0;
@@ -275,6 +276,15 @@ struct DCGridSim : SimBase {
     DCG_TRY(residentGrid(k_advect_density<BW, true, false, false, DGT<SF>>, WGA, B, n));
     DCG_TRY(residentGrid(k_advect_density<BW, false, true, false, DGT<SF>>, WGA, B, n));
     DCG_TRY(residentGrid(k_advect_density<BW, false, false, false, DGT<SF>>, WGA, B, n));
+    if (twoCellStencils()) {
+      constexpr int WGC = CptGeo<BW, CPT2>::WG;
+      DCG_TRY(residentGrid(k_divergence<BW, true, CPT2, DGT<SF>>, WGC, B, n));
+      DCG_TRY(residentGrid(k_divergence<BW, false, CPT2, DGT<SF>>, WGC, B, n));
+      DCG_TRY(residentGrid(k_apply_pressure<BW, true, CPT2, DGT<SF>>, WGC, B, n));
+      DCG_TRY(residentGrid(k_apply_pressure<BW, false, CPT2, DGT<SF>>, WGC, B, n));
+      DCG_TRY(residentGrid(k_features<BW, true, CPT2, DGT<SF>>, WGC, B, n));
+      DCG_TRY(residentGrid(k_features<BW, false, CPT2, DGT<SF>>, WGC, B, n));
+    }
     return DCG_OK;
   }
 
@@ -338,12 +348,13 @@ struct DCGridSim : SimBase {
   int adaptTopology() override { return adapt(nullptr); }
   int adaptTopologyCounted() override { return adapt(cellStepCounter); }
   int adapt(unsigned long long* counter) {
-    const unsigned nWG = slotGrid<BW>(B);
     if (P.score_mode == DCG_SCORE_FEATURES) {  // vorticity and subblock features in one pass
       if (fluidityOn())
-        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_features", (k_features<BW, true><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
+        DCG_TRY(stencilLaunch(DCG_K_TOPOLOGY, "k_features", k_features<BW, true, 1, DGT<SF>>,
+                                    k_features<BW, true, CPT2, DGT<SF>>, P, feat));
       else
-        DCG_LAUNCH(DCG_K_TOPOLOGY, "k_features", (k_features<BW, false><<<nWG, G::WG, 0, stream>>>(g, P, feat)));
+        DCG_TRY(stencilLaunch(DCG_K_TOPOLOGY, "k_features", k_features<BW, false, 1, DGT<SF>>,
+                                    k_features<BW, false, CPT2, DGT<SF>>, P, feat));
       DCG_LAUNCH(DCG_K_TOPOLOGY, "k_scores", (k_scores<BW, true><<<blocksFor(B, 256), 256, 0, stream>>>(g, feat, sbs, bs)));
     } else {
       DCG_LAUNCH(DCG_K_TOPOLOGY, "k_scores", (k_scores<BW, false><<<blocksFor(B, 256), 256, 0, stream>>>(g, feat, sbs, bs)));
@@ -418,6 +429,25 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
 
+  // Stencil walks over all allocated blocks take two cells per thread (BW 8) when the budget is
+  // large enough that workgroups walk many blocks; with a few thousand blocks a workgroup
+  // handles about one, and one cell per thread is faster.
+  static constexpr uint32_t kTwoCellBudget = 16384;
+  static constexpr int CPT2 = BW == 8 ? 2 : 1;  // the two-cell shape (BW 8 only)
+  bool twoCellStencils() const { return CPT2 == 2 && B >= kTwoCellBudget; }
+  template <class K1, class K2, class... Args>
+  int stencilLaunch(int kind, const char* name, K1 one, K2 two, Args... args) {
+    if (twoCellStencils()) {
+      constexpr int WGC = CptGeo<BW, CPT2>::WG;
+      unsigned nWG = 0;
+      DCG_TRY(residentGrid(two, WGC, B, nWG));
+      DCG_LAUNCH(kind, name, (two<<<nWG, WGC, 0, stream>>>(g, args...)));
+    } else {
+      DCG_LAUNCH(kind, name, (one<<<slotGrid<BW>(B), G::WG, 0, stream>>>(g, args...)));
+    }
+    return DCG_OK;
+  }
+
   // advectVelocity(), fluid_simulation_dcgrid.cu:263-268
   int advectVelocity() override {
     const bool buoy = thermo(), fl = fluidityOn();
@@ -467,12 +497,11 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
   int divergenceKernel() {
-    const unsigned nWG = slotGrid<BW>(B);
     if (fluidityOn())
-      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
-    else
-      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_divergence", (k_divergence<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
-    return DCG_OK;
+      return stencilLaunch(DCG_K_DIVERGENCE, "k_divergence", k_divergence<BW, true, 1, DGT<SF>>,
+                                  k_divergence<BW, true, CPT2, DGT<SF>>, P);
+    return stencilLaunch(DCG_K_DIVERGENCE, "k_divergence", k_divergence<BW, false, 1, DGT<SF>>,
+                                k_divergence<BW, false, CPT2, DGT<SF>>, P);
   }
   int divergence() {
     DCG_TRY(divergenceKernel());
@@ -571,12 +600,11 @@ struct DCGridSim : SimBase {
     return DCG_OK;
   }
   int applyKernel() {
-    const unsigned nWG = slotGrid<BW>(B);
     if (fluidityOn())
-      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, true><<<nWG, G::WG, 0, stream>>>(g, P)));
-    else
-      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_apply_pressure", (k_apply_pressure<BW, false><<<nWG, G::WG, 0, stream>>>(g, P)));
-    return DCG_OK;
+      return stencilLaunch(DCG_K_APPLY_PRESSURE, "k_apply_pressure", k_apply_pressure<BW, true, 1, DGT<SF>>,
+                                  k_apply_pressure<BW, true, CPT2, DGT<SF>>, P);
+    return stencilLaunch(DCG_K_APPLY_PRESSURE, "k_apply_pressure", k_apply_pressure<BW, false, 1, DGT<SF>>,
+                                k_apply_pressure<BW, false, CPT2, DGT<SF>>, P);
   }
   int applyPressure() {
     DCG_TRY(applyKernel());

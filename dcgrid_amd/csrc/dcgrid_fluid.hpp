@@ -137,6 +137,16 @@ __device__ __forceinline__ void tileFluidity(const float* ft, int a, float w[6])
   w[5] = ft[a + 1];      // f
 }
 
+// Stencil kernels with a CPT parameter: a thread takes CPT cells of its block, so a block is
+// handled by BV / CPT threads and CPT times as many blocks are resident per CU at the same thread
+// count, each thread keeping CPT cells' loads in flight. That pays when workgroups walk many
+// blocks; with about one block per workgroup the longer thread chain costs more (CPT 1).
+template <int BW, int CPT>
+struct CptGeo {
+  static_assert(CPT == 1 || (BW == 8 && CPT == 2), "two cells per thread for BW 8 only");
+  static constexpr int WG = Geo<BW>::WG / CPT;
+};
+
 // ------------------------------------------------------------------ Jacobi half-sweep
 // k_dcgrid_jacobi / k_dcgrid_jacobi_inv (dcgrid_multigrid_solver.cu:5-41):
 // out = (sum of 6 apron neighbours of `in` - (scale*dx)^2 * div) / 6 on every cell of every
@@ -185,31 +195,37 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(G g, int level, const ty
 // ------------------------------------------------------------------ divergence
 // k_dcgrid_calc_divergence (dcgrid_fluid.cu:174-230): zero p, p~ and div on all cells; stage the
 // velocity with boundary conditions on the apron faces; central divergence on leaf cells.
-template <int BW, bool FLUIDITY, class G>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(G g, dcg_params P) {
+template <int BW, bool FLUIDITY, int CPT, class G>
+__global__ void __launch_bounds__((CptGeo<BW, CPT>::WG)) k_divergence(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
-                SBB = Geo<BW>::SBB;
+                SBB = Geo<BW>::SBB, N = BV / CPT;
   __shared__ float3 tiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
-    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);  // c.t: the thread's first cell
     float3* tile = tiles[c.sub];
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     const int scale = 1 << c.level;
-    const uint32_t cell = c.b * BV + c.t;
-    bool leaf = false;
+    bool leaf[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; j++) leaf[j] = false;
     if (c.valid) {
       const int3 p0 = g.pos[c.b];
-      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
-      g.pre[cell] = 0.f;
-      g.tpre[cell] = 0.f;
-      g.div[cell] = 0.f;
-      const int ac = apronIndexOfLocal<BW>(c.t);
-      tile[ac] = g.vel[cell];
-      if (FLUIDITY) ft[ac] = g.flu[cell];
-      for (int fc = c.t; fc < FACE; fc += BV) {
+#pragma unroll
+      for (int j = 0; j < CPT; j++) {
+        const int t = c.t + j * N;
+        const uint32_t cell = c.b * BV + t;
+        leaf[j] = g.chi[8 * c.b + (t >> SBB)] == NFU;
+        g.pre[cell] = 0.f;
+        g.tpre[cell] = 0.f;
+        g.div[cell] = 0.f;
+        const int ac = apronIndexOfLocal<BW>(t);
+        tile[ac] = g.vel[cell];
+        if (FLUIDITY) ft[ac] = g.flu[cell];
+      }
+      for (int fc = c.t; fc < FACE; fc += N) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
         const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
@@ -219,41 +235,55 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_divergence(G g, dcg_params P) {
       }
     }
     __syncthreads();
-    if (!leaf) continue;
     const float alpha = .5f * P.rdx / (float)scale;
-    const int a = apronIndexOfLocal<BW>(c.t);
-    float w[6];
-    tileFluidity<BW, FLUIDITY>(ft, a, w);
-    const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
-                 vf = tile[a + 1];
-    g.div[cell] = alpha * (w[1] * vr.x - w[0] * vl.x + w[3] * vu.y - w[2] * vd.y + w[5] * vf.z - w[4] * vb.z);
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      if (!leaf[j]) continue;
+      const int t = c.t + j * N;
+      const int a = apronIndexOfLocal<BW>(t);
+      float w[6];
+      tileFluidity<BW, FLUIDITY>(ft, a, w);
+      const float3 vl = tile[a - A * A], vr = tile[a + A * A], vd = tile[a - A], vu = tile[a + A], vb = tile[a - 1],
+                   vf = tile[a + 1];
+      g.div[c.b * BV + t] =
+          alpha * (w[1] * vr.x - w[0] * vl.x + w[3] * vu.y - w[2] * vd.y + w[5] * vf.z - w[4] * vb.z);
+    }
   }
 }
 
 // ------------------------------------------------------------------ gradient subtraction
 // k_dcgrid_apply_pressure (dcgrid_fluid.cu:232-259), leaf cells only.
-template <int BW, bool FLUIDITY, class G>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(G g, dcg_params P) {
+template <int BW, bool FLUIDITY, int CPT, class G>
+__global__ void __launch_bounds__((CptGeo<BW, CPT>::WG)) k_apply_pressure(G g, dcg_params P) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
-                SBB = Geo<BW>::SBB;
+                SBB = Geo<BW>::SBB, N = BV / CPT;
   __shared__ float tiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles are no longer read
-    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);  // c.t: the thread's first cell
     float* tile = tiles[c.sub];
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
-    const uint32_t cell = c.b * BV + c.t;
-    bool leaf = false;
-    float3 v = make_float3(0.f, 0.f, 0.f);
+    bool leaf[CPT];
+    float3 v[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      leaf[j] = false;
+      v[j] = make_float3(0.f, 0.f, 0.f);
+    }
     if (c.valid) {
-      leaf = g.chi[8 * c.b + (c.t >> SBB)] == NFU;
-      if (leaf) v = g.vel[cell];  // only this thread reads and writes its cell's velocity
-      const int ac = apronIndexOfLocal<BW>(c.t);
-      tile[ac] = g.pre[cell];
-      if (FLUIDITY) ft[ac] = g.flu[cell];
-      for (int fc = c.t; fc < FACE; fc += BV) {
+#pragma unroll
+      for (int j = 0; j < CPT; j++) {
+        const int t = c.t + j * N;
+        const uint32_t cell = c.b * BV + t;
+        leaf[j] = g.chi[8 * c.b + (t >> SBB)] == NFU;
+        if (leaf[j]) v[j] = g.vel[cell];  // only this thread reads and writes its cell's velocity
+        const int ac = apronIndexOfLocal<BW>(t);
+        tile[ac] = g.pre[cell];
+        if (FLUIDITY) ft[ac] = g.flu[cell];
+      }
+      for (int fc = c.t; fc < FACE; fc += N) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
         const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
@@ -263,16 +293,21 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_apply_pressure(G g, dcg_params 
       }
     }
     __syncthreads();
-    if (!leaf) continue;
     const float alpha = .5f * P.rdx / (float)(1 << c.level);
-    const int a = apronIndexOfLocal<BW>(c.t);
-    float w[6];
-    tileFluidity<BW, FLUIDITY>(ft, a, w);
-    const float p = tile[a];
-    v.x = v.x - alpha * (w[1] * (tile[a + A * A] - p) + w[0] * (p - tile[a - A * A]));
-    v.y = v.y - alpha * (w[3] * (tile[a + A] - p) + w[2] * (p - tile[a - A]));
-    v.z = v.z - alpha * (w[5] * (tile[a + 1] - p) + w[4] * (p - tile[a - 1]));
-    g.vel[cell] = v;
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      if (!leaf[j]) continue;
+      const int t = c.t + j * N;
+      const int a = apronIndexOfLocal<BW>(t);
+      float w[6];
+      tileFluidity<BW, FLUIDITY>(ft, a, w);
+      const float p = tile[a];
+      float3 u = v[j];
+      u.x = u.x - alpha * (w[1] * (tile[a + A * A] - p) + w[0] * (p - tile[a - A * A]));
+      u.y = u.y - alpha * (w[3] * (tile[a + A] - p) + w[2] * (p - tile[a - A]));
+      u.z = u.z - alpha * (w[5] * (tile[a + 1] - p) + w[4] * (p - tile[a - 1]));
+      g.vel[c.b * BV + t] = u;
+    }
   }
 }
 

@@ -104,10 +104,10 @@ __global__ void __launch_bounds__(256) k_sub_scores_geo(DGBase g, float* __restr
 // uses the stored vorticity, and the subblock maximum is an LDS reduction. Computed once per
 // adaptation: the fields and the (pre-refresh) apron do not change between the score pass before
 // the moves and the one before refinement, so both passes see the same feature values.
-template <int BW, bool FLUIDITY, class G>
-__global__ void __launch_bounds__(Geo<BW>::WG) k_features(G g, dcg_params P, float* __restrict__ feat) {
+template <int BW, bool FLUIDITY, int CPT, class G>
+__global__ void __launch_bounds__((CptGeo<BW, CPT>::WG), (CPT == 2 ? 8 : 1)) k_features(G g, dcg_params P, float* __restrict__ feat) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE,
-                SBB = Geo<BW>::SBB;
+                SBB = Geo<BW>::SBB, N = BV / CPT;
   __shared__ float3 vtiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   __shared__ float dtiles[BPW][A3];
@@ -115,19 +115,24 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_features(G g, dcg_params P, flo
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();  // the previous block's tiles and maxima are no longer read
-    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);
-    const int sub = c.sub, t = c.t;
-    const uint32_t b = c.b, cell = b * BV + t;
+    const BlockCtx<BW> c = slotCtx<BW>(g, -1, base, total);  // c.t: the thread's first cell
+    const int sub = c.sub;
+    const uint32_t b = c.b;
     float3* vt = vtiles[sub];
     float* ft = FLUIDITY ? ftiles[sub] : nullptr;
     float* dt_ = dtiles[sub];
-    if (t < 8) smax[sub][t] = 0u;
+    if (c.t < 8) smax[sub][c.t] = 0u;
     if (c.valid) {
-      const int ac = apronIndexOfLocal<BW>(t);
-      vt[ac] = g.vel[cell];
-      if (FLUIDITY) ft[ac] = g.flu[cell];
-      dt_[ac] = g.den[cell];
-      for (int fc = t; fc < FACE; fc += BV) {
+#pragma unroll
+      for (int j = 0; j < CPT; j++) {
+        const int t = c.t + j * N;
+        const uint32_t cell = b * BV + t;
+        const int ac = apronIndexOfLocal<BW>(t);
+        vt[ac] = g.vel[cell];
+        if (FLUIDITY) ft[ac] = g.flu[cell];
+        dt_[ac] = g.den[cell];
+      }
+      for (int fc = c.t; fc < FACE; fc += N) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
         const uint32_t src = apronCell<BW>(g, b, ai, aj, ak);
@@ -138,33 +143,38 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_features(G g, dcg_params P, flo
       }
     }
     __syncthreads();
-    float cs = 0.f;
-    if (c.valid) {
-      const float alpha = .5f * P.rdx / (float)(1 << c.level);
-      const int a = apronIndexOfLocal<BW>(t);
-      // vorticity (k_vorticity)
-      float w[6];
-      tileFluidity<BW, FLUIDITY>(ft, a, w);
-      const float3 vl = vt[a - A * A], vr = vt[a + A * A], vd = vt[a - A], vu = vt[a + A], vb = vt[a - 1],
-                   vf = vt[a + 1];
-      const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
-      g.vort[cell] = make_float3(alpha * ((wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y)),
-                                 alpha * ((wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z)),
-                                 alpha * ((wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)));
-      const float3 om = g.vort[cell];  // the stored value (rounded as stored)
-      // density gradient
-      const float gxv = alpha * (dt_[a + A * A] - dt_[a - A * A]);
-      const float gyv = alpha * (dt_[a + A] - dt_[a - A]);
-      const float gzv = alpha * (dt_[a + 1] - dt_[a - 1]);
-      const float gm = __builtin_sqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
-      const float wm = __builtin_sqrtf(om.x * om.x + om.y * om.y + om.z * om.z);
-      const float fg = gm / P.grad_threshold, fw = wm / P.vort_threshold;
-      cs = fg > fw ? fg : fw;
+    const float alpha = .5f * P.rdx / (float)(1 << c.level);
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+      const int t = c.t + j * N;
+      const uint32_t cell = b * BV + t;
+      float cs = 0.f;
+      if (c.valid) {
+        const int a = apronIndexOfLocal<BW>(t);
+        // vorticity (k_vorticity)
+        float w[6];
+        tileFluidity<BW, FLUIDITY>(ft, a, w);
+        const float3 vl = vt[a - A * A], vr = vt[a + A * A], vd = vt[a - A], vu = vt[a + A], vb = vt[a - 1],
+                     vf = vt[a + 1];
+        const float wl = w[0], wr = w[1], wd = w[2], wu = w[3], wb = w[4], wf = w[5];
+        g.vort[cell] = make_float3(alpha * ((wu * vu.z - wd * vd.z) - (wf * vf.y - wb * vb.y)),
+                                   alpha * ((wf * vf.x - wb * vb.x) - (wr * vr.z - wl * vl.z)),
+                                   alpha * ((wr * vr.y - wl * vl.y) - (wu * vu.x - wd * vd.x)));
+        const float3 om = g.vort[cell];  // the stored value (rounded as stored)
+        // density gradient
+        const float gxv = alpha * (dt_[a + A * A] - dt_[a - A * A]);
+        const float gyv = alpha * (dt_[a + A] - dt_[a - A]);
+        const float gzv = alpha * (dt_[a + 1] - dt_[a - 1]);
+        const float gm = __builtin_sqrtf(gxv * gxv + gyv * gyv + gzv * gzv);
+        const float wm = __builtin_sqrtf(om.x * om.x + om.y * om.y + om.z * om.z);
+        const float fg = gm / P.grad_threshold, fw = wm / P.vort_threshold;
+        cs = fg > fw ? fg : fw;
+      }
+      // scores are >= 0, so their bit patterns order like their values
+      atomicMax(&smax[sub][t >> SBB], __float_as_uint(cs));
     }
-    // scores are >= 0, so their bit patterns order like their values
-    atomicMax(&smax[sub][t >> SBB], __float_as_uint(cs));
     __syncthreads();
-    if (t < 8 && c.valid) feat[8 * b + t] = __uint_as_float(smax[sub][t]);
+    if (c.t < 8 && c.valid) feat[8 * b + c.t] = __uint_as_float(smax[sub][c.t]);
   }
 }
 

@@ -78,6 +78,9 @@ def dc_workload(name):
     if name == "cfg1_256_b12000":  # a sparse level of 7392 blocks and a dense one of 4096: two per iteration
         w = scenes.config(1)
         return w.kind, w.size, w.params, 12000, True
+    if name == "cfg1_256_b20000":  # a budget of >= 16384 blocks: the two-cell stencils, no fluidity
+        w = scenes.config(1)
+        return w.kind, w.size, w.params, 20000, True
     if name == "cfg4_128_sparse":  # the same scene with sparse finer levels (moves, refinement)
         w = scenes.config(4, scale=8)
         w.params.score_mode = scenes.SCORE_GEOMETRIC
@@ -110,13 +113,15 @@ DC_CASES = ["ref64_bw4", "ref64_bw4_solids", "ref64_bw8", "cfg1_64", "cfg1_128",
             "cfg4_128_sparse"]
 
 
-@pytest.mark.parametrize("name", ["cfg4_256", "cfg4_512", "cfg1_256_b12000"])
+@pytest.mark.parametrize("name", ["cfg4_256", "cfg4_512", "cfg1_256_b12000", "cfg1_256_b20000"])
 def test_dcgrid_large_level_steps_bitwise(name):
     """Levels with at least two (sixteen) rounds of the launch grid take the Jacobi variant with
     two (four) blocks per iteration: config 4 at 256x64x256 has 8192 dense blocks at level 0, at
     512x128x512 65536 at level 0 (four) and 8192 at level 1; the config 1 scene with a 12000-block
-    budget has a sparse level 0 of 7392 blocks and a dense level 1 of 4096. Two steps bit-exact
-    against the FP32 oracle."""
+    budget has a sparse level 0 of 7392 blocks and a dense level 1 of 4096. Budgets of 16384
+    blocks and more (config 4 at 512x128x512 with fluidity, config 1 with 20000 blocks without)
+    run the divergence, gradient and feature kernels with two cells per thread. Two steps
+    bit-exact against the FP32 oracle."""
     g, c = pair(name)
     for step in range(2):
         say(f"[case] {name} step {step}")
