@@ -6,9 +6,11 @@
 // (src/dcgrid/dcgrid_adaptation.cu:92-143).
 //
 // Stencil kernels run one workgroup per block (BW=8: 512 threads) or per four blocks (BW=4:
-// 4x64 threads), one thread per cell. The block's cells are contiguous in HBM (coalesced) and
-// are staged with the six face layers of the apron in LDS, so each neighbour read is an LDS
-// read; face cells are fetched through the block's neighbour table.
+// 4x64 threads), one thread per cell; the advection (always) and the divergence, gradient and
+// feature kernels (large budgets) run BW-8 blocks with 256 threads of two cells each. The block's
+// cells are contiguous in HBM (coalesced) and are staged with the six face layers of the apron
+// in LDS, so each neighbour read is an LDS read; face cells are fetched through the block's
+// neighbour table.
 #pragma once
 
 #include "dcgrid_dev.hpp"
