@@ -194,6 +194,26 @@ __device__ __forceinline__ uint32_t apronCell(const DGBase& g, uint32_t b, int a
   return ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> sh, (aj - 1 - d.z) >> sh, (ak - 1 - d.w) >> sh);
 }
 
+// apronCell for a border entry (at least one of ai, aj, ak is 0 or BW+1), without branches: the
+// descriptor is read with one load and both forms are computed and selected, so a caller can
+// issue this load together with its others (same cell as apronCell).
+template <int BW>
+__device__ __forceinline__ uint32_t borderApronCell(const DGBase& g, uint32_t b, int ai, int aj, int ak) {
+  constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
+  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
+  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
+  const int4 d = g.nd[b * 27 + dirIndex(dx, dy, dz)];
+  const uint32_t sh = ndShift(d);
+  const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
+  const uint32_t spread = loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
+  const uint32_t clamped = loc<BB>(ci - 1, cj - 1, ck - 1);
+  const uint32_t outCell = b * BV + (d.y ? spread : clamped);
+  const uint32_t s = sh == ND_OUT ? 0u : sh;
+  const uint32_t inCell = ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> s, (aj - 1 - d.z) >> s, (ak - 1 - d.w) >> s);
+  return sh == ND_OUT ? outCell : inCell;
+}
+
 // Apron cell of a block on a dense level (l >= S). Every block of such a level is allocated at
 // the dense index (bx*ry + by)*rz + bz and never moves, so its neighbour descriptors are fixed:
 // the same-level neighbour at the dense index, or the clamped own cell outside the domain (the
@@ -221,17 +241,14 @@ __device__ __forceinline__ int3 denseOrigin(const DGBase& g, int l, uint32_t b) 
   return make_int3(BW * (int)(q / ry), BW * (int)(q % ry), BW * (int)(k - q * rz));
 }
 // Apron coordinates of face cell f in [0, 6*BW*BW): faces x-,x+,y-,y+,z-,z+.
+// Written with selects, not a switch: no branch between the loads a caller issues around it.
 template <int BW>
 __device__ __forceinline__ void faceCell(int f, int& ai, int& aj, int& ak) {
   const int face = f / (BW * BW), uv = f % (BW * BW), u = 1 + uv / BW, v = 1 + uv % BW;
-  switch (face) {
-    case 0: ai = 0; aj = u; ak = v; break;
-    case 1: ai = BW + 1; aj = u; ak = v; break;
-    case 2: ai = u; aj = 0; ak = v; break;
-    case 3: ai = u; aj = BW + 1; ak = v; break;
-    case 4: ai = u; aj = v; ak = 0; break;
-    default: ai = u; aj = v; ak = BW + 1; break;
-  }
+  const int axis = face >> 1, side = (face & 1) ? BW + 1 : 0;
+  ai = axis == 0 ? side : u;
+  aj = axis == 0 ? u : (axis == 1 ? side : v);
+  ak = axis == 2 ? side : v;
 }
 
 }  // namespace dcg

@@ -160,35 +160,76 @@ struct CptGeo {
 template <int BW, int NB, bool DENSE, class G>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_jacobi(G g, int level, const typename G::Scalar* __restrict__ in,
                                                          typename G::Scalar* __restrict__ out, float alpha) {
-  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, FACE = Geo<BW>::FACE;
+  constexpr int FPT = (FACE + BV - 1) / BV;  // face cells per thread
   __shared__ float tiles[NB][BPW][A3];
   const uint32_t total = slotTotal(g, level);
   const uint32_t stride = slotStride<BW>();
+  const uint32_t off = g.offs[level];
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  const int ac = apronIndexOfLocal<BW>(t);
   for (uint32_t base = firstSlot<BW>(); base < total; base += NB * stride) {
     __syncthreads();  // the previous blocks' tiles are no longer read
-    BlockCtx<BW> c[NB];
-    float dv[NB];
+    // Every load of the iteration is issued before the first is used, with no branch between
+    // them: slots past the level's end read its last block (in bounds) and store nothing.
+    uint32_t b[NB];
+    bool valid[NB];
 #pragma unroll
     for (int k = 0; k < NB; k++) {
-      c[k] = slotCtx<BW>(g, level, base + k * stride, total);
-      dv[k] = c[k].valid ? (float)g.div[c[k].b * BV + c[k].t] : 0.f;
+      const uint32_t slot = base + k * stride + sub;
+      valid[k] = slot < total;
+      b[k] = off + (valid[k] ? slot : total - 1);
     }
+    // face cells: on a dense level from the block index (integer arithmetic, before any load),
+    // else through the neighbour descriptor (its load issued with the block's)
+    uint32_t src[NB][FPT];
+    auto faceSources = [&]() {
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        const int3 p0 = DENSE ? denseOrigin<BW>(g, level, b[k]) : make_int3(0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < FPT; j++) {
+          const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;  // lanes without one repeat the last
+          int ai, aj, ak;
+          faceCell<BW>(fc, ai, aj, ak);
+          src[k][j] =
+              DENSE ? denseApronCell<BW>(g, level, b[k], p0, ai, aj, ak) : borderApronCell<BW>(g, b[k], ai, aj, ak);
+        }
+      }
+    };
+    if (DENSE) faceSources();
+    float dv[NB], pc[NB], fv[NB][FPT];
 #pragma unroll
     for (int k = 0; k < NB; k++) {
-      if (!c[k].valid) continue;
-      if (DENSE)
-        stageScalarDense<BW>(g, level, in, tiles[k][c[k].sub], c[k].b, c[k].t);
-      else
-        stageScalar<BW>(g, in, tiles[k][c[k].sub], c[k].b, c[k].t);
+      dv[k] = (float)g.div[b[k] * BV + t];
+      pc[k] = (float)in[b[k] * BV + t];
+    }
+    if (!DENSE) faceSources();
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+#pragma unroll
+      for (int j = 0; j < FPT; j++) fv[k][j] = (float)in[src[k][j]];
+    // keep the LDS stores below the loads (a store hoisted above would wait for them)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      float* tile = tiles[k][sub];
+      tile[ac] = pc[k];
+#pragma unroll
+      for (int j = 0; j < FPT; j++) {  // lanes without a face cell store the last one's value again
+        const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
+        int ai, aj, ak;
+        faceCell<BW>(fc, ai, aj, ak);
+        tile[(ai * A + aj) * A + ak] = fv[k][j];
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NB; k++) {
-      if (!c[k].valid) continue;
-      const float* tile = tiles[k][c[k].sub];
-      const int a = apronIndexOfLocal<BW>(c[k].t);
-      out[c[k].b * BV + c[k].t] =
-          (tile[a - A * A] + tile[a + A * A] + tile[a - A] + tile[a + A] + tile[a - 1] + tile[a + 1] - alpha * dv[k]) /
+      if (!valid[k]) continue;
+      const float* tile = tiles[k][sub];
+      out[b[k] * BV + t] =
+          (tile[ac - A * A] + tile[ac + A * A] + tile[ac - A] + tile[ac + A] + tile[ac - 1] + tile[ac + 1] - alpha * dv[k]) /
           6.f;
     }
   }
