@@ -103,7 +103,7 @@ __device__ __forceinline__ void stageScalar(const DGBase& g, const SF* __restric
   for (int fc = t; fc < FACE; fc += BV) {
     int ai, aj, ak;
     faceCell<BW>(fc, ai, aj, ak);
-    tile[(ai * A + aj) * A + ak] = f[apronCell<BW>(g, b, ai, aj, ak)];
+    tile[(ai * A + aj) * A + ak] = f[borderApronCell<BW>(g, b, ai, aj, ak)];
   }
 }
 // stageScalar on a dense level (l >= S): the faces come from denseApronCell, the block origin
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__((CptGeo<BW, CPT>::WG)) k_divergence(G g, dcg_p
       for (int fc = c.t; fc < FACE; fc += N) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
+        const uint32_t src = borderApronCell<BW>(g, c.b, ai, aj, ak);
         const int a = (ai * A + aj) * A + ak;
         tile[a] = velocityBC(P, (float3)g.vel[src], p0.x + ai - 1, p0.y + aj - 1, p0.z + ak - 1, scale);
         if (FLUIDITY) ft[a] = g.flu[src];
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__((CptGeo<BW, CPT>::WG)) k_apply_pressure(G g, d
       for (int fc = c.t; fc < FACE; fc += N) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
+        const uint32_t src = borderApronCell<BW>(g, c.b, ai, aj, ak);
         const int a = (ai * A + aj) * A + ak;
         tile[a] = g.pre[src];
         if (FLUIDITY) ft[a] = g.flu[src];
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_vorticity(G g, dcg_params P) {
       for (int fc = c.t; fc < FACE; fc += BV) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        const uint32_t src = apronCell<BW>(g, c.b, ai, aj, ak);
+        const uint32_t src = borderApronCell<BW>(g, c.b, ai, aj, ak);
         const int a = (ai * A + aj) * A + ak;
         tile[a] = g.vel[src];
         if (FLUIDITY) ft[a] = g.flu[src];
@@ -726,7 +726,7 @@ __device__ __forceinline__ void forApron(const DGBase& g, uint32_t b, int t, Op 
   for (int a = t; a < A3; a += N) {
     const int ai = a / (A * A), aj = (a / A) % A, ak = a % A;
     if (ai > 0 && ai <= BW && aj > 0 && aj <= BW && ak > 0 && ak <= BW) continue;
-    op(a, apronCell<BW>(g, b, ai, aj, ak));
+    op(a, borderApronCell<BW>(g, b, ai, aj, ak));
   }
 }
 

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__((CptGeo<BW, CPT>::WG), (CPT == 2 ? 8 : 1)) k_f
       for (int fc = c.t; fc < FACE; fc += N) {
         int ai, aj, ak;
         faceCell<BW>(fc, ai, aj, ak);
-        const uint32_t src = apronCell<BW>(g, b, ai, aj, ak);
+        const uint32_t src = borderApronCell<BW>(g, b, ai, aj, ak);
         const int a = (ai * A + aj) * A + ak;
         vt[a] = g.vel[src];
         if (FLUIDITY) ft[a] = g.flu[src];
