@@ -14,7 +14,9 @@ import os
 from dcgrid_amd._abi import Handle, Info, Params, RenderParams, bind
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# DCG_ORACLE_LIB: another build of the oracle (e.g. one with a deliberately planted mistake, to
+# show that a pin catches it)
+LIB_PATH = os.environ.get("DCG_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 _fns = None
 
@@ -42,6 +44,10 @@ def library():
         lib.orc_lookup.restype = C.c_int64
         lib.orc_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]
         lib.orc_render.restype = C.c_int
+        lib.orc_scores.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_scores.restype = C.c_int
+        lib.orc_import_tables.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_import_tables.restype = C.c_int
         lib.orc_plan.argtypes = [C.POINTER(C.c_int32), C.c_uint64, C.POINTER(Params), C.POINTER(Info)]
         lib.orc_plan.restype = C.c_int
         for name, args, res in [
@@ -105,6 +111,40 @@ class OracleSim(Handle):
     def lookup(self, x, y, z, level=0):
         """getBlockIndexDeep on the oracle's hash tables (-1 when nothing contains the cell)."""
         return int(_lib.orc_lookup(self._h, x, y, z, level))
+
+    def scores(self):
+        """(subblock scores [8B + 1], block scores [B]) the next adapt_topology starts from."""
+        import numpy as np
+        B = int(self.info().max_num_blocks)
+        sub = np.empty(8 * B + 1, np.float32)
+        blk = np.empty(B, np.float32)
+        self._check(_lib.orc_scores(self._h, C.c_void_p(sub.ctypes.data), C.c_void_p(blk.ctypes.data)), "scores")
+        return sub, blk
+
+    def import_tables(self, tables, loads, move_limits):
+        """Replace the block tables (the dict export_tables returns), loads and move limits."""
+        import numpy as np
+        from dcgrid_amd._abi import Tables, _ptr
+        arr = {
+            "block_positions": np.ascontiguousarray(tables["block_positions"], np.int32),
+            "block_levels": np.ascontiguousarray(tables["block_levels"], np.uint8),
+            "block_flags": np.ascontiguousarray(tables["block_flags"], np.uint8),
+            "parent_indices": np.ascontiguousarray(tables["parent_indices"], np.uint64),
+            "child_indices": np.ascontiguousarray(tables["child_indices"], np.uint64),
+            "cell_indices": np.ascontiguousarray(tables["cell_indices"], np.uint64),
+            "hash_key": np.ascontiguousarray(tables["hash_key"], np.uint32),
+            "hash_val": np.ascontiguousarray(tables["hash_val"], np.uint64),
+        }
+        t = Tables(_ptr(arr["block_positions"], C.c_int32), _ptr(arr["block_levels"], C.c_uint8),
+                   _ptr(arr["block_flags"], C.c_uint8), _ptr(arr["parent_indices"], C.c_uint64),
+                   _ptr(arr["child_indices"], C.c_uint64), _ptr(arr["cell_indices"], C.c_uint64),
+                   _ptr(arr["hash_key"], C.c_uint32), _ptr(arr["hash_val"], C.c_uint64))
+        ld = np.zeros(16, np.uint64)
+        ml = np.zeros(16, np.uint64)
+        ld[:len(loads)] = loads
+        ml[:len(move_limits)] = move_limits
+        st = _lib.orc_import_tables(self._h, C.byref(t), C.c_void_p(ld.ctypes.data), C.c_void_p(ml.ctypes.data))
+        self._check(st, "import_tables")
 
     def export_field_f64(self, field):
         import numpy as np

@@ -60,6 +60,12 @@ int orc_tape_backward(orc_sim* s, const double* gden, const double* gvel, const 
 int orc_run_stage(orc_sim* s, int32_t stage, int32_t arg);
 /* getBlockIndexDeep (dcgrid_utils.cuh:201-232) on the oracle's hash tables; -1 if none. */
 int64_t orc_lookup(orc_sim* s, int32_t x, int32_t y, int32_t z, int32_t level);
+/* Test hooks (DCGrid). orc_scores: the subblock scores (8B + 1 floats, the last one the
+ * -FLT_MAX pad the block score's ninth term reads) and block scores (B floats) that the next
+ * adaptTopology starts from (dcgrid_adaptation.cu:10-63). orc_import_tables: replace the block
+ * tables (reference semantics, every member required), the per-level loads and move limits. */
+int orc_scores(orc_sim* s, float* sub, float* blk);
+int orc_import_tables(orc_sim* s, const dcg_tables* t, const uint64_t* loads, const uint64_t* move_limits);
 /* Level/budget planner of the DCGrid constructor only (no allocation). */
 int orc_plan(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p, dcg_info* out);
 /* Number of OpenMP threads used by the oracle (0 = runtime default). */

@@ -184,6 +184,16 @@ int64_t orc_lookup(orc_sim* s, int32_t x, int32_t y, int32_t z, int32_t level) {
   if (!s) return -1;
   return s->impl->lookup(x, y, z, level);
 }
+int orc_scores(orc_sim* s, float* sub, float* blk) {
+  IMPL(s);
+  if (!sub || !blk) return DCG_E_ARG;
+  return I->scores(sub, blk);
+}
+int orc_import_tables(orc_sim* s, const dcg_tables* t, const uint64_t* loads, const uint64_t* move_limits) {
+  IMPL(s);
+  if (!t) return DCG_E_ARG;
+  return I->importTables(t, loads, move_limits);
+}
 int orc_plan(const int32_t size[3], uint64_t max_num_blocks, const dcg_params* p, dcg_info* out) {
   if (!size || !p || !out) return DCG_E_ARG;
   return orc::planDCGrid(size, max_num_blocks, *p, out, &g_create_error);

@@ -305,6 +305,12 @@ struct Sim {
   // getBlockIndexDeep (DCGrid only): finest allocated block at level >= `level` containing the
   // level-local cell (x,y,z); -1 when not applicable.
   virtual int64_t lookup(int x, int y, int z, int level) { return -1; }
+  // Test hooks (DCGrid only): the scores the next adaptation starts from; replacing the block
+  // tables, loads and move limits.
+  virtual int scores(float* sub, float* blk) { return fail(DCG_E_UNSUPPORTED, "scores: DCGrid only"); }
+  virtual int importTables(const dcg_tables* t, const uint64_t* loads, const uint64_t* moveLimits) {
+    return fail(DCG_E_UNSUPPORTED, "import_tables: DCGrid only");
+  }
   // Projection with the level systems solved to solve_tol (extension; DCGrid only).
   virtual int projectSolve() { return fail(DCG_E_UNSUPPORTED, "the projection solve is implemented for the DCGrid"); }
   virtual int solveReport(dcg_solve_info* out) {

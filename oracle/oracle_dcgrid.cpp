@@ -1322,6 +1322,39 @@ struct DCGrid : Sim {
     const size_t b = getBlockIndexDeep(x, y, z, l);
     return b == NF ? -1 : (int64_t)b;
   }
+  // Test hook: the subblock and block scores the next adaptTopology starts from (its first two
+  // kernels, dcgrid_adaptation.cu:10-63, after the vorticity pass in feature mode).
+  int scores(float* sub, float* blk) override {
+    if (P.score_mode == DCG_SCORE_FEATURES) calcVorticity();
+    calcSubblockScores();
+    accumulateSubblockScores();
+    std::copy(sbs.begin(), sbs.begin() + 8 * B + 1, sub);
+    std::copy(bs.begin(), bs.begin() + B, blk);
+    return DCG_OK;
+  }
+  // Test hook: replace the block tables (reference semantics, as exportTables writes them), the
+  // level loads and the move limits, e.g. with a state written by another implementation.
+  int importTables(const dcg_tables* t, const uint64_t* ld, const uint64_t* ml) override {
+    if (!t->block_positions || !t->block_levels || !t->block_flags || !t->parent_indices ||
+        !t->child_indices || !t->cell_indices || !t->hash_key || !t->hash_val || !ld || !ml)
+      return fail(DCG_E_ARG, "import_tables: every table is required");
+    auto sz = [](uint64_t v) { return v == UINT64_MAX ? NF : (size_t)v; };
+    for (int l = 0; l < L; l++)
+      if (ld[l] > maxL[l]) return fail(DCG_E_ARG, "import_tables: load above the level's capacity");
+    std::copy(t->block_positions, t->block_positions + 3 * B, pos.begin());
+    std::copy(t->block_levels, t->block_levels + B, lev.begin());
+    std::copy(t->block_flags, t->block_flags + B, flg.begin());
+    for (size_t i = 0; i < B; i++) par[i] = sz(t->parent_indices[i]);
+    for (size_t i = 0; i < 8 * B; i++) chi[i] = sz(t->child_indices[i]);
+    for (size_t i = 0; i < cidx.size(); i++) cidx[i] = sz(t->cell_indices[i]);
+    std::copy(t->hash_key, t->hash_key + H, hkey.begin());
+    for (size_t i = 0; i < H; i++) hval[i] = sz(t->hash_val[i]);
+    for (int l = 0; l < L; l++) {
+      loads[l] = (size_t)ld[l];
+      moveLimit[l] = (size_t)ml[l];
+    }
+    return DCG_OK;
+  }
   int exportTables(dcg_tables* t) override {
     auto u64 = [](size_t v) { return v == NF ? UINT64_MAX : (uint64_t)v; };
     if (t->block_positions) std::copy(pos.begin(), pos.end(), t->block_positions);
