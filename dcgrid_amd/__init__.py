@@ -67,6 +67,8 @@ def library():
             f = getattr(lib, name)
             f.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]
             f.restype = C.c_int
+        lib.dcg_state_check.argtypes = [C.c_void_p, C.c_uint64]
+        lib.dcg_state_check.restype = C.c_int
         lib.dcg_state_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         lib.dcg_state_size.restype = C.c_int
         for name in ("dcg_save_state", "dcg_load_state"):
@@ -75,6 +77,18 @@ def library():
             f.restype = C.c_int
         _lib, _fns = lib, fns
     return _lib
+
+
+def state_check(buf):
+    """Check a DCGSTATE buffer (bytes-like or numpy uint8) without a simulation or a device:
+    returns (status, message); status 0 = the tables are consistent with the grid the header
+    describes (dcg_state_check)."""
+    import numpy as np
+    library()
+    a = np.ascontiguousarray(np.frombuffer(buf, dtype=np.uint8) if not isinstance(buf, np.ndarray) else buf)
+    st = _lib.dcg_state_check(C.c_void_p(a.ctypes.data), a.size)
+    msg = _lib.dcg_last_error(None)
+    return st, (msg.decode() if (st and msg) else "")
 
 
 def default_params():

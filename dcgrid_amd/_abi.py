@@ -120,7 +120,8 @@ class Params(C.Structure):
         ("wind", C.c_float * 3),
         ("adiabatic_rate", C.c_float),
         ("storage", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("solve_cold", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
     def copy(self):

@@ -32,6 +32,7 @@ SimBase::~SimBase() {
   }
   for (auto e : eventPool) hipEventDestroy(e);
   for (void* p : allocations) hipFree(p);
+  for (void* p : hostAllocations) (void)hipHostFree(p);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -463,7 +464,85 @@ int stateLayout(dcg::SimBase* I, StateHeader& h, std::vector<StateEntry>& ents,
   total = off;
   return DCG_OK;
 }
+
+// A state buffer checked on its own: header, section table inside the buffer and, for a DCGrid,
+// the block tables against the plan of the grid the header describes (every index a kernel
+// follows from them stays inside its array). Nothing here needs a device.
+int checkStateBuffer(const char* in, uint64_t bytes, std::string& err) {
+  StateHeader fh;
+  if (bytes < sizeof(fh)) {
+    err = "state truncated (header)";
+    return DCG_E_ARG;
+  }
+  std::memcpy(&fh, in, sizeof(fh));
+  if (std::memcmp(fh.magic, kStateMagic, 8) != 0) {
+    err = "not a DCGSTATE buffer";
+    return DCG_E_ARG;
+  }
+  if (fh.version != DCG_STATE_VERSION) {
+    err = "state version " + std::to_string(fh.version) + " (this library reads " + std::to_string(DCG_STATE_VERSION) + ")";
+    return DCG_E_UNSUPPORTED;
+  }
+  if (fh.paramsBytes != sizeof(dcg_params) || fh.sectionCount > 64 ||
+      fh.headerBytes != sizeof(StateHeader) + sizeof(dcg_params) + (uint64_t)fh.sectionCount * sizeof(StateEntry)) {
+    err = "state header does not match this library's parameter layout";
+    return DCG_E_ARG;
+  }
+  if (bytes < fh.headerBytes) {
+    err = "state truncated (section table)";
+    return DCG_E_ARG;
+  }
+  std::vector<StateEntry> fe(fh.sectionCount);
+  std::memcpy(fe.data(), in + sizeof(fh) + sizeof(dcg_params), fe.size() * sizeof(StateEntry));
+  for (const auto& e : fe)
+    if (e.offset < fh.headerBytes || e.offset > bytes || e.bytes > bytes - e.offset) {
+      err = "state truncated (section data)";
+      return DCG_E_ARG;
+    }
+  if (fh.kind != DCG_KIND_DCGRID) return DCG_OK;
+  dcg_params p;
+  std::memcpy(&p, in + sizeof(fh), sizeof(p));
+  dcg::DCGridPlan pl;
+  std::string perr;
+  if (dcg::planDCGrid(fh.blockWidth, fh.size, fh.maxNumBlocks, p, pl, perr) != DCG_OK) {
+    err = "state describes a grid the planner rejects: " + perr;
+    return DCG_E_ARG;
+  }
+  if (pl.L != fh.levels) {
+    err = "state level count does not match its grid";
+    return DCG_E_ARG;
+  }
+  const uint64_t B = pl.B;
+  struct Need {
+    uint32_t id;
+    uint64_t bytes;
+    const void* at;
+  } need[] = {{DCG_STATE_CONTROL, 16 * 4, nullptr},       {DCG_STATE_POSITIONS, 12 * B, nullptr},
+              {DCG_STATE_LEVELS, B, nullptr},             {DCG_STATE_PARENTS, 4 * B, nullptr},
+              {DCG_STATE_CHILDREN, 32 * B, nullptr},      {DCG_STATE_NEIGHBOURS, 27 * 16 * B, nullptr},
+              {DCG_STATE_HASH_VALUES, 4 * (uint64_t)pl.H, nullptr}};
+  for (auto& n : need) {
+    for (const auto& e : fe)
+      if (e.id == n.id && (n.id == DCG_STATE_CONTROL ? e.bytes >= n.bytes : e.bytes == n.bytes)) n.at = in + e.offset;
+    if (!n.at) {
+      err = "state lacks block table section " + std::to_string(n.id) + " of its grid's size";
+      return DCG_E_ARG;
+    }
+  }
+  dcg::StateTables t{(const uint32_t*)need[0].at, (const int32_t*)need[1].at, (const uint8_t*)need[2].at,
+                (const uint32_t*)need[3].at, (const uint32_t*)need[4].at, (const int32_t*)need[5].at,
+                (const uint32_t*)need[6].at};
+  return dcg::checkDCGridTables(pl, t, err);
+}
 }  // namespace
+
+int dcg_state_check(const void* src, uint64_t bytes) {
+  if (!src) return DCG_E_ARG;
+  std::string err;
+  const int st = checkStateBuffer((const char*)src, bytes, err);
+  g_create_error = st ? err : std::string();
+  return st;
+}
 
 int dcg_state_size(dcg_sim* s, uint64_t* bytes) {
   IMPL(s);
@@ -529,6 +608,11 @@ int dcg_load_state(dcg_sim* s, const void* src, uint64_t bytes) {
         fe[i].offset != ents[i].offset)
       return I->fail(DCG_E_ARG, "state section " + std::to_string(i) + " does not match this grid");
     if (fe[i].offset + fe[i].bytes > bytes) return I->fail(DCG_E_ARG, "state truncated (section data)");
+  }
+  {
+    std::string cerr;
+    const int cst = checkStateBuffer(in, bytes, cerr);
+    if (cst) return I->fail(cst, cerr);
   }
   dcg_params p;
   std::memcpy(&p, in + sizeof(fh), sizeof(p));

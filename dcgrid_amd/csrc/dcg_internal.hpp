@@ -23,6 +23,28 @@ namespace dcg {
     if (s_) return s_;    \
   } while (0)
 
+// Level and budget plan of a DCGrid (fluid_simulation_dcgrid.cu:9-140); no device needed.
+struct DCGridPlan {
+  int bw, L, S;
+  uint32_t B, H;
+  uint32_t maxL[DCG_MAX_LEVELS], fullL[DCG_MAX_LEVELS], offs[DCG_MAX_LEVELS];
+  uint32_t hsize[DCG_MAX_LEVELS], hoff[DCG_MAX_LEVELS];
+  int rx[DCG_MAX_LEVELS], ry[DCG_MAX_LEVELS], rz[DCG_MAX_LEVELS];
+};
+int planDCGrid(int bw, const int32_t size[3], uint64_t maxBlocks, const dcg_params& p, DCGridPlan& out,
+               std::string& err);
+// Host copies of the block tables of a DCGrid state (the sections of dcg_save_state).
+struct StateTables {
+  const uint32_t* ctl;  // control words, CTL_LOADS first
+  const int32_t* pos;   // 3 per block
+  const uint8_t* lev;
+  const uint32_t* par;
+  const uint32_t* chi;  // 8 per block
+  const int32_t* nd;    // 27 x 4 per block
+  const uint32_t* hval;
+};
+int checkDCGridTables(const DCGridPlan& pl, const StateTables& t, std::string& err);
+
 struct SimBase {
   dcg_params P{};
   std::string err;
@@ -150,6 +172,18 @@ struct SimBase {
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(DCG_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     allocations.push_back(q);
+    *p = (T*)q;
+    return DCG_OK;
+  }
+  // Pinned host allocation owned by the handle (hipHostMalloc: coherent, mapped into the device
+  // address space; freed in the destructor).
+  std::vector<void*> hostAllocations;
+  template <class T>
+  int allocHost(T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipHostMalloc(&q, count * sizeof(T) > 0 ? count * sizeof(T) : 16, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(DCG_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    hostAllocations.push_back(q);
     *p = (T*)q;
     return DCG_OK;
   }

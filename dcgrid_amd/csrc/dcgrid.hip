@@ -157,50 +157,21 @@ struct DCGridSim : SimBase {
     gx = size[0];
     gy = size[1];
     gz = size[2];
-    if (gx <= 0 || gy <= 0 || gz <= 0 || gx != P.gx || gy != P.gy || gz != P.gz)
-      return fail(DCG_E_ARG, "size must be positive and equal params gx,gy,gz");
-    const int minDim = std::min(gx, std::min(gy, gz));
-    int cellSize = 2;
-    L = 1;
-    while (gx % cellSize == 0 && gy % cellSize == 0 && gz % cellSize == 0 && cellSize * BW <= minDim) {
-      L++;
-      cellSize *= 2;
-    }
-    if (P.max_levels > 0 && L > P.max_levels) L = P.max_levels;
-    if (L > DCG_MAX_LEVELS) return fail(DCG_E_UNSUPPORTED, "too many levels");
-    const int top = BW << (L - 1);
-    if (gx % top || gy % top || gz % top)
-      return fail(DCG_E_UNSUPPORTED, "domain must be a multiple of the coarsest block extent");
-    uint64_t full64[DCG_MAX_LEVELS];
-    for (int l = 0, cs = 1; l < L; l++, cs *= 2)
-      full64[l] = (uint64_t)iDivUpH(gx, cs * BW) * iDivUpH(gy, cs * BW) * iDivUpH(gz, cs * BW);
-    if (maxBlocks < full64[L - 1]) return fail(DCG_E_BUDGET, "too few blocks to fill lowest resolution");
-    if (maxBlocks * (uint64_t)G::BV >= 0xFFFFFFFFull || maxBlocks > ND_BLOCK_MASK)
-      return fail(DCG_E_UNSUPPORTED, "budget exceeds 32-bit cell indexing");
-    B = (uint32_t)maxBlocks;
-    uint64_t mx[DCG_MAX_LEVELS];
-    mx[L - 1] = full64[L - 1];
-    uint64_t left = maxBlocks - mx[L - 1];
-    for (int l = L - 2; l >= 0; l--) {
-      mx[l] = std::min(left / (uint64_t)(l + 1), full64[l]);
-      left -= mx[l];
-    }
-    if (mx[0] == 0) return fail(DCG_E_BUDGET, "too few blocks to reach highest resolution");
-    for (int l = 0; l < L; l++)
-      if (mx[l] == 0) return fail(DCG_E_BUDGET, "a level between the finest and the coarsest gets no blocks");
+    DCGridPlan pl;
+    std::string perr;
+    const int pst = planDCGrid(BW, size, maxBlocks, P, pl, perr);
+    if (pst) return fail(pst, perr);
+    L = pl.L;
+    S = pl.S;
+    B = pl.B;
+    H = pl.H;
     for (int l = 0; l < L; l++) {
-      maxL[l] = (uint32_t)mx[l];
-      fullL[l] = (uint32_t)full64[l];
+      maxL[l] = pl.maxL[l];
+      fullL[l] = pl.fullL[l];
+      offs[l] = pl.offs[l];
+      hsize[l] = pl.hsize[l];
+      hoff[l] = pl.hoff[l];
     }
-    offs[0] = 0;
-    for (int l = 1; l < L; l++) offs[l] = offs[l - 1] + maxL[l - 1];
-    S = 0;
-    for (int l = 0; l < L; l++)
-      if (maxL[l] < fullL[l]) S = l + 1;
-    H = 4 * B;
-    for (int l = 0; l < L; l++) hsize[l] = 4 * maxL[l];
-    hoff[0] = 0;
-    for (int l = 1; l < L; l++) hoff[l] = hoff[l - 1] + hsize[l - 1];
     C = (size_t)B * G::BV;
 
     g.gx = gx;
@@ -320,6 +291,7 @@ struct DCGridSim : SimBase {
     DCG_HIP(hipMemcpyAsync(g.ctl + CTL_LOADS, loads, sizeof(loads), hipMemcpyHostToDevice, stream));
     DCG_TRY(sync());
     customFluidity = false;
+    warmValid = false;
     return init();
   }
   // init(), fluid_simulation_dcgrid.cu:190-210
@@ -628,44 +600,79 @@ struct DCGridSim : SimBase {
     return applyPressure();
   }
   // ---------------------------------------------------------------- projection solve (extension)
-  double* xd = nullptr;        // FP64 pressure of all cells (mixed and FP64 solves)
+  void* xBuf[2] = {nullptr, nullptr};  // iterates of every cell (TX): the canonical one and the
+                                       // other of the update pair x' = x + e
   double* rhoBuf = nullptr;    // right-hand side / residual in density units (T = float or double)
   double* eBuf = nullptr;      // multigrid corrections (T)
-  uint8_t* reachBuf = nullptr; // finest level with cells inside each cell
+  uint8_t* reachBuf = nullptr;   // finest level with cells inside each cell
+  uint8_t* breachBuf = nullptr;  // finest level with cells inside each block
+  uint32_t* svList = nullptr;    // active blocks per level solve, grid and colour (k_sv_lists)
+  uint32_t* svCount = nullptr;   // their counts, device
+  uint32_t* svCountMirror = nullptr;     // and in pinned host memory
+  uint32_t* svCountMirrorDev = nullptr;  // (device address of the mirror)
+  std::vector<uint32_t> lc;      // the counts of the current solve, on the host
   double* ueBuf = nullptr;     // pyramid of the top level: corrections, right-hand sides, activity
   double* urhoBuf = nullptr;
   uint8_t* uactBuf = nullptr;
-  double* partBuf = nullptr;
-  std::vector<UDim> pyr;       // pyramid grids below the top level (cell size doubling)
+  double* partBuf = nullptr;   // per-workgroup partial norms of the residual kernel
+  SolveCtl* ctlBuf = nullptr;  // refinement state per level
+  SolveCtl* ctlMirror = nullptr;     // its copy in pinned host memory (written by k_sv_decide)
+  SolveCtl* ctlMirrorDev = nullptr;  // the device address of that copy
+  std::vector<UDimArg> pyr;    // pyramid grids below the top level (cell size doubling)
+  int pyrCoarse = 0;           // first pyramid grid of the single-workgroup coarse kernel
   dcg_solve_info rep{};
-  uint32_t hostLoads[DCG_MAX_LEVELS] = {};
-  static constexpr unsigned kSolveWG = 1024;  // workgroups of the cell loops with reductions
+  int predCycles[DCG_MAX_LEVELS] = {};  // cycles the last solve of each level took
+  bool warmValid = false;  // the pressure field holds a projection's solution (warm start)
+  static constexpr unsigned kSolveWG = 1024;  // workgroups of the residual kernel at most
 
   unsigned cellGrid(int level) const {
     const unsigned n = blocksFor((size_t)maxL[level] * G::BV, 256);
     return n < 2048u ? n : 2048u;
   }
   int solveSetup() {
-    if (xd) return DCG_OK;
+    if (xBuf[0]) return DCG_OK;
     const int top = L - 1;
     const int tx = g.rx[top] * BW, ty = g.ry[top] * BW, tz = g.rz[top] * BW;
     if ((tx << top) != g.gx || (ty << top) != g.gy || (tz << top) != g.gz)
       return fail(DCG_E_UNSUPPORTED, "projection solve: the grid size must be a multiple of the top level's blocks");
     const size_t C = (size_t)B * G::BV;
-    DCG_TRY(alloc(&xd, C));
+    double* x0 = nullptr;
+    double* x1 = nullptr;
+    DCG_TRY(alloc(&x0, C));
+    DCG_TRY(alloc(&x1, C));
+    xBuf[0] = x0;
+    xBuf[1] = x1;
     DCG_TRY(alloc(&rhoBuf, C));
     DCG_TRY(alloc(&eBuf, C));
     DCG_TRY(alloc(&reachBuf, C));
-    DCG_TRY(alloc(&partBuf, 4 * (size_t)kSolveWG));
+    DCG_TRY(alloc(&breachBuf, B));
+    DCG_TRY(alloc(&svList, (size_t)2 * L * B));
+    DCG_TRY(alloc(&svCount, (size_t)2 * L * L));
+    DCG_TRY(allocHost(&svCountMirror, (size_t)2 * L * L));
+    DCG_HIP(hipHostGetDevicePointer((void**)&svCountMirrorDev, svCountMirror, 0));
+    DCG_TRY(alloc(&partBuf, 5 * (size_t)kSolveWG));
+    DCG_TRY(alloc(&ctlBuf, (size_t)L));
+    DCG_TRY(fill32(ctlBuf, 0u, (size_t)L * sizeof(SolveCtl) / 4));
+    DCG_TRY(allocHost(&ctlMirror, (size_t)L));
+    std::memset((void*)ctlMirror, 0, (size_t)L * sizeof(SolveCtl));
+    DCG_HIP(hipHostGetDevicePointer((void**)&ctlMirrorDev, ctlMirror, 0));
     // pyramid: halve while every size stays even and at least 2
     size_t off = 0;
     int w = tx, h = ty, d = tz;
-    while (w % 2 == 0 && h % 2 == 0 && d % 2 == 0 && std::min(w, std::min(h, d)) >= 4) {
+    while (w % 2 == 0 && h % 2 == 0 && d % 2 == 0 && std::min(w, std::min(h, d)) >= 4 && (int)pyr.size() < kMaxPyr) {
       w /= 2;
       h /= 2;
       d /= 2;
-      pyr.push_back(UDim{w, h, d, (uint32_t)off});
+      pyr.push_back(UDimArg{w, h, d, (uint32_t)off});
       off += (size_t)w * h * d;
+    }
+    // the coarse end that fits the single-workgroup kernel
+    size_t tail = 0;
+    pyrCoarse = (int)pyr.size();
+    for (int j = (int)pyr.size() - 1; j >= 0; j--) {
+      tail += (size_t)pyr[j].w * pyr[j].h * pyr[j].d;
+      if (tail > (size_t)kCoarseCells) break;
+      pyrCoarse = j;
     }
     DCG_TRY(alloc(&ueBuf, off + 1));
     DCG_TRY(alloc(&urhoBuf, off + 1));
@@ -685,185 +692,249 @@ struct DCGridSim : SimBase {
     const float s = (float)(1 << level);
     return s * s * P.dx * P.dx;  // alpha of k_dcgrid_jacobi, dcgrid_multigrid_solver.cu:11
   }
+  // List of the active blocks of grid k in the V-cycle of level lmin, colour c, and its length.
+  const uint32_t* listOf(int lmin, int c) const { return svList + (size_t)(lmin * 2 + c) * B; }
+  uint32_t countOf(int lmin, int k, int c) const { return lc[(size_t)(lmin * L + k) * 2 + c]; }
+  unsigned listGrid(uint32_t n) const {
+    const unsigned w = blocksFor(n, G::BPW);
+    return w < 1024u ? w : 1024u;
+  }
+  // Smoothing of a grid before or after its coarse correction: svPasses passes (solve_smooth,
+  // default 2, at most 5), each the blocks of one colour then the other with kSvInner
+  // red-black sweeps in LDS, i.e. 2 * kSvInner * passes <= 20 sweeps per grid and cycle. One
+  // sweep per pass smooths too little on the sparse levels of the BW-4 reference scene (its
+  // finest level's V-cycle diverged at 2 passes x 1 and 1 x 2 sweeps, took 19 cycles at 3 x 1);
+  // 2 x 2 converges every benchmark and test scene in 6-9 cycles per level.
+  static constexpr int kSvInner = 2;
+  int smoothSweeps() const { return P.solve_smooth > 0 ? std::min(P.solve_smooth, 5) : 2; }
+  int svInner = kSvInner, svPasses = 2;
   template <class T>
-  int smooth(int level, int lmin, int sweeps) {
-    T* e = (T*)eBuf;
-    const T* rho = (const T*)rhoBuf;
-    for (int s = 0; s < sweeps; s++)
-      for (int color = 0; color < 2; color++)
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_rbgs",
-                   (k_sv_rbgs<BW, T><<<cellGrid(level), 256, 0, stream>>>(g, level, lmin, color, e, rho, reachBuf, levelH2(level), ghostFactor(level - lmin))));
+  int smoothBlocks(int k, int lmin, int nu, bool down, const SolveCtl* ctl) {
+    (void)nu;
+    const float h2 = levelH2(k), gm = ghostFactor(k - lmin);
+    for (int pass = 0; pass < svPasses; pass++)
+      for (int colour = 0; colour < 2; colour++) {
+        const uint32_t n = countOf(lmin, k, colour);
+        if (n == 0) continue;
+        // the first pass of the down leg starts from a zero correction: the first colour's
+        // halos are zero too (the second colour's are what the first wrote)
+        const bool zero = down && pass == 0;
+        const bool haloZero = zero && (colour == 0 || countOf(lmin, k, 0) == 0);
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_smooth",
+                   (k_sv_smooth<BW, T><<<listGrid(n), G::WG, 0, stream>>>(
+                       g, k, lmin, listOf(lmin, colour), n, svInner, zero ? 1 : 0, haloZero ? 1 : 0, (T*)eBuf,
+                       (const T*)rhoBuf, reachBuf, h2, gm, ctl)));
+      }
     return DCG_OK;
   }
   template <class T>
-  int smoothU(int j, int lmin, int sweeps) {
-    const UDim& D = pyr[j];
-    const unsigned n = blocksFor((size_t)D.w * D.h * D.d, 256);
+  int smoothPyr(int j, int lmin, int nu, bool down, const SolveCtl* ctl) {
+    (void)nu;
+    const UDimArg& D = pyr[j];
+    const unsigned tiles = (unsigned)(((D.w + 7) / 8) * ((D.h + 7) / 8) * ((D.d + 7) / 8));
+    const unsigned n = tiles < 1024u ? tiles : 1024u;
     const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
     const float gm = ghostFactor(L + j - lmin);
-    for (int k = 0; k < sweeps; k++)
-      for (int color = 0; color < 2; color++)
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_rbgs", (k_svu_rbgs<T><<<n, 256, 0, stream>>>(D, color, (T*)ueBuf, (const T*)urhoBuf, uactBuf, h2, gm)));
+    for (int pass = 0; pass < svPasses; pass++)
+      for (int colour = 0; colour < 2; colour++) {
+        const bool zero = down && pass == 0;
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_smooth",
+                   (k_svu_smooth<T><<<n, 512, 0, stream>>>(D, colour, svInner, zero ? 1 : 0, zero && colour == 0 ? 1 : 0,
+                                                          (T*)ueBuf, (const T*)urhoBuf, uactBuf, h2, gm, ctl)));
+      }
     return DCG_OK;
   }
   // One V-cycle for the correction of level lmin (right-hand side in rho on its cells).
   template <class T>
-  int vcycle(int lmin) {
+  int vcycle(int lmin, const SolveCtl* ctl) {
     T* e = (T*)eBuf;
     T* rho = (T*)rhoBuf;
-    const int nu = P.solve_smooth > 0 ? std::min(P.solve_smooth, 10) : 3;
+    const int nu = smoothSweeps();
     const int top = L - 1;
-    DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, T><<<cellGrid(lmin), 256, 0, stream>>>(g, lmin, e, e)));
     for (int k = lmin; k < top; k++) {
-      DCG_TRY(smooth<T>(k, lmin, nu));
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, T><<<cellGrid(k + 1), 256, 0, stream>>>(g, k + 1, e, rho)));
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
-                 (k_sv_restrict<BW, T><<<cellGrid(k), 256, 0, stream>>>(g, k, lmin, e, rho, reachBuf, levelH2(k), ghostFactor(k - lmin))));
+      DCG_TRY(smoothBlocks<T>(k, lmin, nu, true, ctl));
+      const uint32_t n0 = countOf(lmin, k, 0), n1 = countOf(lmin, k, 1);
+      if (n0 + n1)
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
+                   (k_sv_restrict<BW, T, false><<<listGrid(n0 + n1), G::WG, 0, stream>>>(
+                       g, k, lmin, listOf(lmin, 0), listOf(lmin, 1), n0, n1, e, rho, reachBuf, levelH2(k),
+                       ghostFactor(k - lmin), nullptr, nullptr, UDimArg{}, ctl)));
     }
-    DCG_TRY(smooth<T>(top, lmin, nu));
+    DCG_TRY(smoothBlocks<T>(top, lmin, nu, true, ctl));
     if (!pyr.empty()) {
       T* ue = (T*)ueBuf;
       T* ur = (T*)urhoBuf;
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict_top",
-                 (k_sv_restrict_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, rho, reachBuf, ue, ur, uactBuf, pyr[0], levelH2(top), ghostFactor(top - lmin))));
-      const int J = (int)pyr.size();
-      for (int j = 0; j + 1 < J; j++) {
-        DCG_TRY(smoothU<T>(j, lmin, nu));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
+                 (k_sv_restrict<BW, T, true><<<slotGrid<BW>(maxL[top]), G::WG, 0, stream>>>(
+                     g, top, lmin, nullptr, nullptr, 0u, 0u, e, rho, reachBuf, levelH2(top), ghostFactor(top - lmin),
+                     ur, uactBuf, pyr[0], ctl)));
+      // grids 0..jd-1 by the multi-workgroup kernels, jd.. by the coarse kernel (or, when even
+      // the coarsest grid exceeds its LDS, many smoothing passes on that grid)
+      const int J = (int)pyr.size(), jc = pyrCoarse, jd = jc < J ? jc : J - 1;
+      for (int j = 0; j < jd; j++) {
+        DCG_TRY(smoothPyr<T>(j, lmin, nu, true, ctl));
         const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
-        const UDim& Dc = pyr[j + 1];
+        const UDimArg& Dc = pyr[j + 1];
         DCG_LAUNCH(DCG_K_OTHER, "k_svu_restrict",
-                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(pyr[j], Dc, ue, ur, uactBuf, h2, ghostFactor(L + j - lmin))));
+                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(
+                       pyr[j], Dc, ue, ur, uactBuf, h2, ghostFactor(L + j - lmin), ctl)));
       }
-      {
-        const float s = (float)(1 << (L + J - 1)), h2 = s * s * P.dx * P.dx;
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_bottom",
-                   (k_svu_bottom<T><<<1, 1024, 0, stream>>>(pyr[J - 1], ue, ur, uactBuf, h2, ghostFactor(L + J - 1 - lmin), 40)));
+      if (jc < J) {
+        CoarseArgs ca{};
+        uint32_t loff = 0;
+        for (int j = jc; j < J; j++) {
+          ca.D[j - jc] = UDimArg{pyr[j].w, pyr[j].h, pyr[j].d, loff};
+          loff += (uint32_t)(pyr[j].w * pyr[j].h * pyr[j].d);
+          const float s = (float)(1 << (L + j));
+          ca.h2[j - jc] = s * s * P.dx * P.dx;
+          ca.gm[j - jc] = ghostFactor(L + j - lmin);
+        }
+        ca.goff = pyr[jc].off;
+        ca.n = J - jc;
+        ca.sweeps = svPasses * svInner;
+        ca.bottom = 8;  // converged the same as 40 on the benchmark scenes, at less than half the cost
+        DCG_LAUNCH(DCG_K_OTHER, "k_svu_coarse",
+                   (k_svu_coarse<T><<<1, kCoarseThreads, 0, stream>>>(ca, ue, ur, uactBuf, ctl)));
+      } else {
+        for (int i = 0; i < 20; i++) DCG_TRY(smoothPyr<T>(jd, lmin, nu, i == 0, ctl));
       }
-      for (int j = J - 2; j >= 0; j--) {
-        const UDim& D = pyr[j];
+      for (int j = jd - 1; j >= 0; j--) {
+        const UDimArg& D = pyr[j];
         DCG_LAUNCH(DCG_K_OTHER, "k_svu_prolong",
-                   (k_svu_prolong<T><<<blocksFor((size_t)D.w * D.h * D.d, 256), 256, 0, stream>>>(D, pyr[j + 1], ue, uactBuf)));
-        DCG_TRY(smoothU<T>(j, lmin, nu));
+                   (k_svu_prolong<T><<<blocksFor((size_t)D.w * D.h * D.d, 256), 256, 0, stream>>>(D, pyr[j + 1], ue,
+                                                                                                  uactBuf, ctl)));
+        DCG_TRY(smoothPyr<T>(j, lmin, nu, false, ctl));
       }
       DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_top",
-                 (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, uactBuf, pyr[0])));
-      DCG_TRY(smooth<T>(top, lmin, nu));
+                 (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, uactBuf, pyr[0],
+                                                                             ctl)));
+      DCG_TRY(smoothBlocks<T>(top, lmin, nu, false, ctl));
     } else {
-      DCG_TRY(smooth<T>(top, lmin, 20 * nu));
+      for (int i = 1; i < 20; i++) DCG_TRY(smoothBlocks<T>(top, lmin, nu, false, ctl));
     }
     for (int k = top - 1; k >= lmin; k--) {
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong", (k_sv_prolong<BW, T><<<cellGrid(k), 256, 0, stream>>>(g, k, lmin, e, reachBuf)));
-      DCG_TRY(smooth<T>(k, lmin, nu));
+      const uint32_t n0 = countOf(lmin, k, 0), n1 = countOf(lmin, k, 1);
+      if (n0 + n1)
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong",
+                   (k_sv_prolong<BW, T><<<listGrid(n0 + n1), G::WG, 0, stream>>>(
+                       g, k, lmin, listOf(lmin, 0), listOf(lmin, 1), n0, n1, e, reachBuf, ctl)));
+      DCG_TRY(smoothBlocks<T>(k, lmin, nu, false, ctl));
     }
     return DCG_OK;
   }
-  // Sums of the partials of the last residual kernel.
-  int readPartials(int count, double* out) {
-    std::vector<double> h((size_t)count * kSolveWG);
-    DCG_HIP(hipMemcpyAsync(h.data(), partBuf, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-    DCG_TRY(sync());
-    for (int k = 0; k < count; k++) out[k] = 0;
-    for (unsigned w = 0; w < kSolveWG; w++)
-      for (int k = 0; k < count; k++) out[k] += h[(size_t)w * count + k];
-    return DCG_OK;
-  }
+  // Residual of the updated iterate and the decision on the level's next cycle.
   template <class TX, class T>
-  int solveLevel(int l, TX* x) {
-    T* rho = (T*)rhoBuf;
+  int residualLaunch(int l, bool first, const TX* xin, TX* xout, SolveCtl* ctl) {
     const float alpha = levelH2(l);
-    const bool singular = hostLoads[l] == maxL[l] && maxL[l] == fullL[l];
-    const double N = (double)hostLoads[l] * G::BV;
-    double r0 = -1, rr = 0, prev = 0, shift = 0;
-    int cycles = 0, stall = 0;
     const double tol = P.solve_tol > 0 ? (double)P.solve_tol : 1e-8;
     const int maxCycles = P.solve_max_cycles > 0 ? P.solve_max_cycles : 50;
+    const unsigned n = std::min(slotGrid<BW>(maxL[l]), kSolveWG);
+    DCG_LAUNCH(DCG_K_OTHER, "k_sv_residual",
+               (k_sv_residual<BW, TX, T><<<n, G::WG, 0, stream>>>(g, l, (const TX*)xBuf[0], xin, xout,
+                                                                  (const T*)eBuf, first ? 1 : 0, (T*)rhoBuf, partBuf,
+                                                                  alpha, alpha, ctl)));
+    DCG_LAUNCH(DCG_K_OTHER, "k_sv_decide",
+               (k_sv_decide<<<1, 256, 0, stream>>>(g, l, G::BV, first ? 1 : 0, (int)n, partBuf, alpha, tol, maxCycles,
+                                                   ctl, ctlMirrorDev + l)));
+    return DCG_OK;
+  }
+  // Iterative refinement of level l (x of the level initialised in the canonical iterate). The
+  // loop runs on the device; the host enqueues as many cycles as the level took last time, then
+  // one more at a time until the level's control block says it is done.
+  template <class TX, class T>
+  int solveLevel(int l) {
+    TX* xc = (TX*)xBuf[0];
+    TX* x2 = (TX*)xBuf[1];
+    SolveCtl* ctl = ctlBuf + l;
+    const double tol = P.solve_tol > 0 ? (double)P.solve_tol : 1e-8;
+    const int maxCycles = P.solve_max_cycles > 0 ? P.solve_max_cycles : 50;
+    DCG_TRY((residualLaunch<TX, T>(l, true, xc, xc, ctl)));
+    int issued = 0;
+    int n = predCycles[l] > 0 ? predCycles[l] : 6;
+    SolveCtl h{};
     for (;;) {
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_residual",
-                 (k_sv_residual<BW, TX, T><<<kSolveWG, 256, 0, stream>>>(g, l, x, rho, partBuf, alpha, alpha, shift)));
-      double s[4];
-      DCG_TRY(readPartials(4, s));
-      double m = 0;
-      if (singular) {
-        m = s[0] / N;
-        const double m0 = s[2] / N;
-        rr = std::sqrt(std::max(s[1] - N * m * m, 0.0));
-        if (r0 < 0) r0 = std::sqrt(std::max(s[3] - N * m0 * m0, 0.0));
-      } else {
-        rr = std::sqrt(s[1]);
-        if (r0 < 0) r0 = std::sqrt(s[3]);
+      for (int i = 0; i < n && issued <= maxCycles; i++, issued++) {
+        DCG_TRY(vcycle<T>(l, ctl));
+        // cycle i+1 reads the iterate of cycle i and writes the other buffer
+        TX* xin = issued % 2 == 0 ? xc : x2;
+        TX* xout = issued % 2 == 0 ? x2 : xc;
+        DCG_TRY((residualLaunch<TX, T>(l, false, xin, xout, ctl)));
       }
-      if (rr <= tol * r0 || cycles >= maxCycles) break;
-      if (cycles > 0 && rr > 0.7 * prev) {
-        if (++stall >= 3) break;  // the working precision is exhausted
-      } else {
-        stall = 0;
-      }
-      prev = rr;
-      // the mean of R (the right-hand side's inconsistency) stays the same from cycle to cycle:
-      // the residual kernel subtracts the last one, this corrects what is left
-      if (singular && m != shift) {
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_shift", (k_sv_shift<BW, T><<<cellGrid(l), 256, 0, stream>>>(g, l, rho, (m - shift) / (double)alpha)));
-        shift = m;
-      }
-      DCG_TRY(vcycle<T>(l));
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_update", (k_sv_update<BW, TX, T><<<cellGrid(l), 256, 0, stream>>>(g, l, x, (const T*)eBuf, reachBuf)));
-      cycles++;
+      DCG_TRY(sync());  // the decide kernel mirrored the control block into pinned host memory
+      std::memcpy(&h, (const void*)(ctlMirror + l), sizeof(h));
+      if (h.done) break;
+      if (issued > maxCycles) return fail(DCG_E_HIP, "projection solve: the level's loop did not stop");
+      n = 1;
     }
-    if (singular) {  // zero-mean normalisation of a pure Neumann level
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_sum", (k_sv_sum<BW, TX><<<kSolveWG, 256, 0, stream>>>(g, l, x, partBuf)));
-      double sx;
-      DCG_TRY(readPartials(1, &sx));
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_shift", (k_sv_shift<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, x, sx / N)));
+    predCycles[l] = (int)h.cycles;
+    const TX* xf = h.cycles % 2 ? x2 : xc;
+    DCG_LAUNCH(DCG_K_OTHER, "k_sv_finalize",
+               (k_sv_finalize<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xf, xc, (const SolveCtl*)ctl)));
+    rep.cycles[l] = (int)h.cycles;
+    rep.rel_residual[l] = h.r0 > 0 ? h.rr / h.r0 : 0.0;
+    rep.initial_residual[l] = h.r0;
+    rep.singular[l] = (int)h.singular;
+    if (!(h.rr <= tol * h.r0)) rep.converged = 0;
+    return DCG_OK;
+  }
+  // Initial guess of every level (warm: the last projection's pressure where blocks kept their
+  // place, read before divergence() clears it; cold: zero on the top level), then the levels
+  // from the coarsest.
+  template <class TX, class T>
+  int solveAll(bool warm) {
+    TX* xc = (TX*)xBuf[0];
+    for (int l = L - 1; l >= 0; l--) {
+      if (l == L - 1) {
+        if (!warm) DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xc)));
+      } else {
+        DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_x",
+                   (k_sv_prolong_x<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xc, warm ? 1 : 0)));
+      }
+      DCG_TRY((solveLevel<TX, T>(l)));
     }
-    rep.cycles[l] = cycles;
-    rep.rel_residual[l] = r0 > 0 ? rr / r0 : 0.0;
-    rep.initial_residual[l] = r0;
-    rep.singular[l] = singular ? 1 : 0;
-    if (!(rr <= tol * r0)) rep.converged = 0;
     return DCG_OK;
   }
   template <class TX, class T>
-  int solveAll(TX* x) {
-    for (int l = L - 1; l >= 0; l--) {
-      if (l == L - 1)
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, x, x)));
-      else
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_x", (k_sv_prolong_x<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, x)));
-      DCG_TRY((solveLevel<TX, T>(l, x)));
+  int solveWithGuess() {
+    const bool warm = warmValid && !P.solve_cold;
+    if (warm) {
+      unsigned n = blocksFor((size_t)B * G::BV, 256);
+      if (n > 2048u) n = 2048u;
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_warm", (k_sv_warm<BW, TX><<<n, 256, 0, stream>>>(g, (TX*)xBuf[0])));
     }
-    return DCG_OK;
+    DCG_TRY(divergence());
+    for (int l = 0; l < L; l++)
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_reach",
+                 (k_sv_reach<BW><<<slotGrid<BW>(maxL[l]), G::WG, 0, stream>>>(g, l, reachBuf, breachBuf)));
+    // the active blocks of every level solve's grids, and their counts on the host (which sizes
+    // every launch of the solve)
+    DCG_TRY(fill32(svCount, 0u, (size_t)2 * L * L));
+    {
+      unsigned n = blocksFor(maxL[0], 256);
+      for (int l = 1; l < L; l++) n = std::max(n, blocksFor(maxL[l], 256));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_lists", (k_sv_lists<BW><<<std::min(n, 1024u), 256, 0, stream>>>(g, breachBuf, svList, svCount)));
+      DCG_LAUNCH(DCG_K_OTHER, "k_sv_mirror_counts",
+                 (k_sv_mirror_counts<<<1, 1, 0, stream>>>(svCount, svCountMirrorDev, 2 * L * L)));
+    }
+    DCG_TRY(sync());
+    lc.assign(svCountMirror, svCountMirror + (size_t)2 * L * L);
+    return solveAll<TX, T>(warm);
   }
   int projectSolve() override {
     DCG_TRY(solveSetup());
-    DCG_TRY(divergence());
-    {
-      uint32_t h[DCG_MAX_LEVELS];
-      DCG_HIP(hipMemcpyAsync(h, g.ctl + CTL_LOADS, sizeof(h), hipMemcpyDeviceToHost, stream));
-      DCG_TRY(sync());
-      for (int l = 0; l < L; l++) hostLoads[l] = h[l];
-    }
-    for (int l = 0; l < L; l++)
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_reach", (k_sv_reach<BW><<<cellGrid(l), 256, 0, stream>>>(g, l, reachBuf)));
+    svPasses = smoothSweeps();
+    svInner = kSvInner;
     rep = dcg_solve_info{};
     rep.levels = L;
     rep.converged = 1;
-    if (P.solve_precision == DCG_SOLVE_FP32) {
-      if constexpr (std::is_same<SF, float>::value) {
-        DCG_TRY((solveAll<float, float>(g.pre)));
-      } else {  // FP32 iterate in the FP64 buffer, rounded to the field storage at the end
-        float* x32 = (float*)xd;
-        DCG_TRY((solveAll<float, float>(x32)));
-        for (int l = 0; l < L; l++)
-          DCG_LAUNCH(DCG_K_OTHER, "k_sv_store", (k_sv_store<BW, float><<<cellGrid(l), 256, 0, stream>>>(g, l, x32)));
-      }
-    } else {
-      if (P.solve_precision == DCG_SOLVE_FP64)
-        DCG_TRY((solveAll<double, double>(xd)));
-      else
-        DCG_TRY((solveAll<double, float>(xd)));
-      for (int l = 0; l < L; l++)
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_store", (k_sv_store<BW, double><<<cellGrid(l), 256, 0, stream>>>(g, l, xd)));
-    }
+    // the guess of a solve in one precision is the pressure field, whatever precision wrote it
+    if (P.solve_precision == DCG_SOLVE_FP32)
+      DCG_TRY((solveWithGuess<float, float>()));
+    else if (P.solve_precision == DCG_SOLVE_FP64)
+      DCG_TRY((solveWithGuess<double, double>()));
+    else
+      DCG_TRY((solveWithGuess<double, float>()));
+    warmValid = true;
     return applyPressure();
   }
   int solveReport(dcg_solve_info* out) override {
@@ -973,9 +1044,10 @@ struct DCGridSim : SimBase {
       out.push_back({(uint32_t)(DCG_STATE_FIELD + f), es, fieldPtr(f), comps * C * es});
     }
   }
-  uint32_t stateFlags() const override { return customFluidity ? 1u : 0u; }
+  uint32_t stateFlags() const override { return (customFluidity ? 1u : 0u) | (warmValid ? 2u : 0u); }
   void afterStateLoad(uint32_t flags) override {
     customFluidity = (flags & 1u) != 0;
+    warmValid = (flags & 2u) != 0;
     dropGraph();
   }
 

@@ -8,41 +8,50 @@
 // FP64 (or FP32 / FP64 throughout), the correction e of A e = R comes from one multigrid
 // V-cycle in FP32 whose grids are the level itself, the coarser levels restricted to the
 // ancestors of the level's cells ("reach"), and below the top level a uniform pyramid of the
-// top level (dense by construction). Smoothing is red-black Gauss-Seidel; restriction is the
-// mean of the 8 cells of a restriction group, prolongation the trilinear 27/9/3/1 stencil.
-// Residuals on coarse grids are kept in density units (R / h^2), so restriction is a plain
-// mean and each grid uses its own cell size in the operator.
+// top level (dense by construction). Restriction is the mean of the 8 cells of a restriction
+// group, prolongation the trilinear 27/9/3/1 stencil. Residuals on coarse grids are kept in
+// density units (R / h^2), so restriction is a plain mean and each grid uses its own cell size
+// in the operator.
+//
+// Smoother: red-black Gauss-Seidel inside each 8^3 (BW^3) block, blocks in two colours (block
+// coordinate parity). Face-adjacent blocks always have opposite colours and the operator couples
+// face neighbours only, so the blocks of one colour are independent: a launch stages each of its
+// blocks with the apron faces (the other colour's current values) in LDS and runs all its inner
+// sweeps there. One smoothing pass is two launches (one per colour) instead of two launches per
+// sweep, and the result does not depend on the order in which workgroups run.
+//
+// The refinement loop runs on the device: the residual kernel's last workgroup sums the
+// partial norms, decides convergence (or the stall of the working precision) and publishes it
+// in the level's SolveCtl; every later kernel of the level's loop returns at once when the
+// level is done, so the host enqueues cycles ahead and reads the control block once it expects
+// the level to have converged.
 #pragma once
 
 #include "dcgrid_fluid.hpp"
 
 namespace dcg {
 
-// Face neighbour (dir 0..5: x-, x+, y-, y+, z-, z+) of local cell t of block b, as the apron
-// table holds it; `coarser` tells whether it lies on a coarser level.
-template <int BW>
-__device__ __forceinline__ uint32_t faceNb(const DGBase& g, uint32_t b, int t, int dir, bool& coarser) {
-  constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
-  int x, y, z;
-  locPos<BB>((uint32_t)t, x, y, z);
-  int ai = x + 1, aj = y + 1, ak = z + 1;
-  switch (dir) {
-    case 0: ai--; break;
-    case 1: ai++; break;
-    case 2: aj--; break;
-    case 3: aj++; break;
-    case 4: ak--; break;
-    default: ak++; break;
-  }
-  coarser = false;
-  if (ai >= 1 && ai <= BW && aj >= 1 && aj <= BW && ak >= 1 && ak <= BW) return b * BV + loc<BB>(ai - 1, aj - 1, ak - 1);
-  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
-  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
-  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
-  const uint32_t sh = ndShift(g.nd[b * 27 + dirIndex(dx, dy, dz)]);
-  coarser = sh != 0 && sh != ND_OUT;
-  return apronCell<BW>(g, b, ai, aj, ak);
-}
+// A grid of the uniform pyramid below the top level: size and offset in the pyramid arrays.
+struct UDimArg {
+  int w, h, d;
+  uint32_t off;
+};
+
+// Refinement state of one level (device memory, one per level).
+struct SolveCtl {
+  double r0;      // |R(0)|: residual norm of a zero level pressure
+  double rr;      // |R| of the last residual
+  double prev;    // |R| of the residual before
+  double shift;   // mean of R the residual kernel subtracts before rounding (singular levels)
+  double delta;   // (mean - shift) / h^2 still to take from rho on the level's cells
+  double mean;    // mean of x after the last update (normalisation of a singular level)
+  uint32_t cycles, stall, done, singular;
+  uint32_t ticket;  // arrivals of the residual's workgroups (last one reduces)
+  uint32_t pad[3];
+};
+static_assert(sizeof(SolveCtl) == 80, "SolveCtl layout");
+
+__device__ __forceinline__ bool solveDone(const SolveCtl* c) { return *(volatile const uint32_t*)&c->done != 0; }
 
 // Parent cell (level + 1) of cell t of block b: the restriction-group mapping of accumulate<T>.
 template <int BW>
@@ -52,204 +61,303 @@ __device__ __forceinline__ uint32_t parentCellOf(const DGBase& g, uint32_t b, in
   return (ps / 8) * BV + ((ps % 8) << SBB) + (uint32_t)(t >> 3);
 }
 
-// reach[c] = finest level with cells inside cell c (its own level for a leaf cell), for the
-// cells of `level`; levels are processed finest first.
+// borderApronCell with the level shift of the descriptor (ND_OUT outside the domain).
 template <int BW>
-__global__ void __launch_bounds__(256) k_sv_reach(DGBase g, int level, uint8_t* __restrict__ reach) {
-  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t c = g.offs[level] * BV + i;
-    const uint32_t cb = g.chi[c >> SBB];
-    uint8_t r = (uint8_t)level;
-    if (cb != NFU) {
-      const uint32_t first = cb * BV + 8 * (c & ((1u << SBB) - 1));
-#pragma unroll
-      for (int j = 0; j < 8; j++) r = min(r, reach[first + j]);
-    }
-    reach[c] = r;
-  }
+__device__ __forceinline__ uint32_t borderApronCellShift(const DGBase& g, uint32_t b, int ai, int aj, int ak,
+                                                         uint32_t& sh) {
+  constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
+  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
+  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
+  const int4 d = g.nd[b * 27 + dirIndex(dx, dy, dz)];
+  sh = ndShift(d);
+  const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
+  const uint32_t spread = loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
+  const uint32_t clamped = loc<BB>(ci - 1, cj - 1, ck - 1);
+  const uint32_t outCell = b * BV + (d.y ? spread : clamped);
+  const uint32_t s = sh == ND_OUT ? 0u : sh;
+  const uint32_t inCell = ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> s, (aj - 1 - d.z) >> s, (ak - 1 - d.w) >> s);
+  return sh == ND_OUT ? outCell : inCell;
 }
 
-// Fine-level residual R = sum(apron x) - 6 x - alpha div (precision TX), stored as
-// rho = (R - shift) / h^2 (precision T); per-workgroup partial sums of R, R^2 and of the
-// zero-guess residual R0 and R0^2 go to part[4 * blockIdx.x ...]. A singular level passes the
-// mean of R as `shift`, so the zero-mean part is formed before the rounding to T: rounding R
-// first would leave an error of T's epsilon times that mean, which does not shrink with R.
-template <int BW, class TX, class T, class G>
-__global__ void __launch_bounds__(256) k_sv_residual(G g, int level, const TX* __restrict__ x,
-                                                     T* __restrict__ rho, double* __restrict__ part, float alpha,
-                                                     float h2, double shift) {
-  constexpr int BV = Geo<BW>::BV;
-  __shared__ double red[4][256];
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  double s[4] = {0, 0, 0, 0};
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t b = g.offs[level] + i / BV;
-    const int t = (int)(i % BV);
-    const uint32_t c = b * BV + t;
-    TX sum = 0, ghosts = 0;
-#pragma unroll
-    for (int d = 0; d < 6; d++) {
-      bool co;
-      const uint32_t nb = faceNb<BW>(g, b, t, d, co);
-      const TX v = x[nb];
-      sum += v;
-      if (co) ghosts += v;
-    }
-    const TX ad = (TX)alpha * (TX)g.div[c];
-    const TX R = sum - (TX)6 * x[c] - ad;
-    const TX R0 = ghosts - ad;
-    rho[c] = (T)((R - (TX)shift) / (TX)h2);
-    s[0] += (double)R;
-    s[1] += (double)R * (double)R;
-    s[2] += (double)R0;
-    s[3] += (double)R0 * (double)R0;
-  }
-#pragma unroll
-  for (int k = 0; k < 4; k++) red[k][threadIdx.x] = s[k];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+// Block origin and colour (parity of the block coordinates) of block b of `level`.
+template <int BW>
+__device__ __forceinline__ int3 svOrigin(const DGBase& g, int level, uint32_t b) {
+  return level >= g.S ? denseOrigin<BW>(g, level, b) : g.pos[b];
+}
+template <int BW>
+__device__ __forceinline__ int blockColour(int3 p0) {
+  constexpr int BB = Geo<BW>::BB;
+  return ((p0.x >> BB) + (p0.y >> BB) + (p0.z >> BB)) & 1;
+}
+
+// reach[c] = finest level with cells inside cell c (its own level for a leaf cell), and
+// breach[b] = the finest of its block; levels are processed finest first.
+template <int BW>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_sv_reach(DGBase g, int level, uint8_t* __restrict__ reach,
+                                                        uint8_t* __restrict__ breach) {
+  constexpr int BV = Geo<BW>::BV, SBB = Geo<BW>::SBB, BPW = Geo<BW>::BPW;
+  __shared__ int bmin[BPW];
+  const uint32_t total = slotTotal(g, level);
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();
-  }
-  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
-}
-
-// rho -= shift on the cells of a level (zero-mean right-hand side of a singular level).
-template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_shift(DGBase g, int level, T* __restrict__ v, double shift) {
-  constexpr int BV = Geo<BW>::BV;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t c = g.offs[level] * BV + i;
-    v[c] = (T)((double)v[c] - shift);
-  }
-}
-
-// Sum of v over the cells of a level (per-workgroup partials).
-template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_sum(DGBase g, int level, const T* __restrict__ v, double* __restrict__ part) {
-  constexpr int BV = Geo<BW>::BV;
-  __shared__ double red[256];
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  double s = 0;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += (double)v[g.offs[level] * BV + i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if (t == 0) bmin[sub] = 255;
     __syncthreads();
+    const uint32_t slot = base + sub;
+    int r = 255;
+    uint32_t b = 0;
+    if (slot < total) {
+      b = g.offs[level] + slot;
+      const uint32_t c = b * BV + t;
+      const uint32_t cb = g.chi[c >> SBB];
+      r = level;
+      if (cb != NFU) {
+        const uint32_t first = cb * BV + 8 * (c & ((1u << SBB) - 1));
+#pragma unroll
+        for (int j = 0; j < 8; j++) r = min(r, (int)reach[first + j]);
+      }
+      reach[c] = (uint8_t)r;
+      atomicMin(&bmin[sub], r);
+    }
+    __syncthreads();
+    if (slot < total && t == 0) breach[b] = (uint8_t)bmin[sub];
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// e = 0, rho = 0 on the cells of a level.
+// Active blocks of the V-cycle grids of every level solve: for lmin <= k, the blocks of level k
+// with breach <= lmin, per colour c, at list[(lmin * 2 + c) * B + offs[k] + i] for
+// i < count[(lmin * L + k) * 2 + c] (2 L B entries in all). The order inside a list follows the atomic increments and
+// does not matter: the blocks of one list are independent.
+template <int BW>
+__global__ void __launch_bounds__(256) k_sv_lists(DGBase g, const uint8_t* __restrict__ breach,
+                                                  uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+  for (int k = 0; k < g.L; k++) {
+    const uint32_t n = g.ctl[CTL_LOADS + k];
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+      const uint32_t b = g.offs[k] + i;
+      const int c = blockColour<BW>(svOrigin<BW>(g, k, b));
+      for (int lmin = breach[b]; lmin <= k; lmin++) {
+        const uint32_t slot = atomicAdd(&count[(lmin * g.L + k) * 2 + c], 1u);
+        list[(size_t)(lmin * 2 + c) * g.B + g.offs[k] + slot] = b;
+      }
+    }
+  }
+}
+// The list counts into the host-visible mirror (one thread, after k_sv_lists).
+__global__ void k_sv_mirror_counts(const uint32_t* __restrict__ count, uint32_t* __restrict__ mirror, int n) {
+  for (int i = 0; i < n; i++) mirror[i] = count[i];
+}
+
+// ---------------------------------------------------------------- block staging of a correction
+// Coupling codes of a staged block tile (apron index a in [0, A^3)): a value >= 0 is the tile
+// entry whose value the neighbour contributes (the entry itself for interior and same-level
+// halo cells, an own interior cell for the domain-edge clamp of blocks placed by a move or
+// refinement); SV_NEU marks the clamp onto the asking cell itself (Neumann: the diagonal loses
+// 1); SV_GHOST a coarser or inactive neighbour (the Dirichlet boundary of the solved region: the
+// diagonal loses gm). Tile entry 0 (a corner, never a face) holds 0 for uncoupled neighbours.
+enum : int { SV_NEU = -1, SV_GHOST = -2 };
+
+// Stages the correction e of block b around local cell t: V (values as T), K (codes), and for
+// the calling thread its own cell's reach and rho. Loads of a thread are issued together.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_zero(DGBase g, int level, T* __restrict__ e, T* __restrict__ rho) {
-  constexpr int BV = Geo<BW>::BV;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t c = g.offs[level] * BV + i;
-    e[c] = (T)0;
-    rho[c] = (T)0;
+__device__ __forceinline__ void svStageBlock(const DGBase& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
+                                             const uint8_t* __restrict__ reach, bool ownZero, bool haloZero,
+                                             bool valid, T* V, int16_t* K) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE, BB = Geo<BW>::BB;
+  constexpr int FPT = (FACE + BV - 1) / BV;
+  const uint32_t c = b * BV + t;
+  const int a = apronIndexOfLocal<BW>(t);
+  uint32_t nb[FPT], sh[FPT];
+  int ai[FPT], aj[FPT], ak[FPT];
+#pragma unroll
+  for (int j = 0; j < FPT; j++) {
+    const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
+    faceCell<BW>(fc, ai[j], aj[j], ak[j]);
+    nb[j] = borderApronCellShift<BW>(g, b, ai[j], aj[j], ak[j], sh[j]);
+  }
+  const int rc = reach[c];
+  const T vc = ownZero ? (T)0 : e[c];
+  int rn[FPT];
+  T vn[FPT];
+#pragma unroll
+  for (int j = 0; j < FPT; j++) {
+    rn[j] = reach[nb[j]];
+    vn[j] = haloZero ? (T)0 : e[nb[j]];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (!valid) return;
+  V[a] = vc;
+  K[a] = rc <= lmin ? (int16_t)a : (int16_t)SV_GHOST;
+  if (t == 0) {
+    V[0] = (T)0;
+    K[0] = (int16_t)SV_GHOST;
+  }
+#pragma unroll
+  for (int j = 0; j < FPT; j++) {
+    if (t + j * BV >= FACE) continue;
+    const int fa = (ai[j] * A + aj[j]) * A + ak[j];
+    int code;
+    if (sh[j] == ND_OUT) {
+      // the clamp onto an own cell: the asking cell itself (Neumann) or another own cell
+      const uint32_t own = nb[j] - b * BV;
+      const uint32_t ask = loc<BB>(clampi(ai[j], 1, BW) - 1, clampi(aj[j], 1, BW) - 1, clampi(ak[j], 1, BW) - 1);
+      code = own == ask ? SV_NEU : (rn[j] <= lmin ? apronIndexOfLocal<BW>((int)own) : SV_GHOST);
+    } else if (sh[j] != 0) {
+      code = SV_GHOST;
+    } else {
+      code = rn[j] <= lmin ? fa : SV_GHOST;
+      V[fa] = vn[j];
+    }
+    K[fa] = (int16_t)code;
   }
 }
 
-// (A e)_c / h^2 pieces of an active cell: sum of the coupled neighbours and the diagonal.
-// Neighbours on coarser levels and inactive ones are the Dirichlet boundary of the region being
-// solved. On the level itself they hold a zero correction; on coarser grids the zero lies
-// between the boundary cell and that neighbour (where the finest grid puts it), which a ghost
-// value gm * e_c (gm <= 0, from the grid's depth) reproduces.
-template <int BW, class T>
-__device__ __forceinline__ void svStencil(const DGBase& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
-                                          const uint8_t* __restrict__ reach, T gm, T& sum, T& diag) {
-  const uint32_t c = b * Geo<BW>::BV + t;
-  sum = (T)0;
-  diag = (T)6;
+// Neighbour tile entries (uncoupled ones point at the zero entry) and diagonal of the cell at
+// tile entry a.
+template <int A, class T>
+__device__ __forceinline__ T svCoupling(const int16_t* K, int a, T gm, int idx[6]) {
+  const int n[6] = {a - A * A, a + A * A, a - A, a + A, a - 1, a + 1};
+  T diag = (T)6;
 #pragma unroll
   for (int d = 0; d < 6; d++) {
-    bool co;
-    const uint32_t nb = faceNb<BW>(g, b, t, d, co);
-    if (nb == c) {
-      diag -= (T)1;
-    } else if (!co && reach[nb] <= lmin) {
-      sum += e[nb];
-    } else {
-      diag -= gm;
+    const int k = K[n[d]];
+    idx[d] = k >= 0 ? k : 0;
+    diag -= k >= 0 ? (T)0 : (k == SV_NEU ? (T)1 : gm);
+  }
+  return diag;
+}
+
+// One smoothing pass on the listed blocks (one colour of grid `level`, active cells: reach <=
+// lmin): nu red-black Gauss-Seidel sweeps of the block in LDS against its faces. ownZero: the
+// block's correction starts at 0 this pass; haloZero: so do its neighbours'. rho is in density
+// units (R / h^2); on the solved level itself the pending mean correction delta is taken from it.
+template <int BW, class T>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_sv_smooth(DGBase g, int level, int lmin, const uint32_t* __restrict__ list,
+                                                         uint32_t count, int nu, int ownZero, int haloZero,
+                                                         T* __restrict__ e, const T* __restrict__ rho,
+                                                         const uint8_t* __restrict__ reach, float h2, float gm,
+                                                         const SolveCtl* __restrict__ ctl) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
+  __shared__ T Vs[BPW][A3];
+  __shared__ int16_t Ks[BPW][A3];
+  if (solveDone(ctl)) return;
+  const T delta = level == lmin ? (T)ctl->delta : (T)0;
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  const int a = apronIndexOfLocal<BW>(t);
+  int lx, ly, lz;
+  locPos<BB>((uint32_t)t, lx, ly, lz);
+  const int par = (lx + ly + lz) & 1;
+  T* V = Vs[sub];
+  int16_t* K = Ks[sub];
+  for (uint32_t base = blockIdx.x * BPW; base < count; base += gridDim.x * BPW) {
+    const uint32_t i = base + sub;
+    const bool work = i < count;
+    const uint32_t b = list[g.offs[level] + (work ? i : count - 1)];
+    __syncthreads();  // the previous block's tile is no longer read
+    const T r = work ? rho[b * BV + t] - delta : (T)0;
+    svStageBlock<BW, T>(g, b, t, lmin, e, reach, ownZero != 0, haloZero != 0, work, V, K);
+    __syncthreads();
+    int idx[6];
+    const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
+    const bool act = work && K[a] >= 0;
+    const T hr = (T)h2 * r;
+    const T inv = (T)1 / diag;
+    for (int s = 0; s < nu; s++) {
+#pragma unroll
+      for (int col = 0; col < 2; col++) {
+        if (act && par == col) V[a] = (V[idx[0]] + V[idx[1]] + V[idx[2]] + V[idx[3]] + V[idx[4]] + V[idx[5]] + hr) * inv;
+        __syncthreads();
+      }
+    }
+    if (act) e[b * BV + t] = V[a];
+  }
+}
+
+// Residual of the correction equation on the active cells of grid `level`, restricted to the
+// parent cells: the mean over each restriction group (inactive cells contribute 0). TOP: every
+// block of the (dense) top level, the parents are the first pyramid grid D1, whose activity is
+// recorded too (it changes with lmin); else the listed blocks (both colours, n0 + n1) and the
+// cells of level + 1.
+template <int BW, class T, bool TOP>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_sv_restrict(DGBase g, int level, int lmin, const uint32_t* __restrict__ list0,
+                                                           const uint32_t* __restrict__ list1, uint32_t n0, uint32_t n1,
+                                                           const T* __restrict__ e, T* __restrict__ rho,
+                                                           const uint8_t* __restrict__ reach, float h2, float gm,
+                                                           T* __restrict__ urho, uint8_t* __restrict__ uact,
+                                                           UDimArg D1, const SolveCtl* __restrict__ ctl) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
+  __shared__ T Vs[BPW][A3];
+  __shared__ int16_t Ks[BPW][A3];
+  __shared__ T Rs[BPW][BV];
+  if (solveDone(ctl)) return;
+  const T delta = level == lmin ? (T)ctl->delta : (T)0;
+  const uint32_t total = TOP ? g.ctl[CTL_LOADS + level] : n0 + n1;
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  const int a = apronIndexOfLocal<BW>(t);
+  T* V = Vs[sub];
+  int16_t* K = Ks[sub];
+  for (uint32_t base = blockIdx.x * BPW; base < total; base += gridDim.x * BPW) {
+    const uint32_t i = base + sub;
+    bool work = i < total;
+    const uint32_t j = work ? i : total - 1;
+    const uint32_t b = TOP ? g.offs[level] + j : (j < n0 ? list0[g.offs[level] + j] : list1[g.offs[level] + j - n0]);
+    if (!TOP && work) work = g.par[b] != NFU;
+    __syncthreads();
+    const T r = work ? rho[b * BV + t] - delta : (T)0;
+    svStageBlock<BW, T>(g, b, t, lmin, e, reach, false, false, work, V, K);
+    __syncthreads();
+    int idx[6];
+    const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
+    const bool act = work && K[a] >= 0;
+    const T sum = V[idx[0]] + V[idx[1]] + V[idx[2]] + V[idx[3]] + V[idx[4]] + V[idx[5]];
+    Rs[sub][t] = act ? r - (diag * V[a] - sum) / (T)h2 : (T)0;
+    __syncthreads();
+    if (work && t < BV / 8) {
+      T acc = (T)0;
+      bool anyAct = false;
+#pragma unroll
+      for (int j8 = 0; j8 < 8; j8++) {
+        acc += Rs[sub][8 * t + j8];
+        anyAct |= K[apronIndexOfLocal<BW>(8 * t + j8)] >= 0;
+      }
+      if (TOP) {
+        const int3 p0 = svOrigin<BW>(g, level, b);
+        int x, y, z;
+        locPos<BB>((uint32_t)(8 * t), x, y, z);
+        const uint32_t u = D1.off + ((uint32_t)((p0.z + z) >> 1) * D1.h + ((p0.y + y) >> 1)) * D1.w + ((p0.x + x) >> 1);
+        urho[u] = acc * (T).125;
+        uact[u] = anyAct ? 1 : 0;
+      } else {
+        rho[parentCellOf<BW>(g, b, 8 * t)] = acc * (T).125;
+      }
     }
   }
 }
 
-// One red-black Gauss-Seidel half-sweep on the active cells of `level` with parity `color`.
+// e += P e(parent level) on the active cells of the listed blocks of `level` (both colours):
+// the 27/9/3/1 trilinear stencil of k_prolongate over the parent's apron, with any stencil cell
+// that is not an active cell of the parent level (coarser, or outside the region being solved)
+// replaced by the own parent cell.
 template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_rbgs(DGBase g, int level, int lmin, int color, T* __restrict__ e,
-                                                 const T* __restrict__ rho, const uint8_t* __restrict__ reach,
-                                                 float h2, float gm) {
-  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t b = g.offs[level] + i / BV;
-    const int t = (int)(i % BV);
-    const uint32_t c = b * BV + t;
-    if (reach[c] > lmin) continue;
-    const int3 p0 = g.pos[b];
-    int x, y, z;
-    locPos<BB>((uint32_t)t, x, y, z);
-    if (((p0.x + x + p0.y + y + p0.z + z) & 1) != color) continue;
-    T sum, diag;
-    svStencil<BW, T>(g, b, t, lmin, e, reach, (T)gm, sum, diag);
-    e[c] = (sum + (T)h2 * rho[c]) / diag;
-  }
-}
-
-// Residual of the correction equation on the active cells of `level`, restricted to the parent
-// cells (level + 1): the mean over each restriction group (inactive cells contribute 0). The
-// parents' correction starts at 0.
-template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_restrict(DGBase g, int level, int lmin, T* __restrict__ e,
-                                                     T* __restrict__ rho, const uint8_t* __restrict__ reach,
-                                                     float h2, float gm) {
-  constexpr int BV = Geo<BW>::BV, GPB = BV / 8;
-  const uint32_t groups = g.ctl[CTL_LOADS + level] * GPB;
-  const T inv = (T)1 / (T)h2;
-  for (uint32_t gi = blockIdx.x * 256 + threadIdx.x; gi < groups; gi += gridDim.x * 256) {
-    const uint32_t b = g.offs[level] + gi / GPB;
-    const int t0 = 8 * (int)(gi % GPB);
-    if (g.par[b] == NFU) continue;
-    T acc = (T)0;
-    for (int j = 0; j < 8; j++) {
-      const uint32_t c = b * BV + t0 + j;
-      if (reach[c] > lmin) continue;
-      T sum, diag;
-      svStencil<BW, T>(g, b, t0 + j, lmin, e, reach, (T)gm, sum, diag);
-      acc += rho[c] - (diag * e[c] - sum) * inv;
-    }
-    const uint32_t pc = parentCellOf<BW>(g, b, t0);
-    rho[pc] = acc * (T).125;
-    e[pc] = (T)0;
-  }
-}
-
-// e += P e(parent level) on the active cells of `level`: the 27/9/3/1 trilinear stencil of
-// k_prolongate over the parent's apron, with any stencil cell that is not an active cell of the
-// parent level (coarser, or outside the region being solved) replaced by the own parent cell.
-template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_prolong(DGBase g, int level, int lmin, T* __restrict__ e,
-                                                    const uint8_t* __restrict__ reach) {
-  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t b = g.offs[level] + i / BV;
-    const int t = (int)(i % BV);
+__global__ void __launch_bounds__(Geo<BW>::WG) k_sv_prolong(DGBase g, int level, int lmin, const uint32_t* __restrict__ list0,
+                                                          const uint32_t* __restrict__ list1, uint32_t n0, uint32_t n1,
+                                                          T* __restrict__ e, const uint8_t* __restrict__ reach,
+                                                          const SolveCtl* __restrict__ ctl) {
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW;
+  if (solveDone(ctl)) return;
+  const uint32_t total = n0 + n1;
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  int lx, ly, lz;
+  locPos<BB>((uint32_t)t, lx, ly, lz);
+  for (uint32_t base = blockIdx.x * BPW; base < total; base += gridDim.x * BPW) {
+    const uint32_t j = base + sub;
+    if (j >= total) continue;
+    const uint32_t b = j < n0 ? list0[g.offs[level] + j] : list1[g.offs[level] + j - n0];
     const uint32_t c = b * BV + t;
     const uint32_t ps = g.par[b];
     if (reach[c] > lmin || ps == NFU) continue;
     const int3 p0 = g.pos[b];
-    int lx, ly, lz;
-    locPos<BB>((uint32_t)t, lx, ly, lz);
     uint32_t q[8];
     parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
     T v[8];
@@ -263,54 +371,15 @@ __global__ void __launch_bounds__(256) k_sv_prolong(DGBase g, int level, int lmi
 }
 
 // ---------------------------------------------------------------- uniform pyramid of the top level
-struct UDim {
-  int w, h, d;
-  uint32_t off;
-};
-__device__ __forceinline__ uint32_t uAt(const UDim& D, int x, int y, int z) {
+__device__ __forceinline__ uint32_t uAt(const UDimArg& D, int x, int y, int z) {
   return D.off + ((uint32_t)z * D.h + y) * D.w + x;
-}
-
-// Top-level residual restricted into the first pyramid grid (dims of the top level / 2): the
-// 8 cells of a restriction group form a 2x2x2 cube.
-template <int BW, class T>
-__global__ void __launch_bounds__(256) k_sv_restrict_top(DGBase g, int lmin, const T* __restrict__ e,
-                                                         const T* __restrict__ rho,
-                                                         const uint8_t* __restrict__ reach, T* __restrict__ ue,
-                                                         T* __restrict__ urho, uint8_t* __restrict__ uact, UDim D1,
-                                                         float h2, float gm) {
-  constexpr int BV = Geo<BW>::BV, GPB = BV / 8, BB = Geo<BW>::BB;
-  const int top = g.L - 1;
-  const uint32_t groups = g.ctl[CTL_LOADS + top] * GPB;
-  const T inv = (T)1 / (T)h2;
-  for (uint32_t gi = blockIdx.x * 256 + threadIdx.x; gi < groups; gi += gridDim.x * 256) {
-    const uint32_t b = g.offs[top] + gi / GPB;
-    const int t0 = 8 * (int)(gi % GPB);
-    T acc = (T)0;
-    bool any = false;
-    for (int j = 0; j < 8; j++) {
-      const uint32_t c = b * BV + t0 + j;
-      if (reach[c] > lmin) continue;
-      any = true;
-      T sum, diag;
-      svStencil<BW, T>(g, b, t0 + j, lmin, e, reach, (T)gm, sum, diag);
-      acc += rho[c] - (diag * e[c] - sum) * inv;
-    }
-    const int3 p0 = g.pos[b];
-    int x, y, z;
-    locPos<BB>((uint32_t)t0, x, y, z);
-    const uint32_t u = uAt(D1, (p0.x + x) >> 1, (p0.y + y) >> 1, (p0.z + z) >> 1);
-    urho[u] = acc * (T).125;
-    ue[u] = (T)0;
-    uact[u] = any ? 1 : 0;
-  }
 }
 
 // Trilinear 27/9/3/1 prolongation from pyramid grid Dc at fine cell (x,y,z) of a grid of size
 // (w,h,d) (edge rows collapse onto their own parent, as k_u_prolongate); inactive coarse cells
 // are replaced by the own parent.
 template <class T>
-__device__ __forceinline__ T uProlong(const UDim& Dc, int x, int y, int z, int w, int h, int d,
+__device__ __forceinline__ T uProlong(const UDimArg& Dc, int x, int y, int z, int w, int h, int d,
                                       const T* __restrict__ ue, const uint8_t* __restrict__ uact) {
   const int X = x >> 1, Y = y >> 1, Z = z >> 1;
   const int sx = (x == 0 || x == w - 1) ? 0 : 2 * (x & 1) - 1;
@@ -335,8 +404,10 @@ __device__ __forceinline__ T uProlong(const UDim& Dc, int x, int y, int z, int w
 template <int BW, class T>
 __global__ void __launch_bounds__(256) k_sv_prolong_top(DGBase g, int lmin, T* __restrict__ e,
                                                         const uint8_t* __restrict__ reach, const T* __restrict__ ue,
-                                                        const uint8_t* __restrict__ uact, UDim D1) {
+                                                        const uint8_t* __restrict__ uact, UDimArg D1,
+                                                        const SolveCtl* __restrict__ ctl) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
+  if (solveDone(ctl)) return;
   const int top = g.L - 1;
   const uint32_t n = g.ctl[CTL_LOADS + top] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -352,9 +423,9 @@ __global__ void __launch_bounds__(256) k_sv_prolong_top(DGBase g, int lmin, T* _
 }
 
 // Pyramid stencil: clamped (Neumann) at the domain walls; inactive cells are the Dirichlet
-// boundary of the solved region (ghost gm * e_c, as in svStencil).
+// boundary of the solved region (ghost gm * e_c).
 template <class T>
-__device__ __forceinline__ void uStencil(const UDim& D, int x, int y, int z, const T* __restrict__ e,
+__device__ __forceinline__ void uStencil(const UDimArg& D, int x, int y, int z, const T* __restrict__ e,
                                          const uint8_t* __restrict__ act, T gm, T& sum, T& diag) {
   sum = (T)0;
   diag = (T)6;
@@ -373,25 +444,87 @@ __device__ __forceinline__ void uStencil(const UDim& D, int x, int y, int z, con
   }
 }
 
+// One smoothing pass on the tiles of `colour` of pyramid grid D (8^3 tiles, colour = tile
+// coordinate parity): nu red-black sweeps of the tile in LDS against its faces, as k_sv_smooth.
 template <class T>
-__global__ void __launch_bounds__(256) k_svu_rbgs(UDim D, int color, T* __restrict__ e, const T* __restrict__ rho,
-                                                  const uint8_t* __restrict__ act, float h2, float gm) {
-  const uint32_t n = (uint32_t)D.w * D.h * D.d;
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int x = (int)(i % D.w), y = (int)((i / D.w) % D.h), z = (int)(i / ((uint32_t)D.w * D.h));
-  if (((x + y + z) & 1) != color) return;
-  const uint32_t u = D.off + i;
-  if (!act[u]) return;
-  T sum, diag;
-  uStencil<T>(D, x, y, z, e, act, (T)gm, sum, diag);
-  e[u] = (sum + (T)h2 * rho[u]) / diag;
+__global__ void __launch_bounds__(512) k_svu_smooth(UDimArg D, int colour, int nu, int ownZero, int haloZero,
+                                                    T* __restrict__ e, const T* __restrict__ rho,
+                                                    const uint8_t* __restrict__ act, float h2, float gm,
+                                                    const SolveCtl* __restrict__ ctl) {
+  constexpr int TW = 8, A = TW + 2, A3 = A * A * A, FACE = 6 * TW * TW;
+  __shared__ T V[A3];
+  __shared__ int16_t K[A3];
+  if (solveDone(ctl)) return;
+  const int ntx = (D.w + TW - 1) / TW, nty = (D.h + TW - 1) / TW, ntz = (D.d + TW - 1) / TW;
+  const int tiles = ntx * nty * ntz;
+  const int t = threadIdx.x;
+  const int lx = t % TW, ly = (t / TW) % TW, lz = t / (TW * TW);
+  const int a = ((lx + 1) * A + (ly + 1)) * A + (lz + 1);
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    if (((tx + ty + tz) & 1) != colour) continue;  // uniform over the workgroup
+    __syncthreads();
+    const int x0 = tx * TW, y0 = ty * TW, z0 = tz * TW;
+    // own cell
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    const bool in = x < D.w && y < D.h && z < D.d;
+    const uint32_t u = in ? uAt(D, x, y, z) : 0;
+    const int ac = in ? act[u] : 0;
+    const T vc = (in && !ownZero) ? e[u] : (T)0;
+    const T r = in ? rho[u] : (T)0;
+    // face entries of the halo (the clamp and inactive codes as in uStencil)
+    int fx = 0, fy = 0, fz = 0, fa = 0, fcode = SV_GHOST;
+    T fv = (T)0;
+    if (t < FACE) {
+      int ai, aj, ak;
+      faceCell<TW>(t, ai, aj, ak);
+      fa = (ai * A + aj) * A + ak;
+      fx = x0 + ai - 1;
+      fy = y0 + aj - 1;
+      fz = z0 + ak - 1;
+      if (fx < 0 || fy < 0 || fz < 0 || fx >= D.w || fy >= D.h || fz >= D.d) {
+        fcode = SV_NEU;
+      } else {
+        const uint32_t fu = uAt(D, fx, fy, fz);
+        const int fact = act[fu];
+        fv = haloZero ? (T)0 : e[fu];
+        fcode = fact ? fa : SV_GHOST;
+      }
+    }
+    V[a] = vc;
+    K[a] = in ? (ac ? (int16_t)a : (int16_t)SV_GHOST) : (int16_t)SV_NEU;
+    if (t < FACE) {
+      V[fa] = fv;
+      K[fa] = (int16_t)fcode;
+    }
+    if (t == 0) {
+      V[0] = (T)0;
+      K[0] = (int16_t)SV_GHOST;
+    }
+    __syncthreads();
+    int idx[6];
+    const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
+    const bool live = in && ac;
+    const T hr = (T)h2 * r;
+    const T inv = (T)1 / diag;
+    const int par = (x + y + z) & 1;
+    for (int s = 0; s < nu; s++) {
+#pragma unroll
+      for (int col = 0; col < 2; col++) {
+        if (live && par == col) V[a] = (V[idx[0]] + V[idx[1]] + V[idx[2]] + V[idx[3]] + V[idx[4]] + V[idx[5]] + hr) * inv;
+        __syncthreads();
+      }
+    }
+    if (live) e[u] = V[a];
+  }
 }
 
 // Residual of pyramid grid D restricted into the next grid Dc (thread per coarse cell).
 template <class T>
-__global__ void __launch_bounds__(256) k_svu_restrict(UDim D, UDim Dc, T* __restrict__ e, T* __restrict__ rho,
-                                                      uint8_t* __restrict__ act, float h2, float gm) {
+__global__ void __launch_bounds__(256) k_svu_restrict(UDimArg D, UDimArg Dc, const T* __restrict__ e,
+                                                      T* __restrict__ rho, uint8_t* __restrict__ act, float h2, float gm,
+                                                      const SolveCtl* __restrict__ ctl) {
+  if (solveDone(ctl)) return;
   const uint32_t n = (uint32_t)Dc.w * Dc.h * Dc.d;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -409,12 +542,13 @@ __global__ void __launch_bounds__(256) k_svu_restrict(UDim D, UDim Dc, T* __rest
     acc += rho[u] - (diag * e[u] - sum) * inv;
   }
   rho[Dc.off + i] = acc * (T).125;
-  e[Dc.off + i] = (T)0;
   act[Dc.off + i] = any ? 1 : 0;
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_svu_prolong(UDim D, UDim Dc, T* __restrict__ e, const uint8_t* __restrict__ act) {
+__global__ void __launch_bounds__(256) k_svu_prolong(UDimArg D, UDimArg Dc, T* __restrict__ e,
+                                                     const uint8_t* __restrict__ act, const SolveCtl* __restrict__ ctl) {
+  if (solveDone(ctl)) return;
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -424,48 +558,389 @@ __global__ void __launch_bounds__(256) k_svu_prolong(UDim D, UDim Dc, T* __restr
   e[u] = e[u] + uProlong<T>(Dc, x, y, z, D.w, D.h, D.d, e, act);
 }
 
-// Coarsest pyramid grid: `sweeps` red-black Gauss-Seidel sweeps in one workgroup.
+// The coarse end of the pyramid (grids jc..J-1, at most kCoarseCells cells together) in one
+// workgroup with every grid in LDS: sweeps, restriction of the residual, down to the coarsest
+// grid (bottom sweeps there), then prolongation and sweeps back up. Reads rho and the active
+// cells of grid jc, writes its correction. Each grid's couplings (a neighbour flag per face,
+// parity, activity) and inverse diagonal are computed once per call; grids of at most 64 cells
+// are swept by one wavefront without workgroup barriers.
+static constexpr int kCoarseCells = 4800;
+static constexpr int kCoarseThreads = 1024;
+static constexpr int kMaxPyr = 12;
+struct CoarseArgs {
+  UDimArg D[kMaxPyr];   // grids jc.., offsets into the workgroup's LDS arrays
+  float h2[kMaxPyr];
+  float gm[kMaxPyr];
+  uint32_t goff;        // offset of grid jc in the pyramid arrays
+  int n, sweeps, bottom;
+};
+__device__ __forceinline__ void cCoord(const UDimArg& D, uint32_t i, int& x, int& y, int& z) {
+  x = (int)(i % (uint32_t)D.w);
+  const uint32_t q = i / (uint32_t)D.w;
+  y = (int)(q % (uint32_t)D.h);
+  z = (int)(q / (uint32_t)D.h);
+}
+// fl: bits 0-5 neighbour d (x-, x+, y-, y+, z-, z+) coupled (inside the grid and active), bit 6
+// parity, bit 7 active; inv = 1 / diagonal (walls Neumann, inactive neighbours ghosts gm e_c).
 template <class T>
-__global__ void __launch_bounds__(1024) k_svu_bottom(UDim D, T* __restrict__ e, const T* __restrict__ rho,
-                                                     const uint8_t* __restrict__ act, float h2, float gm, int sweeps) {
+__device__ __forceinline__ void cPrepare(const UDimArg& D, const uint8_t* act, uint8_t* fl, T* inv, T gm) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    int x, y, z;
+    cCoord(D, i, x, y, z);
+    const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
+    uint32_t f = 0;
+    T diag = (T)6;
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      if (nx[d] < 0 || ny[d] < 0 || nz[d] < 0 || nx[d] >= D.w || ny[d] >= D.h || nz[d] >= D.d) {
+        diag -= (T)1;
+      } else if (act[uAt(D, nx[d], ny[d], nz[d])]) {
+        f |= 1u << d;
+      } else {
+        diag -= gm;
+      }
+    }
+    const uint32_t u = D.off + i;
+    f |= (uint32_t)((x + y + z) & 1) << 6;
+    f |= act[u] ? 0x80u : 0u;
+    fl[u] = (uint8_t)f;
+    inv[u] = (T)1 / diag;
+  }
+}
+// Sum of the coupled neighbours of LDS cell u: all six loads issued unconditionally (indices
+// clamped into the arrays) and selected by the flags, so a cell costs one round of LDS latency.
+template <class T>
+__device__ __forceinline__ T cNeighbourSum(const UDimArg& D, uint32_t u, uint32_t f, const T* e) {
+  const int sy = D.w, sz = D.w * D.h, last = kCoarseCells - 1, iu = (int)u;
+  const T a0 = e[max(iu - 1, 0)], a1 = e[min(iu + 1, last)];
+  const T a2 = e[max(iu - sy, 0)], a3 = e[min(iu + sy, last)];
+  const T a4 = e[max(iu - sz, 0)], a5 = e[min(iu + sz, last)];
+  T sum = (f & 1u) ? a0 : (T)0;
+  sum += (f & 2u) ? a1 : (T)0;
+  sum += (f & 4u) ? a2 : (T)0;
+  sum += (f & 8u) ? a3 : (T)0;
+  sum += (f & 16u) ? a4 : (T)0;
+  sum += (f & 32u) ? a5 : (T)0;
+  return sum;
+}
+template <class T>
+__device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, const uint8_t* fl, const T* inv, T h2,
+                                        int sweeps) {
+  const uint32_t n = (uint32_t)D.w * D.h * D.d;
+  const bool wave = n <= 64;  // one wavefront, ordered LDS accesses: no workgroup barrier
+  if (wave && threadIdx.x >= 64) {
+    __syncthreads();
+    return;
+  }
   for (int s = 0; s < 2 * sweeps; s++) {
-    const int color = s & 1;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-      const int x = (int)(i % D.w), y = (int)((i / D.w) % D.h), z = (int)(i / ((uint32_t)D.w * D.h));
-      if (((x + y + z) & 1) != color) continue;
+    const uint32_t colour = (uint32_t)(s & 1);
+    for (uint32_t i = threadIdx.x; i < n; i += (wave ? 64u : blockDim.x)) {
       const uint32_t u = D.off + i;
-      if (!act[u]) continue;
-      T sum, diag;
-      uStencil<T>(D, x, y, z, e, act, (T)gm, sum, diag);
-      e[u] = (sum + (T)h2 * rho[u]) / diag;
+      const uint32_t f = fl[u];
+      const T v = (cNeighbourSum<T>(D, u, f, e) + h2 * rho[u]) * inv[u];
+      if ((f & 0x80u) && ((f >> 6) & 1u) == colour) e[u] = v;
+    }
+    if (wave) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
+  }
+  if (wave) __syncthreads();
+}
+template <class T>
+__global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T* __restrict__ ue, const T* __restrict__ urho,
+                                                               const uint8_t* __restrict__ uact,
+                                                               const SolveCtl* __restrict__ ctl) {
+  __shared__ T e[kCoarseCells];
+  __shared__ T rho[kCoarseCells];
+  __shared__ T inv[kCoarseCells];
+  __shared__ uint8_t act[kCoarseCells];
+  __shared__ uint8_t fl[kCoarseCells];
+  if (solveDone(ctl)) return;
+  const UDimArg D0 = ca.D[0];
+  const uint32_t n0 = (uint32_t)D0.w * D0.h * D0.d;
+  for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
+    e[i] = (T)0;
+    rho[i] = urho[ca.goff + i];
+    act[i] = uact[ca.goff + i];
+  }
+  __syncthreads();
+  for (int j = 0; j + 1 < ca.n; j++) {
+    const UDimArg F = ca.D[j], Cg = ca.D[j + 1];
+    const T h2j = (T)ca.h2[j];
+    cPrepare<T>(F, act, fl, inv, (T)ca.gm[j]);
+    __syncthreads();
+    cSweeps<T>(F, e, rho, fl, inv, h2j, ca.sweeps);
+    // residual restricted into the next grid (mean over the 2x2x2 children, inactive ones 0)
+    const uint32_t nc = (uint32_t)Cg.w * Cg.h * Cg.d;
+    const T rh2 = (T)1 / h2j;
+    for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
+      int X, Y, Z;
+      cCoord(Cg, i, X, Y, Z);
+      T acc = (T)0;
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t u = uAt(F, 2 * X + ((k >> 2) & 1), 2 * Y + ((k >> 1) & 1), 2 * Z + (k & 1));
+        const uint32_t f = fl[u];
+        const T r = rho[u] - (e[u] / inv[u] - cNeighbourSum<T>(F, u, f, e)) * rh2;
+        const bool a = (f & 0x80u) != 0;
+        any |= a;
+        acc += a ? r : (T)0;
+      }
+      rho[Cg.off + i] = acc * (T).125;
+      e[Cg.off + i] = (T)0;
+      act[Cg.off + i] = any ? 1 : 0;
     }
     __syncthreads();
   }
+  {
+    const int l = ca.n - 1;
+    const UDimArg Dl = ca.D[l];
+    cPrepare<T>(Dl, act, fl, inv, (T)ca.gm[l]);
+    __syncthreads();
+    cSweeps<T>(Dl, e, rho, fl, inv, (T)ca.h2[l], ca.bottom);
+  }
+  for (int j = ca.n - 2; j >= 0; j--) {
+    const UDimArg F = ca.D[j], Cg = ca.D[j + 1];
+    const uint32_t nf = (uint32_t)F.w * F.h * F.d;
+    for (uint32_t i = threadIdx.x; i < nf; i += blockDim.x) {
+      const uint32_t u = F.off + i;
+      if (!act[u]) continue;
+      int x, y, z;
+      cCoord(F, i, x, y, z);
+      e[u] = e[u] + uProlong<T>(Cg, x, y, z, F.w, F.h, F.d, e, act);
+    }
+    __syncthreads();
+    cSweeps<T>(F, e, rho, fl, inv, (T)ca.h2[j], ca.sweeps);
+  }
+  for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x)
+    if (act[i]) ue[ca.goff + i] = e[i];
 }
 
-// x += e on the cells of a level (the refinement update; FP64 in the mixed solve).
-template <int BW, class TX, class T>
-__global__ void __launch_bounds__(256) k_sv_update(DGBase g, int level, TX* __restrict__ x, const T* __restrict__ e,
-                                                   const uint8_t* __restrict__ reach) {
-  constexpr int BV = Geo<BW>::BV;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t c = g.offs[level] * BV + i;
-    x[c] = x[c] + (TX)e[c];
+// ---------------------------------------------------------------- refinement loop of a level
+// Residual after the update x' = x + e (first: x' = x, no correction yet) on the cells of
+// `level`, block by block with the faces staged in LDS: writes x' to xout,
+// rho = (R - shift) / h^2, and per workgroup the partial sums of R, R^2, R0, R0^2 and x' (R0:
+// the residual of a zero level pressure). Same-level neighbours come from xin (+ e), coarser
+// ones (already solved ghosts) from xc. The partials are combined by k_sv_decide in the next
+// launch: a kernel boundary makes them visible without a device-scope fence per workgroup (on
+// MI355X such a fence writes the L2 back, which cost this kernel 130 us per launch).
+template <int BW, class TX, class T, class G>
+__global__ void __launch_bounds__(Geo<BW>::WG) k_sv_residual(G g, int level, const TX* __restrict__ xc,
+                                                           const TX* __restrict__ xin, TX* __restrict__ xout,
+                                                           const T* __restrict__ e, int first,
+                                                           T* __restrict__ rho, double* __restrict__ part, float alpha,
+                                                           float h2, const SolveCtl* __restrict__ ctl) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, WG = Geo<BW>::WG;
+  constexpr int FACE = Geo<BW>::FACE, FPT = (FACE + BV - 1) / BV;
+  __shared__ TX Xs[BPW][A3];
+  __shared__ uint8_t Gs[BPW][A3];  // 1: coarser-level ghost
+  __shared__ double red[5][WG / 64];
+  if (!first && solveDone(ctl)) return;
+  const double shift = first ? 0.0 : ctl->shift;
+  const uint32_t total = slotTotal(g, level);
+  const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
+  const int a = apronIndexOfLocal<BW>(t);
+  TX* X = Xs[sub];
+  uint8_t* Gh = Gs[sub];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
+    const uint32_t slot = base + sub;
+    const bool valid = slot < total;
+    const uint32_t b = g.offs[level] + (valid ? slot : total - 1);
+    const uint32_t c = b * BV + t;
+    __syncthreads();
+    uint32_t nb[FPT], sh[FPT];
+    int fa[FPT];
+#pragma unroll
+    for (int j = 0; j < FPT; j++) {
+      const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
+      int ai, aj, ak;
+      faceCell<BW>(fc, ai, aj, ak);
+      fa[j] = (ai * A + aj) * A + ak;
+      nb[j] = borderApronCellShift<BW>(g, b, ai, aj, ak, sh[j]);
+    }
+    const TX xv = xin[c] + (first ? (TX)0 : (TX)e[c]);
+    const float dv = (float)g.div[c];
+    TX fv[FPT];
+#pragma unroll
+    for (int j = 0; j < FPT; j++) {
+      const bool co = sh[j] != 0 && sh[j] != ND_OUT;
+      fv[j] = co ? xc[nb[j]] : xin[nb[j]] + (first ? (TX)0 : (TX)e[nb[j]]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    X[a] = xv;
+    Gh[a] = 0;
+#pragma unroll
+    for (int j = 0; j < FPT; j++) {
+      if (t + j * BV >= FACE) continue;
+      X[fa[j]] = fv[j];
+      Gh[fa[j]] = sh[j] != 0 && sh[j] != ND_OUT;
+    }
+    __syncthreads();
+    if (!valid) continue;
+    if (!first) xout[c] = xv;
+    const int n[6] = {a - A * A, a + A * A, a - A, a + A, a - 1, a + 1};
+    TX sum = 0, ghosts = 0;
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      sum += X[n[d]];
+      ghosts += Gh[n[d]] ? X[n[d]] : (TX)0;
+    }
+    const TX ad = (TX)alpha * (TX)dv;
+    const TX R = sum - (TX)6 * xv - ad;
+    const TX R0 = ghosts - ad;
+    rho[c] = (T)((R - (TX)shift) / (TX)h2);
+    s[0] += (double)R;
+    s[1] += (double)R * (double)R;
+    s[2] += (double)R0;
+    s[3] += (double)R0 * (double)R0;
+    s[4] += (double)xv;
+  }
+  // workgroup sums in a fixed order: wavefront, then across wavefronts
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    double v = s[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double v = 0;
+    for (int w = 0; w < WG / 64; w++) v += red[threadIdx.x][w];
+    part[5 * blockIdx.x + threadIdx.x] = v;
   }
 }
 
-// Initial guess of a level from its parent level: the 27/9/3/1 stencil of k_prolongate on x.
+// One workgroup after k_sv_residual (nparts of its workgroups): sums the partials in workgroup
+// order (the result does not depend on scheduling) and decides in ctl whether the level is done
+// (|R| <= tol |R(0)|, the cycle limit, or three cycles that did not reduce |R| by 0.7 — the
+// working precision is exhausted); if not, the next cycle runs. A level that covers the domain
+// (loads == capacity == full) is pure Neumann: its residual's mean is removed. The control block
+// is mirrored into host memory (`mirror`), which the host reads after a synchronisation.
+__global__ void __launch_bounds__(256) k_sv_decide(DGBase g, int level, int bv, int first, int nparts,
+                                                   const double* __restrict__ part, float h2, double tol,
+                                                   int maxCycles, SolveCtl* __restrict__ ctl,
+                                                   SolveCtl* __restrict__ mirror) {
+  __shared__ double red[5][4];
+  if (!first && solveDone(ctl)) return;
+  double S[5] = {0, 0, 0, 0, 0};
+  for (int w = threadIdx.x; w < nparts; w += 256)
+#pragma unroll
+    for (int k = 0; k < 5; k++) S[k] += part[5 * w + k];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    double v = S[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double T5[5];
+  for (int k = 0; k < 5; k++) T5[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+  const uint32_t loads = g.ctl[CTL_LOADS + level];
+  const bool singular = loads == g.maxL[level] && g.maxL[level] == g.fullL[level];
+  const double N = (double)loads * bv;
+  double rr, r0c;
+  const double m = N > 0 ? T5[0] / N : 0.0;
+  if (singular) {
+    const double m0 = N > 0 ? T5[2] / N : 0.0;
+    rr = sqrt(fmax(T5[1] - N * m * m, 0.0));
+    r0c = sqrt(fmax(T5[3] - N * m0 * m0, 0.0));
+  } else {
+    rr = sqrt(T5[1]);
+    r0c = sqrt(T5[3]);
+  }
+  SolveCtl c = *ctl;
+  if (first) {
+    c.r0 = r0c;
+    c.cycles = 0;
+    c.stall = 0;
+    c.prev = 0;
+    c.shift = 0;
+    c.singular = singular ? 1u : 0u;
+  }
+  c.mean = N > 0 ? T5[4] / N : 0.0;
+  c.rr = rr;
+  bool done = rr <= tol * c.r0 || (int)c.cycles >= maxCycles;
+  if (!done) {
+    if (c.cycles > 0 && rr > 0.7 * c.prev) {
+      if (++c.stall >= 3) done = true;
+    } else {
+      c.stall = 0;
+    }
+  }
+  c.prev = rr;
+  c.delta = 0;
+  if (!done) {
+    // the mean of R (the right-hand side's inconsistency) stays the same from cycle to cycle:
+    // the residual kernel subtracts the last one, the level's kernels take what is left
+    if (singular && m != c.shift) {
+      c.delta = (m - c.shift) / (double)h2;
+      c.shift = m;
+    }
+    c.cycles++;
+  }
+  c.done = done ? 1u : 0u;
+  *ctl = c;
+  *mirror = c;
+}
+
+// End of a level's loop: x (the final iterate, in xsrc) minus its mean on a singular level,
+// into the canonical iterate xdst and the pressure field.
+template <int BW, class TX, class G>
+__global__ void __launch_bounds__(256) k_sv_finalize(G g, int level, const TX* __restrict__ xsrc, TX* __restrict__ xdst,
+                                                     const SolveCtl* __restrict__ ctl) {
+  constexpr int BV = Geo<BW>::BV;
+  const TX m = ctl->singular ? (TX)ctl->mean : (TX)0;
+  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const uint32_t c = g.offs[level] * BV + i;
+    const TX v = xsrc[c] - m;
+    xdst[c] = v;
+    g.pre[c] = (float)v;
+  }
+}
+
+// x = 0 on the cells of a level.
+template <int BW, class T>
+__global__ void __launch_bounds__(256) k_sv_zero(DGBase g, int level, T* __restrict__ x) {
+  constexpr int BV = Geo<BW>::BV;
+  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[g.offs[level] * BV + i] = (T)0;
+}
+
+// Warm start: x = the pressure of the last projection on every allocated block that kept its
+// place in this step's adaptation (the others are prolonged from their parent level once it is
+// solved, k_sv_prolong_x with movedOnly).
+template <int BW, class TX, class G>
+__global__ void __launch_bounds__(256) k_sv_warm(G g, TX* __restrict__ x) {
+  constexpr int BV = Geo<BW>::BV;
+  for (int l = 0; l < g.L; l++) {
+    const uint32_t n = g.ctl[CTL_LOADS + l] * BV;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+      const uint32_t c = g.offs[l] * BV + i;
+      if (!(g.flg[c / BV] & MOVED)) x[c] = (TX)(float)g.pre[c];
+    }
+  }
+}
+
+// Initial guess of a level from its parent level: the 27/9/3/1 stencil of k_prolongate on x
+// (movedOnly: only the blocks moved or created in this step's adaptation).
 template <int BW, class TX>
-__global__ void __launch_bounds__(256) k_sv_prolong_x(DGBase g, int level, TX* __restrict__ x) {
+__global__ void __launch_bounds__(256) k_sv_prolong_x(DGBase g, int level, TX* __restrict__ x, int movedOnly) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const uint32_t b = g.offs[level] + i / BV;
     const int t = (int)(i % BV);
     const uint32_t ps = g.par[b];
-    if (ps == NFU) continue;
+    if (ps == NFU || (movedOnly && !(g.flg[b] & MOVED))) continue;
     const int3 p0 = g.pos[b];
     int lx, ly, lz;
     locPos<BB>((uint32_t)t, lx, ly, lz);
@@ -474,17 +949,6 @@ __global__ void __launch_bounds__(256) k_sv_prolong_x(DGBase g, int level, TX* _
     x[b * BV + t] = ((TX)27 * x[q[0]] + (TX)9 * (x[q[1]] + x[q[2]] + x[q[3]]) + (TX)3 * (x[q[4]] + x[q[5]] + x[q[6]]) +
                      x[q[7]]) /
                     (TX)64;
-  }
-}
-
-// pressure = (float) x on the cells of a level.
-template <int BW, class TX, class G>
-__global__ void __launch_bounds__(256) k_sv_store(G g, int level, const TX* __restrict__ x) {
-  constexpr int BV = Geo<BW>::BV;
-  const uint32_t n = g.ctl[CTL_LOADS + level] * BV;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    const uint32_t c = g.offs[level] * BV + i;
-    g.pre[c] = (float)x[c];
   }
 }
 

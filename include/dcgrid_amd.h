@@ -137,8 +137,9 @@ typedef struct dcg_params {
   int32_t solve_precision;  /* DCG_SOLVE_* */
   float solve_tol;          /* per-level relative residual |r| / |r(0)| to reach (1e-8; <= 0: 1e-8) */
   int32_t solve_max_cycles; /* refinement cycles per level before giving up (50; <= 0: 50) */
-  int32_t solve_smooth;     /* red-black Gauss-Seidel sweeps before and after each coarse
-                               correction (3; <= 0: 3; at most 10) */
+  int32_t solve_smooth;     /* smoothing passes before and after each coarse correction, each
+                               two red-black Gauss-Seidel sweeps per block colour
+                               (2; <= 0: 2; at most 5) */
 
   /* Terrain and moist atmosphere (extension). */
   int32_t solid_mode;     /* DCG_SOLID_* */
@@ -155,7 +156,11 @@ typedef struct dcg_params {
   float wind[3];          /* ambient wind, world units per second (initial field, lateral inflow) */
   float adiabatic_rate;   /* cooling of rising air per world unit of ascent (moist only) */
   int32_t storage;        /* DCG_STORAGE_* of the fields, fixed at creation (DCGrid only) */
-  int32_t reserved[2];
+  int32_t solve_cold;     /* DCG_PROJECT_SOLVE initial guess: 0 = the pressure of the last
+                             projection on every block that kept its place (blocks moved or
+                             created since are prolonged from their parent level), 1 = zero on
+                             the top level and prolongation below */
+  int32_t reserved[1];
 } dcg_params;
 
 /* Outcome of the last DCG_PROJECT_SOLVE projection, per level (0 finest). */
@@ -355,7 +360,8 @@ int dcg_render_host(dcg_sim* s, const dcg_render_params* r, float* rgba_host);
  *   12  uint32   header bytes = 72 + params bytes + 24 * sections
  *   16  int32    kind (DCG_KIND_*), storage (DCG_STORAGE_*), size[3], block_width, levels
  *               (the dcg_info values of the grid)
- *   44  uint32   flags (bit 0: the fluidity field was imported)
+ *   44  uint32   flags (bit 0: the fluidity field was imported; bit 1: the pressure field holds
+ *               a projection solve's solution, the next solve starts from it)
  *   48  uint32   params bytes (= sizeof(dcg_params)), 52 uint32 section count
  *   56  uint64   max_num_blocks, 64 uint64 num_cells (dcg_info)
  *   72  dcg_params
@@ -382,6 +388,13 @@ enum {
 int dcg_state_size(dcg_sim* s, uint64_t* bytes);
 int dcg_save_state(dcg_sim* s, void* dst, uint64_t bytes);
 int dcg_load_state(dcg_sim* s, const void* src, uint64_t bytes);
+/* A state buffer checked without a simulation (and without a device): header, section table
+ * and, for a DCGrid, the block tables against the planner's sizes of the grid the header
+ * describes: block loads within the level capacities (dense levels full), block levels
+ * matching the loads, positions inside their level, parent, child, neighbour and hash links
+ * pointing at allocated blocks of the right level. DCG_E_ARG on the first violation (message in
+ * dcg_last_error(NULL)). dcg_load_state runs the same check before it copies anything. */
+int dcg_state_check(const void* src, uint64_t bytes);
 
 /* Per-kernel timing with HIP events recorded on the handle's stream around every launch of the
  * kernel classes selected by `mask` (bit k = kernel id k below; 0 disables). Setting the mask
