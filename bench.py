@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the DCGrid simulation step on one MI355X.
 
-Workload (default): the adaptive smoke plume on a 256^3 virtual domain of 8^3-cell blocks, 3
-levels, a 4096-block budget, feature-driven refinement (density gradient 0.05, vorticity 0.5),
-the buoyant sphere source scaled 4x from the 64^3 plume, and the seeded 3-octave value-noise
-initial velocity of amplitude 0.2. Block tables are rebuilt every step. Synthetic, seeded inputs.
+Workload (default, configuration 2): the mixed-precision adaptive smoke plume on a 512^3 virtual
+domain of 8^3-cell blocks, 4 levels, a 32768-block budget, feature-driven refinement, the buoyant
+sphere source and the seeded value-noise initial velocity. Fields are stored in BF16 (FP32
+arithmetic); the projection of every step solves each level's pressure system to a 1e-8 relative
+residual by FP64 iterative refinement around FP32 multigrid cycles. Block tables are rebuilt
+every step. Synthetic, seeded inputs, resident in HBM before the timed region.
 
 One step = advectVelocity -> adaptTopology -> project -> advectDensity (src/simulation.cpp:104-111)
-through the C ABI. Prints one JSON line (see README of the contract in DESIGN.md):
-  value          active cell-steps/s (sum of allocated cells over the timed steps / time)
-  roofline       the Jacobi half-sweep kernel (the projection's hot loop): algorithmic bytes
-                 (12 B per cell per sweep: read p and div, write p~) over its HIP-event-timed
-                 average duration, against the 8 TB/s HBM peak
+through the C ABI. Prints one JSON line:
+  value          active cell-steps/s (allocated cells summed over the timed steps / time)
+  roofline       the step's dominant kernel with a byte model: algorithmic bytes over its
+                 HIP-event-timed average launch duration, against the 8 TB/s HBM peak
+  kernels        per kernel class: time per step, launches, algorithmic GB/s where defined
+  solve          the same projection solved from a cold start in mixed precision, all-FP32 and
+                 all-FP64: time, cycles per level, reached residual, pressure error against FP64
   cpu_baseline   the CPU oracle (FP32 restatement, OpenMP over all host cores) on a bounded
-                 sample of the same workload
+                 sample of the same scene
+
+Other workloads: --config 0..4 (3 = the adjoint rollout), --project reference (the reference's
+Jacobi schedule), --storage fp32/fp16/bf16, --jacobi-micro (uniform 512^3 Jacobi sweeps, a
+working set that does not fit the 256 MB Infinity Cache).
 """
 import argparse
 import json
@@ -25,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "active cell-steps/s + %HBM peak; mixed-prec vs FP32 projection at 1e-8 resid"
 
 
 def log(*a):
@@ -33,11 +42,12 @@ def log(*a):
 
 STORAGE_BYTES = {0: 4, 1: 2, 2: 2}  # dcg_params.storage -> bytes per stored scalar
 STORAGE_NAMES = {"fp32": 0, "fp16": 1, "bf16": 2}
+DTYPE = {0: "f32", 1: "f16", 2: "bf16"}
 
 
 def jacobi_bytes(info, params):
-    """Algorithmic bytes and launch count of the Jacobi half-sweeps of one project() call: each
-    half-sweep reads p and div and writes p (three stored scalars per cell)."""
+    """Algorithmic bytes and launch count of the Jacobi half-sweeps of one reference project()
+    call: each half-sweep reads p and div and writes p (three stored scalars per cell)."""
     S = STORAGE_BYTES[params.storage]
     bv = int(info.block_width) ** 3
     L = int(info.levels)
@@ -51,25 +61,45 @@ def jacobi_bytes(info, params):
     return total_bytes, launches
 
 
-def step_bytes(info, params):
-    """Algorithmic HBM bytes of one step (per-stage compulsory field traffic, no solids); S is the
-    stored scalar size (4 for FP32 fields, 2 for FP16/BF16), V = 3 S."""
-    bv = int(info.block_width) ** 3
-    L = int(info.levels)
+def per_cell_bytes(params, moist=False):
+    """Algorithmic bytes per active cell of the step's field kernels (S = stored scalar, V = 3 S,
+    F = fluidity when solids are on, k = advected scalars)."""
     S = STORAGE_BYTES[params.storage]
     V = 3 * S
+    F = S if params.enable_additional_solids else 0
+    src = params.source_mode != 0 or params.moist
+    k = (3 if params.moist else 2) if src else 1
+    return {
+        "advect_velocity": 2 * V + F + (2 * S if src else 0),
+        "advect_density": V + 2 * k * S + F,
+        "divergence": V + F + S,
+        "apply_pressure": 2 * V + F + S,
+    }
+
+
+def step_bytes(info, params, uniform=False):
+    """Algorithmic HBM bytes of one reference-schedule step (per-stage compulsory field traffic)."""
+    S = STORAGE_BYTES[params.storage]
+    V = 3 * S
+    pc = per_cell_bytes(params)
+    if uniform:
+        act = int(info.active_cells)
+        sweeps = 2 * (params.project_local_pairs if params.project_mode == 1 else params.project_level_pairs)
+        return act * (pc["advect_velocity"] + pc["advect_density"] + pc["divergence"] + pc["apply_pressure"] +
+                      12 * sweeps)
+    bv = int(info.block_width) ** 3
+    L = int(info.levels)
     src = params.source_mode != 0
-    k = 2 if src else 1  # advected scalars
+    k = 2 if src else 1
     act = sum(int(info.block_loads[l]) for l in range(L)) * bv
     below = sum(int(info.block_loads[l]) for l in range(L - 1)) * bv
     jb, _ = jacobi_bytes(info, params)
-    b = act * (2 * V + (2 * S if src else 0))          # advect u (+ buoyancy)
-    b += below * 9 * V // 8 * 2                         # restrict u twice
-    b += act * (V + S) + below * 9 * S // 8             # divergence + restrict div
-    b += jb                                             # Jacobi sweeps
-    b += sum(int(info.block_loads[l]) for l in range(L - 1)) * bv * S * 9 // 8  # prolongate
-    b += act * (2 * V + S)                              # gradient subtraction
-    b += act * (V + 2 * k * S) + below * 9 * k * S // 8  # advect scalars + restrict
+    b = act * (pc["advect_velocity"] + pc["divergence"] + pc["apply_pressure"] + pc["advect_density"])
+    b += below * 9 * V // 8 * 2          # restrict u twice
+    b += below * 9 * S // 8              # restrict div
+    b += jb                              # Jacobi sweeps
+    b += below * S * 9 // 8              # prolongate
+    b += below * 9 * k * S // 8          # restrict scalars
     return b
 
 
@@ -103,19 +133,20 @@ def measured_traffic(workload, kernel):
         return None
 
 
-def make_sim(cls, w, **kw):
-    s = cls(w.kind, w.size, w.params, w.max_num_blocks if w.kind == "dcgrid" else None, **kw)
+def make_sim(cls, w, params=None, **kw):
+    p = w.params if params is None else params
+    s = cls(w.kind, w.size, p, w.max_num_blocks if w.kind == "dcgrid" else None, **kw)
     s.seed_scene()
     return s
 
 
-def cpu_baseline(w, warm_steps, budget_s):
-    """The oracle (FP32) on the same seeded workload: warm_steps untimed, then steps until
-    about budget_s seconds of CPU work; reports active cell-steps/s."""
+def cpu_baseline(w, params, warm_steps, budget_s):
+    """The oracle (FP32) on the same seeded scene: warm_steps untimed, then steps until about
+    budget_s seconds of CPU work; reports active cell-steps/s."""
     import oracle
     oracle.library()
     cores = oracle.get_threads()
-    c = make_sim(oracle.OracleSim, w, precision=32)
+    c = make_sim(oracle.OracleSim, w, params, precision=32)
     c.step(warm_steps)
     n = 0
     cells = 0
@@ -135,9 +166,9 @@ def cpu_baseline(w, warm_steps, budget_s):
         "unit": "active cell-steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{n} steps of {w.name} after {warm_steps} untimed steps from the same seeded scene "
-                  f"({cells / max(n, 1):.0f} active cells/step), FP32 oracle, OpenMP {cores} threads",
         "seconds": round(dt, 3),
+        "cells_per_step": round(cells / max(n, 1)),
+        "steps": n,
     }
 
 
@@ -209,31 +240,21 @@ def adjoint_bench(w, args):
     }
 
 
-def solve_bench(w, args):
-    """Projection solve of the adaptive workload: after `warmup` reference steps, the same
-    velocity field is projected with every level solved to a 1e-8 relative residual by FP64
-    iterative refinement around FP32 multigrid (mixed), and by the all-FP32 and all-FP64
-    variants; wall time of each projection (median of `steps` repetitions from the same
-    state), cycles and reached residual."""
-    import dcgrid_amd
+def solve_compare(sim, params, reps=3):
+    """The projection of the simulation's current velocity, solved from a cold start (zero top
+    level, prolongation below) to 1e-8 in mixed precision, all-FP32 and all-FP64: wall time
+    (median of `reps`, each from the same velocity), cycles and residual per level, and the
+    relative L2 distance of the pressure to the FP64 solution on the allocated cells."""
+    import numpy as np
     from dcgrid_amd._abi import SOLVE_FP32, SOLVE_FP64, SOLVE_MIXED
-    p = w.params.copy()
-    p.solve_tol = 1e-8
-    p.solve_max_cycles = 60
-    sim = make_sim(dcgrid_amd.Simulation, w)
-    sim.set_params(p)
-    log(f"[bench] solve: {args.warmup} reference steps")
-    sim.step(args.warmup)
     vel = sim.export_field("velocity")
-    inf = sim.info()
-    out = {"metric": "projection solve time to 1e-8 relative residual", "unit": "ms",
-           "higher_is_better": False, "n_gpus": 1, "data": "synthetic (seeded plume after warmup steps)",
-           "config": {"workload": w.name, "domain": list(w.size), "block_loads": inf.levels_list("block_loads"),
-                      "active_cells": int(inf.active_cells)}}
-    reps = max(1, min(args.steps, 5))
-    for name, prec in (("mixed", SOLVE_MIXED), ("fp32", SOLVE_FP32), ("fp64", SOLVE_FP64)):
-        q = p.copy()
+    mask = (sim.export_tables()["block_levels"] != 0xFF).repeat(int(sim.info().block_width) ** 3)
+    out = {}
+    press = {}
+    for name, prec in (("fp64", SOLVE_FP64), ("mixed", SOLVE_MIXED), ("fp32", SOLVE_FP32)):
+        q = params.copy()
         q.solve_precision = prec
+        q.solve_tol = 1e-8
         sim.set_params(q)
         times = []
         for _ in range(reps):
@@ -244,37 +265,77 @@ def solve_bench(w, args):
             sim.synchronize()
             times.append(1e3 * (time.perf_counter() - t0))
         rep = sim.solve_report()
+        press[name] = sim.export_field("pressure")[mask].astype(np.float64)
         times.sort()
         out[name] = {"ms": round(times[len(times) // 2], 3), "cycles": rep["cycles"],
                      "rel_residual": [float(f"{r:.3e}") for r in rep["rel_residual"]],
-                     "converged": rep["converged"]}
-        log(f"[bench] {name}: {out[name]}")
-    out["value"] = out["mixed"]["ms"]
-    sim.close()
+                     "converged": bool(rep["converged"])}
+        log(f"[bench] solve {name}: {out[name]}")
+    ref = press["fp64"]
+    for name in ("mixed", "fp32"):
+        out[name]["pressure_rel_l2_vs_fp64"] = float(f"{np.linalg.norm(press[name] - ref) / np.linalg.norm(ref):.3e}")
+    out["mixed_over_fp64_time"] = round(out["mixed"]["ms"] / out["fp64"]["ms"], 3)
+    sim.set_params(params)
+    sim.import_field("velocity", vel)
     return out
+
+
+def jacobi_micro(args):
+    """Jacobi half-sweeps of the uniform grid's finest level at 512^3 (134 M cells, 1.6 GB per
+    sweep: beyond the 256 MB Infinity Cache), timed with HIP events; 12 B per cell per sweep."""
+    import numpy as np
+    import dcgrid_amd
+    from dcgrid_amd import scenes
+    from dcgrid_amd._abi import STAGE_JACOBI, STAGE_JACOBI_INV
+    n = 512
+    p = scenes.reference_params(n, kind="uniform")
+    sim = dcgrid_amd.Simulation("uniform", (n, n, n), p)
+    rng = np.random.default_rng(7)
+    sim.import_field("divergence", rng.standard_normal(sim.field_size("divergence")).astype(np.float32))
+    for k in range(4):
+        sim.run_stage(STAGE_JACOBI if k % 2 == 0 else STAGE_JACOBI_INV, 0)
+    sim.synchronize()
+    sim.set_timing(["jacobi"])
+    launches = 2 * max(args.steps // 2, 10)
+    for k in range(launches):
+        sim.run_stage(STAGE_JACOBI if k % 2 == 0 else STAGE_JACOBI_INV, 0)
+    sim.synchronize()
+    ms, cnt = sim.timing()["jacobi"]
+    per_launch = 12 * n ** 3
+    achieved = per_launch / (ms / cnt * 1e-3) / 1e9
+    sim.close()
+    return {"metric": "uniform Jacobi half-sweep bandwidth (cache-busting)", "value": round(achieved, 1),
+            "unit": "GB/s", "n_gpus": 1, "steps": int(cnt), "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic (seeded normal divergence)",
+            "config": {"workload": "uniform512_jacobi", "domain": [n, n, n]},
+            "roofline": {"kernel": "k_u_jacobi", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic("uniform512_jacobi", "k_u_jacobi"),
+                         "avg_launch_us": round(1e3 * ms / cnt, 3), "algorithmic_bytes_per_launch": per_launch}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # default: the configuration's full 200-step run (20 untimed + 180 timed)
-    ap.add_argument("--steps", type=int, default=180)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", type=int, default=1)
+    # default: configuration 2's 100-step run (10 untimed + 90 timed)
+    ap.add_argument("--steps", type=int, default=90)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--storage", choices=sorted(STORAGE_NAMES), default=None,
+                    help="field storage (default: bf16 for configuration 2, else fp32)")
+    ap.add_argument("--project", choices=["solve", "reference"], default=None,
+                    help="solve: every level solved to 1e-8 (mixed precision, default for configurations 2 and "
+                         "4); reference: the reference's Jacobi schedule (default for 0, 1)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing pass")
-    ap.add_argument("--timing-steps", type=int, default=20,
-                    help="steps of the separate pass that times every Jacobi launch with HIP events")
-    ap.add_argument("--storage", choices=sorted(STORAGE_NAMES), default="fp32",
-                    help="field storage format in HBM (FP32 arithmetic either way)")
+    ap.add_argument("--timing-steps", type=int, default=5,
+                    help="steps of the separate pass that times every launch with HIP events")
+    ap.add_argument("--no-solve-compare", action="store_true")
     ap.add_argument("--graph", action="store_true",
-                    help="replay the timed steps in pairs from one captured HIP graph (measured no "
-                         "faster than eager launches on MI355X for this step)")
-    ap.add_argument("--solve", action="store_true",
-                    help="time the projection solve to 1e-8 (mixed, FP32, FP64) instead of the step")
+                    help="replay the timed steps in pairs from one captured HIP graph (reference schedule)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device-to-device copy first (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--jacobi-micro", action="store_true", help="uniform 512^3 Jacobi bandwidth only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -283,30 +344,38 @@ def main():
 
     import dcgrid_amd
     from dcgrid_amd import scenes
+    from dcgrid_amd._abi import PROJECT_SOLVE, SOLVE_MIXED
+    if args.copy_probe:
+        copy_probe()
+    if args.jacobi_micro:
+        print(json.dumps(jacobi_micro(args)), flush=True)
+        return
     w = scenes.config(args.config)
-    w.params.storage = STORAGE_NAMES[args.storage]
-    if args.storage != "fp32" and w.kind != "dcgrid":
+    storage = args.storage or ("bf16" if args.config == 2 else "fp32")
+    project = args.project or ("solve" if args.config in (2, 4) else "reference")
+    p = w.params.copy()
+    p.storage = STORAGE_NAMES[storage]
+    if storage != "fp32" and w.kind != "dcgrid":
         print(json.dumps({"error": "16-bit field storage is implemented for the DCGrid configurations"}))
         sys.exit(2)
     if args.config == 3:
         print(json.dumps(adjoint_bench(w, args)), flush=True)
         return
-    if args.solve:
-        print(json.dumps(solve_bench(w, args)), flush=True)
-        return
+    solve = project == "solve" and w.kind == "dcgrid"
+    if solve:
+        p.project_mode = PROJECT_SOLVE
+        p.solve_precision = SOLVE_MIXED
+        p.solve_tol = 1e-8
+        p.solve_max_cycles = 50
 
-    if args.copy_probe:
-        copy_probe()
-    log(f"[bench] creating {w.name} {w.size} budget={w.max_num_blocks}")
-    sim = make_sim(dcgrid_amd.Simulation, w)
+    log(f"[bench] creating {w.name} {w.size} budget={w.max_num_blocks} storage={storage} project={project}")
+    sim = make_sim(dcgrid_amd.Simulation, w, p)
     log(f"[bench] warmup {args.warmup} steps")
     sim.step(args.warmup)
     sim.synchronize()
-    info0 = sim.info()
-    act0 = int(info0.active_cells)
-    # timed steps: whole-step throughput (per-kernel event timing runs in a pass of its own)
-    sim.set_graph(args.graph)
-    log(f"[bench] timing {args.steps} steps (active cells {act0}, graph {args.graph})")
+    act0 = int(sim.info().active_cells)
+    sim.set_graph(args.graph and not solve)
+    log(f"[bench] timing {args.steps} steps (active cells {act0})")
     cs0 = sim.cell_steps()
     sim.synchronize()
     t0 = time.perf_counter()
@@ -319,47 +388,68 @@ def main():
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = cell_steps / elapsed
+    last_solve = sim.solve_report() if solve else None
 
-    roof = None
-    if not args.no_timing and args.timing_steps > 0:
-        # roofline pass: the following steps with every Jacobi launch bracketed by HIP events on
-        # the simulation's stream
+    # per-kernel pass: every launch of every class bracketed by HIP events on the library's stream
+    kernels, roof = {}, None
+    if args.timing_steps > 0:
         sim.set_graph(False)
-        sim.set_timing(["jacobi"])
+        sim.set_timing(list(dcgrid_amd.K_NAMES))
         cs1 = sim.cell_steps()
         sim.step(args.timing_steps)
         sim.synchronize()
         tcs = sim.cell_steps() - cs1
-        info2 = sim.info()
         tm = sim.timing()
-        j_ms, j_n = tm["jacobi"]
-        p = w.params
-        if p.project_top_pairs == p.project_level_pairs:
-            # every active cell is swept 2*pairs times per step: exact over the varying topology
-            jb = 2 * p.project_level_pairs * 3 * STORAGE_BYTES[p.storage] * tcs
-            jl = j_n
-        else:
-            jb, jl = jacobi_bytes(info2, p)
-        if j_n > 0 and j_ms > 0:
-            per_launch_bytes = jb / jl
-            avg_ms = j_ms / j_n
-            achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
-            roof = {
-                "kernel": "k_jacobi (all levels)",
-                "bound": "hbm",
-                "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS,
-                "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(w.name, "k_jacobi"),
-                "avg_launch_us": round(avg_ms * 1e3, 3),
-                "launches": int(j_n),
-                "timed_steps": args.timing_steps,
-                "algorithmic_bytes_per_launch": int(per_launch_bytes),
-            }
-    sb = step_bytes(info1, w.params)
+        tb = sim.timing_bytes()
+        pc = per_cell_bytes(p)
+        for name, (ms, n) in tm.items():
+            if n == 0:
+                continue
+            entry = {"ms_per_step": round(ms / args.timing_steps, 4), "launches_per_step": round(n / args.timing_steps, 1),
+                     "avg_launch_us": round(1e3 * ms / n, 3)}
+            bytes_ = None
+            if name in pc and w.kind == "dcgrid":
+                bytes_ = pc[name] * tcs  # every allocated cell once per step
+            elif name == "jacobi" and not solve and w.kind == "dcgrid":
+                if p.project_top_pairs == p.project_level_pairs:
+                    bytes_ = 2 * p.project_level_pairs * 3 * STORAGE_BYTES[p.storage] * tcs
+                else:
+                    bytes_ = jacobi_bytes(info1, p)[0] * args.timing_steps
+            elif name == "jacobi" and w.kind == "uniform" and p.project_mode == 1:
+                bytes_ = 12 * int(info1.active_cells) * n
+            elif tb.get(name):
+                bytes_ = tb[name]
+            if bytes_:
+                entry["algorithmic_GBps"] = round(bytes_ / (ms * 1e-3) / 1e9, 1)
+                entry["algorithmic_bytes_per_launch"] = int(bytes_ / n)
+            kernels[name] = entry
+        modelled = {k: v for k, v in kernels.items() if "algorithmic_GBps" in v}
+        if modelled:
+            dom = max(modelled, key=lambda k: modelled[k]["ms_per_step"])
+            e = modelled[dom]
+            kname = {"solve_smooth": "k_sv_smooth", "jacobi": "k_jacobi" if w.kind == "dcgrid" else "k_u_jacobi",
+                     "advect_velocity": "k_advect_velocity", "advect_density": "k_advect_density",
+                     "divergence": "k_divergence", "apply_pressure": "k_apply_pressure"}[dom]
+            roof = {"kernel": kname, "bound": "hbm", "achieved": e["algorithmic_GBps"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(e["algorithmic_GBps"] / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(w.name, kname), "avg_launch_us": e["avg_launch_us"],
+                    "launches": int(e["launches_per_step"] * args.timing_steps), "timed_steps": args.timing_steps,
+                    "algorithmic_bytes_per_launch": e["algorithmic_bytes_per_launch"],
+                    "share_of_step_kernel_time": round(e["ms_per_step"] / sum(v["ms_per_step"] for v in kernels.values()), 3)}
+    solve_cmp = None
+    if solve and not args.no_solve_compare:
+        # compared on FP32 field storage (16-bit storage rounds the pressures of every variant
+        # to the same values), on the same scene after the same number of warm-up steps
+        log("[bench] solve comparison (FP32 storage, mixed / FP32 / FP64)")
+        sim.close()
+        q = p.copy()
+        q.storage = STORAGE_NAMES["fp32"]
+        sim = make_sim(dcgrid_amd.Simulation, w, q)
+        sim.step(args.warmup)
+        solve_cmp = solve_compare(sim, q)
+        solve_cmp["field_storage"] = "fp32"
     out = {
-        "metric": "active cell-steps/s",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "active cell-steps/s",
         "n_gpus": 1,
@@ -369,29 +459,49 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded sphere source + value-noise velocity)",
+        "dtype": DTYPE[p.storage],
+        "arithmetic": "f32, f64 residual / norms / update of the refinement" if solve else "f32",
+        "data": "synthetic (seeded sphere source + value-noise velocity)" if args.config != 4 else
+                "synthetic (seeded fBm terrain, moist layer, ambient wind)",
         "config": {
             "workload": w.name,
             "domain": list(w.size),
-            "block_width": w.params.block_width,
+            "block_width": p.block_width if w.kind == "dcgrid" else None,
             "levels": int(info1.levels),
-            "budget_blocks": w.max_num_blocks,
+            "budget_blocks": w.max_num_blocks if w.kind == "dcgrid" else None,
+            "field_storage": storage,
+            "projection": "each level solved to 1e-8 relative residual (FP64 refinement around FP32 multigrid), "
+                          "every step" if solve else "reference Jacobi schedule",
             "active_cells_first_last": [act0, act1],
             "mean_active_cells": round(cell_steps / args.steps, 1),
-            "block_loads": info1.levels_list("block_loads"),
-            "step_algorithmic_bytes": sb,
-            "step_algorithmic_GBps": round(sb / (ms_per_step * 1e-3) / 1e9, 1),
-            "launch": "hip graph (pairs of steps)" if args.graph else "eager",
-            "field_storage": args.storage,
+            "block_loads": info1.levels_list("block_loads") if w.kind == "dcgrid" else None,
+            "launch": "hip graph (pairs of steps)" if (args.graph and not solve) else "eager",
         },
         "roofline": roof,
+        "kernels": kernels,
+        "last_step_solve": last_solve,
+        "solve": solve_cmp,
         "cpu_baseline": None,
     }
+    if not solve:
+        sb = step_bytes(info1, p, uniform=w.kind == "uniform")
+        out["config"]["step_algorithmic_bytes"] = sb
+        out["config"]["step_algorithmic_GBps"] = round(sb / (ms_per_step * 1e-3) / 1e9, 1)
     sim.close()
     if not args.no_cpu:
         log(f"[bench] CPU baseline (~{args.cpu_seconds}s)")
-        out["cpu_baseline"] = cpu_baseline(w, 2, args.cpu_seconds)
+        cp = p.copy()
+        note = ""
+        if solve:
+            # the oracle's solve is an exact-to-1e-12 SOR reference (a correctness check, minutes
+            # per step at this size): the CPU runs the same scene with the reference projection
+            cp.project_mode = 0
+            note = "; projection: the reference's Jacobi schedule (the CPU oracle has no multigrid solve)"
+        cb = cpu_baseline(w, cp, 2, args.cpu_seconds)
+        cb["sample"] = (f"{cb.pop('steps')} steps of {w.name} after 2 untimed steps from the same seeded scene "
+                        f"({cb.pop('cells_per_step')} active cells/step), FP32 oracle, {storage} field rounding, "
+                        f"OpenMP {cb['cores']} threads{note}")
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

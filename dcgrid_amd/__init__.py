@@ -49,6 +49,8 @@ def library():
         lib.dcg_set_graph.restype = C.c_int
         lib.dcg_timing_get.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
         lib.dcg_timing_get.restype = C.c_int
+        lib.dcg_timing_bytes.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]
+        lib.dcg_timing_bytes.restype = C.c_int
         lib.dcg_get_params.argtypes = [C.c_void_p, C.POINTER(Params)]
         lib.dcg_get_params.restype = C.c_int
         lib.dcg_cell_steps.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
@@ -135,6 +137,16 @@ class Simulation(Handle):
             n = C.c_uint64()
             self._check(_lib.dcg_timing_get(self._h, k, C.byref(ms), C.byref(n)), "timing_get")
             out[name] = (ms.value, n.value)
+        return out
+
+    def timing_bytes(self):
+        """{kernel name: algorithmic bytes of the timed launches} for the classes the library
+        counts itself (the solve's smoothing), else 0."""
+        out = {}
+        for k, name in enumerate(K_NAMES):
+            v = C.c_uint64()
+            self._check(_lib.dcg_timing_bytes(self._h, k, C.byref(v)), "timing_bytes")
+            out[name] = v.value
         return out
 
     # -- rendering (FluidSimulation::render)

@@ -64,7 +64,7 @@ STAGE_APPLY_PRESSURE = 5
 STAGE_VORTICITY = 6
 
 K_NAMES = ["advect_velocity", "advect_density", "divergence", "jacobi", "prolongate",
-           "apply_pressure", "restrict", "topology", "other"]
+           "apply_pressure", "restrict", "topology", "other", "solve_smooth", "solve"]
 
 
 class Params(C.Structure):
@@ -120,7 +120,7 @@ class Params(C.Structure):
         ("wind", C.c_float * 3),
         ("adiabatic_rate", C.c_float),
         ("storage", C.c_int32),
-        ("solve_cold", C.c_int32),
+        ("solve_warm", C.c_int32),
         ("reserved", C.c_int32 * 1),
     ]
 

@@ -114,6 +114,29 @@ int SimBase::beginTimed(int k, Pending* ev) {
   DCG_HIP(hipEventRecord(ev->a, stream));
   return DCG_OK;
 }
+int SimBase::takeEvents(int k, Pending* ev) {
+  ev->k = -1;
+  if (!(timingMask & (1u << k))) return DCG_OK;
+  for (int i = 0; i < 2; i++) {
+    hipEvent_t e;
+    if (!eventPool.empty()) {
+      e = eventPool.back();
+      eventPool.pop_back();
+    } else {
+      DCG_HIP(hipEventCreate(&e));
+    }
+    (i == 0 ? ev->a : ev->b) = e;
+  }
+  ev->k = k;
+  ev->n = 1;
+  return DCG_OK;
+}
+int SimBase::pushPending(Pending* ev) {
+  if (ev->k < 0) return DCG_OK;
+  pending.push_back(*ev);
+  if (pending.size() > 4096) return collectTiming();
+  return DCG_OK;
+}
 int SimBase::endTimed(int, Pending* ev, uint32_t launches) {
   if (ev->k < 0) return DCG_OK;
   ev->n = launches;
@@ -647,6 +670,7 @@ int dcg_set_timing(dcg_sim* s, int32_t mask) {
   for (int k = 0; k < DCG_NUM_KERNEL_IDS; k++) {
     I->timingMs[k] = 0;
     I->timingLaunches[k] = 0;
+    I->timingBytes[k] = 0;
   }
   return DCG_OK;
 }
@@ -657,6 +681,12 @@ int dcg_timing_get(dcg_sim* s, int32_t kernel, double* total_ms, uint64_t* launc
   if (e) return e;
   *total_ms = I->timingMs[kernel];
   *launches = I->timingLaunches[kernel];
+  return DCG_OK;
+}
+int dcg_timing_bytes(dcg_sim* s, int32_t kernel, uint64_t* bytes) {
+  IMPL(s);
+  if (kernel < 0 || kernel >= DCG_NUM_KERNEL_IDS || !bytes) return DCG_E_ARG;
+  *bytes = I->timingBytes[kernel];
   return DCG_OK;
 }
 const char* dcg_last_error(const dcg_sim* s) { return s ? s->impl->err.c_str() : g_create_error.c_str(); }

@@ -63,6 +63,7 @@ struct SimBase {
   std::vector<hipEvent_t> eventPool;
   double timingMs[DCG_NUM_KERNEL_IDS] = {};
   uint64_t timingLaunches[DCG_NUM_KERNEL_IDS] = {};
+  uint64_t timingBytes[DCG_NUM_KERNEL_IDS] = {};  // algorithmic bytes, where the library counts them
 
   virtual ~SimBase();
   virtual int kind() const = 0;
@@ -204,6 +205,12 @@ struct SimBase {
 
   // Launch bracket: records events around the launch when kernel id k is timed.
   int beginTimed(int k, Pending* ev);
+  // Events of a launch timed by the dispatch's own begin and end (hipExtLaunchKernelGGL): the
+  // kernel duration rocprofv3 reports, without the gap between dispatches that events recorded
+  // around a launch include. takeEvents gives the pair (ev->k = -1 when class k is not timed),
+  // pushPending files it after the launch.
+  int takeEvents(int k, Pending* ev);
+  int pushPending(Pending* ev);
   int endTimed(int k, Pending* ev, uint32_t launches = 1);
   int collectTiming();
   int checkLaunch(const char* what) {

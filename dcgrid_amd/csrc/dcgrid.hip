@@ -7,6 +7,8 @@
 #include <cstring>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "dcg_internal.hpp"
 #include "dcgrid_fluid.hpp"
 #include "dcgrid_solve.hpp"
@@ -720,10 +722,24 @@ struct DCGridSim : SimBase {
         // halos are zero too (the second colour's are what the first wrote)
         const bool zero = down && pass == 0;
         const bool haloZero = zero && (colour == 0 || countOf(lmin, k, 0) == 0);
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_smooth",
-                   (k_sv_smooth<BW, T><<<listGrid(n), G::WG, 0, stream>>>(
-                       g, k, lmin, listOf(lmin, colour), n, svInner, zero ? 1 : 0, haloZero ? 1 : 0, (T*)eBuf,
-                       (const T*)rhoBuf, reachBuf, h2, gm, ctl)));
+        if (timingMask & (1u << DCG_K_SOLVE_SMOOTH))  // e read (unless zero), rho read, e written
+          timingBytes[DCG_K_SOLVE_SMOOTH] += (uint64_t)n * G::BV * sizeof(T) * (zero ? 2 : 3);
+        Pending ev;
+        DCG_TRY(takeEvents(DCG_K_SOLVE_SMOOTH, &ev));
+        const DGBase gb = g;
+        const uint32_t* lst = listOf(lmin, colour);
+        const int oz = zero ? 1 : 0, hz = haloZero ? 1 : 0;
+        T* ep = (T*)eBuf;
+        const T* rp = (const T*)rhoBuf;
+        const uint8_t* reach = reachBuf;
+        if (ev.k >= 0)
+          hipExtLaunchKernelGGL(k_sv_smooth<BW, T>, dim3(listGrid(n)), dim3(G::WG), 0u, stream, ev.a, ev.b, 0u, gb, k,
+                                lmin, lst, n, svInner, oz, hz, ep, rp, reach, h2, gm, ctl);
+        else
+          k_sv_smooth<BW, T><<<listGrid(n), G::WG, 0, stream>>>(gb, k, lmin, lst, n, svInner, oz, hz, ep, rp, reach,
+                                                                h2, gm, ctl);
+        DCG_TRY(checkLaunch("k_sv_smooth"));
+        DCG_TRY(pushPending(&ev));
       }
     return DCG_OK;
   }
@@ -738,7 +754,7 @@ struct DCGridSim : SimBase {
     for (int pass = 0; pass < svPasses; pass++)
       for (int colour = 0; colour < 2; colour++) {
         const bool zero = down && pass == 0;
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_smooth",
+        DCG_LAUNCH(DCG_K_SOLVE, "k_svu_smooth",
                    (k_svu_smooth<T><<<n, 512, 0, stream>>>(D, colour, svInner, zero ? 1 : 0, zero && colour == 0 ? 1 : 0,
                                                           (T*)ueBuf, (const T*)urhoBuf, uactBuf, h2, gm, ctl)));
       }
@@ -755,7 +771,7 @@ struct DCGridSim : SimBase {
       DCG_TRY(smoothBlocks<T>(k, lmin, nu, true, ctl));
       const uint32_t n0 = countOf(lmin, k, 0), n1 = countOf(lmin, k, 1);
       if (n0 + n1)
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
+        DCG_LAUNCH(DCG_K_SOLVE, "k_sv_restrict",
                    (k_sv_restrict<BW, T, false><<<listGrid(n0 + n1), G::WG, 0, stream>>>(
                        g, k, lmin, listOf(lmin, 0), listOf(lmin, 1), n0, n1, e, rho, reachBuf, levelH2(k),
                        ghostFactor(k - lmin), nullptr, nullptr, UDimArg{}, ctl)));
@@ -764,7 +780,7 @@ struct DCGridSim : SimBase {
     if (!pyr.empty()) {
       T* ue = (T*)ueBuf;
       T* ur = (T*)urhoBuf;
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_restrict",
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_restrict",
                  (k_sv_restrict<BW, T, true><<<slotGrid<BW>(maxL[top]), G::WG, 0, stream>>>(
                      g, top, lmin, nullptr, nullptr, 0u, 0u, e, rho, reachBuf, levelH2(top), ghostFactor(top - lmin),
                      ur, uactBuf, pyr[0], ctl)));
@@ -775,7 +791,7 @@ struct DCGridSim : SimBase {
         DCG_TRY(smoothPyr<T>(j, lmin, nu, true, ctl));
         const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
         const UDimArg& Dc = pyr[j + 1];
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_restrict",
+        DCG_LAUNCH(DCG_K_SOLVE, "k_svu_restrict",
                    (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(
                        pyr[j], Dc, ue, ur, uactBuf, h2, ghostFactor(L + j - lmin), ctl)));
       }
@@ -792,20 +808,22 @@ struct DCGridSim : SimBase {
         ca.goff = pyr[jc].off;
         ca.n = J - jc;
         ca.sweeps = svPasses * svInner;
-        ca.bottom = 8;  // converged the same as 40 on the benchmark scenes, at less than half the cost
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_coarse",
+        // coarsest grid: 8 sweeps converge 2^3 (as 40 did), flat grids such as 8x2x8 need 40
+        const UDimArg& Db = pyr[J - 1];
+        ca.bottom = std::max(Db.w, std::max(Db.h, Db.d)) <= 2 ? 8 : 40;
+        DCG_LAUNCH(DCG_K_SOLVE, "k_svu_coarse",
                    (k_svu_coarse<T><<<1, kCoarseThreads, 0, stream>>>(ca, ue, ur, uactBuf, ctl)));
       } else {
         for (int i = 0; i < 20; i++) DCG_TRY(smoothPyr<T>(jd, lmin, nu, i == 0, ctl));
       }
       for (int j = jd - 1; j >= 0; j--) {
         const UDimArg& D = pyr[j];
-        DCG_LAUNCH(DCG_K_OTHER, "k_svu_prolong",
+        DCG_LAUNCH(DCG_K_SOLVE, "k_svu_prolong",
                    (k_svu_prolong<T><<<blocksFor((size_t)D.w * D.h * D.d, 256), 256, 0, stream>>>(D, pyr[j + 1], ue,
                                                                                                   uactBuf, ctl)));
         DCG_TRY(smoothPyr<T>(j, lmin, nu, false, ctl));
       }
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_top",
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_prolong_top",
                  (k_sv_prolong_top<BW, T><<<cellGrid(top), 256, 0, stream>>>(g, lmin, e, reachBuf, ue, uactBuf, pyr[0],
                                                                              ctl)));
       DCG_TRY(smoothBlocks<T>(top, lmin, nu, false, ctl));
@@ -815,7 +833,7 @@ struct DCGridSim : SimBase {
     for (int k = top - 1; k >= lmin; k--) {
       const uint32_t n0 = countOf(lmin, k, 0), n1 = countOf(lmin, k, 1);
       if (n0 + n1)
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong",
+        DCG_LAUNCH(DCG_K_SOLVE, "k_sv_prolong",
                    (k_sv_prolong<BW, T><<<listGrid(n0 + n1), G::WG, 0, stream>>>(
                        g, k, lmin, listOf(lmin, 0), listOf(lmin, 1), n0, n1, e, reachBuf, ctl)));
       DCG_TRY(smoothBlocks<T>(k, lmin, nu, false, ctl));
@@ -829,11 +847,11 @@ struct DCGridSim : SimBase {
     const double tol = P.solve_tol > 0 ? (double)P.solve_tol : 1e-8;
     const int maxCycles = P.solve_max_cycles > 0 ? P.solve_max_cycles : 50;
     const unsigned n = std::min(slotGrid<BW>(maxL[l]), kSolveWG);
-    DCG_LAUNCH(DCG_K_OTHER, "k_sv_residual",
+    DCG_LAUNCH(DCG_K_SOLVE, "k_sv_residual",
                (k_sv_residual<BW, TX, T><<<n, G::WG, 0, stream>>>(g, l, (const TX*)xBuf[0], xin, xout,
                                                                   (const T*)eBuf, first ? 1 : 0, (T*)rhoBuf, partBuf,
                                                                   alpha, alpha, ctl)));
-    DCG_LAUNCH(DCG_K_OTHER, "k_sv_decide",
+    DCG_LAUNCH(DCG_K_SOLVE, "k_sv_decide",
                (k_sv_decide<<<1, 256, 0, stream>>>(g, l, G::BV, first ? 1 : 0, (int)n, partBuf, alpha, tol, maxCycles,
                                                    ctl, ctlMirrorDev + l)));
     return DCG_OK;
@@ -868,7 +886,7 @@ struct DCGridSim : SimBase {
     }
     predCycles[l] = (int)h.cycles;
     const TX* xf = h.cycles % 2 ? x2 : xc;
-    DCG_LAUNCH(DCG_K_OTHER, "k_sv_finalize",
+    DCG_LAUNCH(DCG_K_SOLVE, "k_sv_finalize",
                (k_sv_finalize<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xf, xc, (const SolveCtl*)ctl)));
     rep.cycles[l] = (int)h.cycles;
     rep.rel_residual[l] = h.r0 > 0 ? h.rr / h.r0 : 0.0;
@@ -885,9 +903,9 @@ struct DCGridSim : SimBase {
     TX* xc = (TX*)xBuf[0];
     for (int l = L - 1; l >= 0; l--) {
       if (l == L - 1) {
-        if (!warm) DCG_LAUNCH(DCG_K_OTHER, "k_sv_zero", (k_sv_zero<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xc)));
+        if (!warm) DCG_LAUNCH(DCG_K_SOLVE, "k_sv_zero", (k_sv_zero<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xc)));
       } else {
-        DCG_LAUNCH(DCG_K_OTHER, "k_sv_prolong_x",
+        DCG_LAUNCH(DCG_K_SOLVE, "k_sv_prolong_x",
                    (k_sv_prolong_x<BW, TX><<<cellGrid(l), 256, 0, stream>>>(g, l, xc, warm ? 1 : 0)));
       }
       DCG_TRY((solveLevel<TX, T>(l)));
@@ -896,15 +914,15 @@ struct DCGridSim : SimBase {
   }
   template <class TX, class T>
   int solveWithGuess() {
-    const bool warm = warmValid && !P.solve_cold;
+    const bool warm = warmValid && P.solve_warm;
     if (warm) {
       unsigned n = blocksFor((size_t)B * G::BV, 256);
       if (n > 2048u) n = 2048u;
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_warm", (k_sv_warm<BW, TX><<<n, 256, 0, stream>>>(g, (TX*)xBuf[0])));
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_warm", (k_sv_warm<BW, TX><<<n, 256, 0, stream>>>(g, (TX*)xBuf[0])));
     }
     DCG_TRY(divergence());
     for (int l = 0; l < L; l++)
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_reach",
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_reach",
                  (k_sv_reach<BW><<<slotGrid<BW>(maxL[l]), G::WG, 0, stream>>>(g, l, reachBuf, breachBuf)));
     // the active blocks of every level solve's grids, and their counts on the host (which sizes
     // every launch of the solve)
@@ -912,8 +930,8 @@ struct DCGridSim : SimBase {
     {
       unsigned n = blocksFor(maxL[0], 256);
       for (int l = 1; l < L; l++) n = std::max(n, blocksFor(maxL[l], 256));
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_lists", (k_sv_lists<BW><<<std::min(n, 1024u), 256, 0, stream>>>(g, breachBuf, svList, svCount)));
-      DCG_LAUNCH(DCG_K_OTHER, "k_sv_mirror_counts",
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_lists", (k_sv_lists<BW><<<std::min(n, 1024u), 256, 0, stream>>>(g, breachBuf, svList, svCount)));
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_mirror_counts",
                  (k_sv_mirror_counts<<<1, 1, 0, stream>>>(svCount, svCountMirrorDev, 2 * L * L)));
     }
     DCG_TRY(sync());

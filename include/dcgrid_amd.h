@@ -156,10 +156,11 @@ typedef struct dcg_params {
   float wind[3];          /* ambient wind, world units per second (initial field, lateral inflow) */
   float adiabatic_rate;   /* cooling of rising air per world unit of ascent (moist only) */
   int32_t storage;        /* DCG_STORAGE_* of the fields, fixed at creation (DCGrid only) */
-  int32_t solve_cold;     /* DCG_PROJECT_SOLVE initial guess: 0 = the pressure of the last
+  int32_t solve_warm;     /* DCG_PROJECT_SOLVE initial guess: 0 = zero on the top level and the
+                             solved parent level prolonged below; 1 = the pressure of the last
                              projection on every block that kept its place (blocks moved or
-                             created since are prolonged from their parent level), 1 = zero on
-                             the top level and prolongation below */
+                             created since are prolonged). The prolonged parent is the better
+                             guess on dense scenes (config 4: 6 cycles per level against 16) */
   int32_t reserved[1];
 } dcg_params;
 
@@ -402,6 +403,11 @@ int dcg_state_check(const void* src, uint64_t bytes);
  * launch count of one class since then. Used by the benchmark for roofline figures. */
 int dcg_set_timing(dcg_sim* s, int32_t mask);
 int dcg_timing_get(dcg_sim* s, int32_t kernel, double* total_ms, uint64_t* launches);
+/* Algorithmic HBM bytes of the launches timed in class `kernel` since dcg_set_timing, where the
+ * library counts them itself (DCG_K_SOLVE_SMOOTH: the correction read and written and the
+ * right-hand side read, per cell of the listed blocks; 0 for the other classes, whose work is a
+ * function of the block loads the caller can read). */
+int dcg_timing_bytes(dcg_sim* s, int32_t kernel, uint64_t* bytes);
 
 /* Kernel ids for dcg_timing_get. */
 enum {
@@ -414,7 +420,9 @@ enum {
   DCG_K_RESTRICT = 6,
   DCG_K_TOPOLOGY = 7,
   DCG_K_OTHER = 8,
-  DCG_NUM_KERNEL_IDS = 9
+  DCG_K_SOLVE_SMOOTH = 9, /* projection solve: block Gauss-Seidel smoothing of the block levels */
+  DCG_K_SOLVE = 10,       /* projection solve: every other kernel of the solve */
+  DCG_NUM_KERNEL_IDS = 11
 };
 
 const char* dcg_last_error(const dcg_sim* s);

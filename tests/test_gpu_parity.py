@@ -187,7 +187,7 @@ def test_dcgrid_fp64_drift():
             out.append((n, r))
     say("[result] drift FP32 GPU vs FP64 oracle (velocity rel L2):", out)
     assert out[0][1] <= 1e-5
-    assert out[-1][1] <= 1e-3
+    assert out[-1][1] <= 5e-6  # measured 3.6e-7 after 10 steps
 
 
 @pytest.mark.parametrize("name", ["ref64_bw4", "cfg1_64", "ref64_bw8"])

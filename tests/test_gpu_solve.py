@@ -79,7 +79,8 @@ def test_mixed_solve_reaches_tolerance_and_matches_oracle(name):
     pg, pc = g.export_field("pressure")[mask], c.export_field_f64("pressure")[mask]
     vg, vc = g.export_field("velocity")[mask], c.export_field_f64("velocity")[mask]
     say("pressure rel l2", rel_l2(pg, pc), "velocity rel l2", rel_l2(vg, vc))
-    assert rel_l2(pg, pc) <= 1e-5
+    # measured: pressure 2.8e-8 / 8.7e-8 / 2.9e-8, velocity 1.0e-7 / 3.1e-8 / 3.5e-7
+    assert rel_l2(pg, pc) <= 2e-7
     assert rel_l2(vg, vc) <= 1e-6
 
 
@@ -104,7 +105,7 @@ def test_fixed_point_of_the_reference_jacobi_sweep():
             d = d - d.mean()
         err = np.sqrt(np.mean(d * d)) / scale
         say("level", level, "singular", rep["singular"][level], "jacobi change", err)
-        assert err <= 2e-6
+        assert err <= 3e-7  # measured about 6e-8 on every level
 
 
 def test_precision_variants():
@@ -127,7 +128,7 @@ def test_precision_variants():
     say("fp32 report", res[SOLVE_FP32][0])
     assert res[SOLVE_FP64][0]["converged"]
     assert max(res[SOLVE_FP64][0]["rel_residual"]) <= 1e-8
-    assert e_64 <= 1e-5 and e_mixed <= 1e-5
+    assert e_64 <= 2e-7 and e_mixed <= 2e-7  # measured 2.8e-8 (both)
     assert max(res[SOLVE_FP32][0]["rel_residual"]) > 1e-8 or e_32 <= 1e-5
 
 
