@@ -65,7 +65,7 @@ def library():
         lib.dcg_tape_backward.restype = C.c_int
         lib.dcg_default_render_params.argtypes = [C.c_void_p, C.POINTER(RenderParams)]
         lib.dcg_default_render_params.restype = None
-        for name in ("dcg_render", "dcg_render_host"):
+        for name in ("dcg_render", "dcg_render_host", "dcg_render_host_noalloc"):
             f = getattr(lib, name)
             f.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.c_void_p]
             f.restype = C.c_int
@@ -162,6 +162,14 @@ class Simulation(Handle):
         r = r or self.default_render_params()
         out = np.empty((r.height, r.width, 4), dtype=np.float32)
         self._check(_lib.dcg_render_host(self._h, C.byref(r), C.c_void_p(out.ctypes.data)), "render")
+        return out
+
+    def render_noalloc(self, r=None):
+        """render() through dcg_render_host_noalloc (no device allocation; at most 1024 x 1024)."""
+        import numpy as np
+        r = r or self.default_render_params()
+        out = np.empty((r.height, r.width, 4), dtype=np.float32)
+        self._check(_lib.dcg_render_host_noalloc(self._h, C.byref(r), C.c_void_p(out.ctypes.data)), "render_noalloc")
         return out
 
     def render_device(self, r, ptr):

@@ -2,6 +2,7 @@
 // the shared host machinery of SimBase (stream, fills, timing).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "dcg_internal.hpp"
 
@@ -77,6 +78,7 @@ int SimBase::steps(int n) {
 }
 
 int SimBase::createStream() {
+  DCG_HIP(hipGetDevice(&device));
   // Default priority on purpose.
   DCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   return DCG_OK;
@@ -438,6 +440,53 @@ int dcg_render_host(dcg_sim* s, const dcg_render_params* r, float* rgba_host) {
   const int sst = I->sync();
   (void)hipFree(d);
   return st ? st : sst;
+}
+
+// ------------------------------------------------------------------ allocation-free render
+// dcg_render_host stages the image in device memory it allocates per call. Callers that cannot
+// allocate device memory render through this file-scope array instead: one copy per GPU, held
+// until the process ends, its device address taken once per device with that device current.
+// In host code `scratch` names a host-side stand-in, never a device address: only the pointers
+// hipGetSymbolAddress returns reach kernels and copies.
+static constexpr size_t kScratchPixels = (size_t)1024 * 1024;  // 16 MiB of float4
+static __device__ __align__(256) float4 scratch[kScratchPixels];
+static std::mutex g_scratchMutex;
+static constexpr int kScratchDevices = 64;
+static void* g_scratchPtr[kScratchDevices] = {};
+
+static int scratchAddress(SimBase* I, float** out) {
+  if (I->device < 0 || I->device >= kScratchDevices) return I->fail(DCG_E_UNSUPPORTED, "device index beyond 64");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[I->device]) {
+    int prev = 0;
+    hipError_t de = hipGetDevice(&prev);
+    if (de == hipSuccess && prev != I->device) de = hipSetDevice(I->device);
+    if (de != hipSuccess) return I->fail(DCG_E_HIP, std::string("select device: ") + hipGetErrorString(de));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != I->device) (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p)
+      return I->fail(DCG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratchPtr[I->device] = p;
+  }
+  *out = (float*)g_scratchPtr[I->device];
+  return DCG_OK;
+}
+
+int dcg_render_host_noalloc(dcg_sim* s, const dcg_render_params* r, float* rgba_host) {
+  IMPL(s);
+  if (!r || !rgba_host) return DCG_E_ARG;
+  DCG_TRY(checkRender(I, r));
+  const size_t pixels = (size_t)r->width * r->height;
+  if (pixels > kScratchPixels)
+    return I->fail(DCG_E_ARG, "dcg_render_host_noalloc: image of " + std::to_string(pixels) +
+                                  " pixels exceeds the 1048576-pixel (1024 x 1024) render scratch");
+  float* d = nullptr;
+  DCG_TRY(scratchAddress(I, &d));
+  DCG_TRY(I->render(*r, d));  // every pixel of the image is written: nothing to clear
+  const hipError_t e = hipMemcpyAsync(rgba_host, d, pixels * 4 * sizeof(float), hipMemcpyDeviceToHost, I->stream);
+  if (e != hipSuccess) return I->fail(DCG_E_HIP, std::string("render copy: ") + hipGetErrorString(e));
+  return I->sync();
 }
 
 // ------------------------------------------------------------------ state format

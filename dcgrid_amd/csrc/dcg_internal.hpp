@@ -49,6 +49,7 @@ struct SimBase {
   dcg_params P{};
   std::string err;
   hipStream_t stream = nullptr;
+  int device = 0;  // the GPU the handle was created on (its stream and memory)
   std::vector<void*> allocations;
 
   // Per-kernel timing with events on the handle's stream.

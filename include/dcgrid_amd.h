@@ -347,6 +347,13 @@ typedef struct dcg_render_params {
 void dcg_default_render_params(const dcg_sim* s, dcg_render_params* r);
 int dcg_render(dcg_sim* s, const dcg_render_params* r, float* rgba_device);
 int dcg_render_host(dcg_sim* s, const dcg_render_params* r, float* rgba_host);
+/* No counterpart in the reference: dcg_render_host for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, hosts that forbid hipMalloc). The
+ * image is staged in a file-scope device array of at most 1024 x 1024 pixels (16 MiB per GPU,
+ * held until the process ends) instead of per-call device memory; larger images are DCG_E_ARG.
+ * It allocates nothing and gives the image dcg_render_host gives. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (several handles or threads) race. */
+int dcg_render_host_noalloc(dcg_sim* s, const dcg_render_params* r, float* rgba_host);
 
 /* Versioned state of a simulation (extension: the reference keeps its state in device memory
  * only, src/dcgrid/dcgrid.h:5-43 and src/uniformgrid/uniformgrid.h:31-46). A state holds
