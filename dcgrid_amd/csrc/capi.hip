@@ -41,26 +41,39 @@ int SimBase::steps(int n) {
   const bool useGraph = graphOn && graphCapable() && timingMask == 0 && P.project_mode != DCG_PROJECT_SOLVE && n >= 2;
   int i = 0;
   if (useGraph) {
+    for (; i < n && captureUnsettled(); i++) {
+      DCG_TRY(stepOnce());
+      countHost(1);
+    }
+  }
+  if (useGraph && n - i >= 2) {
     std::vector<char> key;
     graphKeyBytes(key);
     if (!graphExec || key != graphKey) {
       dropGraph();
+      std::vector<char> before;
+      hostSnapshot(before);
       DCG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       int e = stepOnce();
       if (!e) e = stepOnce();
       hipGraph_t captured = nullptr;
       const hipError_t ce = hipStreamEndCapture(stream, &captured);
-      if (e) {
-        if (captured) hipGraphDestroy(captured);
+      if (e) {  // nothing ran: the host state returns to where the capture began
+        if (captured) (void)hipGraphDestroy(captured);
+        hostRestore(before);
         return e;
       }
-      if (ce != hipSuccess) return fail(DCG_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      if (ce != hipSuccess) {
+        hostRestore(before);
+        return fail(DCG_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      }
       graph = captured;
       DCG_HIP(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
       std::vector<char> after;
       graphKeyBytes(after);
       if (after != key) {  // the captured pair must leave the recorded state as it found it
         dropGraph();
+        hostRestore(before);
         return fail(DCG_E_HIP, "step capture changed the recorded state");
       }
       graphKey = key;

@@ -145,6 +145,13 @@ struct SimBase {
   std::vector<char> graphKey;
   virtual bool graphCapable() const { return false; }
   virtual void graphKeyBytes(std::vector<char>& out) const { out.clear(); }
+  // Host-side work a step would do only once (the uniform grid's fluidity pyramid after an
+  // import or a parameter change): while it is pending, steps run eagerly, since a capture
+  // records the launches of one pass but replays them every time.
+  virtual bool captureUnsettled() const { return false; }
+  // Host state a step changes (field-buffer swaps, caches), restored when a capture fails.
+  virtual void hostSnapshot(std::vector<char>& out) const { graphKeyBytes(out); }
+  virtual void hostRestore(const std::vector<char>&) {}
   void dropGraph() {
     if (graphExec) hipGraphExecDestroy(graphExec);
     if (graph) hipGraphDestroy(graph);

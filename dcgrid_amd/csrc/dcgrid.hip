@@ -564,6 +564,14 @@ struct DCGridSim : SimBase {
     put(&P, sizeof(P));
     put(&customFluidity, sizeof(customFluidity));
   }
+  void hostSnapshot(std::vector<char>& out) const override {
+    out.assign((const char*)&g, (const char*)&g + sizeof(g));
+    out.push_back(customFluidity ? 1 : 0);
+  }
+  void hostRestore(const std::vector<char>& in) override {
+    std::memcpy(&g, in.data(), sizeof(g));
+    customFluidity = in[sizeof(g)] != 0;
+  }
   // The step's active cells are counted by the refresh launch of adaptTopologyCounted().
   int countStep() override { return DCG_OK; }
   int cellSteps(uint64_t* out) override {

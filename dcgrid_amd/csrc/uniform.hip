@@ -607,6 +607,21 @@ struct UniformSim : SimBase {
     put(&P, sizeof(P));
     put(&customFluidity, sizeof(customFluidity));
   }
+  bool fluiditySettled() const {
+    return !customFluidity && fluKey.size() == sizeof(P) && std::memcmp(fluKey.data(), &P, sizeof(P)) == 0;
+  }
+  bool captureUnsettled() const override { return !fluiditySettled(); }
+  void hostSnapshot(std::vector<char>& out) const override {
+    out.assign((const char*)&g, (const char*)&g + sizeof(g));
+    out.push_back(customFluidity ? 1 : 0);
+    out.insert(out.end(), fluKey.begin(), fluKey.end());
+  }
+  void hostRestore(const std::vector<char>& in) override {
+    std::memcpy(&g, in.data(), sizeof(g));
+    customFluidity = in[sizeof(g)] != 0;
+    fluKey.assign(in.begin() + sizeof(g) + 1, in.end());
+  }
+
   int cellSteps(uint64_t* out) override {
     *out = cellStepCount;
     return DCG_OK;

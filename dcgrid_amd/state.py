@@ -134,3 +134,68 @@ def write(path, st):
         out[o:o + nb] = np.ascontiguousarray(st.sections[names[i]]).view(np.uint8).ravel()
     out.tofile(path)
     return off
+
+
+def _morton(x, y, z, bb):
+    """Local cell index of block coordinates (x, y, z): bit b of x at 3b + 2, of y at 3b + 1, of
+    z at 3b (src/dcgrid/dcgrid_utils.cuh:50-58,69-81), for int64 arrays (negative values keep
+    their two's-complement low bits, as the device's int arithmetic does)."""
+    r = np.zeros(np.broadcast(x, y, z).shape, dtype=np.int64)
+    for b in range(bb):
+        r |= (((x >> b) & 1) << (3 * b + 2)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b))
+    return r
+
+
+def reference_tables(st):
+    """The block tables of a DCGrid state in the reference's form (what dcg_export_tables
+    returns, src/dcgrid/dcgrid.h:5-73): uint64 links with 2^64 - 1 for none, uint8 flags, and the
+    (BW+2)^3 apron cell indices of every block rebuilt from its 27 neighbour descriptors (an
+    entry of direction d is the descriptor's block at the offset shifted to its level, or, outside
+    the domain, the block's own clamped cell; free blocks have no border entries)."""
+    bw = st.block_width
+    bb = {4: 2, 8: 3}[bw]
+    bv, A = bw ** 3, bw + 2
+    lev = st.table("levels")
+    B = lev.size
+    none = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def u64(a):
+        out = a.astype(np.uint64)
+        out[a == 0xFFFFFFFF] = none
+        return out
+
+    nd = st.table("neighbours").astype(np.int64)  # (B, 27, 4)
+    i, j, k = (a.ravel() for a in np.meshgrid(np.arange(A), np.arange(A), np.arange(A), indexing="ij"))
+    dx = np.where(i == 0, -1, np.where(i == bw + 1, 1, 0))
+    dy = np.where(j == 0, -1, np.where(j == bw + 1, 1, 0))
+    dz = np.where(k == 0, -1, np.where(k == bw + 1, 1, 0))
+    interior = (dx == 0) & (dy == 0) & (dz == 0)
+    d = nd[:, (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1), :]  # (B, A^3, 4)
+    x0 = d[..., 0] & 0xFFFFFFFF
+    sh, nb = x0 >> 27, x0 & 0x07FFFFFF
+    b = np.arange(B, dtype=np.int64)[:, None]
+    ci, cj, ck = np.clip(i, 1, bw), np.clip(j, 1, bw), np.clip(k, 1, bw)
+    spread = _morton(ci & (bw - 1), cj & (bw - 1), ck & (bw - 1), bb)
+    clamped = _morton(ci - 1, cj - 1, ck - 1, bb)
+    out_cell = b * bv + np.where(d[..., 1] != 0, spread[None, :], clamped[None, :])
+    s = np.where(sh == 31, 0, sh)
+    in_cell = nb * bv + _morton((i - 1 - d[..., 1]) >> s, (j - 1 - d[..., 2]) >> s, (k - 1 - d[..., 3]) >> s, bb)
+    cell = np.where(sh == 31, out_cell, in_cell)
+    cell = np.where(interior[None, :], b * bv + _morton(i - 1, j - 1, k - 1, bb)[None, :], cell).astype(np.uint64)
+    cell[(lev == 0xFF)[:, None] & ~interior[None, :]] = none
+    return {
+        "block_positions": st.table("positions").astype(np.int32).reshape(B, 3),
+        "block_levels": lev.astype(np.uint8),
+        "block_flags": st.table("flags").astype(np.uint8),
+        "parent_indices": u64(st.table("parents")),
+        "child_indices": u64(st.table("children")),
+        "cell_indices": cell.reshape(B, A ** 3),
+        "hash_key": st.table("hash_keys").astype(np.uint32),
+        "hash_val": u64(st.table("hash_values")),
+    }
+
+
+def control_words(st):
+    """(block loads, move limits) per level from a DCGrid state's control words."""
+    ctl = st.table("control")
+    return [int(v) for v in ctl[0:st.levels]], [int(v) for v in ctl[16:16 + st.levels]]

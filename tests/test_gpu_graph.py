@@ -86,3 +86,25 @@ def test_graph_replay_matches_eager(name):
     same(state(g), state(e))
     g.close()
     e.close()
+
+
+def test_graph_after_imported_fluidity():
+    """An imported fluidity field on the uniform grid: the first step after the import rebuilds
+    the fluidity pyramid on the host's order (host state a captured pass would not replay), so
+    that step runs eagerly and the capture follows; replay after replay equals the eager run."""
+    g, e = make("uniform32"), make("uniform32")
+    n = g.field_size("fluidity")
+    rng = np.random.default_rng(5)
+    flu = (0.5 + 0.5 * rng.random(n)).astype(np.float32)
+    for s in (g, e):
+        s.import_field("fluidity", flu)
+    g.set_graph(True)
+    for rep in range(3):
+        say(f"[case] imported fluidity replay {rep}")
+        g.step(2)
+        e.step(2)
+        g.synchronize()
+        same(state(g), state(e))
+        assert np.array_equal(g.export_field("fluidity").view(np.uint32), e.export_field("fluidity").view(np.uint32))
+    g.close()
+    e.close()

@@ -152,6 +152,27 @@ def test_dcgrid_steps_bitwise_vs_fp32_oracle(name):
         assert g.debug_stats() == c.debug_stats()
 
 
+@pytest.mark.parametrize("name", ["cfg1_64", "ref64_bw4_solids"])
+def test_dcgrid_project_local_bitwise(name):
+    """projectLocal (fluid_simulation_dcgrid.cu:296-311): divergence, project_local_pairs Jacobi
+    pairs on every level from the top without prolongation, gradient subtraction and velocity
+    restriction; after two reference steps, the GPU's velocity, pressure and divergence equal the
+    FP32 oracle's bit for bit (a sparse scene, and BW 4 with sphere solids)."""
+    g, c = pair(name)
+    g.step(2)
+    c.step(2)
+    assert_tables_equal(g, c)
+    for rep in range(2):
+        say(f"[case] {name} project_local {rep}")
+        g.project_local()
+        c.project_local()
+        m = allocated_cells(c)
+        for f in ("velocity", "pressure", "t_pressure", "divergence"):
+            assert bitwise(g.export_field(f), c.export_field(f), m), (name, rep, f)
+    g.close()
+    c.close()
+
+
 @pytest.mark.parametrize("name", ["ref64_bw4", "ref64_bw4_solids", "cfg1_128"])
 def test_dcgrid_fp64_per_step(name):
     """From the same input state, one GPU step is within 1e-5 relative L2 of the FP64 oracle."""

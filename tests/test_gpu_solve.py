@@ -31,6 +31,9 @@ def workload(name):
     if name == "ref64_bw4":
         p = scenes.reference_params(64)
         return "dcgrid", (64, 64, 64), p, 1500
+    if name == "cfg4_128":  # terrain solids: fluidity in the divergence and the gradient
+        w = scenes.config(4, scale=8)
+        return w.kind, w.size, w.params, w.max_num_blocks
     raise KeyError(name)
 
 
@@ -64,7 +67,7 @@ def level_cells(sim, level):
     return np.repeat(t["block_levels"] == level, inf.block_width ** 3)
 
 
-@pytest.mark.parametrize("name", ["cfg2_128", "ref64_bw4", "cfg1_128"])
+@pytest.mark.parametrize("name", ["cfg2_128", "ref64_bw4", "cfg1_128", "cfg4_128"])
 def test_mixed_solve_reaches_tolerance_and_matches_oracle(name):
     g, c = prepared(name, SOLVE_MIXED)
     g.project_solve()

@@ -189,3 +189,39 @@ def test_load_rejects_foreign_or_damaged_states(tmp_path):
     a.close()
     b.close()
     c.close()
+
+
+@pytest.mark.parametrize("name", ["cfg4_128_sparse", "cfg1_64", "ref64_bw4"])
+def test_oracle_continues_a_gpu_written_state(name, tmp_path):
+    """The oracle leg of the format: a state the GPU wrote is read with numpy alone, its block
+    tables rebuilt in the reference's form (dcgrid_amd.state.reference_tables, equal to what the
+    C ABI exports), loaded into the CPU oracle with its fields, and the oracle then steps bit for
+    bit with the GPU resumed from the same file (tables, hash map and fields)."""
+    import oracle
+    kind, size, p, b, seed = workload(name)
+    say(f"[case] oracle resumes {name}")
+    g = make(name, seed_steps=3)
+    path = tmp_path / "s"
+    g.save_state(str(path))
+    st = state.read(str(path))
+    t = state.reference_tables(st)
+    assert compare_tables(t, g.export_tables(), TABLE_KEYS + ("hash_key", "hash_val")) == []
+    c = oracle.OracleSim(kind, size, st.params, b, precision=32)
+    loads, limits = state.control_words(st)
+    c.import_tables(t, loads, limits)
+    for f in fields_of(g):
+        c.import_field(f, st.field(f).reshape(-1))
+    r = dcgrid_amd.Simulation(kind, size, p, b)
+    r.load_state(str(path))
+    g.close()
+    for k in range(3):
+        r.step(1)
+        c.step(1)
+        tr, tc = r.export_tables(), c.export_tables()
+        assert compare_tables(tr, tc, TABLE_KEYS) == [], (name, k)
+        assert hash_map(tr, r.info()) == hash_map(tc, c.info()), (name, k)
+        m = mask_of(r)
+        for f in ("density", "velocity", "pressure", "temperature", "fluidity"):
+            assert same_field(r.export_field(f), np.asarray(c.export_field(f), np.float32), m), (name, k, f)
+    r.close()
+    c.close()

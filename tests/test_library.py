@@ -79,3 +79,25 @@ def test_create_without_gpu_fails_loudly():
     with pytest.raises(dcgrid_amd.DCGError) as e:
         dcgrid_amd.Simulation("uniform", (16, 16, 16), scenes.config(0, scale=4).params)
     assert e.value.code == -3  # DCG_E_HIP
+
+
+@pytest.mark.parametrize("size,bw,levels,budget,code", [
+    ((96, 96, 96), 8, 4, 4000, -4),    # not a multiple of the 64-cell top block: DCG_E_UNSUPPORTED
+    ((96, 64, 64), 4, 0, 4000, -4),    # the reference's level rule gives 5 levels; 96 is not a multiple of 64
+    ((64, 64, 64), 8, 3, 7, -2),       # fewer blocks than the dense top level: DCG_E_BUDGET
+    ((64, 64, 64), 8, 3, 8, -2),       # the top level takes the whole budget, level 0 gets none
+])
+def test_planner_deviations_are_reported(size, bw, levels, budget, code):
+    """Grids this build does not plan (documented deviations from fluid_simulation_dcgrid.cu:29-53,
+    which sizes partial coarse blocks with iDivUp) fail at creation with a status code, before
+    any device call (so also without a GPU), and the oracle's planner agrees."""
+    import oracle
+    p = scenes.reference_params(size[0], size[1], size[2])
+    p.block_width = bw
+    p.max_levels = levels
+    with pytest.raises(dcgrid_amd.DCGError) as e:
+        dcgrid_amd.Simulation("dcgrid", size, p, budget)
+    assert e.value.code == code
+    with pytest.raises(dcgrid_amd.DCGError) as e2:
+        oracle.plan(size, budget, p)
+    assert e2.value.code == code

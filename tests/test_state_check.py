@@ -149,3 +149,21 @@ def test_header_describing_an_impossible_grid(lib, tmp_path):
     state.write(str(path), st)
     code, msg = dcgrid_amd.state_check(path.read_bytes())
     assert code == DCG_E_ARG and "planner" in msg
+
+
+def test_reference_tables_match_the_oracle_activation(lib, tmp_path):
+    """dcgrid_amd.state.reference_tables rebuilds the reference-form tables from a state's
+    neighbour descriptors: on the all-dense grid they equal the CPU oracle's own tables after
+    its activation (k_dcgrid_activate_level semantics), apron entry by apron entry."""
+    import oracle
+    oracle.library()
+    path = tmp_path / "d"
+    state.write(str(path), dense_state())
+    t = state.reference_tables(state.read(str(path)))
+    p = scenes.reference_params(16)
+    p.block_width = 8
+    c = oracle.OracleSim("dcgrid", (16, 16, 16), p, 9, precision=32)
+    tc = c.export_tables()
+    for k in ("block_positions", "block_levels", "parent_indices", "child_indices", "cell_indices"):
+        assert np.array_equal(t[k], tc[k]), k
+    c.close()
