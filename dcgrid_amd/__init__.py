@@ -209,7 +209,7 @@ class Simulation(Handle):
         self._check(_lib.dcg_get_params(self._h, C.byref(p)), "get_params")
         return p
 
-    # -- device-pointer access and the differentiable rollout (uniform grid)
+    # -- device-pointer access and the differentiable rollout (uniform grid; DCGrid on a frozen topology)
     def export_field_device(self, field, ptr, nfloats):
         """Copy a field into caller-owned device memory at `ptr` (an int address)."""
         f = FIELDS.get(field, field)

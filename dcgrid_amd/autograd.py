@@ -1,10 +1,12 @@
-"""torch.autograd binding of the differentiable rollout (uniform grid).
+"""torch.autograd binding of the differentiable rollout (uniform grid; DCGrid on a frozen block
+topology, FP32 field storage).
 
 `rollout(sim, density0, velocity0, steps, checkpoint_every)` returns the density and velocity
 after `steps` steps of the simulation step; its backward runs the hand-written adjoint kernels
 of libdcgrid_amd through the C ABI (dcg_tape_forward / dcg_tape_backward). Tensors are float32
-CUDA tensors: density [N] and velocity [N, 3] in the grid's x-fastest cell order. Temperature is
-held at the simulation's current field and receives no gradient here.
+CUDA tensors: density [N] and velocity [N, 3] in the grid's cell order (uniform: x fastest;
+DCGrid: block-major, the order of dcg_export_field). Temperature is held at the simulation's
+current field and receives no gradient here.
 """
 import torch
 
@@ -51,5 +53,5 @@ class Rollout(torch.autograd.Function):
 
 
 def rollout(sim, density0, velocity0, steps, checkpoint_every=8):
-    """Differentiable `steps`-step rollout of `sim` (a uniform-grid dcgrid_amd.Simulation)."""
+    """Differentiable `steps`-step rollout of `sim` (a dcgrid_amd.Simulation)."""
     return Rollout.apply(density0, velocity0, sim, steps, checkpoint_every)

@@ -112,13 +112,9 @@ struct SimBase {
     return fail(DCG_E_UNSUPPORTED, "device field copies are not available for this grid");
   }
   // Differentiable rollout (extension).
-  virtual int tapeForward(int steps, int checkpointEvery) {
-    return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
-  }
+  virtual int tapeForward(int steps, int checkpointEvery) = 0;
   virtual int tapeBackward(const float* gden, const float* gvel, const float* gtem, float* oden, float* ovel,
-                           float* otem) {
-    return fail(DCG_E_UNSUPPORTED, "the adjoint is implemented for the uniform grid");
-  }
+                           float* otem) = 0;
 
   // Ray-marched image of the fields into caller-owned device memory (float4 per pixel).
   virtual int render(const dcg_render_params& r, float* rgbaDevice) = 0;

@@ -13,6 +13,7 @@
 #include "dcgrid_fluid.hpp"
 #include "dcgrid_solve.hpp"
 #include "dcgrid_topo.hpp"
+#include "dcgrid_adjoint.hpp"
 #include "render.hpp"
 
 namespace dcg {
@@ -1177,6 +1178,229 @@ struct DCGridSim : SimBase {
       }
     }
     return DCG_OK;
+  }
+
+  // ---------------------------------------------------------------- differentiable rollout
+  // (extension). Steps on a frozen block topology: advectVelocity, the Jacobi projection,
+  // advectDensity, without adaptTopology. Checkpoints of (density, temperature, velocity) every
+  // K steps; the backward pass recomputes each segment from its checkpoint, keeping the
+  // segment's states, and runs the transposed kernels of dcgrid_adjoint.hpp in reverse.
+  float* ckBuf = nullptr;   // checkpoint states, 5 C floats each
+  float* segBuf = nullptr;  // the states of the segment being reversed
+  int tapeCk = 0, tapeSeg = 0, tapeN = 0, tapeK = 0;
+  bool tapeValid = false;
+  float *lq = nullptr, *lt = nullptr, *lq0 = nullptr, *lt0 = nullptr;  // density / temperature adjoints
+  float3 *lu = nullptr, *lu0 = nullptr;                                 // velocity adjoints
+  float *lp = nullptr, *ltp = nullptr, *ld = nullptr;                   // pressure pair, divergence
+  float* stateDen(float* base, int k) { return base + (size_t)k * 5 * C; }
+  float* stateTem(float* base, int k) { return base + (size_t)k * 5 * C + C; }
+  float3* stateVel(float* base, int k) { return (float3*)(base + (size_t)k * 5 * C + 2 * C); }
+  int copyDD(void* dst, const void* src, size_t bytes) {
+    DCG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
+    return DCG_OK;
+  }
+  // Temperature changes only with a source (it is advected then); otherwise every state holds
+  // the same temperature, which is neither saved nor restored.
+  int saveState(float* base, int k) {
+    DCG_TRY(copyDD(stateDen(base, k), g.den, C * 4));
+    if (thermo()) DCG_TRY(copyDD(stateTem(base, k), g.tem, C * 4));
+    return copyDD(stateVel(base, k), g.vel, C * 12);
+  }
+  int loadState(float* base, int k) {
+    DCG_TRY(copyDD(g.den, stateDen(base, k), C * 4));
+    if (thermo()) DCG_TRY(copyDD(g.tem, stateTem(base, k), C * 4));
+    return copyDD(g.vel, stateVel(base, k), C * 12);
+  }
+  int frozenStep() {
+    DCG_TRY(advectVelocity());
+    DCG_TRY(P.project_mode == DCG_PROJECT_LOCAL ? projectLocal() : project());
+    return advectDensity();
+  }
+  unsigned cellWalkGrid(size_t cells) const {
+    const unsigned n = blocksFor(cells, 256);
+    return n < 4096u ? (n ? n : 1u) : 4096u;
+  }
+  template <class T>
+  int restrictAllT(T* lam) {
+    for (int l = L - 2; l >= 0; l--) {
+      const size_t groups = (size_t)maxL[l] * (G::BV / 8);
+      const unsigned n = blocksFor(groups, 256) < 2048 ? blocksFor(groups, 256) : 2048;
+      DCG_LAUNCH(DCG_K_RESTRICT, "k_da_restrict_T", (k_da_restrict_T<BW, T><<<n, 256, 0, stream>>>(g, l, lam)));
+    }
+    return DCG_OK;
+  }
+  int tapeForward(int steps, int K) override {
+    if constexpr (!std::is_same<SF, float>::value) {
+      return fail(DCG_E_UNSUPPORTED, "the adjoint runs on FP32 field storage");
+    } else {
+      if (steps < 1 || K < 1) return fail(DCG_E_ARG, "steps and checkpoint interval must be positive");
+      if (P.project_mode == DCG_PROJECT_SOLVE) return fail(DCG_E_UNSUPPORTED, "the adjoint follows the Jacobi schedules");
+      if (P.moist) return fail(DCG_E_UNSUPPORTED, "the adjoint does not differentiate the moist atmosphere");
+      if (K > steps) K = steps;
+      const int ck = (steps + K - 1) / K + 1;  // states 0, K, 2K, ... below steps, and the final one
+      if (ck != tapeCk || K + 1 != tapeSeg) {
+        release(ckBuf);
+        release(segBuf);
+        ckBuf = segBuf = nullptr;
+        tapeCk = tapeSeg = 0;
+        DCG_TRY(alloc(&ckBuf, (size_t)ck * 5 * C));
+        DCG_TRY(alloc(&segBuf, (size_t)(K + 1) * 5 * C));
+        tapeCk = ck;
+        tapeSeg = K + 1;
+      }
+      if (!lq) {
+        DCG_TRY(alloc(&lq, C));
+        DCG_TRY(alloc(&lt, C));
+        DCG_TRY(alloc(&lq0, C));
+        DCG_TRY(alloc(&lt0, C));
+        DCG_TRY(alloc(&lu, C));
+        DCG_TRY(alloc(&lu0, C));
+        DCG_TRY(alloc(&lp, C));
+        DCG_TRY(alloc(&ltp, C));
+        DCG_TRY(alloc(&ld, C));
+      }
+      dropGraph();  // the pointer swaps of the steps below move the live fields
+      tapeValid = false;
+      DCG_TRY(saveState(ckBuf, 0));
+      for (int k = 0; k < steps; k++) {
+        DCG_TRY(frozenStep());
+        if ((k + 1) % K == 0 && k + 1 < steps) DCG_TRY(saveState(ckBuf, (k + 1) / K));
+      }
+      DCG_TRY(saveState(ckBuf, ck - 1));
+      tapeN = steps;
+      tapeK = K;
+      tapeValid = true;
+      return sync();
+    }
+  }
+  // Adjoint of one step between segment states s (before) and s + 1 (after).
+  int adjointStep(int s) {
+    const bool src = thermo(), fl = fluidityOn();
+    const float* den0 = stateDen(segBuf, s);
+    const float* tem0 = stateTem(segBuf, s);
+    const float3* vel0 = stateVel(segBuf, s);
+    const float3* vel1 = stateVel(segBuf, s + 1);
+    const unsigned nAll = cellWalkGrid(C);
+    // advectDensity: the restriction of the advected scalars, then the transposed gather
+    DCG_TRY(restrictAllT(lq));
+    if (src) DCG_TRY(restrictAllT(lt));
+    DCG_TRY(fill32(lq0, 0, C));
+    if (src) DCG_TRY(fill32(lt0, 0, C));
+#define DA_ADV_D(F, S)                                                                         \
+  DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_da_advect_density_T",                                    \
+             (k_da_advect_density_T<BW, F, S><<<nAll, 256, 0, stream>>>(g, P, den0, tem0, vel1, lq, lt, lq0, lt0, lu)))
+    if (fl && src) DA_ADV_D(true, true);
+    else if (fl) DA_ADV_D(true, false);
+    else if (src) DA_ADV_D(false, true);
+    else DA_ADV_D(false, false);
+#undef DA_ADV_D
+    // projection: restriction of the velocity, gradient, Jacobi schedule, restriction of the
+    // divergence, divergence, each reversed
+    DCG_TRY(restrictAllT(lu));
+    DCG_TRY(fill32(lp, 0, C));
+    DCG_TRY(fill32(ltp, 0, C));
+    DCG_TRY(fill32(ld, 0, C));
+    if (fl)
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_da_grad_T", (k_da_grad_T<BW, true><<<nAll, 256, 0, stream>>>(g, P, lu, lp)));
+    else
+      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_da_grad_T", (k_da_grad_T<BW, false><<<nAll, 256, 0, stream>>>(g, P, lu, lp)));
+    auto pairsT = [&](int level, int pairs) -> int {
+      const int sc = 1 << level;
+      const float alpha = (float)(sc * sc) * P.dx * P.dx;
+      const unsigned n = cellWalkGrid((size_t)maxL[level] * G::BV);
+      for (int i = 0; i < pairs; i++) {
+        // jacobi(level, t_pre, pre) then jacobi(level, pre, t_pre), reversed
+        DCG_LAUNCH(DCG_K_JACOBI, "k_da_jacobi_T", (k_da_jacobi_T<BW><<<n, 256, 0, stream>>>(g, level, lp, ltp, ld, alpha)));
+        DCG_LAUNCH(DCG_K_JACOBI, "k_da_jacobi_T", (k_da_jacobi_T<BW><<<n, 256, 0, stream>>>(g, level, ltp, lp, ld, alpha)));
+      }
+      return DCG_OK;
+    };
+    if (P.project_mode == DCG_PROJECT_LOCAL) {
+      for (int l = 0; l < L; l++) DCG_TRY(pairsT(l, P.project_local_pairs));
+    } else {
+      for (int l = 0; l <= L - 2; l++) {
+        DCG_TRY(pairsT(l, P.project_level_pairs));
+        DCG_LAUNCH(DCG_K_PROLONGATE, "k_da_prolongate_T",
+                   (k_da_prolongate_T<BW><<<cellWalkGrid((size_t)maxL[l] * G::BV), 256, 0, stream>>>(g, l, lp)));
+      }
+      DCG_TRY(pairsT(L - 1, P.project_top_pairs));
+    }
+    DCG_TRY(restrictAllT(ld));
+    if (fl)
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_da_div_T", (k_da_div_T<BW, true><<<nAll, 256, 0, stream>>>(g, P, ld, lu)));
+    else
+      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_da_div_T", (k_da_div_T<BW, false><<<nAll, 256, 0, stream>>>(g, P, ld, lu)));
+    // advectVelocity: the restriction of the advected velocity, then the transposed gather
+    DCG_TRY(restrictAllT(lu));
+    DCG_TRY(fill32(lu0, 0, 3 * C));
+#define DA_ADV_V(F, Bu)                                                                        \
+  DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_da_advect_velocity_T",                                  \
+             (k_da_advect_velocity_T<BW, F, Bu><<<nAll, 256, 0, stream>>>(g, P, vel0, lu, lu0, lq0, lt0)))
+    if (fl && src) DA_ADV_V(true, true);
+    else if (fl) DA_ADV_V(true, false);
+    else if (src) DA_ADV_V(false, true);
+    else DA_ADV_V(false, false);
+#undef DA_ADV_V
+    std::swap(lq, lq0);
+    std::swap(lu, lu0);
+    if (src) std::swap(lt, lt0);  // without a source the temperature adjoint passes through
+    return DCG_OK;
+  }
+  int tapeBackward(const float* gden, const float* gvel, const float* gtem, float* oden, float* ovel,
+                   float* otem) override {
+    if constexpr (!std::is_same<SF, float>::value) {
+      return fail(DCG_E_UNSUPPORTED, "the adjoint runs on FP32 field storage");
+    } else {
+      if (!tapeValid) return fail(DCG_E_ARG, "no recorded rollout (dcg_tape_forward)");
+      DCG_TRY(copyDD(lq, gden, C * 4));
+      DCG_TRY(copyDD(lu, gvel, C * 12));
+      if (gtem)
+        DCG_TRY(copyDD(lt, gtem, C * 4));
+      else
+        DCG_TRY(fill32(lt, 0, C));
+      const int segs = (tapeN + tapeK - 1) / tapeK;
+      for (int sg = segs - 1; sg >= 0; sg--) {
+        const int k0 = sg * tapeK, k1 = std::min(tapeN, k0 + tapeK);
+        DCG_TRY(loadState(ckBuf, sg));
+        DCG_TRY(saveState(segBuf, 0));
+        for (int k = k0; k < k1; k++) {
+          DCG_TRY(frozenStep());
+          DCG_TRY(saveState(segBuf, k - k0 + 1));
+        }
+        for (int k = k1 - 1; k >= k0; k--) DCG_TRY(adjointStep(k - k0));
+      }
+      DCG_TRY(copyDD(oden, lq, C * 4));
+      DCG_TRY(copyDD(ovel, lu, C * 12));
+      if (otem) DCG_TRY(copyDD(otem, lt, C * 4));
+      DCG_LAUNCH(DCG_K_OTHER, "k_da_mask_free",
+                 (k_da_mask_free<BW><<<cellWalkGrid(C), 256, 0, stream>>>(g, oden, (float3*)ovel, otem, C)));
+      DCG_TRY(loadState(ckBuf, tapeCk - 1));  // the live state is the rollout's final state again
+      return sync();
+    }
+  }
+  // Device-memory field copies (FP32 storage), for the torch binding of the rollout.
+  int exportFieldDevice(int f, float* dst, uint64_t n) override {
+    if constexpr (!std::is_same<SF, float>::value) {
+      return fail(DCG_E_UNSUPPORTED, "device field copies need FP32 field storage");
+    } else {
+      uint64_t need;
+      DCG_TRY(fieldSize(f, &need));
+      if (n != need || !dst) return fail(DCG_E_ARG, "size mismatch");
+      DCG_TRY(copyDD(dst, fieldPtr(f), n * 4));
+      return sync();
+    }
+  }
+  int importFieldDevice(int f, const float* src, uint64_t n) override {
+    if constexpr (!std::is_same<SF, float>::value) {
+      return fail(DCG_E_UNSUPPORTED, "device field copies need FP32 field storage");
+    } else {
+      uint64_t need;
+      DCG_TRY(fieldSize(f, &need));
+      if (n != need || !src) return fail(DCG_E_ARG, "size mismatch");
+      DCG_TRY(copyDD(fieldPtr(f), src, n * 4));
+      if (f == DCG_FIELD_FLUIDITY) customFluidity = true;
+      return sync();
+    }
   }
 };
 

@@ -291,21 +291,26 @@ int dcg_field_size(dcg_sim* s, int32_t field, uint64_t* nfloats);
 int dcg_export_field(dcg_sim* s, int32_t field, float* dst, uint64_t nfloats);
 int dcg_import_field(dcg_sim* s, int32_t field, const float* src, uint64_t nfloats);
 /* The same copies with caller-owned DEVICE memory (e.g. torch.Tensor.data_ptr()); the caller
- * finishes its own work on that memory first; both synchronise the handle's stream.
- * Uniform grid. Extension; no counterpart in the reference. */
+ * finishes its own work on that memory first; both synchronise the handle's stream. Uniform
+ * grid, and DCGrid with FP32 field storage. Extension; no counterpart in the reference. */
 int dcg_export_field_device(dcg_sim* s, int32_t field, float* dst_device, uint64_t nfloats);
 int dcg_import_field_device(dcg_sim* s, int32_t field, const float* src_device, uint64_t nfloats);
 
-/* Differentiable rollout on the uniform grid (extension; the reference has no derivatives).
- * dcg_tape_forward runs `steps` steps of dcg_step from the current state and keeps a
- * checkpoint of the state (density, temperature, velocity) every `checkpoint_every` steps.
- * dcg_tape_backward takes the adjoint of the final state (d loss / d density, velocity as
+/* Differentiable rollout (extension; the reference has no derivatives).
+ * dcg_tape_forward runs `steps` steps from the current state and keeps a checkpoint of the
+ * state (density, temperature, velocity) every `checkpoint_every` steps. Uniform grid: the steps
+ * of dcg_step. DCGrid: the steps on a frozen block topology (advectVelocity, the Jacobi
+ * projection, advectDensity; no adaptTopology), so the block tables and fluidity stay as they
+ * are. dcg_tape_backward takes the adjoint of the final state (d loss / d density, velocity as
  * xyz-interleaved float3, temperature; g_temperature may be NULL for zero) and writes the
  * gradient with respect to the state at the start of the rollout (d_temperature may be NULL),
  * recomputing each segment forward from its checkpoint. All six pointers are caller-owned
- * device memory of N (3N for velocity) floats. The fluidity field is held fixed. Returns
- * DCG_E_UNSUPPORTED on a DCGrid handle, or when a foot moves more than 4 cells in one step.
- * Afterwards the live state is the rollout's final state. Synchronises. */
+ * device memory of N (3N for velocity) floats, N the field size of dcg_field_size. On the DCGrid
+ * the gradient's entries for cells of free blocks are 0, and its sums are formed with FP32
+ * atomics (reproducible to rounding, not bitwise). Returns DCG_E_UNSUPPORTED with the projection
+ * solve (DCG_PROJECT_SOLVE), with the moist atmosphere, for DCGrid 16-bit field storage, or
+ * (uniform grid) when a foot moves more than 4 cells in one step. Afterwards the live state is
+ * the rollout's final state. Synchronises. */
 int dcg_tape_forward(dcg_sim* s, int32_t steps, int32_t checkpoint_every);
 int dcg_tape_backward(dcg_sim* s, const float* g_density, const float* g_velocity, const float* g_temperature,
                       float* d_density, float* d_velocity, float* d_temperature);

@@ -167,7 +167,8 @@ class OracleSim(Handle):
         self._check(st, "import_field_f64")
 
     def tape_forward(self, steps):
-        """Record `steps` steps from the current state (differentiable rollout, uniform grid)."""
+        """Record `steps` steps from the current state (differentiable rollout; DCGrid: on a frozen
+        block topology)."""
         self._check(_lib.orc_tape_forward(self._h, int(steps)), "tape_forward")
 
     def tape_backward(self, g_density, g_velocity, g_temperature=None):

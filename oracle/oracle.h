@@ -49,10 +49,10 @@ int orc_export_field_f64(orc_sim* s, int32_t field, double* dst, uint64_t nfloat
 int orc_import_field(orc_sim* s, int32_t field, const float* src, uint64_t nfloats);
 int orc_import_field_f64(orc_sim* s, int32_t field, const double* src, uint64_t nfloats);
 int orc_export_tables(orc_sim* s, dcg_tables* t);
-/* Differentiable rollout on the uniform grid (extension): record n steps from the current
- * state, then map the adjoint of the final state (density, velocity xyz-interleaved,
- * temperature; gtem/otem may be NULL) to the gradient with respect to the recorded initial
- * state. FP64 in the FP64 instance: the reference-logic gradient. */
+/* Differentiable rollout (extension): record n steps from the current state (DCGrid: on a
+ * frozen block topology, no adaptTopology), then map the adjoint of the final state (density,
+ * velocity xyz-interleaved, temperature; gtem/otem may be NULL) to the gradient with respect to
+ * the recorded initial state. FP64 in the FP64 instance: the reference-logic gradient. */
 int orc_tape_forward(orc_sim* s, int32_t n);
 int orc_tape_backward(orc_sim* s, const double* gden, const double* gvel, const double* gtem, double* oden,
                       double* ovel, double* otem);

@@ -321,14 +321,12 @@ struct Sim {
     if (P.project_mode == DCG_PROJECT_SOLVE) return projectSolve();
     return project();
   }
-  // Differentiable rollout (extension; uniform grid only): record n steps, then pull the
-  // adjoint of the final state back to the recorded initial state. Null temperature pointers
-  // mean a zero seed / no output.
-  virtual int tapeForward(int n) { return fail(DCG_E_UNSUPPORTED, "adjoint: uniform grid only"); }
+  // Differentiable rollout (extension): record n steps (on the DCGrid with the block topology
+  // frozen), then pull the adjoint of the final state back to the recorded initial state. Null
+  // temperature pointers mean a zero seed / no output.
+  virtual int tapeForward(int n) = 0;
   virtual int tapeBackward(const double* gden, const double* gvel, const double* gtem, double* oden,
-                           double* ovel, double* otem) {
-    return fail(DCG_E_UNSUPPORTED, "adjoint: uniform grid only");
-  }
+                           double* ovel, double* otem) = 0;
   int fail(int code, const std::string& m) {
     err = m;
     return code;

@@ -1110,6 +1110,330 @@ struct DCGrid : Sim {
     return DCG_OK;
   }
 
+  // ---------------------------------------------------------------- adjoint (extension)
+  // Reverse mode of the step on a frozen block topology: advectVelocity, the Jacobi projection
+  // (project or projectLocal), advectDensity, without adaptTopology, so the block tables, apron
+  // indices and fluidity are fixed over the rollout. Each forward loop `out = f(in)` is mirrored,
+  // in reverse order, by `lam_in += (df/din)^T lam_out`, then `lam_out = 0` where the output
+  // overwrote its previous value. Plain serial scatters in a fixed order. Storage rounding is
+  // taken as the identity. Cells of free blocks take no part in a step (nothing reads them and
+  // their values after a step are undefined), so their gradient entries are 0.
+  struct Snap {
+    std::vector<R> den, tem;
+    std::vector<V3<R>> vel;
+  };
+  std::vector<Snap> tape;
+
+  int tapeForward(int n) override {
+    if (n < 0) return fail(DCG_E_ARG, "negative step count");
+    if (P.moist) return fail(DCG_E_UNSUPPORTED, "the adjoint does not differentiate the moist atmosphere");
+    if (P.project_mode == DCG_PROJECT_SOLVE) return fail(DCG_E_UNSUPPORTED, "the adjoint follows the Jacobi schedules");
+    tape.clear();
+    tape.push_back(Snap{den, tem, vel});
+    for (int k = 0; k < n; k++) {
+      if (int e = advectVelocity()) return e;
+      if (int e = projectByMode()) return e;
+      if (int e = advectDensity()) return e;
+      tape.push_back(Snap{den, tem, vel});
+    }
+    return DCG_OK;
+  }
+
+  // accumulate() reversed: levels from coarse to fine, a parent cell's adjoint shared equally by
+  // its 8 children, then cleared (the restriction overwrote the parent cell).
+  template <class T>
+  void restrictAdj(std::vector<T>& lam) {
+    for (int level = L - 2; level >= 0; level--)
+      for (size_t b = offs[level]; b < offs[level] + maxL[level]; b++) {
+        if (lev[b] != level || par[b] == NF) continue;
+        const size_t ps = par[b];
+        for (size_t g = 0; g < (size_t)BV / 8; g++) {
+          const size_t pc = (ps / 8) * BV + ((ps % 8) << SBB) + g;
+          const T G = mulT(lam[pc], (R).125f);
+          for (size_t i = 0; i < 8; i++) lam[b * BV + 8 * g + i] = addT(lam[b * BV + 8 * g + i], G);
+          lam[pc] = T{};
+        }
+      }
+  }
+  // Sample of the foot of leaf cell c under velocity v (the backtrace expressions), with the
+  // unnormalised corner weights f_c t_c, d t_c / d frac_a, and k = d frac / d v.
+  struct FootD {
+    Sample s;
+    R f[8], t[8], dtd[8][3], W, k;
+  };
+  void footD(size_t c, const V3<R>& v, FootD& F) const {
+    const R scale = (R)(1 << lev[c / BV]);
+    const R alpha = (R)P.dt * (R)P.rdx;
+    int cx, cy, cz;
+    cellPos(c, cx, cy, cz);
+    const R px_ = ((R)cx + (R).5) * scale - v.x * alpha;
+    const R py_ = ((R)cy + (R).5) * scale - v.y * alpha;
+    const R pz_ = ((R)cz + (R).5) * scale - v.z * alpha;
+    sample(px_, py_, pz_, F.s);
+    const R si = (R)1 / (R)F.s.scale;
+    const R xs = px_ * si - (R).5, ys = py_ * si - (R).5, zs = pz_ * si - (R).5;
+    const R d[3] = {xs - std::floor(xs), ys - std::floor(ys), zs - std::floor(zs)};
+    F.W = 0;
+    for (int q = 0; q < 8; q++) {
+      const int bit[3] = {(q >> 2) & 1, (q >> 1) & 1, q & 1};
+      R tau[3];
+      for (int a = 0; a < 3; a++) tau[a] = bit[a] ? d[a] : (R)1 - d[a];
+      F.t[q] = tau[0] * tau[1] * tau[2];
+      for (int a = 0; a < 3; a++) {
+        R g = bit[a] ? (R)1 : (R)-1;
+        for (int b = 0; b < 3; b++)
+          if (b != a) g *= tau[b];
+        F.dtd[q][a] = g;
+      }
+      F.f[q] = flu[F.s.i[q]];
+      F.W += F.f[q] * F.t[q];
+    }
+    F.k = -alpha * si;
+  }
+  // d(sample)/d(v) of the normalised interpolant of the BC-applied corner values vc.
+  void footGrad(const FootD& F, const R vc[8], R out[3]) const {
+    R val = 0;
+    for (int q = 0; q < 8; q++) val += F.f[q] * F.t[q] * vc[q];
+    val /= F.W;
+    for (int a = 0; a < 3; a++) {
+      R num = 0, den_ = 0;
+      for (int q = 0; q < 8; q++) {
+        num += F.f[q] * vc[q] * F.dtd[q][a];
+        den_ += F.f[q] * F.dtd[q][a];
+      }
+      out[a] = F.k * (num - val * den_) / F.W;
+    }
+  }
+  // Corner q of a sample is inside the domain: its BC passes the stored value through
+  // (outside it, and on the inlet, which lies outside, the BCs give constants).
+  bool cornerPasses(const Sample& s, int q) const {
+    return !outside(P, s.x0 + ((q >> 2) & 1), s.y0 + ((q >> 1) & 1), s.z0 + (q & 1), s.scale);
+  }
+
+  // advectDensity reversed: lq, lt (after) -> lq0, lt0 (before); foot terms into lu, the adjoint
+  // of the velocity the step advected with.
+  void advectDensityAdj(const Snap& S0, const Snap& S1, std::vector<R>& lq, std::vector<R>& lt,
+                        std::vector<V3<R>>& lu, std::vector<R>& lq0, std::vector<R>& lt0) {
+    const bool src = thermo(), sphere = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
+    restrictAdj(lq);
+    if (src) restrictAdj(lt);
+    lq0.assign(C, (R)0);
+    if (src)
+      lt0.assign(C, (R)0);
+    else
+      lt0 = lt;  // temperature is only advected with a source
+    for (size_t b = 0; b < B; b++) {
+      if (lev[b] == 0xFF) continue;
+      const int level = lev[b], scale = 1 << level;
+      for (int l = 0; l < BV; l++) {
+        const size_t c = b * BV + l;
+        if (!isLeaf(c)) continue;
+        if (src && sphere) {
+          int cx, cy, cz;
+          cellPos(c, cx, cy, cz);
+          if (inSource(P, ((float)cx + .5f) * (float)scale, ((float)cy + .5f) * (float)scale,
+                       ((float)cz + .5f) * (float)scale))
+            continue;  // overwritten by the source's constants
+        }
+        FootD F;
+        footD(c, S1.vel[c], F);
+        if (F.s.zero) continue;
+        R qc[8], tc[8];
+        for (int q = 0; q < 8; q++) {
+          const int x = F.s.x0 + ((q >> 2) & 1), y = F.s.y0 + ((q >> 1) & 1), z = F.s.z0 + (q & 1);
+          const size_t i = F.s.i[q];
+          qc[q] = densityBC(P, S0.den[i], x, y, z, F.s.scale);
+          tc[q] = temperatureBC(P, S0.tem[i], x, y, z, F.s.scale);
+          if (cornerPasses(F.s, q)) {
+            lq0[i] += F.s.w[q] * lq[c];
+            if (src) lt0[i] += F.s.w[q] * lt[c];
+          }
+        }
+        R g[3];
+        footGrad(F, qc, g);
+        lu[c] = add(lu[c], V3<R>{g[0] * lq[c], g[1] * lq[c], g[2] * lq[c]});
+        if (src) {
+          footGrad(F, tc, g);
+          lu[c] = add(lu[c], V3<R>{g[0] * lt[c], g[1] * lt[c], g[2] * lt[c]});
+        }
+      }
+    }
+  }
+
+  // project() / projectLocal() reversed: lu holds the adjoint of the velocity after the
+  // projection on entry and before it on exit.
+  void projectAdj(std::vector<V3<R>>& lu) {
+    std::vector<R> lp(C, (R)0), ltp(C, (R)0), ld(C, (R)0);
+    restrictAdj(lu);
+    // applyPressure: v -= alpha (f_r (p_r - p) + f_l (p - p_l)) per axis; v passes through
+    for (size_t b = 0; b < B; b++) {
+      if (lev[b] == 0xFF) continue;
+      const R alpha = (R).5f * (R)P.rdx / (R)(1 << lev[b]);
+      const size_t* ci = &cidx[b * A3];
+      for (int l = 0; l < BV; l++) {
+        const size_t c = b * BV + l;
+        if (!isLeaf(c)) continue;
+        const int a = apronOf(c);
+        const size_t nb[3][2] = {{ci[a - A2], ci[a + A2]}, {ci[a - A], ci[a + A]}, {ci[a - 1], ci[a + 1]}};
+        const R lam[3] = {lu[c].x, lu[c].y, lu[c].z};
+        for (int ax = 0; ax < 3; ax++) {
+          const size_t il = nb[ax][0], ir = nb[ax][1];
+          const R fl = flu[il], fr = flu[ir];
+          lp[ir] += -alpha * fr * lam[ax];
+          lp[c] += alpha * fr * lam[ax];
+          lp[c] += -alpha * fl * lam[ax];
+          lp[il] += alpha * fl * lam[ax];
+        }
+      }
+    }
+    // jacobi(level, in, out) reversed
+    auto jacobiAdj = [&](int level, std::vector<R>& lin, std::vector<R>& lout) {
+      const int bs_ = 1 << level;
+      const R alpha = (R)(bs_ * bs_) * (R)P.dx * (R)P.dx;
+      for (size_t b = offs[level]; b < offs[level] + maxL[level]; b++) {
+        if (lev[b] != level) continue;
+        const size_t* ci = &cidx[b * A3];
+        for (int l = 0; l < BV; l++) {
+          const size_t c = b * BV + l;
+          const int a = apronOf(c);
+          const R G = lout[c] / (R)6;
+          lin[ci[a - A2]] += G;
+          lin[ci[a + A2]] += G;
+          lin[ci[a - A]] += G;
+          lin[ci[a + A]] += G;
+          lin[ci[a - 1]] += G;
+          lin[ci[a + 1]] += G;
+          ld[c] += -alpha * G;
+          lout[c] = 0;
+        }
+      }
+    };
+    auto pairsAdj = [&](int level, int pairs) {
+      for (int i = 0; i < pairs; i++) {
+        jacobiAdj(level, ltp, lp);  // jacobi(level, t_pre, pre)
+        jacobiAdj(level, lp, ltp);  // jacobi(level, pre, t_pre)
+      }
+    };
+    if (P.project_mode == DCG_PROJECT_LOCAL) {
+      for (int level = 0; level < L; level++) pairsAdj(level, P.project_local_pairs);
+    } else {
+      for (int level = 0; level <= L - 2; level++) {
+        pairsAdj(level, P.project_level_pairs);
+        // prolongate(level): pre = (27, 9, 3, 1)/64 weights of the parent stencil
+        for (size_t b = offs[level]; b < offs[level] + maxL[level]; b++) {
+          if (lev[b] != level || par[b] == NF) continue;
+          for (int l = 0; l < BV; l++) {
+            const size_t c = b * BV + l;
+            size_t q[8];
+            parentStencil(c, q);
+            const R G = lp[c] / (R)64;
+            lp[q[0]] += (R)27 * G;
+            lp[q[1]] += (R)9 * G;
+            lp[q[2]] += (R)9 * G;
+            lp[q[3]] += (R)9 * G;
+            lp[q[4]] += (R)3 * G;
+            lp[q[5]] += (R)3 * G;
+            lp[q[6]] += (R)3 * G;
+            lp[q[7]] += G;
+            lp[c] = 0;
+          }
+        }
+      }
+      pairsAdj(L - 1, P.project_top_pairs);
+    }
+    restrictAdj(ld);
+    // calcDivergence: div = alpha (w_r v_r.x - w_l v_l.x + ...) over leaf cells, neighbours with BCs
+    for (size_t b = 0; b < B; b++) {
+      if (lev[b] == 0xFF) continue;
+      const int scale = 1 << lev[b];
+      const R alpha = (R).5f * (R)P.rdx / (R)scale;
+      const size_t* ci = &cidx[b * A3];
+      for (int l = 0; l < BV; l++) {
+        const size_t c = b * BV + l;
+        if (!isLeaf(c)) continue;
+        int x, y, z;
+        cellPos(c, x, y, z);
+        const int a = apronOf(c);
+        const R G = alpha * ld[c];
+        const size_t il = ci[a - A2], ir = ci[a + A2], id = ci[a - A], iu = ci[a + A], ib = ci[a - 1], iF = ci[a + 1];
+        if (!outside(P, x + 1, y, z, scale)) lu[ir].x += flu[ir] * G;
+        if (!outside(P, x - 1, y, z, scale)) lu[il].x += -flu[il] * G;
+        if (!outside(P, x, y + 1, z, scale)) lu[iu].y += flu[iu] * G;
+        if (!outside(P, x, y - 1, z, scale)) lu[id].y += -flu[id] * G;
+        if (!outside(P, x, y, z + 1, scale)) lu[iF].z += flu[iF] * G;
+        if (!outside(P, x, y, z - 1, scale)) lu[ib].z += -flu[ib] * G;
+      }
+    }
+  }
+
+  // advectVelocity reversed: lu (after) -> lu0 (before); buoyancy into lq0, lt0.
+  void advectVelocityAdj(const Snap& S0, std::vector<V3<R>>& lu, std::vector<V3<R>>& lu0, std::vector<R>& lq0,
+                         std::vector<R>& lt0) {
+    const bool buoy = thermo();
+    const R dt = (R)P.dt;
+    restrictAdj(lu);
+    lu0.assign(C, V3<R>{0, 0, 0});
+    for (size_t b = 0; b < B; b++) {
+      if (lev[b] == 0xFF) continue;
+      for (int l = 0; l < BV; l++) {
+        const size_t c = b * BV + l;
+        if (!isLeaf(c)) continue;
+        if (buoy) {
+          lq0[c] += -dt * (R)P.buoy_alpha * lu[c].y;
+          lt0[c] += dt * (R)P.buoy_beta * lu[c].y;
+        }
+        FootD F;
+        footD(c, S0.vel[c], F);
+        if (F.s.zero) continue;
+        R vc[3][8];
+        for (int q = 0; q < 8; q++) {
+          const size_t i = F.s.i[q];
+          const V3<R> v = velocityBC(P, S0.vel[i], F.s.x0 + ((q >> 2) & 1), F.s.y0 + ((q >> 1) & 1), F.s.z0 + (q & 1),
+                                     F.s.scale);
+          vc[0][q] = v.x;
+          vc[1][q] = v.y;
+          vc[2][q] = v.z;
+          if (cornerPasses(F.s, q)) lu0[i] = add(lu0[i], mul(lu[c], F.s.w[q]));
+        }
+        const R lam[3] = {lu[c].x, lu[c].y, lu[c].z};
+        for (int comp = 0; comp < 3; comp++) {
+          R g[3];
+          footGrad(F, vc[comp], g);
+          lu0[c] = add(lu0[c], V3<R>{g[0] * lam[comp], g[1] * lam[comp], g[2] * lam[comp]});
+        }
+      }
+    }
+  }
+
+  int tapeBackward(const double* gden, const double* gvel, const double* gtem, double* oden, double* ovel,
+                   double* otem) override {
+    if (tape.empty()) return fail(DCG_E_ARG, "no recorded rollout");
+    std::vector<R> lq(C), lt(C, (R)0), lq0, lt0;
+    std::vector<V3<R>> lu(C), lu0;
+    for (size_t i = 0; i < C; i++) {
+      lq[i] = (R)gden[i];
+      lu[i] = V3<R>{(R)gvel[3 * i], (R)gvel[3 * i + 1], (R)gvel[3 * i + 2]};
+      if (gtem) lt[i] = (R)gtem[i];
+    }
+    for (int k = (int)tape.size() - 2; k >= 0; k--) {
+      advectDensityAdj(tape[k], tape[k + 1], lq, lt, lu, lq0, lt0);
+      projectAdj(lu);
+      advectVelocityAdj(tape[k], lu, lu0, lq0, lt0);
+      lq.swap(lq0);
+      lt.swap(lt0);
+      lu.swap(lu0);
+    }
+    for (size_t i = 0; i < C; i++) {
+      const bool on = lev[i / BV] != 0xFF;
+      oden[i] = on ? (double)lq[i] : 0.0;
+      ovel[3 * i] = on ? (double)lu[i].x : 0.0;
+      ovel[3 * i + 1] = on ? (double)lu[i].y : 0.0;
+      ovel[3 * i + 2] = on ? (double)lu[i].z : 0.0;
+      if (otem) otem[i] = on ? (double)lt[i] : 0.0;
+    }
+    return DCG_OK;
+  }
+
   // k_dcgrid_debug_stats + host sum, dcgrid_structure.cu:224-251, fluid_simulation_dcgrid.cu:517-528
   // Samplers of src/dcgrid/dcgrid_rendering.cu:6-64 over the reference-semantics tables.
   struct MarchSampler {

@@ -11,6 +11,7 @@ pytestmark = pytest.mark.gpu
 
 import dcgrid_amd  # noqa: E402
 from dcgrid_amd import scenes  # noqa: E402
+from dcgrid_amd._abi import PROJECT_SOLVE  # noqa: E402
 from parity_util import rel_l2  # noqa: E402
 
 
@@ -220,9 +221,11 @@ def test_autograd_function_and_descent():
 
 
 def test_adjoint_errors():
-    """DCGrid handles have no adjoint; backward before forward is an error."""
-    w = scenes.config(1, scale=4)
-    sim = dcgrid_amd.Simulation(w.kind, w.size, w.params, w.max_num_blocks)
+    """The projection solve has no adjoint (DCG_E_UNSUPPORTED); backward before forward is an
+    error."""
+    w = scenes.config(0, scale=4)
+    w.params.project_mode = PROJECT_SOLVE
+    sim = dcgrid_amd.Simulation(w.kind, w.size, w.params)
     with pytest.raises(dcgrid_amd.DCGError) as e:
         sim.tape_forward(2, 1)
     assert e.value.code == -4

@@ -1,0 +1,166 @@
+"""GPU parity of the differentiable rollout on the DCGrid (frozen block topology) through the
+C ABI: the HIP adjoint against the oracle's adjoint (FP32 instance: the same rollout, only the
+summation order of the scatters differs; FP64 instance: the reference-logic gradient, which the
+oracle tests pin with finite differences and the dot-product identity), checkpoint-interval
+invariance, the live state after the backward pass, the torch.autograd binding on a DCGrid
+handle, and the unsupported modes."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import dcgrid_amd  # noqa: E402
+import oracle  # noqa: E402
+from dcgrid_amd._abi import PROJECT_SOLVE, STORAGE_FP16  # noqa: E402
+from dcgrid_adjoint_cases import NAMES, allocated, prepared, workload  # noqa: E402
+from parity_util import compare_tables, rel_l2  # noqa: E402
+
+# Against the FP32 oracle only the order of the scatters' sums differs (hardware atomics here,
+# sequential loops there). Against the FP64 oracle the gradient also carries the FP32 forward's
+# rounding, which is the FP32 oracle's own distance to the FP64 one (measured: 1e-4 to 3e-4 in
+# velocity on cfg1_64, 8.4e-3 on ref64_bw4_solids, whose fluidity weights and inlet make FP32
+# rollouts part from FP64 ones fastest), so the GPU is held to that distance plus TOL_FP32.
+TOL_FP32 = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def gpu_sim(kind, size, p, b):
+    return dcgrid_amd.Simulation(kind, size, p, b)
+
+
+def gpu_backward(sim, gd, gv, gt):
+    torch = torch_cuda()
+    dev = torch.device("cuda")
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)  # noqa: E731
+    N = gd.size
+    g_d, g_v, g_t = t(gd), t(gv), t(gt)
+    o_d = torch.empty(N, device=dev)
+    o_v = torch.empty(N, 3, device=dev)
+    o_t = torch.empty(N, device=dev)
+    torch.cuda.synchronize()
+    sim.tape_backward(g_d.data_ptr(), g_v.data_ptr(), g_t.data_ptr(), o_d.data_ptr(), o_v.data_ptr(), o_t.data_ptr())
+    return o_d.cpu().numpy(), o_v.cpu().numpy(), o_t.cpu().numpy()
+
+
+def seeds(N, seed=5):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.normal(size=N), rng.normal(size=(N, 3)), rng.normal(size=N)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_vs_oracle_adjoint(name):
+    """3 steps with checkpoints every 2 (a partial last segment); random seeds on all three
+    final fields; gradients compared on the allocated cells, zero on the cells of free blocks."""
+    steps, K = 3, 2
+    say(f"[case] {name} GPU rollout")
+    g = prepared(gpu_sim, name)
+    o32 = prepared(lambda k, s, p, b: oracle.OracleSim(k, s, p, b, precision=32), name)
+    assert compare_tables(g.export_tables(), o32.export_tables()) == []
+    m = allocated(g)
+    state0 = {f: g.export_field(f) for f in ("density", "velocity", "temperature")}
+    g.tape_forward(steps, K)
+    N = m.size
+    gd0, gv0, gt0 = seeds(N)
+    gd, gv, gt = gpu_backward(g, gd0, gv0, gt0)
+    g.close()
+    assert not gd[~m].any() and not gv[~m].any() and not gt[~m].any()
+    say(f"[case] {name} oracle adjoints")
+    o64 = prepared(lambda k, s, p, b: oracle.OracleSim(k, s, p, b, precision=64), name)
+    assert compare_tables(o64.export_tables(), o32.export_tables()) == []
+    for f, v in state0.items():  # the FP64 instance starts from the same (FP32) state
+        o64.import_field(f, v)
+    ref = {}
+    for o, tag in ((o32, "fp32"), (o64, "fp64")):
+        o.tape_forward(steps)
+        ref[tag] = o.tape_backward(gd0, gv0, gt0)
+        o.close()
+    ours = (gd, gv, gt)
+    for k, f in enumerate(("density", "velocity", "temperature")):
+        r32 = rel_l2(ours[k][m], ref["fp32"][k][m])
+        r64 = rel_l2(ours[k][m], ref["fp64"][k][m])
+        spread = rel_l2(ref["fp32"][k][m], ref["fp64"][k][m])
+        say(f"[result] {name} {f}: vs FP32 oracle {r32:.3e}, vs FP64 oracle {r64:.3e} (FP32 oracle vs FP64: {spread:.3e})")
+        assert r32 <= TOL_FP32, (f, r32)
+        assert r64 <= spread + TOL_FP32, (f, r64, spread)
+
+
+def test_checkpoint_interval_and_final_state():
+    """K = 1 and K = 3 over 3 steps give the same gradient (up to the atomics' summation
+    order), and after each backward pass the live state is the rollout's final state, bit for
+    bit (cells of free blocks included: they hold whatever the buffers held, possibly NaN
+    patterns, so the comparison is of the bits)."""
+    name = "cfg1_64"
+    say(f"[case] {name} checkpoint intervals")
+    out, finals = [], []
+    for K in (1, 3):
+        g = prepared(gpu_sim, name)
+        m = allocated(g)
+        g.tape_forward(3, K)
+        fin = {f: g.export_field(f) for f in ("density", "velocity", "temperature")}
+        N = fin["density"].size
+        out.append(gpu_backward(g, *seeds(N, 9)))
+        after = {f: g.export_field(f) for f in fin}
+        for f in fin:
+            assert np.array_equal(fin[f].view(np.uint32), after[f].view(np.uint32)), (K, f)
+        finals.append({f: v[m] for f, v in fin.items()})
+        g.close()
+    for f in finals[0]:
+        assert np.array_equal(finals[0][f], finals[1][f]), f
+    for a, b in zip(out[0], out[1]):
+        assert rel_l2(a, b) <= 1e-6
+
+
+def test_autograd_rollout_on_dcgrid():
+    """dcgrid_amd.autograd.rollout on a DCGrid handle: the gradient of
+    L = 0.5 |rho_N|^2 + <w, u_N> equals the tape's backward pass with seeds (rho_N, w)."""
+    torch = torch_cuda()
+    from dcgrid_amd.autograd import rollout
+    name = "cfg1_64"
+    say(f"[case] {name} autograd")
+    g = prepared(gpu_sim, name)
+    dev = torch.device("cuda")
+    d0 = torch.as_tensor(g.export_field("density"), device=dev).requires_grad_(True)
+    v0 = torch.as_tensor(g.export_field("velocity").reshape(-1, 3), device=dev).requires_grad_(True)
+    w = torch.as_tensor(np.random.Generator(np.random.PCG64(2)).normal(size=tuple(v0.shape)), dtype=torch.float32,
+                        device=dev)
+    den, vel = rollout(g, d0, v0, 3, 2)
+    loss = 0.5 * (den * den).sum() + (w * vel).sum()
+    loss.backward()
+    gd_auto, gv_auto = d0.grad.cpu().numpy(), v0.grad.cpu().numpy()
+    gd, gv, _ = gpu_backward(g, den.detach().cpu().numpy(), w.cpu().numpy(), np.zeros(den.numel()))
+    g.close()
+    assert np.linalg.norm(gv_auto) > 0
+    assert rel_l2(gd_auto, gd) <= 1e-6
+    assert rel_l2(gv_auto, gv) <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["solve", "fp16", "no_tape"])
+def test_unsupported(mode):
+    kind, size, p, b, _ = workload("cfg1_64")
+    if mode == "solve":
+        p.project_mode = PROJECT_SOLVE
+    if mode == "fp16":
+        p.storage = STORAGE_FP16
+    g = dcgrid_amd.Simulation(kind, size, p, b)
+    g.seed_scene()
+    with pytest.raises(dcgrid_amd.DCGError):
+        if mode == "no_tape":
+            torch = torch_cuda()
+            N = g.export_field("density").size
+            z = torch.zeros(3 * N, device="cuda")
+            g.tape_backward(z.data_ptr(), z.data_ptr(), None, z.data_ptr(), z.data_ptr(), None)
+        else:
+            g.tape_forward(1, 1)
+    g.close()
