@@ -21,7 +21,8 @@ through the C ABI. Prints one JSON line:
 
 Other workloads: --config 0..4 (3 = the adjoint rollout), --project reference (the reference's
 Jacobi schedule), --storage fp32/fp16/bf16, --jacobi-micro (uniform 512^3 Jacobi sweeps, a
-working set that does not fit the 256 MB Infinity Cache).
+working set that does not fit the 256 MB Infinity Cache), --adjoint (DCGrid configurations: the
+differentiable rollout on a frozen block topology, backward against forward time).
 """
 import argparse
 import json
@@ -240,6 +241,68 @@ def adjoint_bench(w, args):
     }
 
 
+def adjoint_dcgrid_bench(w, p, args):
+    """Differentiable rollout on the DCGrid (frozen block topology): the workload's scene after
+    its warm-up steps with adaptation, then `--adjoint-steps` steps with checkpoints every 4.
+    Times the rollout recorded with only its end states (the plain forward), the recorded
+    rollout with checkpoints, and the backward pass (segment recomputation + adjoint kernels)
+    for the seed d(0.5 |rho_N|^2); value = backward / plain forward."""
+    import torch
+    import dcgrid_amd
+    steps, k = args.adjoint_steps, 4
+    sim = make_sim(dcgrid_amd.Simulation, w, p)
+    log(f"[bench] adjoint: {args.warmup} adaptive warm-up steps")
+    sim.step(args.warmup)
+    sim.synchronize()
+    N = sim.export_field("density").size
+    dev = torch.device("cuda")
+    d0, v0, t0s = (torch.empty(N, device=dev), torch.empty(N, 3, device=dev), torch.empty(N, device=dev))
+    sim.export_field_device("density", d0.data_ptr(), N)
+    sim.export_field_device("velocity", v0.data_ptr(), 3 * N)
+    sim.export_field_device("temperature", t0s.data_ptr(), N)
+
+    def load():
+        torch.cuda.synchronize()
+        sim.import_field_device("density", d0.data_ptr(), N)
+        sim.import_field_device("velocity", v0.data_ptr(), 3 * N)
+        sim.import_field_device("temperature", t0s.data_ptr(), N)
+
+    times = {}
+    for name, kk in (("warm", steps), ("plain", steps), ("tape", k)):
+        load()
+        t = time.perf_counter()
+        sim.tape_forward(steps, kk)
+        times[name] = time.perf_counter() - t
+    dN = torch.empty_like(d0)
+    sim.export_field_device("density", dN.data_ptr(), N)
+    gv, od, ov, ot = torch.zeros_like(v0), torch.empty_like(d0), torch.empty_like(v0), torch.empty_like(d0)
+    torch.cuda.synchronize()
+    log(f"[bench] adjoint: backward over {steps} steps")
+    t = time.perf_counter()
+    sim.tape_backward(dN.data_ptr(), gv.data_ptr(), None, od.data_ptr(), ov.data_ptr(), ot.data_ptr())
+    t_bwd = time.perf_counter() - t
+    cells = int(sim.info().active_cells)
+    sim.close()
+    return {
+        "metric": "adjoint/forward time ratio",
+        "value": round(t_bwd / times["plain"], 3),
+        "unit": "ratio",
+        "n_gpus": 1,
+        "steps": steps,
+        "higher_is_better": False,
+        "dtype": "f32",
+        "data": "synthetic (the seeded scene after its warm-up steps)",
+        "config": {"workload": w.name, "domain": list(w.size), "budget": w.max_num_blocks,
+                   "checkpoint_every": k, "frozen_topology": True, "active_cells": cells},
+        "forward_ms": round(1e3 * times["plain"], 3),
+        "recorded_forward_ms": round(1e3 * times["tape"], 3),
+        "backward_ms": round(1e3 * t_bwd, 3),
+        "adjoint_steps_ms": round(1e3 * (t_bwd - times["tape"]), 3),
+        "adjoint_step_ratio": round((t_bwd - times["tape"]) / times["plain"], 3),
+        "grad_norm_velocity": float(torch.linalg.vector_norm(ov)),
+    }
+
+
 def solve_compare(sim, params, reps=3):
     """The projection of the simulation's current velocity, solved from a cold start (zero top
     level, prolongation below) to 1e-8 in mixed precision, all-FP32 and all-FP64: wall time
@@ -336,6 +399,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device-to-device copy first (calibrates rocprofv3 byte counters)")
     ap.add_argument("--jacobi-micro", action="store_true", help="uniform 512^3 Jacobi bandwidth only")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="DCGrid configurations: time the differentiable rollout (reference projection)")
+    ap.add_argument("--adjoint-steps", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -351,7 +417,7 @@ def main():
         print(json.dumps(jacobi_micro(args)), flush=True)
         return
     w = scenes.config(args.config)
-    storage = args.storage or ("bf16" if args.config == 2 else "fp32")
+    storage = args.storage or ("bf16" if args.config == 2 and not args.adjoint else "fp32")
     project = args.project or ("solve" if args.config in (2, 4) else "reference")
     p = w.params.copy()
     p.storage = STORAGE_NAMES[storage]
@@ -360,6 +426,12 @@ def main():
         sys.exit(2)
     if args.config == 3:
         print(json.dumps(adjoint_bench(w, args)), flush=True)
+        return
+    if args.adjoint:
+        if w.kind != "dcgrid" or storage != "fp32":
+            print(json.dumps({"error": "--adjoint times the DCGrid configurations with FP32 storage"}))
+            sys.exit(2)
+        print(json.dumps(adjoint_dcgrid_bench(w, p, args)), flush=True)
         return
     solve = project == "solve" and w.kind == "dcgrid"
     if solve:

@@ -1192,6 +1192,10 @@ struct DCGridSim : SimBase {
   float *lq = nullptr, *lt = nullptr, *lq0 = nullptr, *lt0 = nullptr;  // density / temperature adjoints
   float3 *lu = nullptr, *lu0 = nullptr;                                 // velocity adjoints
   float *lp = nullptr, *ltp = nullptr, *ld = nullptr;                   // pressure pair, divergence
+  // transposed advection: per-block apron halos (B (BW+2)^3 entries) and far-contribution
+  // buffers (per cell, kept zero between uses) of density, temperature and velocity
+  float *hq = nullptr, *ht = nullptr, *fq = nullptr, *ft = nullptr;
+  float3 *hu = nullptr, *fu = nullptr;
   float* stateDen(float* base, int k) { return base + (size_t)k * 5 * C; }
   float* stateTem(float* base, int k) { return base + (size_t)k * 5 * C + C; }
   float3* stateVel(float* base, int k) { return (float3*)(base + (size_t)k * 5 * C + 2 * C); }
@@ -1201,15 +1205,20 @@ struct DCGridSim : SimBase {
   }
   // Temperature changes only with a source (it is advected then); otherwise every state holds
   // the same temperature, which is neither saved nor restored.
+  // Only the allocated blocks' cells are copied (the others take no part in a step).
+  int copyState(const float* d0, const float* t0, const float3* v0, float* d1, float* t1, float3* v1) {
+    const unsigned n = blocksFor(C, 256) < 4096 ? blocksFor(C, 256) : 4096;
+    DCG_LAUNCH(DCG_K_OTHER, "k_da_copy_state",
+               (k_da_copy_state<BW><<<n, 256, 0, stream>>>(g, d0, thermo() ? t0 : nullptr, v0, d1, t1, v1)));
+    return DCG_OK;
+  }
   int saveState(float* base, int k) {
-    DCG_TRY(copyDD(stateDen(base, k), g.den, C * 4));
-    if (thermo()) DCG_TRY(copyDD(stateTem(base, k), g.tem, C * 4));
-    return copyDD(stateVel(base, k), g.vel, C * 12);
+    return copyState((const float*)g.den, (const float*)g.tem, (const float3*)g.vel, stateDen(base, k),
+                     stateTem(base, k), stateVel(base, k));
   }
   int loadState(float* base, int k) {
-    DCG_TRY(copyDD(g.den, stateDen(base, k), C * 4));
-    if (thermo()) DCG_TRY(copyDD(g.tem, stateTem(base, k), C * 4));
-    return copyDD(g.vel, stateVel(base, k), C * 12);
+    return copyState(stateDen(base, k), stateTem(base, k), stateVel(base, k), (float*)g.den, (float*)g.tem,
+                     (float3*)g.vel);
   }
   int frozenStep() {
     DCG_TRY(advectVelocity());
@@ -1258,6 +1267,16 @@ struct DCGridSim : SimBase {
         DCG_TRY(alloc(&lp, C));
         DCG_TRY(alloc(&ltp, C));
         DCG_TRY(alloc(&ld, C));
+        const size_t halo = (size_t)B * G::A3;
+        DCG_TRY(alloc(&hq, halo));
+        DCG_TRY(alloc(&ht, halo));
+        DCG_TRY(alloc(&hu, halo));
+        DCG_TRY(alloc(&fq, C));
+        DCG_TRY(alloc(&ft, C));
+        DCG_TRY(alloc(&fu, C));
+        DCG_TRY(fill32(fq, 0, C));
+        DCG_TRY(fill32(ft, 0, C));
+        DCG_TRY(fill32(fu, 0, 3 * C));
       }
       dropGraph();  // the pointer swaps of the steps below move the live fields
       tapeValid = false;
@@ -1280,38 +1299,54 @@ struct DCGridSim : SimBase {
     const float* tem0 = stateTem(segBuf, s);
     const float3* vel0 = stateVel(segBuf, s);
     const float3* vel1 = stateVel(segBuf, s + 1);
-    const unsigned nAll = cellWalkGrid(C);
+    const unsigned nAll = slotGrid<BW>(B);  // block walks over all allocated blocks
     // advectDensity: the restriction of the advected scalars, then the transposed gather
     DCG_TRY(restrictAllT(lq));
     if (src) DCG_TRY(restrictAllT(lt));
-    DCG_TRY(fill32(lq0, 0, C));
-    if (src) DCG_TRY(fill32(lt0, 0, C));
 #define DA_ADV_D(F, S)                                                                         \
   DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_da_advect_density_T",                                    \
-             (k_da_advect_density_T<BW, F, S><<<nAll, 256, 0, stream>>>(g, P, den0, tem0, vel1, lq, lt, lq0, lt0, lu)))
+             (k_da_advect_density_T<BW, F, S><<<nAll, G::WG, 0, stream>>>(g, P, den0, tem0, vel1, lq, lt, lq0, lt0, lu, \
+                                                                          hq, ht, fq, ft)))
     if (fl && src) DA_ADV_D(true, true);
     else if (fl) DA_ADV_D(true, false);
     else if (src) DA_ADV_D(false, true);
     else DA_ADV_D(false, false);
 #undef DA_ADV_D
+    const unsigned nCells = blocksFor(C, 256) < 4096 ? blocksFor(C, 256) : 4096;
+    DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_da_halo_gather", (k_da_halo_gather<BW, float><<<nCells, 256, 0, stream>>>(g, lq0, hq, fq)));
+    if (src)
+      DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_da_halo_gather", (k_da_halo_gather<BW, float><<<nCells, 256, 0, stream>>>(g, lt0, ht, ft)));
     // projection: restriction of the velocity, gradient, Jacobi schedule, restriction of the
     // divergence, divergence, each reversed
     DCG_TRY(restrictAllT(lu));
     DCG_TRY(fill32(lp, 0, C));
     DCG_TRY(fill32(ltp, 0, C));
     DCG_TRY(fill32(ld, 0, C));
-    if (fl)
-      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_da_grad_T", (k_da_grad_T<BW, true><<<nAll, 256, 0, stream>>>(g, P, lu, lp)));
-    else
-      DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_da_grad_T", (k_da_grad_T<BW, false><<<nAll, 256, 0, stream>>>(g, P, lu, lp)));
+    for (int l = 0; l < L; l++) {  // finest first: coarser cells take atomic terms
+      const unsigned n = slotGrid<BW>(maxL[l]);
+      if (fl)
+        DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_da_grad_T", (k_da_grad_T<BW, true><<<n, G::WG, 0, stream>>>(g, P, l, lu, lp)));
+      else
+        DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_da_grad_T", (k_da_grad_T<BW, false><<<n, G::WG, 0, stream>>>(g, P, l, lu, lp)));
+    }
     auto pairsT = [&](int level, int pairs) -> int {
       const int sc = 1 << level;
       const float alpha = (float)(sc * sc) * P.dx * P.dx;
-      const unsigned n = cellWalkGrid((size_t)maxL[level] * G::BV);
+      const unsigned n = slotGrid<BW>(maxL[level]);
+      auto sweepT = [&](const float* lout, float* lin, int accumulate) -> int {
+        if (level >= S)
+          DCG_LAUNCH(DCG_K_JACOBI, "k_da_jacobi_dense_T",
+                     (k_da_jacobi_dense_T<BW><<<n, G::WG, 0, stream>>>(g, level, lout, lin, ld, alpha, accumulate)));
+        else
+          DCG_LAUNCH(DCG_K_JACOBI, "k_da_jacobi_T",
+                     (k_da_jacobi_T<BW><<<n, G::WG, 0, stream>>>(g, level, lout, lin, ld, alpha, accumulate)));
+        return DCG_OK;
+      };
       for (int i = 0; i < pairs; i++) {
-        // jacobi(level, t_pre, pre) then jacobi(level, pre, t_pre), reversed
-        DCG_LAUNCH(DCG_K_JACOBI, "k_da_jacobi_T", (k_da_jacobi_T<BW><<<n, 256, 0, stream>>>(g, level, lp, ltp, ld, alpha)));
-        DCG_LAUNCH(DCG_K_JACOBI, "k_da_jacobi_T", (k_da_jacobi_T<BW><<<n, 256, 0, stream>>>(g, level, ltp, lp, ld, alpha)));
+        // jacobi(level, t_pre, pre) then jacobi(level, pre, t_pre), reversed; the level's first
+        // launch adds to what lin holds, the others store
+        DCG_TRY(sweepT(lp, ltp, i == 0 ? 1 : 0));
+        DCG_TRY(sweepT(ltp, lp, 0));
       }
       return DCG_OK;
     };
@@ -1321,26 +1356,29 @@ struct DCGridSim : SimBase {
       for (int l = 0; l <= L - 2; l++) {
         DCG_TRY(pairsT(l, P.project_level_pairs));
         DCG_LAUNCH(DCG_K_PROLONGATE, "k_da_prolongate_T",
-                   (k_da_prolongate_T<BW><<<cellWalkGrid((size_t)maxL[l] * G::BV), 256, 0, stream>>>(g, l, lp)));
+                   (k_da_prolongate_T<BW><<<slotGrid<BW>(maxL[l]), G::WG, 0, stream>>>(g, l, lp)));
       }
       DCG_TRY(pairsT(L - 1, P.project_top_pairs));
     }
     DCG_TRY(restrictAllT(ld));
-    if (fl)
-      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_da_div_T", (k_da_div_T<BW, true><<<nAll, 256, 0, stream>>>(g, P, ld, lu)));
-    else
-      DCG_LAUNCH(DCG_K_DIVERGENCE, "k_da_div_T", (k_da_div_T<BW, false><<<nAll, 256, 0, stream>>>(g, P, ld, lu)));
+    for (int l = 0; l < L; l++) {  // finest first: coarser cells take atomic terms
+      const unsigned n = slotGrid<BW>(maxL[l]);
+      if (fl)
+        DCG_LAUNCH(DCG_K_DIVERGENCE, "k_da_div_T", (k_da_div_T<BW, true><<<n, G::WG, 0, stream>>>(g, P, l, ld, lu)));
+      else
+        DCG_LAUNCH(DCG_K_DIVERGENCE, "k_da_div_T", (k_da_div_T<BW, false><<<n, G::WG, 0, stream>>>(g, P, l, ld, lu)));
+    }
     // advectVelocity: the restriction of the advected velocity, then the transposed gather
     DCG_TRY(restrictAllT(lu));
-    DCG_TRY(fill32(lu0, 0, 3 * C));
 #define DA_ADV_V(F, Bu)                                                                        \
   DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_da_advect_velocity_T",                                  \
-             (k_da_advect_velocity_T<BW, F, Bu><<<nAll, 256, 0, stream>>>(g, P, vel0, lu, lu0, lq0, lt0)))
+             (k_da_advect_velocity_T<BW, F, Bu><<<nAll, G::WG, 0, stream>>>(g, P, vel0, lu, lu0, lq0, lt0, hu, fu)))
     if (fl && src) DA_ADV_V(true, true);
     else if (fl) DA_ADV_V(true, false);
     else if (src) DA_ADV_V(false, true);
     else DA_ADV_V(false, false);
 #undef DA_ADV_V
+    DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_da_halo_gather", (k_da_halo_gather<BW, float3><<<nCells, 256, 0, stream>>>(g, lu0, hu, fu)));
     std::swap(lq, lq0);
     std::swap(lu, lu0);
     if (src) std::swap(lt, lt0);  // without a source the temperature adjoint passes through
