@@ -17,12 +17,14 @@
 //    always the cell itself) is added in LDS to that cell of the block, and a coarser ghost cell
 //    gets a global FP32 atomic (global_atomic_add_f32). Levels are launched from the finest up,
 //    so a cell is never stored and atomically updated in the same launch.
-//  * The semi-Lagrangian advection and the prolongation read arbitrary corners / parent cells:
-//    their transposes scatter. A workgroup takes one block (four at BW 4, as the forward
-//    stencils), adds its cells' contributions into a tile of the block's apron (or, for the
-//    prolongation, of the parent region the block reads) with LDS atomics, and then adds each
-//    nonzero tile entry to the cell that entry stands for with one global atomic; a foot outside
-//    the thread's own block adds its corners with global atomics directly.
+//  * The semi-Lagrangian advection reads arbitrary corners: its transpose scatters. A workgroup
+//    takes one block (four at BW 4, as the forward stencils), adds its cells' contributions into
+//    an LDS tile of the block's apron with LDS atomics, and writes the tile out with few global
+//    atomics (interior stored, same-level halo entries through a halo buffer and a gather; see
+//    the section below).
+//  * The prolongation reads a fixed parent stencil: its transpose is a fixed 4-tap gather per
+//    axis over the child block's cells, staged in LDS; the parent region entries it produces are
+//    shared between neighbouring child blocks and are added with global atomics.
 //  * Restriction hands a parent cell's lam/8 to each of its 8 children (a gather).
 // Atomic sums are formed in an order that can change from run to run: the gradient is
 // reproducible to FP32 rounding, not bitwise.
@@ -721,53 +723,56 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_jacobi_dense_T(DGBase g, int
   }
 }
 
-// prolongate(level) reversed: pre = (27 q0 + 9 (q1+q2+q3) + 3 (q4+q5+q6) + q7) / 64 of the parent
-// stencil, so each parent stencil cell gets its weight times lp/64, and lp is cleared. The tile
-// is the (BW/2 + 2)^3 region of the parent's apron that k_prolongate stages.
+// prolongate(level) reversed: a child cell's pre = (27 q0 + 9 (q1+q2+q3) + 3 (q4+q5+q6) + q7) / 64
+// of its parent stencil, i.e. per axis 3/4 of its parent cell and 1/4 of the parent cell's
+// neighbour on its side. Transposed per axis, parent region entry r (the (BW/2 + 2)^3 region of
+// the parent's apron that k_prolongate stages, r = 1 + x/2 for child x) receives from children
+// 2r-3, 2r-2, 2r-1, 2r with weights 1/4, 3/4, 3/4, 1/4. A workgroup stages its child block's
+// lp/64, gathers each region entry over that 4x4x4 window (no LDS atomics), adds the nonzero
+// entries to the parent cells they stand for (global atomics: neighbouring child blocks share
+// region entries) and clears lp on its cells.
 template <int BW>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_da_prolongate_T(DGBase g, int level, float* __restrict__ lp) {
   constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW, H = BW / 2 + 2, H3 = H * H * H;
-  __shared__ float tiles[BPW][H3];
+  __shared__ float stage[BPW][BV];  // lp/64 by block-local raster position (x BW + y) BW + z
   const uint32_t total = slotTotal(g, level);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();
     const BlockCtx<BW> c = slotCtx<BW>(g, level, base, total);
     const uint32_t ps = c.valid ? g.par[c.b] : NFU;
-    float* tile = tiles[c.sub];
-    for (int e = c.t; e < H3; e += BV) tile[e] = 0.f;
-    __syncthreads();
-    int ox = 0, oy = 0, oz = 0;  // parent apron coordinates of the region's first entry
+    float* st = stage[c.sub];
+    int lx, ly, lz;
+    locPos<BB>((uint32_t)c.t, lx, ly, lz);
     if (ps != NFU) {
-      const int3 p0 = g.pos[c.b];
-      ox = (p0.x / 2) % BW;
-      oy = (p0.y / 2) % BW;
-      oz = (p0.z / 2) % BW;
       const uint32_t cell = c.b * BV + c.t;
-      const float G = lp[cell] / 64.f;
-      if (G != 0.f) {
-        int lx, ly, lz;
-        locPos<BB>((uint32_t)c.t, lx, ly, lz);
-        const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
-        const int ti = 1 + (x / 2) % BW - ox, tj = 1 + (y / 2) % BW - oy, tk = 1 + (z / 2) % BW - oz;
-        const int si = x % 2 ? 1 : -1, sj = y % 2 ? 1 : -1, sk = z % 2 ? 1 : -1;
-        auto at = [&](int i, int j, int k) -> float* { return &tile[(i * H + j) * H + k]; };
-        ldsAdd(at(ti, tj, tk), 27.f * G);
-        ldsAdd(at(ti, tj, tk + sk), 9.f * G);
-        ldsAdd(at(ti, tj + sj, tk), 9.f * G);
-        ldsAdd(at(ti + si, tj, tk), 9.f * G);
-        ldsAdd(at(ti, tj + sj, tk + sk), 3.f * G);
-        ldsAdd(at(ti + si, tj, tk + sk), 3.f * G);
-        ldsAdd(at(ti + si, tj + sj, tk), 3.f * G);
-        ldsAdd(at(ti + si, tj + sj, tk + sk), G);
-        lp[cell] = 0.f;
-      }
+      const float v = lp[cell];
+      st[(lx * BW + ly) * BW + lz] = v / 64.f;
+      if (v != 0.f) lp[cell] = 0.f;
     }
     __syncthreads();
-    if (ps != NFU)
-      for (int e = c.t; e < H3; e += BV) {
-        const float v = tile[e];
-        if (v != 0.f) atomicAddF(&lp[apronCell<BW>(g, ps / 8, ox + e / (H * H), oy + (e / H) % H, oz + e % H)], v);
+    if (ps == NFU) continue;
+    const int3 p0 = g.pos[c.b];
+    const int ox = (p0.x / 2) % BW, oy = (p0.y / 2) % BW, oz = (p0.z / 2) % BW;
+    for (int e = c.t; e < H3; e += BV) {
+      const int r[3] = {e / (H * H), (e / H) % H, e % H};
+      float v = 0.f;
+      for (int a = 0; a < 4; a++) {
+        const int x = 2 * r[0] - 3 + a;
+        if (x < 0 || x >= BW) continue;
+        const float wx = (a == 1 || a == 2) ? 3.f : 1.f;
+        for (int b2 = 0; b2 < 4; b2++) {
+          const int y = 2 * r[1] - 3 + b2;
+          if (y < 0 || y >= BW) continue;
+          const float wxy = wx * ((b2 == 1 || b2 == 2) ? 3.f : 1.f);
+          for (int d = 0; d < 4; d++) {
+            const int z = 2 * r[2] - 3 + d;
+            if (z < 0 || z >= BW) continue;
+            v += wxy * ((d == 1 || d == 2) ? 3.f : 1.f) * st[(x * BW + y) * BW + z];
+          }
+        }
       }
+      if (v != 0.f) atomicAddF(&lp[apronCell<BW>(g, ps / 8, ox + r[0], oy + r[1], oz + r[2])], v);
+    }
   }
 }
 
