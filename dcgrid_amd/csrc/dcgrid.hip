@@ -1402,8 +1402,13 @@ struct DCGridSim : SimBase {
         DCG_TRY(loadState(ckBuf, sg));
         DCG_TRY(saveState(segBuf, 0));
         for (int k = k0; k < k1; k++) {
-          DCG_TRY(frozenStep());
-          DCG_TRY(saveState(segBuf, k - k0 + 1));
+          if (k + 1 == k1)  // the segment's last state is the next checkpoint: no step to re-run
+            DCG_TRY(copyState(stateDen(ckBuf, sg + 1), stateTem(ckBuf, sg + 1), stateVel(ckBuf, sg + 1),
+                              stateDen(segBuf, k - k0 + 1), stateTem(segBuf, k - k0 + 1), stateVel(segBuf, k - k0 + 1)));
+          else {
+            DCG_TRY(frozenStep());
+            DCG_TRY(saveState(segBuf, k - k0 + 1));
+          }
         }
         for (int k = k1 - 1; k >= k0; k--) DCG_TRY(adjointStep(k - k0));
       }

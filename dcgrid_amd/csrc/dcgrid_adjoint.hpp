@@ -505,20 +505,12 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_velocity_T(DG g, dcg_
   }
 }
 
-// Apron cells of the six face neighbours of local cell t of block b: -x, +x, -y, +y, -z, +z.
+// Local index of a cell of its block.
 template <int BW>
-__device__ __forceinline__ void faceNeighbours(const DGBase& g, uint32_t b, int t, uint32_t nb[6]) {
-  int lx, ly, lz;
-  locPos<Geo<BW>::BB>((uint32_t)t, lx, ly, lz);
-  const int ai = lx + 1, aj = ly + 1, ak = lz + 1;
-  nb[0] = apronCell<BW>(g, b, ai - 1, aj, ak);
-  nb[1] = apronCell<BW>(g, b, ai + 1, aj, ak);
-  nb[2] = apronCell<BW>(g, b, ai, aj - 1, ak);
-  nb[3] = apronCell<BW>(g, b, ai, aj + 1, ak);
-  nb[4] = apronCell<BW>(g, b, ai, aj, ak - 1);
-  nb[5] = apronCell<BW>(g, b, ai, aj, ak + 1);
+__device__ __forceinline__ int localOf(uint32_t cell) {
+  return (int)(cell % (uint32_t)Geo<BW>::BV);
 }
-// Tile entries of the six face neighbours of tile entry a, in faceNeighbours' order.
+// Tile entries of the six face neighbours of tile entry a: -x, +x, -y, +y, -z, +z.
 template <int BW>
 __device__ __forceinline__ void faceEntries(int a, int e[6]) {
   constexpr int A = Geo<BW>::A;
@@ -529,45 +521,68 @@ __device__ __forceinline__ void faceEntries(int a, int e[6]) {
   e[4] = a - 1;
   e[5] = a + 1;
 }
-// Kind of the apron entry of local cell t in face direction k (faceNeighbours' order): inside
-// the block (0), a same-level neighbour block (1), a coarser block (2) or outside the domain (3).
+// Kind of the apron entry of a cell in face direction k: inside the block, a same-level
+// neighbour block, a coarser block or outside the domain.
 enum : int { NB_INSIDE = 0, NB_SAME = 1, NB_COARSER = 2, NB_OUT = 3 };
-template <int BW>
-__device__ __forceinline__ int neighbourKind(const DGBase& g, uint32_t b, int t, int k) {
-  int l[3];
-  locPos<Geo<BW>::BB>((uint32_t)t, l[0], l[1], l[2]);
-  const int ax = k >> 1, dir = k & 1 ? 1 : -1;
-  const int q = l[ax] + dir;
-  if (q >= 0 && q < BW) return NB_INSIDE;
-  const int d3[3] = {ax == 0 ? dir : 0, ax == 1 ? dir : 0, ax == 2 ? dir : 0};
-  const uint32_t sh = ndShift(g.nd[b * 27 + dirIndex(d3[0], d3[1], d3[2])]);
-  return sh == ND_OUT ? NB_OUT : (sh == 0 ? NB_SAME : NB_COARSER);
-}
 
-// Stage block b's values of a per-cell quantity (v(cell)) and those of its same-level face
-// neighbours into an apron tile; face entries toward coarser blocks or outside the domain are
-// zero. The adjoint stencils below gather from these entries: a same-level neighbour's apron
-// entry toward this block is this block's cell, so its contribution is picked up here instead
-// of being scattered.
-template <int BW, class T, class F>
-__device__ __forceinline__ void stageSameLevel(const DGBase& g, T* tile, uint32_t b, int t, F v) {
-  constexpr int BV = Geo<BW>::BV, FACE = Geo<BW>::FACE;
-  tile[apronIndexOfLocal<BW>(t)] = v(b * BV + t);
-  for (int fc = t; fc < FACE; fc += BV) {
-    int ai, aj, ak;
-    faceCell<BW>(fc, ai, aj, ak);
-    const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
-    const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
-    const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
-    const int4 d = g.nd[b * 27 + dirIndex(dx, dy, dz)];
-    tile[tix<BW>(ai, aj, ak)] = ndShift(d) == 0 ? v(apronCell<BW>(g, b, ai, aj, ak)) : zeroOf<T>();
-  }
-}
-// Local index of a cell of block b.
+// The neighbour descriptors of a block's six faces (-x, +x, -y, +y, -z, +z), staged in LDS once
+// per block so that the stencil transposes resolve their face entries with integer arithmetic
+// (the apronCell rule) instead of a descriptor load per entry.
 template <int BW>
-__device__ __forceinline__ int localOf(uint32_t cell) {
-  return (int)(cell % (uint32_t)Geo<BW>::BV);
-}
+struct Faces {
+  const int4* d;
+  uint32_t b;
+  static __device__ __forceinline__ int dirOf(int k) {
+    const int s = k & 1 ? 1 : -1, ax = k >> 1;
+    return dirIndex(ax == 0 ? s : 0, ax == 1 ? s : 0, ax == 2 ? s : 0);
+  }
+  static __device__ __forceinline__ void stage(const DGBase& g, int4* d, uint32_t b, int t) {
+    if (t < 6) d[t] = g.nd[b * 27 + dirOf(t)];
+  }
+  __device__ __forceinline__ int kindOf(int k) const {
+    const uint32_t sh = ndShift(d[k]);
+    return sh == ND_OUT ? NB_OUT : (sh == 0 ? NB_SAME : NB_COARSER);
+  }
+  // Cell of apron entry (ai,aj,ak) on face k (the apronCell rule for that face's descriptor).
+  __device__ __forceinline__ uint32_t cellOf(int k, int ai, int aj, int ak) const {
+    constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+    const int4 q = d[k];
+    const uint32_t sh = ndShift(q);
+    if (sh == ND_OUT) {
+      const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
+      if (q.y) return b * BV + loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
+      return b * BV + loc<BB>(ci - 1, cj - 1, ck - 1);
+    }
+    return ndBlock(q) * BV + loc<BB>((ai - 1 - q.y) >> sh, (aj - 1 - q.z) >> sh, (ak - 1 - q.w) >> sh);
+  }
+  // Kind and cell of the face-k neighbour of local cell (lx,ly,lz).
+  __device__ __forceinline__ int neighbour(int lx, int ly, int lz, int k, uint32_t& cell) const {
+    constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+    const int s = k & 1 ? 1 : -1, ax = k >> 1;
+    const int nx = lx + (ax == 0 ? s : 0), ny = ly + (ax == 1 ? s : 0), nz = lz + (ax == 2 ? s : 0);
+    if ((unsigned)nx < (unsigned)BW && (unsigned)ny < (unsigned)BW && (unsigned)nz < (unsigned)BW) {
+      cell = b * BV + loc<BB>(nx, ny, nz);
+      return NB_INSIDE;
+    }
+    cell = cellOf(k, nx + 1, ny + 1, nz + 1);
+    return kindOf(k);
+  }
+  // Stage the block's values v(cell) and those of its same-level face neighbours into an apron
+  // tile; face entries toward coarser blocks or outside the domain are zero. The stencil
+  // transposes gather from these entries: a same-level neighbour's apron entry toward this
+  // block is this block's cell, so its contribution is picked up here instead of scattered.
+  template <class T, class F>
+  __device__ __forceinline__ void stageSameLevel(T* tile, int t, F v) const {
+    constexpr int BV = Geo<BW>::BV, FACE = Geo<BW>::FACE;
+    tile[apronIndexOfLocal<BW>(t)] = v(b * BV + t);
+    for (int fc = t; fc < FACE; fc += BV) {
+      int ai, aj, ak;
+      faceCell<BW>(fc, ai, aj, ak);
+      const int k = fc / (BW * BW);
+      tile[tix<BW>(ai, aj, ak)] = ndShift(d[k]) == 0 ? v(cellOf(k, ai, aj, ak)) : zeroOf<T>();
+    }
+  }
+};
 
 // applyPressure reversed, one level (launched from the finest level up): leaf cell c did
 // v -= alpha (f_r (p_r - p_c) + f_l (p_c - p_l)) per axis, so lp[r] -= alpha f_r lam_c,
@@ -579,10 +594,11 @@ __device__ __forceinline__ int localOf(uint32_t cell) {
 template <int BW, bool FLUIDITY>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_da_grad_T(DG g, dcg_params P, int level, const float3* __restrict__ lu,
                                                            float* __restrict__ lp) {
-  constexpr int BV = Geo<BW>::BV, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  constexpr int BV = Geo<BW>::BV, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
   __shared__ float3 ltiles[BPW][A3];
   __shared__ float ftiles[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   __shared__ float accs[BPW][BV];
+  __shared__ int4 nds[BPW][6];
   const float alpha = .5f * P.rdx / (float)(1 << level);
   const uint32_t total = slotTotal(g, level);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
@@ -592,8 +608,11 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_grad_T(DG g, dcg_params P, i
     float* ft = FLUIDITY ? ftiles[c.sub] : nullptr;
     float* acc = accs[c.sub];
     acc[c.t] = 0.f;
+    if (c.valid) Faces<BW>::stage(g, nds[c.sub], c.b, c.t);
+    __syncthreads();
+    const Faces<BW> fa{nds[c.sub], c.b};
     if (c.valid) {
-      stageSameLevel<BW>(g, lt, c.b, c.t, [&](uint32_t i) { return lu[i]; });
+      fa.stageSameLevel(lt, c.t, [&](uint32_t i) { return lu[i]; });
       if (FLUIDITY)
         forApron<BW, 1>(g, c.b, c.t, [&](int a, uint32_t src) { ft[a] = g.flu[src]; });
     }
@@ -602,8 +621,6 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_grad_T(DG g, dcg_params P, i
     const uint32_t cell = c.b * BV + c.t;
     float own = 0.f;
     if (c.valid) {
-      int e[6];
-      faceEntries<BW>(a, e);
       float fw[6];
       tileFluidity<BW, FLUIDITY>(ft, a, fw);
       const float3 lam3 = lt[a];
@@ -612,19 +629,18 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_grad_T(DG g, dcg_params P, i
       for (int ax = 0; ax < 3; ax++) own += alpha * fw[2 * ax + 1] * lam[ax] - alpha * fw[2 * ax] * lam[ax];
       // terms of this cell for its coarser and out-of-domain neighbours
       if (lam3.x != 0.f || lam3.y != 0.f || lam3.z != 0.f) {
-        uint32_t nb[6];
-        bool have = false;
+        int lx, ly, lz;
+        locPos<BB>((uint32_t)c.t, lx, ly, lz);
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-          const int kind = neighbourKind<BW>(g, c.b, c.t, k);
+          uint32_t nb;
+          const int kind = fa.neighbour(lx, ly, lz, k, nb);
           if (kind != NB_COARSER && kind != NB_OUT) continue;
-          if (!have) faceNeighbours<BW>(g, c.b, c.t, nb);
-          have = true;
           const float v = (k & 1 ? -alpha : alpha) * fw[k] * lam[k >> 1];
           if (kind == NB_OUT)
-            ldsAdd(&acc[localOf<BW>(nb[k])], v);
+            ldsAdd(&acc[localOf<BW>(nb)], v);
           else
-            atomicAddF(&lp[nb[k]], v);
+            atomicAddF(&lp[nb], v);
         }
       }
     }
@@ -653,9 +669,10 @@ template <int BW>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_da_jacobi_T(DGBase g, int level, const float* __restrict__ lout,
                                                              float* __restrict__ lin, float* __restrict__ ld,
                                                              float alpha, int accumulate) {
-  constexpr int BV = Geo<BW>::BV, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  constexpr int BV = Geo<BW>::BV, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
   __shared__ float tiles[BPW][A3];
   __shared__ float accs[BPW][BV];
+  __shared__ int4 nds[BPW][6];
   const uint32_t total = slotTotal(g, level);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();
@@ -663,7 +680,10 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_jacobi_T(DGBase g, int level
     float* tile = tiles[c.sub];
     float* acc = accs[c.sub];
     acc[c.t] = 0.f;
-    if (c.valid) stageSameLevel<BW>(g, tile, c.b, c.t, [&](uint32_t i) { return lout[i] / 6.f; });
+    if (c.valid) Faces<BW>::stage(g, nds[c.sub], c.b, c.t);
+    __syncthreads();
+    const Faces<BW> fa{nds[c.sub], c.b};
+    if (c.valid) fa.stageSameLevel(tile, c.t, [&](uint32_t i) { return lout[i] / 6.f; });
     __syncthreads();
     const int a = apronIndexOfLocal<BW>(c.t);
     const uint32_t cell = c.b * BV + c.t;
@@ -671,18 +691,16 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_jacobi_T(DGBase g, int level
       const float G = tile[a];
       if (G != 0.f) {
         ld[cell] += -alpha * G;
-        uint32_t nb[6];
-        bool have = false;
+        int lx, ly, lz;
+        locPos<BB>((uint32_t)c.t, lx, ly, lz);
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-          const int kind = neighbourKind<BW>(g, c.b, c.t, k);
-          if (kind != NB_COARSER && kind != NB_OUT) continue;
-          if (!have) faceNeighbours<BW>(g, c.b, c.t, nb);
-          have = true;
+          uint32_t nb;
+          const int kind = fa.neighbour(lx, ly, lz, k, nb);
           if (kind == NB_OUT)
-            ldsAdd(&acc[localOf<BW>(nb[k])], G);
-          else
-            atomicAddF(&lin[nb[k]], G);
+            ldsAdd(&acc[localOf<BW>(nb)], G);
+          else if (kind == NB_COARSER)
+            atomicAddF(&lin[nb], G);
         }
       }
     }
@@ -785,15 +803,19 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_prolongate_T(DGBase g, int l
 template <int BW, bool FLUIDITY>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_da_div_T(DG g, dcg_params P, int level, const float* __restrict__ ld,
                                                           float3* __restrict__ lu) {
-  constexpr int BV = Geo<BW>::BV, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW;
+  constexpr int BV = Geo<BW>::BV, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
   __shared__ float tiles[BPW][A3];
+  __shared__ int4 nds[BPW][6];
   const float alpha = .5f * P.rdx / (float)(1 << level);
   const uint32_t total = slotTotal(g, level);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();
     const BlockCtx<BW> c = slotCtx<BW>(g, level, base, total);
     float* tile = tiles[c.sub];
-    if (c.valid) stageSameLevel<BW>(g, tile, c.b, c.t, [&](uint32_t i) { return alpha * ld[i]; });
+    if (c.valid) Faces<BW>::stage(g, nds[c.sub], c.b, c.t);
+    __syncthreads();
+    const Faces<BW> fa{nds[c.sub], c.b};
+    if (c.valid) fa.stageSameLevel(tile, c.t, [&](uint32_t i) { return alpha * ld[i]; });
     __syncthreads();
     if (!c.valid) continue;
     const int a = apronIndexOfLocal<BW>(c.t);
@@ -801,14 +823,15 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_div_T(DG g, dcg_params P, in
     int e[6];
     faceEntries<BW>(a, e);
     const float G = tile[a];
-    uint32_t nb[6];
-    if (G != 0.f || FLUIDITY) faceNeighbours<BW>(g, c.b, c.t, nb);
     if (G != 0.f) {
+      int lx, ly, lz;
+      locPos<BB>((uint32_t)c.t, lx, ly, lz);
 #pragma unroll
       for (int k = 0; k < 6; k++) {
-        if (neighbourKind<BW>(g, c.b, c.t, k) != NB_COARSER) continue;
-        const float f = FLUIDITY ? (float)g.flu[nb[k]] : 1.f;
-        float* comp = reinterpret_cast<float*>(&lu[nb[k]]) + (k >> 1);
+        uint32_t nb;
+        if (fa.neighbour(lx, ly, lz, k, nb) != NB_COARSER) continue;
+        const float f = FLUIDITY ? (float)g.flu[nb] : 1.f;
+        float* comp = reinterpret_cast<float*>(&lu[nb]) + (k >> 1);
         atomicAddF(comp, (k & 1 ? 1.f : -1.f) * f * G);
       }
     }

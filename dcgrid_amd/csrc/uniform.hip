@@ -947,8 +947,13 @@ struct UniformSim : SimBase {
       DCG_TRY(saveState(segBuf, 0));
       DCG_TRY(window(0));
       for (int k = k0; k < k1; k++) {
-        DCG_TRY(plainStep());
-        DCG_TRY(saveState(segBuf, k - k0 + 1));
+        if (k + 1 == k1) {  // the segment's last state is the next checkpoint: no step to re-run
+          DCG_HIP(hipMemcpyAsync(stateDen(segBuf, k - k0 + 1), stateDen(ckBuf, sg + 1), (size_t)5 * N * 4,
+                                 hipMemcpyDeviceToDevice, stream));
+        } else {
+          DCG_TRY(plainStep());
+          DCG_TRY(saveState(segBuf, k - k0 + 1));
+        }
         DCG_TRY(window(k - k0 + 1));
       }
       for (int k = k1 - 1; k >= k0; k--) DCG_TRY(adjointStep(k - k0));
