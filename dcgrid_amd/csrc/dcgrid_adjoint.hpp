@@ -216,8 +216,36 @@ template <int BW>
 __device__ __forceinline__ bool interiorEntry(int i, int j, int k) {
   return i > 0 && i <= BW && j > 0 && j <= BW && k > 0 && k <= BW;
 }
+// The apronCell rule with block b's 27 neighbour descriptors staged in LDS (nd27).
+template <int BW>
+__device__ __forceinline__ uint32_t apronCellStaged(const int4* nd27, uint32_t b, int ai, int aj, int ak) {
+  constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
+  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
+  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
+  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
+  if ((dx | dy | dz) == 0) return b * BV + loc<BB>(ai - 1, aj - 1, ak - 1);
+  const int4 d = nd27[dirIndex(dx, dy, dz)];
+  const uint32_t sh = ndShift(d);
+  if (sh == ND_OUT) {
+    const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
+    if (d.y) return b * BV + loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
+    return b * BV + loc<BB>(ci - 1, cj - 1, ck - 1);
+  }
+  return ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> sh, (aj - 1 - d.z) >> sh, (ak - 1 - d.w) >> sh);
+}
+// forApron (one cell per thread) with the descriptors from LDS.
+template <int BW, class Op>
+__device__ __forceinline__ void forApronStaged(const int4* nd27, uint32_t b, int t, Op op) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3;
+  op(apronIndexOfLocal<BW>(t), b * BV + t);
+  for (int a = t; a < A3; a += BV) {
+    const int ai = a / (A * A), aj = (a / A) % A, ak = a % A;
+    if (ai > 0 && ai <= BW && aj > 0 && aj <= BW && ak > 0 && ak <= BW) continue;
+    op(a, apronCellStaged<BW>(nd27, b, ai, aj, ak));
+  }
+}
 template <int BW, class T>
-__device__ __forceinline__ void storeTile(const DGBase& g, uint32_t b, const T* tile, T* __restrict__ dst,
+__device__ __forceinline__ void storeTile(const int4* nd27, uint32_t b, const T* tile, T* __restrict__ dst,
                                           T* __restrict__ halo, T* __restrict__ far, int t) {
   constexpr int A = Geo<BW>::A, A3 = Geo<BW>::A3, BV = Geo<BW>::BV, BB = Geo<BW>::BB;
   for (int e = t; e < A3; e += BV) {
@@ -228,12 +256,12 @@ __device__ __forceinline__ void storeTile(const DGBase& g, uint32_t b, const T* 
     }
     const int dx = i == 0 ? -1 : (i == BW + 1 ? 1 : 0), dy = j == 0 ? -1 : (j == BW + 1 ? 1 : 0);
     const int dz = k == 0 ? -1 : (k == BW + 1 ? 1 : 0);
-    const uint32_t sh = ndShift(g.nd[b * 27 + dirIndex(dx, dy, dz)]);
+    const uint32_t sh = ndShift(nd27[dirIndex(dx, dy, dz)]);
     if (sh == ND_OUT) continue;
     if (sh == 0)
       halo[(size_t)b * A3 + e] = tile[e];
     else if (nonzero(tile[e]))
-      atomicAddT(&far[apronCell<BW>(g, b, i, j, k)], tile[e]);
+      atomicAddT(&far[apronCellStaged<BW>(nd27, b, i, j, k)], tile[e]);
   }
 }
 // Whether all corners of foot F lie in the apron box of block b (origin p0, level lb), and the
@@ -308,6 +336,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_density_T(DG g, dcg_p
   __shared__ float tstage[SRC ? BPW : 1][SRC ? A3 : 1];
   __shared__ float fstage[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   __shared__ uint32_t chis[BPW][8];
+  __shared__ int4 nd27s[BPW][27];
   const bool sphere = P.source_mode == DCG_SOURCE_SPHERE_BUOYANT;
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
@@ -319,13 +348,18 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_density_T(DG g, dcg_p
     float* ts_ = SRC ? tstage[c.sub] : nullptr;
     float* fs = FLUIDITY ? fstage[c.sub] : nullptr;
     uint32_t* chi8 = chis[c.sub];
+    int4* nd27 = nd27s[c.sub];
     zeroTile<BW>(qt, c.t);
     if (SRC) zeroTile<BW>(tt, c.t);
     int3 p0 = make_int3(0, 0, 0);
     if (c.valid) {
       p0 = g.pos[c.b];
       if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
-      forApron<BW, 1>(g, c.b, c.t, [&](int a, uint32_t src) {
+      if (c.t < 27) nd27[c.t] = g.nd[c.b * 27 + c.t];
+    }
+    __syncthreads();
+    if (c.valid) {
+      forApronStaged<BW>(nd27, c.b, c.t, [&](int a, uint32_t src) {
         ds[a] = den0[src];
         if (SRC) ts_[a] = tem0[src];
         if (FLUIDITY) fs[a] = g.flu[src];
@@ -394,8 +428,8 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_density_T(DG g, dcg_p
     }
     __syncthreads();
     if (c.valid) {
-      storeTile<BW>(g, c.b, qt, lq0, hq, fq, c.t);
-      if (SRC) storeTile<BW>(g, c.b, tt, lt0, ht, ft, c.t);
+      storeTile<BW>(nd27, c.b, qt, lq0, hq, fq, c.t);
+      if (SRC) storeTile<BW>(nd27, c.b, tt, lt0, ht, ft, c.t);
     }
   }
 }
@@ -418,6 +452,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_velocity_T(DG g, dcg_
   __shared__ float3 vstage[BPW][A3];  // staged apron velocity
   __shared__ float fstage[FLUIDITY ? BPW : 1][FLUIDITY ? A3 : 1];
   __shared__ uint32_t chis[BPW][8];
+  __shared__ int4 nd27s[BPW][27];
   const uint32_t total = slotTotal(g, -1);
   for (uint32_t base = firstSlot<BW>(); base < total; base += slotStride<BW>()) {
     __syncthreads();
@@ -426,12 +461,17 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_velocity_T(DG g, dcg_
     float3* vs = vstage[c.sub];
     float* fs = FLUIDITY ? fstage[c.sub] : nullptr;
     uint32_t* chi8 = chis[c.sub];
+    int4* nd27 = nd27s[c.sub];
     zeroTile<BW>(ut, c.t);
     int3 p0 = make_int3(0, 0, 0);
     if (c.valid) {
       p0 = g.pos[c.b];
       if (c.t < 8) chi8[c.t] = g.chi[8 * c.b + c.t];
-      forApron<BW, 1>(g, c.b, c.t, [&](int a, uint32_t src) {
+      if (c.t < 27) nd27[c.t] = g.nd[c.b * 27 + c.t];
+    }
+    __syncthreads();
+    if (c.valid) {
+      forApronStaged<BW>(nd27, c.b, c.t, [&](int a, uint32_t src) {
         vs[a] = vel0[src];
         if (FLUIDITY) fs[a] = g.flu[src];
       });
@@ -501,7 +541,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_da_advect_velocity_T(DG g, dcg_
       }
     }
     __syncthreads();
-    if (c.valid) storeTile<BW>(g, c.b, ut, lu0, hu, fu, c.t);
+    if (c.valid) storeTile<BW>(nd27, c.b, ut, lu0, hu, fu, c.t);
   }
 }
 
