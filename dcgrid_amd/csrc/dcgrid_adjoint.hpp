@@ -265,27 +265,6 @@ __device__ __forceinline__ void storeTile(const int4* nd27, uint32_t b, Val val,
       atomicAddT(&far[apronCellStaged<BW>(nd27, b, i, j, k)], v);
   }
 }
-// Fixed-point accumulation in LDS. LDS float atomics run at about 1/35 of the rate of integer
-// ones on this hardware (measured: 0.2 against 7.4 T lane-ops/s), so the scatter tiles hold
-// 64-bit integers on a per-block scale 2^E, E = 50 - (binary exponent of the largest |lam| the
-// block scatters): a contribution w lam (w <= 1) is rounded to a multiple of 2^-E, i.e. to
-// 2^-50 of the block's largest adjoint value, up to 2^13 such contributions fit in one entry,
-// and the integer sums do not depend on their order.
-struct FixScale {
-  double up, down;
-};
-__device__ __forceinline__ FixScale fixScale(uint32_t maxBits) {
-  int ex = 0;
-  frexpf(__uint_as_float(maxBits), &ex);
-  return FixScale{ldexp(1.0, 50 - ex), ldexp(1.0, ex - 50)};
-}
-__device__ __forceinline__ void fixAdd(unsigned long long* p, float v, const FixScale& sc) {
-  atomicAdd(p, (unsigned long long)__double2ll_rn((double)v * sc.up));
-}
-__device__ __forceinline__ float fixGet(unsigned long long v, const FixScale& sc) {
-  return (float)((double)(long long)v * sc.down);
-}
-__device__ __forceinline__ void ldsMax(uint32_t* p, float v) { atomicMax(p, __float_as_uint(fabsf(v))); }
 // Whether all corners of foot F lie in the apron box of block b (origin p0, level lb), and the
 // tile entry of corner 000 there.
 template <int BW, bool FLUIDITY>

@@ -458,6 +458,8 @@ struct UniformSim : SimBase {
   float* segBuf = nullptr;  // tapeSeg states of 5N floats
   int* winBuf = nullptr;    // window radius per segment state
   int* adjErr = nullptr;
+  float *hq = nullptr, *ht = nullptr;  // ring slots of the transposed advection's tiles
+  float3* hu = nullptr;
   float *lq = nullptr, *lt = nullptr, *lq0 = nullptr, *lt0 = nullptr;
   float3 *lu = nullptr, *lu0 = nullptr;
   float *lp = nullptr, *ltp = nullptr, *ld = nullptr;
@@ -844,6 +846,10 @@ struct UniformSim : SimBase {
       DCG_TRY(alloc(&lp, MC));
       DCG_TRY(alloc(&ltp, MC));
       DCG_TRY(alloc(&ld, MC));
+      const size_t ring = (size_t)adjTiles(g.gx, g.gy, g.gz) * ADJ_SRC;
+      DCG_TRY(alloc(&hq, ring));
+      DCG_TRY(alloc(&ht, ring));
+      DCG_TRY(alloc(&hu, ring));
     }
     tapeValid = false;
     DCG_TRY(saveState(ckBuf, 0));
@@ -871,12 +877,15 @@ struct UniformSim : SimBase {
     const float3* vel0 = stateVel(segBuf, s);
     const float3* vel1 = stateVel(segBuf, s + 1);
     // advectDensity: lq -> lq0 (and lt -> lt0 with the source), foot terms into lu
-#define UA_ADV_D(F, S) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_advect_density", (k_ua_advect_density<F, S><<<adjTiles(g.gx, g.gy, g.gz), 256, 0, stream>>>(P, g, den0, tem0, vel1, lq, lt, lq0, lt0, lu, winBuf + s + 1, adjErr)))
+#define UA_ADV_D(F, S) DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_advect_density", (k_ua_advect_density<F, S><<<adjTiles(g.gx, g.gy, g.gz), 256, 0, stream>>>(P, g, den0, tem0, vel1, lq, lt, lq0, lt0, lu, hq, ht, winBuf + s + 1, adjErr)))
     if (fl && src) UA_ADV_D(true, true);
     else if (fl) UA_ADV_D(true, false);
     else if (src) UA_ADV_D(false, true);
     else UA_ADV_D(false, false);
 #undef UA_ADV_D
+    DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_halo_gather", (k_ua_halo_gather<float><<<nblk(N), 256, 0, stream>>>(g, lq0, hq)));
+    if (src)
+      DCG_LAUNCH(DCG_K_ADVECT_DENSITY, "k_ua_halo_gather", (k_ua_halo_gather<float><<<nblk(N), 256, 0, stream>>>(g, lt0, ht)));
     // projection: lu (after) -> lu (before)
     if (fl)
       DCG_LAUNCH(DCG_K_APPLY_PRESSURE, "k_ua_grad_T", (k_ua_grad_T<true><<<nblk(N), 256, 0, stream>>>(P, g, lu, lp)));
@@ -919,12 +928,13 @@ struct UniformSim : SimBase {
       DCG_LAUNCH(DCG_K_DIVERGENCE, "k_ua_div_T", (k_ua_div_T<false><<<nblk(N), 256, 0, stream>>>(P, g, ld, lu)));
     // advectVelocity: lu -> lu0, buoyancy into lq0 / lt0
     float* ltOut = src ? lt0 : lt;  // without the source temperature is not advected
-#define UA_ADV_V(F, B) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_ua_advect_velocity", (k_ua_advect_velocity<F, B><<<adjTiles(g.gx, g.gy, g.gz), 256, 0, stream>>>(P, g, vel0, lu, lu0, lq0, ltOut, winBuf + s, adjErr)))
+#define UA_ADV_V(F, B) DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_ua_advect_velocity", (k_ua_advect_velocity<F, B><<<adjTiles(g.gx, g.gy, g.gz), 256, 0, stream>>>(P, g, vel0, lu, lu0, lq0, ltOut, hu, winBuf + s, adjErr)))
     if (fl && src) UA_ADV_V(true, true);
     else if (fl) UA_ADV_V(true, false);
     else if (src) UA_ADV_V(false, true);
     else UA_ADV_V(false, false);
 #undef UA_ADV_V
+    DCG_LAUNCH(DCG_K_ADVECT_VELOCITY, "k_ua_halo_gather", (k_ua_halo_gather<float3><<<nblk(N), 256, 0, stream>>>(g, lu0, hu)));
     std::swap(lq, lq0);
     std::swap(lu, lu0);
     if (src) std::swap(lt, lt0);

@@ -5,9 +5,11 @@
 // The step is u* = A(u; u) (+ buoyancy), u' = u* - G P D u* (the fixed multilevel Jacobi
 // schedule P is linear), q' = A(q; u'). Its adjoint:
 //  * transposed advection: lam_q[j] = sum over sources i of w_c(i) lam_q'[i] where corner c of
-//    the trilinear sample of i is cell j. Written as a gather per destination j over the window
-//    of sources whose foot can reach j (radius R from the largest cell offset of any foot,
-//    measured per state): one pass, no atomics, a fixed summation order;
+//    the trilinear sample of i is cell j. With every foot within one cell of its own (radius 1,
+//    measured per state) each cell scatters into an LDS tile in 64-bit fixed point (integer
+//    atomics, order independent), the tiles' rings are exchanged through a halo buffer and added
+//    in a fixed order; wider windows gather per destination over the window of sources whose
+//    foot can reach it;
 //  * foot term: lam_u[i] += d(sample)/d(u(i)) . lam[i] (local to the source);
 //  * projection: G^T, the Jacobi half-sweeps reversed (the clamped 7-point stencil is symmetric,
 //    so each reversed sweep is the same stencil with the div term accumulated), prolongation and
@@ -172,30 +174,24 @@ __device__ __forceinline__ int adjCorner(int packed, int xi, int yi, int zi, int
 __device__ __forceinline__ int adjSlot(int X0, int Y0, int Z0, int xi, int yi, int zi) {
   return ((zi - Z0 + 1) * ADJ_SY + (yi - Y0 + 1)) * ADJ_SX + (xi - X0 + 1);
 }
-// Samples source k of the grown tile under velocity `vel`: normalised weights (0 for a zero
-// sample, a masked source or a cell outside the domain) and the packed base offset.
-template <bool FLUIDITY, bool MASK>
-__device__ __forceinline__ bool adjSampleSource(const dcg_params& P, const UGrid& g, const float3* __restrict__ vel,
-                                                int X0, int Y0, int Z0, int k, float w[8], int& packed, size_t& i) {
-  const int xi = X0 - 1 + k % ADJ_SX, yi = Y0 - 1 + (k / ADJ_SX) % ADJ_SY, zi = Z0 - 1 + k / (ADJ_SX * ADJ_SY);
-#pragma unroll
-  for (int c = 0; c < 8; c++) w[c] = 0.f;
-  packed = adjPack(0, 0, 0);
-  i = 0;
-  if (!uinDomain(g, xi, yi, zi)) return false;
-  i = uidx(g, xi, yi, zi);
-  if (MASK && inSource(P, (float)xi + .5f, (float)yi + .5f, (float)zi + .5f)) return true;
-  float px, py, pz;
-  ufoot(P, xi, yi, zi, vel[i], px, py, pz);
-  packed = adjPack(ubase(px) - xi, ubase(py) - yi, ubase(pz) - zi);
-  USample s;
-  usample<FLUIDITY>(g, px, py, pz, s);
-  if (!s.zero) {
-#pragma unroll
-    for (int c = 0; c < 8; c++) w[c] = s.w[c];
-  }
-  return true;
+// Grown-tile coordinates of slot k (the tile grown by one cell).
+__device__ __forceinline__ void adjSlotCell(int X0, int Y0, int Z0, int k, int& xi, int& yi, int& zi) {
+  xi = X0 - 1 + k % ADJ_SX;
+  yi = Y0 - 1 + (k / ADJ_SX) % ADJ_SY;
+  zi = Z0 - 1 + k / (ADJ_SX * ADJ_SY);
 }
+__device__ __forceinline__ bool adjOwn(int X0, int Y0, int Z0, int xi, int yi, int zi) {
+  return xi >= X0 && xi < X0 + ADJ_TX && yi >= Y0 && yi < Y0 + ADJ_TY && zi >= Z0 && zi < Z0 + ADJ_TZ;
+}
+
+// Transposed trilinear gathers as scatters (window radius 1, CFL < 1): every cell of a
+// workgroup's tile samples its own foot once and adds w_c lam to the corner entries of an LDS
+// tile of the tile grown by one cell, in 64-bit fixed point (fixAdd: integer LDS atomics, order
+// independent); the same sample gives the cell's foot term. The tile is then written out: its
+// own cells stored, the ring into the tile's slots of a halo buffer that k_ua_halo_gather adds
+// to the neighbouring tiles' cells in a fixed order. A launch whose states move a foot further
+// (window radius > 1) gathers per destination over the window instead and leaves its halo slots
+// zero.
 
 // Adjoint of advectDensity under velocity vel1 = u_{k+1}: lq0 = A^T lq (and lt0 = A^T lt with
 // the source), sources masked inside the emitting sphere; lu += foot terms (den0/tem0 = state k).
@@ -205,62 +201,95 @@ __global__ void __launch_bounds__(256) k_ua_advect_density(dcg_params P, UGrid g
                                                            const float3* __restrict__ vel1,
                                                            const float* __restrict__ lq, const float* __restrict__ lt,
                                                            float* __restrict__ lq0, float* __restrict__ lt0,
-                                                           float3* __restrict__ lu, const int* __restrict__ win,
+                                                           float3* __restrict__ lu, float* __restrict__ hq,
+                                                           float* __restrict__ ht, const int* __restrict__ win,
                                                            int* __restrict__ errFlag) {
-  __shared__ float sw[8][ADJ_SRC];
-  __shared__ int sb[ADJ_SRC];
-  __shared__ float sq[ADJ_SRC];
-  __shared__ float st[SRC ? ADJ_SRC : 1];
+  __shared__ unsigned long long tq[ADJ_SRC];
+  __shared__ unsigned long long tt[SRC ? ADJ_SRC : 1];
+  __shared__ uint32_t mq, mt;
   const int R = windowRadius(win, errFlag);
   int X0, Y0, Z0, x, y, z;
   adjTileCell(g, X0, Y0, Z0, x, y, z);
   const bool inside = x < g.gx && y < g.gy && z < g.gz;
-  float aq = 0.f, at = 0.f;
+  const size_t j = inside ? uidx(g, x, y, z) : 0;
+  const bool masked = SRC && inSource(P, (float)x + .5f, (float)y + .5f, (float)z + .5f);
+  const float gq = inside ? lq[j] : 0.f, gt = (SRC && inside) ? lt[j] : 0.f;
+  float* hqt = hq + (size_t)blockIdx.x * ADJ_SRC;
+  float* htt = SRC ? ht + (size_t)blockIdx.x * ADJ_SRC : nullptr;
   if (R <= 1) {  // uniform over the launch
     for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
-      float w[8];
-      int packed;
-      size_t i;
-      const bool dom = adjSampleSource<FLUIDITY, SRC>(P, g, vel1, X0, Y0, Z0, k, w, packed, i);
-#pragma unroll
-      for (int c = 0; c < 8; c++) sw[c][k] = w[c];
-      sb[k] = packed;
-      sq[k] = dom ? lq[i] : 0.f;
-      if (SRC) st[k] = dom ? lt[i] : 0.f;
+      tq[k] = 0ull;
+      if (SRC) tt[k] = 0ull;
+    }
+    if (threadIdx.x == 0) mq = mt = 0u;
+    __syncthreads();
+    if (inside && !masked) {
+      ldsMax(&mq, gq);
+      if (SRC) ldsMax(&mt, gt);
     }
     __syncthreads();
-    if (!inside) return;
-    for (int zi = max(z - 1, 0); zi <= min(z + 1, g.gz - 1); zi++)
-      for (int yi = max(y - 1, 0); yi <= min(y + 1, g.gy - 1); yi++)
-        for (int xi = max(x - 1, 0); xi <= min(x + 1, g.gx - 1); xi++) {
-          const int k = adjSlot(X0, Y0, Z0, xi, yi, zi);
-          const int c = adjCorner(sb[k], xi, yi, zi, x, y, z);
-          if (c < 0) continue;
-          const float w = sw[c][k];
-          if (w != 0.f) {
-            aq += w * sq[k];
-            if (SRC) at += w * st[k];
+    const FixScale sq = fixScale(mq), sc = fixScale(SRC ? mt : 0u);
+    if (inside && !masked && (gq != 0.f || gt != 0.f)) {
+      float px, py, pz;
+      ufoot(P, x, y, z, vel1[j], px, py, pz);
+      UFootD F;
+      ufootD<FLUIDITY>(g, px, py, pz, F);
+      if (!F.s.zero) {
+        float qc[8], tc[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const int cx = F.s.x0 + ((c >> 2) & 1), cy = F.s.y0 + ((c >> 1) & 1), cz = F.s.z0 + (c & 1);
+          qc[c] = densityBC(P, den0[F.s.i[c]], cx, cy, cz, 1);
+          if (SRC) tc[c] = temperatureBC(P, tem0[F.s.i[c]], cx, cy, cz, 1);
+          if (uinDomain(g, cx, cy, cz)) {  // the BC passes the stored value through
+            const int k = adjSlot(X0, Y0, Z0, cx, cy, cz);
+            fixAdd(&tq[k], F.s.w[c] * gq, sq);
+            if (SRC) fixAdd(&tt[k], F.s.w[c] * gt, sc);
           }
         }
-  } else {
-    if (!inside) return;
-    for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
-      for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
-        for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
-          const size_t i = uidx(g, xi, yi, zi);
-          if (SRC && inSource(P, (float)xi + .5f, (float)yi + .5f, (float)zi + .5f)) continue;
-          const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel1[i], x, y, z);
-          if (w != 0.f) {
-            aq += w * lq[i];
-            if (SRC) at += w * lt[i];
-          }
-        }
+        float3 d = ufootGrad(P, F, qc, gq);
+        if (SRC) d = f3add(d, ufootGrad(P, F, tc, gt));
+        lu[j] = f3add(lu[j], d);
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
+      int xi, yi, zi;
+      adjSlotCell(X0, Y0, Z0, k, xi, yi, zi);
+      if (!uinDomain(g, xi, yi, zi)) continue;
+      const float v = fixGet(tq[k], sq), vt = SRC ? fixGet(tt[k], sc) : 0.f;
+      if (adjOwn(X0, Y0, Z0, xi, yi, zi)) {
+        const size_t i = uidx(g, xi, yi, zi);
+        lq0[i] = v;
+        if (SRC) lt0[i] = vt;
+      } else {
+        hqt[k] = v;
+        if (SRC) htt[k] = vt;
+      }
+    }
+    return;
   }
-  const size_t j = uidx(g, x, y, z);
+  for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
+    hqt[k] = 0.f;
+    if (SRC) htt[k] = 0.f;
+  }
+  if (!inside) return;
+  float aq = 0.f, at = 0.f;
+  for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
+    for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
+      for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
+        const size_t i = uidx(g, xi, yi, zi);
+        if (SRC && inSource(P, (float)xi + .5f, (float)yi + .5f, (float)zi + .5f)) continue;
+        const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel1[i], x, y, z);
+        if (w != 0.f) {
+          aq += w * lq[i];
+          if (SRC) at += w * lt[i];
+        }
+      }
   lq0[j] = aq;
   if (SRC) lt0[j] = at;
   // foot term of cell j
-  if (SRC && inSource(P, (float)x + .5f, (float)y + .5f, (float)z + .5f)) return;
+  if (masked) return;
   float px, py, pz;
   ufoot(P, x, y, z, vel1[j], px, py, pz);
   UFootD F;
@@ -284,58 +313,20 @@ template <bool FLUIDITY, bool BUOY>
 __global__ void __launch_bounds__(256) k_ua_advect_velocity(dcg_params P, UGrid g, const float3* __restrict__ vel0,
                                                             const float3* __restrict__ lus,
                                                             float3* __restrict__ lu0, float* __restrict__ lq0,
-                                                            float* __restrict__ lt0, const int* __restrict__ win,
-                                                            int* __restrict__ errFlag) {
-  __shared__ float sw[8][ADJ_SRC];
-  __shared__ int sb[ADJ_SRC];
-  __shared__ float sl[3][ADJ_SRC];
+                                                            float* __restrict__ lt0, float3* __restrict__ hu,
+                                                            const int* __restrict__ win, int* __restrict__ errFlag) {
+  __shared__ unsigned long long tu[3][ADJ_SRC];
+  __shared__ float3 selfs[256];
+  __shared__ uint32_t mu;
   const int R = windowRadius(win, errFlag);
   int X0, Y0, Z0, x, y, z;
   adjTileCell(g, X0, Y0, Z0, x, y, z);
   const bool inside = x < g.gx && y < g.gy && z < g.gz;
-  float3 acc = make_float3(0.f, 0.f, 0.f);
-  if (R <= 1) {  // uniform over the launch
-    for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
-      float w[8];
-      int packed;
-      size_t i;
-      const bool dom = adjSampleSource<FLUIDITY, false>(P, g, vel0, X0, Y0, Z0, k, w, packed, i);
-#pragma unroll
-      for (int c = 0; c < 8; c++) sw[c][k] = w[c];
-      sb[k] = packed;
-      const float3 l = dom ? lus[i] : make_float3(0.f, 0.f, 0.f);
-      sl[0][k] = l.x;
-      sl[1][k] = l.y;
-      sl[2][k] = l.z;
-    }
-    __syncthreads();
-    if (!inside) return;
-    for (int zi = max(z - 1, 0); zi <= min(z + 1, g.gz - 1); zi++)
-      for (int yi = max(y - 1, 0); yi <= min(y + 1, g.gy - 1); yi++)
-        for (int xi = max(x - 1, 0); xi <= min(x + 1, g.gx - 1); xi++) {
-          const int k = adjSlot(X0, Y0, Z0, xi, yi, zi);
-          const int c = adjCorner(sb[k], xi, yi, zi, x, y, z);
-          if (c < 0) continue;
-          const float w = sw[c][k];
-          if (w != 0.f) acc = f3add(acc, f3mul(make_float3(sl[0][k], sl[1][k], sl[2][k]), w));
-        }
-  } else {
-    if (!inside) return;
-    for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
-      for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
-        for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
-          const size_t i = uidx(g, xi, yi, zi);
-          const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel0[i], x, y, z);
-          if (w != 0.f) acc = f3add(acc, f3mul(lus[i], w));
-        }
-  }
-  const size_t j = uidx(g, x, y, z);
-  const float3 lam = lus[j];
-  float px, py, pz;
-  ufoot(P, x, y, z, vel0[j], px, py, pz);
-  UFootD F;
-  ufootD<FLUIDITY>(g, px, py, pz, F);
-  if (!F.s.zero) {
+  const size_t j = inside ? uidx(g, x, y, z) : 0;
+  const float3 lam = inside ? lus[j] : make_float3(0.f, 0.f, 0.f);
+  float3* hut = hu + (size_t)blockIdx.x * ADJ_SRC;
+  // foot term of cell j (every path)
+  auto footTerm = [&](const UFootD& F) {
     float vx[8], vy[8], vz[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -344,15 +335,100 @@ __global__ void __launch_bounds__(256) k_ua_advect_velocity(dcg_params P, UGrid 
       vy[c] = v.y;
       vz[c] = v.z;
     }
-    acc = f3add(acc, ufootGrad(P, F, vx, lam.x));
+    float3 acc = ufootGrad(P, F, vx, lam.x);
     acc = f3add(acc, ufootGrad(P, F, vy, lam.y));
-    acc = f3add(acc, ufootGrad(P, F, vz, lam.z));
-  }
-  lu0[j] = acc;
-  if (BUOY) {
+    return f3add(acc, ufootGrad(P, F, vz, lam.z));
+  };
+  if (inside && BUOY) {
     lq0[j] = lq0[j] + (-P.dt * P.buoy_alpha) * lam.y;
     lt0[j] = lt0[j] + (P.dt * P.buoy_beta) * lam.y;
   }
+  if (R <= 1) {  // uniform over the launch
+    for (int k = threadIdx.x; k < ADJ_SRC; k += 256) tu[0][k] = tu[1][k] = tu[2][k] = 0ull;
+    if (threadIdx.x == 0) mu = 0u;
+    selfs[threadIdx.x] = make_float3(0.f, 0.f, 0.f);
+    __syncthreads();
+    if (inside) ldsMax(&mu, fmaxf(fabsf(lam.x), fmaxf(fabsf(lam.y), fabsf(lam.z))));
+    __syncthreads();
+    const FixScale su = fixScale(mu);
+    if (inside && (lam.x != 0.f || lam.y != 0.f || lam.z != 0.f)) {
+      float px, py, pz;
+      ufoot(P, x, y, z, vel0[j], px, py, pz);
+      UFootD F;
+      ufootD<FLUIDITY>(g, px, py, pz, F);
+      if (!F.s.zero) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const int cx = F.s.x0 + ((c >> 2) & 1), cy = F.s.y0 + ((c >> 1) & 1), cz = F.s.z0 + (c & 1);
+          if (!uinDomain(g, cx, cy, cz)) continue;
+          const int k = adjSlot(X0, Y0, Z0, cx, cy, cz);
+          fixAdd(&tu[0][k], lam.x * F.s.w[c], su);
+          fixAdd(&tu[1][k], lam.y * F.s.w[c], su);
+          fixAdd(&tu[2][k], lam.z * F.s.w[c], su);
+        }
+        selfs[threadIdx.x] = footTerm(F);
+      }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ADJ_SRC; k += 256) {
+      int xi, yi, zi;
+      adjSlotCell(X0, Y0, Z0, k, xi, yi, zi);
+      if (!uinDomain(g, xi, yi, zi)) continue;
+      float3 v = make_float3(fixGet(tu[0][k], su), fixGet(tu[1][k], su), fixGet(tu[2][k], su));
+      if (adjOwn(X0, Y0, Z0, xi, yi, zi)) {
+        const int t = ((zi - Z0) * ADJ_TY + (yi - Y0)) * ADJ_TX + (xi - X0);  // adjTileCell's thread
+        lu0[uidx(g, xi, yi, zi)] = f3add(v, selfs[t]);
+      } else {
+        hut[k] = v;
+      }
+    }
+    return;
+  }
+  for (int k = threadIdx.x; k < ADJ_SRC; k += 256) hut[k] = make_float3(0.f, 0.f, 0.f);
+  if (!inside) return;
+  float3 acc = make_float3(0.f, 0.f, 0.f);
+  for (int zi = max(z - R, 0); zi <= min(z + R, g.gz - 1); zi++)
+    for (int yi = max(y - R, 0); yi <= min(y + R, g.gy - 1); yi++)
+      for (int xi = max(x - R, 0); xi <= min(x + R, g.gx - 1); xi++) {
+        const size_t i = uidx(g, xi, yi, zi);
+        const float w = utransWeight<FLUIDITY>(P, g, xi, yi, zi, vel0[i], x, y, z);
+        if (w != 0.f) acc = f3add(acc, f3mul(lus[i], w));
+      }
+  float px, py, pz;
+  ufoot(P, x, y, z, vel0[j], px, py, pz);
+  UFootD F;
+  ufootD<FLUIDITY>(g, px, py, pz, F);
+  if (!F.s.zero) acc = f3add(acc, footTerm(F));
+  lu0[j] = acc;
+}
+
+__device__ __forceinline__ float uaddv(float a, float b) { return a + b; }
+__device__ __forceinline__ float3 uaddv(float3 a, float3 b) { return f3add(a, b); }
+// Adds to each cell the ring entries of the neighbouring tiles that stand for it (the tiles on
+// the sides where the cell lies on its tile's border), in a fixed order.
+template <class T>
+__global__ void __launch_bounds__(256) k_ua_halo_gather(UGrid g, T* __restrict__ dst, const T* __restrict__ halo) {
+  const size_t n = (size_t)g.gx * g.gy * g.gz;
+  const size_t j = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  int x, y, z;
+  ucoord(j, g.gx, g.gy, x, y, z);
+  const int ntx = (g.gx + ADJ_TX - 1) / ADJ_TX, nty = (g.gy + ADJ_TY - 1) / ADJ_TY, ntz = (g.gz + ADJ_TZ - 1) / ADJ_TZ;
+  const int tx = x / ADJ_TX, ty = y / ADJ_TY, tz = z / ADJ_TZ;
+  const int bx = x % ADJ_TX == 0 ? -1 : (x % ADJ_TX == ADJ_TX - 1 ? 1 : 0);
+  const int by = y % ADJ_TY == 0 ? -1 : (y % ADJ_TY == ADJ_TY - 1 ? 1 : 0);
+  const int bz = z % ADJ_TZ == 0 ? -1 : (z % ADJ_TZ == ADJ_TZ - 1 ? 1 : 0);
+  if (!(bx | by | bz)) return;
+  T v = dst[j];
+  for (int m = 1; m < 8; m++) {
+    const int dx = m & 4 ? bx : 0, dy = m & 2 ? by : 0, dz = m & 1 ? bz : 0;
+    if ((dx == 0 && (m & 4)) || (dy == 0 && (m & 2)) || (dz == 0 && (m & 1))) continue;
+    const int ux = tx + dx, uy = ty + dy, uz = tz + dz;
+    if (ux < 0 || uy < 0 || uz < 0 || ux >= ntx || uy >= nty || uz >= ntz) continue;
+    const int k = adjSlot(ux * ADJ_TX, uy * ADJ_TY, uz * ADJ_TZ, x, y, z);
+    v = uaddv(v, halo[(size_t)((uz * nty + uy) * ntx + ux) * ADJ_SRC + k]);
+  }
+  dst[j] = v;
 }
 
 // G^T: lambda of the level-0 pressure from lambda of the velocity after applyPressure.
