@@ -218,7 +218,7 @@ void dcg_default_params(dcg_params* p) {
   p->solve_precision = DCG_SOLVE_MIXED;
   p->solve_tol = 1e-8f;
   p->solve_max_cycles = 50;
-  p->solve_smooth = 3;
+  p->solve_smooth = 1;
 }
 
 int dcg_create_uniform(const int32_t size[3], const dcg_params* p, dcg_sim** out) {

@@ -618,6 +618,7 @@ struct DCGridSim : SimBase {
   uint8_t* reachBuf = nullptr;   // finest level with cells inside each cell
   uint8_t* breachBuf = nullptr;  // finest level with cells inside each block
   uint32_t* svList = nullptr;    // active blocks per level solve, grid and colour (k_sv_lists)
+  int4* svFaces = nullptr;       // the six face descriptors beside each list entry
   uint32_t* svCount = nullptr;   // their counts, device
   uint32_t* svCountMirror = nullptr;     // and in pinned host memory
   uint32_t* svCountMirrorDev = nullptr;  // (device address of the mirror)
@@ -658,6 +659,7 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&reachBuf, C));
     DCG_TRY(alloc(&breachBuf, B));
     DCG_TRY(alloc(&svList, (size_t)2 * L * B));
+    DCG_TRY(alloc(&svFaces, (size_t)12 * L * B));
     DCG_TRY(alloc(&svCount, (size_t)2 * L * L));
     DCG_TRY(allocHost(&svCountMirror, (size_t)2 * L * L));
     DCG_HIP(hipHostGetDevicePointer((void**)&svCountMirrorDev, svCountMirror, 0));
@@ -705,20 +707,25 @@ struct DCGridSim : SimBase {
   }
   // List of the active blocks of grid k in the V-cycle of level lmin, colour c, and its length.
   const uint32_t* listOf(int lmin, int c) const { return svList + (size_t)(lmin * 2 + c) * B; }
+  const int4* facesOf(int lmin, int c) const { return svFaces + (size_t)(lmin * 2 + c) * B * 6; }
   uint32_t countOf(int lmin, int k, int c) const { return lc[(size_t)(lmin * L + k) * 2 + c]; }
   unsigned listGrid(uint32_t n) const {
     const unsigned w = blocksFor(n, G::BPW);
     return w < 1024u ? w : 1024u;
   }
   // Smoothing of a grid before or after its coarse correction: svPasses passes (solve_smooth,
-  // default 2, at most 5), each the blocks of one colour then the other with kSvInner
-  // red-black sweeps in LDS, i.e. 2 * kSvInner * passes <= 20 sweeps per grid and cycle. One
-  // sweep per pass smooths too little on the sparse levels of the BW-4 reference scene (its
-  // finest level's V-cycle diverged at 2 passes x 1 and 1 x 2 sweeps, took 19 cycles at 3 x 1);
-  // 2 x 2 converges every benchmark and test scene in 6-9 cycles per level.
-  static constexpr int kSvInner = 2;
-  int smoothSweeps() const { return P.solve_smooth > 0 ? std::min(P.solve_smooth, 5) : 2; }
-  int svInner = kSvInner, svPasses = 2;
+  // default 1, at most 2), each the blocks of one colour then the other with kSvInner
+  // red-black sweeps in LDS, i.e. 2 * kSvInner * passes <= 16 sweeps per grid and cycle. A
+  // pass costs a launch per colour and grid while an inner sweep costs a few LDS round trips,
+  // so the default takes one pass of four sweeps. Measured (MI355X, every projection solved to
+  // 1e-8), passes x inner sweeps: 2 x 1 and 1 x 2 diverge on the finest level of the BW-4
+  // reference scene; 1 x 4 converges on every benchmark and test scene in 7-14 cycles per
+  // level, with fewer launches per step than 2 x 2 (configuration 2 at 512^3: 7.9 against
+  // 9.3 ms per step, 34.8 against 31.6 cycles); 1 x 5 .. 1 x 8 take a cycle or two fewer
+  // but more time (the pyramid's sweeps grow with the inner count).
+  static constexpr int kSvInner = 4;
+  int smoothSweeps() const { return P.solve_smooth > 0 ? std::min(P.solve_smooth, 2) : 1; }
+  int svInner = kSvInner, svPasses = 1;
   template <class T>
   int smoothBlocks(int k, int lmin, int nu, bool down, const SolveCtl* ctl) {
     (void)nu;
@@ -737,16 +744,17 @@ struct DCGridSim : SimBase {
         DCG_TRY(takeEvents(DCG_K_SOLVE_SMOOTH, &ev));
         const DGBase gb = g;
         const uint32_t* lst = listOf(lmin, colour);
+        const int4* fcs = facesOf(lmin, colour);
         const int oz = zero ? 1 : 0, hz = haloZero ? 1 : 0;
         T* ep = (T*)eBuf;
         const T* rp = (const T*)rhoBuf;
         const uint8_t* reach = reachBuf;
         if (ev.k >= 0)
           hipExtLaunchKernelGGL(k_sv_smooth<BW, T>, dim3(listGrid(n)), dim3(G::WG), 0u, stream, ev.a, ev.b, 0u, gb, k,
-                                lmin, lst, n, svInner, oz, hz, ep, rp, reach, h2, gm, ctl);
+                                lmin, lst, fcs, n, svInner, oz, hz, ep, rp, reach, h2, gm, ctl);
         else
-          k_sv_smooth<BW, T><<<listGrid(n), G::WG, 0, stream>>>(gb, k, lmin, lst, n, svInner, oz, hz, ep, rp, reach,
-                                                                h2, gm, ctl);
+          k_sv_smooth<BW, T><<<listGrid(n), G::WG, 0, stream>>>(gb, k, lmin, lst, fcs, n, svInner, oz, hz, ep, rp,
+                                                                reach, h2, gm, ctl);
         DCG_TRY(checkLaunch("k_sv_smooth"));
         DCG_TRY(pushPending(&ev));
       }
@@ -782,7 +790,8 @@ struct DCGridSim : SimBase {
       if (n0 + n1)
         DCG_LAUNCH(DCG_K_SOLVE, "k_sv_restrict",
                    (k_sv_restrict<BW, T, false><<<listGrid(n0 + n1), G::WG, 0, stream>>>(
-                       g, k, lmin, listOf(lmin, 0), listOf(lmin, 1), n0, n1, e, rho, reachBuf, levelH2(k),
+                       g, k, lmin, listOf(lmin, 0), listOf(lmin, 1), facesOf(lmin, 0), facesOf(lmin, 1), n0, n1, e,
+                       rho, reachBuf, levelH2(k),
                        ghostFactor(k - lmin), nullptr, nullptr, UDimArg{}, ctl)));
     }
     DCG_TRY(smoothBlocks<T>(top, lmin, nu, true, ctl));
@@ -791,7 +800,8 @@ struct DCGridSim : SimBase {
       T* ur = (T*)urhoBuf;
       DCG_LAUNCH(DCG_K_SOLVE, "k_sv_restrict",
                  (k_sv_restrict<BW, T, true><<<slotGrid<BW>(maxL[top]), G::WG, 0, stream>>>(
-                     g, top, lmin, nullptr, nullptr, 0u, 0u, e, rho, reachBuf, levelH2(top), ghostFactor(top - lmin),
+                     g, top, lmin, nullptr, nullptr, nullptr, nullptr, 0u, 0u, e, rho, reachBuf, levelH2(top),
+                     ghostFactor(top - lmin),
                      ur, uactBuf, pyr[0], ctl)));
       // grids 0..jd-1 by the multi-workgroup kernels, jd.. by the coarse kernel (or, when even
       // the coarsest grid exceeds its LDS, many smoothing passes on that grid)
@@ -939,7 +949,7 @@ struct DCGridSim : SimBase {
     {
       unsigned n = blocksFor(maxL[0], 256);
       for (int l = 1; l < L; l++) n = std::max(n, blocksFor(maxL[l], 256));
-      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_lists", (k_sv_lists<BW><<<std::min(n, 1024u), 256, 0, stream>>>(g, breachBuf, svList, svCount)));
+      DCG_LAUNCH(DCG_K_SOLVE, "k_sv_lists", (k_sv_lists<BW><<<std::min(n, 1024u), 256, 0, stream>>>(g, breachBuf, svList, svFaces, svCount)));
       DCG_LAUNCH(DCG_K_SOLVE, "k_sv_mirror_counts",
                  (k_sv_mirror_counts<<<1, 1, 0, stream>>>(svCount, svCountMirrorDev, 2 * L * L)));
     }

@@ -63,13 +63,8 @@ __device__ __forceinline__ uint32_t parentCellOf(const DGBase& g, uint32_t b, in
 
 // borderApronCell with the level shift of the descriptor (ND_OUT outside the domain).
 template <int BW>
-__device__ __forceinline__ uint32_t borderApronCellShift(const DGBase& g, uint32_t b, int ai, int aj, int ak,
-                                                         uint32_t& sh) {
+__device__ __forceinline__ uint32_t borderApronCellShiftD(uint32_t b, int ai, int aj, int ak, int4 d, uint32_t& sh) {
   constexpr int BB = Geo<BW>::BB, BV = Geo<BW>::BV;
-  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
-  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
-  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
-  const int4 d = g.nd[b * 27 + dirIndex(dx, dy, dz)];
   sh = ndShift(d);
   const int ci = clampi(ai, 1, BW), cj = clampi(aj, 1, BW), ck = clampi(ak, 1, BW);
   const uint32_t spread = loc<BB>(ci & (BW - 1), cj & (BW - 1), ck & (BW - 1));
@@ -78,6 +73,34 @@ __device__ __forceinline__ uint32_t borderApronCellShift(const DGBase& g, uint32
   const uint32_t s = sh == ND_OUT ? 0u : sh;
   const uint32_t inCell = ndBlock(d) * BV + loc<BB>((ai - 1 - d.y) >> s, (aj - 1 - d.z) >> s, (ak - 1 - d.w) >> s);
   return sh == ND_OUT ? outCell : inCell;
+}
+template <int BW>
+__device__ __forceinline__ uint32_t borderApronCellShift(const DGBase& g, uint32_t b, int ai, int aj, int ak,
+                                                         uint32_t& sh) {
+  const int dx = ai == 0 ? -1 : (ai == BW + 1 ? 1 : 0);
+  const int dy = aj == 0 ? -1 : (aj == BW + 1 ? 1 : 0);
+  const int dz = ak == 0 ? -1 : (ak == BW + 1 ? 1 : 0);
+  return borderApronCellShiftD<BW>(b, ai, aj, ak, g.nd[b * 27 + dirIndex(dx, dy, dz)], sh);
+}
+// Neighbour-descriptor direction of block face `face` (0..5: x-, x+, y-, y+, z-, z+; the face
+// numbering of faceCell).
+__device__ __forceinline__ int faceDir(int face) {
+  const int s = (face & 1) ? 1 : -1, axis = face >> 1;
+  return dirIndex(axis == 0 ? s : 0, axis == 1 ? s : 0, axis == 2 ? s : 0);
+}
+// The face descriptors a thread of a block's staging needs (its face cells t, t + BV, ..): from
+// the six listed beside the block's list entry `entry` when `faces` is given (k_sv_lists), else
+// from the block's own descriptors. Neither load depends on the block index of a list.
+template <int BW>
+__device__ __forceinline__ void svFaceDescs(const DGBase& g, const int4* __restrict__ faces, uint32_t entry, uint32_t b,
+                                            int t, int4* dd) {
+  constexpr int BV = Geo<BW>::BV, FACE = Geo<BW>::FACE, FPT = (FACE + BV - 1) / BV;
+#pragma unroll
+  for (int j = 0; j < FPT; j++) {
+    const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
+    const int face = fc / (BW * BW);
+    dd[j] = faces ? faces[(size_t)entry * 6 + face] : g.nd[b * 27 + faceDir(face)];
+  }
 }
 
 // Block origin and colour (parity of the block coordinates) of block b of `level`.
@@ -127,19 +150,28 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_reach(DGBase g, int level, u
 
 // Active blocks of the V-cycle grids of every level solve: for lmin <= k, the blocks of level k
 // with breach <= lmin, per colour c, at list[(lmin * 2 + c) * B + offs[k] + i] for
-// i < count[(lmin * L + k) * 2 + c] (2 L B entries in all). The order inside a list follows the atomic increments and
-// does not matter: the blocks of one list are independent.
+// i < count[(lmin * L + k) * 2 + c] (2 L B entries in all), and beside each entry the block's six
+// face descriptors (faces[entry * 6 + face]), so that a kernel walking a list reads a block's
+// index and its faces together. The order inside a list follows the atomic increments and does
+// not matter: the blocks of one list are independent.
 template <int BW>
 __global__ void __launch_bounds__(256) k_sv_lists(DGBase g, const uint8_t* __restrict__ breach,
-                                                  uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+                                                  uint32_t* __restrict__ list, int4* __restrict__ faces,
+                                                  uint32_t* __restrict__ count) {
   for (int k = 0; k < g.L; k++) {
     const uint32_t n = g.ctl[CTL_LOADS + k];
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
       const uint32_t b = g.offs[k] + i;
       const int c = blockColour<BW>(svOrigin<BW>(g, k, b));
+      int4 fd[6];
+#pragma unroll
+      for (int f = 0; f < 6; f++) fd[f] = g.nd[b * 27 + faceDir(f)];
       for (int lmin = breach[b]; lmin <= k; lmin++) {
         const uint32_t slot = atomicAdd(&count[(lmin * g.L + k) * 2 + c], 1u);
-        list[(size_t)(lmin * 2 + c) * g.B + g.offs[k] + slot] = b;
+        const size_t entry = (size_t)(lmin * 2 + c) * g.B + g.offs[k] + slot;
+        list[entry] = b;
+#pragma unroll
+        for (int f = 0; f < 6; f++) faces[entry * 6 + f] = fd[f];
       }
     }
   }
@@ -163,7 +195,7 @@ enum : int { SV_NEU = -1, SV_GHOST = -2 };
 template <int BW, class T>
 __device__ __forceinline__ void svStageBlock(const DGBase& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
                                              const uint8_t* __restrict__ reach, bool ownZero, bool haloZero,
-                                             bool valid, T* V, int16_t* K) {
+                                             bool valid, T* V, int16_t* K, const int4* dd) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE, BB = Geo<BW>::BB;
   constexpr int FPT = (FACE + BV - 1) / BV;
   const uint32_t c = b * BV + t;
@@ -174,7 +206,7 @@ __device__ __forceinline__ void svStageBlock(const DGBase& g, uint32_t b, int t,
   for (int j = 0; j < FPT; j++) {
     const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
     faceCell<BW>(fc, ai[j], aj[j], ak[j]);
-    nb[j] = borderApronCellShift<BW>(g, b, ai[j], aj[j], ak[j], sh[j]);
+    nb[j] = borderApronCellShiftD<BW>(b, ai[j], aj[j], ak[j], dd[j], sh[j]);
   }
   const int rc = reach[c];
   const T vc = ownZero ? (T)0 : e[c];
@@ -234,15 +266,14 @@ __device__ __forceinline__ T svCoupling(const int16_t* K, int a, T gm, int idx[6
 // units (R / h^2); on the solved level itself the pending mean correction delta is taken from it.
 template <int BW, class T>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_smooth(DGBase g, int level, int lmin, const uint32_t* __restrict__ list,
-                                                         uint32_t count, int nu, int ownZero, int haloZero,
-                                                         T* __restrict__ e, const T* __restrict__ rho,
-                                                         const uint8_t* __restrict__ reach, float h2, float gm,
-                                                         const SolveCtl* __restrict__ ctl) {
+                                                         const int4* __restrict__ faces, uint32_t count, int nu,
+                                                         int ownZero, int haloZero, T* __restrict__ e,
+                                                         const T* __restrict__ rho, const uint8_t* __restrict__ reach,
+                                                         float h2, float gm, const SolveCtl* __restrict__ ctl) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
+  constexpr int FPT = (Geo<BW>::FACE + BV - 1) / BV;
   __shared__ T Vs[BPW][A3];
   __shared__ int16_t Ks[BPW][A3];
-  if (solveDone(ctl)) return;
-  const T delta = level == lmin ? (T)ctl->delta : (T)0;
   const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
   const int a = apronIndexOfLocal<BW>(t);
   int lx, ly, lz;
@@ -250,13 +281,23 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_smooth(DGBase g, int level, 
   const int par = (lx + ly + lz) & 1;
   T* V = Vs[sub];
   int16_t* K = Ks[sub];
+  T delta = (T)0;
+  bool first = true;
   for (uint32_t base = blockIdx.x * BPW; base < count; base += gridDim.x * BPW) {
     const uint32_t i = base + sub;
     const bool work = i < count;
-    const uint32_t b = list[g.offs[level] + (work ? i : count - 1)];
+    const uint32_t entry = g.offs[level] + (work ? i : count - 1);
+    const uint32_t b = list[entry];
+    int4 dd[FPT];
+    svFaceDescs<BW>(g, faces, entry, b, t, dd);
+    if (first) {  // the control block, read while the list loads are in flight
+      if (solveDone(ctl)) return;
+      delta = level == lmin ? (T)ctl->delta : (T)0;
+      first = false;
+    }
     __syncthreads();  // the previous block's tile is no longer read
     const T r = work ? rho[b * BV + t] - delta : (T)0;
-    svStageBlock<BW, T>(g, b, t, lmin, e, reach, ownZero != 0, haloZero != 0, work, V, K);
+    svStageBlock<BW, T>(g, b, t, lmin, e, reach, ownZero != 0, haloZero != 0, work, V, K, dd);
     __syncthreads();
     int idx[6];
     const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
@@ -281,31 +322,42 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_smooth(DGBase g, int level, 
 // cells of level + 1.
 template <int BW, class T, bool TOP>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_restrict(DGBase g, int level, int lmin, const uint32_t* __restrict__ list0,
-                                                           const uint32_t* __restrict__ list1, uint32_t n0, uint32_t n1,
+                                                           const uint32_t* __restrict__ list1,
+                                                           const int4* __restrict__ faces0,
+                                                           const int4* __restrict__ faces1, uint32_t n0, uint32_t n1,
                                                            const T* __restrict__ e, T* __restrict__ rho,
                                                            const uint8_t* __restrict__ reach, float h2, float gm,
                                                            T* __restrict__ urho, uint8_t* __restrict__ uact,
                                                            UDimArg D1, const SolveCtl* __restrict__ ctl) {
   constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, A3 = Geo<BW>::A3, BPW = Geo<BW>::BPW, BB = Geo<BW>::BB;
+  constexpr int FPT = (Geo<BW>::FACE + BV - 1) / BV;
   __shared__ T Vs[BPW][A3];
   __shared__ int16_t Ks[BPW][A3];
   __shared__ T Rs[BPW][BV];
-  if (solveDone(ctl)) return;
-  const T delta = level == lmin ? (T)ctl->delta : (T)0;
   const uint32_t total = TOP ? g.ctl[CTL_LOADS + level] : n0 + n1;
   const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
   const int a = apronIndexOfLocal<BW>(t);
   T* V = Vs[sub];
   int16_t* K = Ks[sub];
+  T delta = (T)0;
+  bool first = true;
   for (uint32_t base = blockIdx.x * BPW; base < total; base += gridDim.x * BPW) {
     const uint32_t i = base + sub;
     bool work = i < total;
     const uint32_t j = work ? i : total - 1;
-    const uint32_t b = TOP ? g.offs[level] + j : (j < n0 ? list0[g.offs[level] + j] : list1[g.offs[level] + j - n0]);
+    const uint32_t entry = TOP ? 0u : g.offs[level] + (j < n0 ? j : j - n0);
+    const uint32_t b = TOP ? g.offs[level] + j : (j < n0 ? list0[entry] : list1[entry]);
+    int4 dd[FPT];
+    svFaceDescs<BW>(g, TOP ? nullptr : (j < n0 ? faces0 : faces1), entry, b, t, dd);
+    if (first) {  // the control block, read while the list loads are in flight
+      if (solveDone(ctl)) return;
+      delta = level == lmin ? (T)ctl->delta : (T)0;
+      first = false;
+    }
     if (!TOP && work) work = g.par[b] != NFU;
     __syncthreads();
     const T r = work ? rho[b * BV + t] - delta : (T)0;
-    svStageBlock<BW, T>(g, b, t, lmin, e, reach, false, false, work, V, K);
+    svStageBlock<BW, T>(g, b, t, lmin, e, reach, false, false, work, V, K, dd);
     __syncthreads();
     int idx[6];
     const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
@@ -580,10 +632,25 @@ __device__ __forceinline__ void cCoord(const UDimArg& D, uint32_t i, int& x, int
   y = (int)(q % (uint32_t)D.h);
   z = (int)(q / (uint32_t)D.h);
 }
+// Position of the calling lane's entry in a list counted by *cnt (LDS) for the lanes whose
+// `take` is set: one atomic per wavefront (the lanes below in the wavefront come first).
+__device__ __forceinline__ uint32_t cWaveAppend(uint32_t* cnt, bool take) {
+  const unsigned long long m = __ballot(take);
+  const int leader = __ffsll((long long)__ballot(1)) - 1;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  uint32_t base = 0;
+  if ((int)lane == leader && m != 0ull) base = atomicAdd(cnt, (uint32_t)__popcll(m));
+  return (uint32_t)__shfl((int)base, leader, 64) + below;
+}
 // fl: bits 0-5 neighbour d (x-, x+, y-, y+, z-, z+) coupled (inside the grid and active), bit 6
 // parity, bit 7 active; inv = 1 / diagonal (walls Neumann, inactive neighbours ghosts gm e_c).
+// The active cells of each parity are listed in lst (the grid's range of the LDS arrays: parity
+// 0 from its start, parity 1 from its end; cnt[0], cnt[1] their numbers, zero on entry). The
+// order within a list does not matter: a half-sweep's cells read only cells of the other parity.
 template <class T>
-__device__ __forceinline__ void cPrepare(const UDimArg& D, const uint8_t* act, uint8_t* fl, T* inv, T gm) {
+__device__ __forceinline__ void cPrepare(const UDimArg& D, const uint8_t* act, uint8_t* fl, T* inv, T gm,
+                                         uint16_t* lst, uint32_t* cnt) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     int x, y, z;
@@ -602,10 +669,15 @@ __device__ __forceinline__ void cPrepare(const UDimArg& D, const uint8_t* act, u
       }
     }
     const uint32_t u = D.off + i;
-    f |= (uint32_t)((x + y + z) & 1) << 6;
-    f |= act[u] ? 0x80u : 0u;
+    const uint32_t par = (uint32_t)((x + y + z) & 1);
+    const bool on = act[u] != 0;
+    f |= par << 6;
+    f |= on ? 0x80u : 0u;
     fl[u] = (uint8_t)f;
     inv[u] = (T)1 / diag;
+    const uint32_t p0 = cWaveAppend(&cnt[0], on && par == 0);
+    const uint32_t p1 = cWaveAppend(&cnt[1], on && par == 1);
+    if (on) lst[par ? D.off + n - 1 - p1 : D.off + p0] = (uint16_t)u;
   }
 }
 // Sum of the coupled neighbours of LDS cell u: all six loads issued unconditionally (indices
@@ -624,23 +696,64 @@ __device__ __forceinline__ T cNeighbourSum(const UDimArg& D, uint32_t u, uint32_
   sum += (f & 32u) ? a5 : (T)0;
   return sum;
 }
+// Red-black sweeps of one grid over its parity lists: each thread keeps its (at most kCSlots)
+// cells of each parity in registers (LDS index, flags, h^2 rho, inverse diagonal), so a
+// half-sweep is one round of LDS loads for all of them, then the stores.
+static constexpr int kCSlots = 3;
+static_assert(kCSlots * kCoarseThreads >= (kCoarseCells + 1) / 2, "a parity of a coarse grid exceeds the slots");
 template <class T>
-__device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, const uint8_t* fl, const T* inv, T h2,
-                                        int sweeps) {
+struct CParity {
+  uint32_t u[kCSlots], f[kCSlots];
+  T hr[kCSlots], iv[kCSlots];
+  int ns;  // slots in use (the same in every thread): loads are issued for these only
+  __device__ __forceinline__ void load(const UDimArg& D, uint32_t n, uint32_t nt, int par, uint32_t count,
+                                       const uint16_t* lst, const uint8_t* fl, const T* rho, const T* inv, T h2) {
+    ns = (int)((count + nt - 1) / nt);
+#pragma unroll
+    for (int k = 0; k < kCSlots; k++) {
+      const uint32_t j = threadIdx.x + (uint32_t)k * nt;
+      const bool ok = j < count;
+      const uint32_t li = ok ? (par ? D.off + n - 1 - j : D.off + j) : D.off;
+      const uint32_t uu = ok ? (uint32_t)lst[li] : D.off;
+      u[k] = uu;
+      f[k] = ok ? (uint32_t)fl[uu] : 0u;
+      hr[k] = h2 * rho[uu];
+      iv[k] = inv[uu];
+    }
+  }
+  template <int N>
+  __device__ __forceinline__ void sweepN(const UDimArg& D, T* e) const {
+    T v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = (cNeighbourSum<T>(D, u[k], f[k], e) + hr[k]) * iv[k];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+      if (f[k] & 0x80u) e[u[k]] = v[k];
+  }
+  __device__ __forceinline__ void sweep(const UDimArg& D, T* e) const {
+    static_assert(kCSlots == 3, "one case per slot count");
+    if (ns == 1)
+      sweepN<1>(D, e);
+    else if (ns == 2)
+      sweepN<2>(D, e);
+    else if (ns >= 3)
+      sweepN<3>(D, e);
+  }
+};
+template <class T>
+__device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, const uint8_t* fl, const T* inv,
+                                        const uint16_t* lst, const uint32_t* cnt, T h2, int sweeps) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
   const bool wave = n <= 64;  // one wavefront, ordered LDS accesses: no workgroup barrier
   if (wave && threadIdx.x >= 64) {
     __syncthreads();
     return;
   }
-  for (int s = 0; s < 2 * sweeps; s++) {
-    const uint32_t colour = (uint32_t)(s & 1);
-    for (uint32_t i = threadIdx.x; i < n; i += (wave ? 64u : blockDim.x)) {
-      const uint32_t u = D.off + i;
-      const uint32_t f = fl[u];
-      const T v = (cNeighbourSum<T>(D, u, f, e) + h2 * rho[u]) * inv[u];
-      if ((f & 0x80u) && ((f >> 6) & 1u) == colour) e[u] = v;
-    }
+  const uint32_t nt = wave ? 64u : blockDim.x;
+  CParity<T> p0, p1;
+  p0.load(D, n, nt, 0, cnt[0], lst, fl, rho, inv, h2);
+  p1.load(D, n, nt, 1, cnt[1], lst, fl, rho, inv, h2);
+  auto sync = [&]() {
     if (wave) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -648,6 +761,12 @@ __device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, co
     } else {
       __syncthreads();
     }
+  };
+  for (int s = 0; s < sweeps; s++) {
+    p0.sweep(D, e);
+    sync();
+    p1.sweep(D, e);
+    sync();
   }
   if (wave) __syncthreads();
 }
@@ -660,6 +779,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T*
   __shared__ T inv[kCoarseCells];
   __shared__ uint8_t act[kCoarseCells];
   __shared__ uint8_t fl[kCoarseCells];
+  __shared__ uint16_t lst[kCoarseCells];
+  __shared__ uint32_t cnt[kMaxPyr][2];
   if (solveDone(ctl)) return;
   const UDimArg D0 = ca.D[0];
   const uint32_t n0 = (uint32_t)D0.w * D0.h * D0.d;
@@ -668,13 +789,14 @@ __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T*
     rho[i] = urho[ca.goff + i];
     act[i] = uact[ca.goff + i];
   }
+  if (threadIdx.x < 2 * kMaxPyr) cnt[threadIdx.x / 2][threadIdx.x % 2] = 0u;
   __syncthreads();
   for (int j = 0; j + 1 < ca.n; j++) {
     const UDimArg F = ca.D[j], Cg = ca.D[j + 1];
     const T h2j = (T)ca.h2[j];
-    cPrepare<T>(F, act, fl, inv, (T)ca.gm[j]);
+    cPrepare<T>(F, act, fl, inv, (T)ca.gm[j], lst, cnt[j]);
     __syncthreads();
-    cSweeps<T>(F, e, rho, fl, inv, h2j, ca.sweeps);
+    cSweeps<T>(F, e, rho, fl, inv, lst, cnt[j], h2j, ca.sweeps);
     // residual restricted into the next grid (mean over the 2x2x2 children, inactive ones 0)
     const uint32_t nc = (uint32_t)Cg.w * Cg.h * Cg.d;
     const T rh2 = (T)1 / h2j;
@@ -701,9 +823,9 @@ __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T*
   {
     const int l = ca.n - 1;
     const UDimArg Dl = ca.D[l];
-    cPrepare<T>(Dl, act, fl, inv, (T)ca.gm[l]);
+    cPrepare<T>(Dl, act, fl, inv, (T)ca.gm[l], lst, cnt[l]);
     __syncthreads();
-    cSweeps<T>(Dl, e, rho, fl, inv, (T)ca.h2[l], ca.bottom);
+    cSweeps<T>(Dl, e, rho, fl, inv, lst, cnt[l], (T)ca.h2[l], ca.bottom);
   }
   for (int j = ca.n - 2; j >= 0; j--) {
     const UDimArg F = ca.D[j], Cg = ca.D[j + 1];
@@ -716,7 +838,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T*
       e[u] = e[u] + uProlong<T>(Cg, x, y, z, F.w, F.h, F.d, e, act);
     }
     __syncthreads();
-    cSweeps<T>(F, e, rho, fl, inv, (T)ca.h2[j], ca.sweeps);
+    cSweeps<T>(F, e, rho, fl, inv, lst, cnt[j], (T)ca.h2[j], ca.sweeps);
   }
   for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x)
     if (act[i]) ue[ca.goff + i] = e[i];

@@ -138,8 +138,8 @@ typedef struct dcg_params {
   float solve_tol;          /* per-level relative residual |r| / |r(0)| to reach (1e-8; <= 0: 1e-8) */
   int32_t solve_max_cycles; /* refinement cycles per level before giving up (50; <= 0: 50) */
   int32_t solve_smooth;     /* smoothing passes before and after each coarse correction, each
-                               two red-black Gauss-Seidel sweeps per block colour
-                               (2; <= 0: 2; at most 5) */
+                               four red-black Gauss-Seidel sweeps per block colour
+                               (1; <= 0: 1; at most 2) */
 
   /* Terrain and moist atmosphere (extension). */
   int32_t solid_mode;     /* DCG_SOLID_* */
