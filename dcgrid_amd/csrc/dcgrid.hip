@@ -811,7 +811,7 @@ struct DCGridSim : SimBase {
         const float s = (float)(1 << (L + j)), h2 = s * s * P.dx * P.dx;
         const UDimArg& Dc = pyr[j + 1];
         DCG_LAUNCH(DCG_K_SOLVE, "k_svu_restrict",
-                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d, 256), 256, 0, stream>>>(
+                   (k_svu_restrict<T><<<blocksFor((size_t)Dc.w * Dc.h * Dc.d * 8, 256), 256, 0, stream>>>(
                        pyr[j], Dc, ue, ur, uactBuf, h2, ghostFactor(L + j - lmin), ctl)));
       }
       if (jc < J) {

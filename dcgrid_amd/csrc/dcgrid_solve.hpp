@@ -390,34 +390,63 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_restrict(DGBase g, int level
 // e += P e(parent level) on the active cells of the listed blocks of `level` (both colours):
 // the 27/9/3/1 trilinear stencil of k_prolongate over the parent's apron, with any stencil cell
 // that is not an active cell of the parent level (coarser, or outside the region being solved)
-// replaced by the own parent cell.
+// replaced by the own parent cell. As in k_prolongate, the (BW/2 + 2)^3 region of the parent's
+// apron around the block's octant is staged once per block (one apron lookup, level and reach
+// check per entry) and every cell takes its eight stencil values from LDS.
 template <int BW, class T>
 __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_prolong(DGBase g, int level, int lmin, const uint32_t* __restrict__ list0,
                                                           const uint32_t* __restrict__ list1, uint32_t n0, uint32_t n1,
                                                           T* __restrict__ e, const uint8_t* __restrict__ reach,
                                                           const SolveCtl* __restrict__ ctl) {
-  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW;
-  if (solveDone(ctl)) return;
+  constexpr int BV = Geo<BW>::BV, BB = Geo<BW>::BB, BPW = Geo<BW>::BPW, H = BW / 2 + 2, H3 = H * H * H;
+  static_assert(H3 <= BV, "a block's threads stage the parent region");
+  __shared__ T vals[BPW][H3];
+  __shared__ uint8_t oks[BPW][H3];
   const uint32_t total = n0 + n1;
   const int sub = threadIdx.x / BV, t = threadIdx.x % BV;
   int lx, ly, lz;
   locPos<BB>((uint32_t)t, lx, ly, lz);
+  bool first = true;
   for (uint32_t base = blockIdx.x * BPW; base < total; base += gridDim.x * BPW) {
     const uint32_t j = base + sub;
-    if (j >= total) continue;
-    const uint32_t b = j < n0 ? list0[g.offs[level] + j] : list1[g.offs[level] + j - n0];
+    const bool work = j < total;
+    const uint32_t jj = work ? j : total - 1;
+    const uint32_t b = jj < n0 ? list0[g.offs[level] + jj] : list1[g.offs[level] + jj - n0];
+    if (first) {  // the control block, read while the list load is in flight
+      if (solveDone(ctl)) return;
+      first = false;
+    }
+    __syncthreads();  // the previous block's region is no longer read
+    const uint32_t ps = work ? g.par[b] : NFU;
     const uint32_t c = b * BV + t;
-    const uint32_t ps = g.par[b];
-    if (reach[c] > lmin || ps == NFU) continue;
-    const int3 p0 = g.pos[b];
-    uint32_t q[8];
-    parentStencil<BW>(g, ps, p0.x | lx, p0.y | ly, p0.z | lz, q);
+    const int rc = reach[c];
+    int3 p0 = make_int3(0, 0, 0);
+    int ox = 0, oy = 0, oz = 0;  // parent apron coordinates of the region's first entry
+    if (ps != NFU) {
+      p0 = g.pos[b];
+      ox = (p0.x / 2) % BW;
+      oy = (p0.y / 2) % BW;
+      oz = (p0.z / 2) % BW;
+      if (t < H3) {
+        const int i = t / (H * H), jy = (t / H) % H, k = t % H;
+        const uint32_t q = apronCell<BW>(g, ps / 8, ox + i, oy + jy, oz + k);
+        oks[sub][t] = g.lev[q / BV] == level + 1 && reach[q] <= lmin;
+        vals[sub][t] = e[q];
+      }
+    }
+    __syncthreads();
+    if (ps == NFU || rc > lmin) continue;
+    const int x = p0.x | lx, y = p0.y | ly, z = p0.z | lz;
+    // parentStencil's apron coordinates, relative to the staged region
+    const int ti = 1 + (x / 2) % BW - ox, tj = 1 + (y / 2) % BW - oy, tk = 1 + (z / 2) % BW - oz;
+    const int si = x % 2 ? 1 : -1, sj = y % 2 ? 1 : -1, sk = z % 2 ? 1 : -1;
+    auto at = [&](int i, int jy, int k) { return (i * H + jy) * H + k; };
+    const int r[8] = {at(ti, tj, tk),           at(ti, tj, tk + sk),      at(ti, tj + sj, tk),
+                      at(ti + si, tj, tk),      at(ti, tj + sj, tk + sk), at(ti + si, tj, tk + sk),
+                      at(ti + si, tj + sj, tk), at(ti + si, tj + sj, tk + sk)};
     T v[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const bool ok = g.lev[q[k] / BV] == level + 1 && reach[q[k]] <= lmin;
-      v[k] = e[ok ? q[k] : q[0]];
-    }
+    for (int m = 0; m < 8; m++) v[m] = vals[sub][oks[sub][r[m]] ? r[m] : r[0]];
     e[c] = e[c] + ((T)27 * v[0] + (T)9 * (v[1] + v[2] + v[3]) + (T)3 * (v[4] + v[5] + v[6]) + v[7]) / (T)64;
   }
 }
@@ -571,30 +600,63 @@ __global__ void __launch_bounds__(512) k_svu_smooth(UDimArg D, int colour, int n
   }
 }
 
-// Residual of pyramid grid D restricted into the next grid Dc (thread per coarse cell).
+// Residual of pyramid grid D restricted into the next grid Dc: a thread per fine cell (8
+// consecutive lanes per coarse cell, child order 000 .. 111), every load of a cell issued at
+// once; the group's first lane sums the active children in child order, as a loop over the
+// children would.
 template <class T>
 __global__ void __launch_bounds__(256) k_svu_restrict(UDimArg D, UDimArg Dc, const T* __restrict__ e,
                                                       T* __restrict__ rho, uint8_t* __restrict__ act, float h2, float gm,
                                                       const SolveCtl* __restrict__ ctl) {
   if (solveDone(ctl)) return;
   const uint32_t n = (uint32_t)Dc.w * Dc.h * Dc.d;
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int X = (int)(i % Dc.w), Y = (int)((i / Dc.w) % Dc.h), Z = (int)(i / ((uint32_t)Dc.w * Dc.h));
-  const T inv = (T)1 / (T)h2;
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t i = t >> 3;  // the same for the 8 lanes of a group
+  const int j = (int)(t & 7u);
+  const bool ok = i < n;
+  const uint32_t ic = ok ? i : 0u;
+  const int X = (int)(ic % Dc.w), Y = (int)((ic / Dc.w) % Dc.h), Z = (int)(ic / ((uint32_t)Dc.w * Dc.h));
+  const int x = 2 * X + ((j >> 2) & 1), y = 2 * Y + ((j >> 1) & 1), z = 2 * Z + (j & 1);
+  const uint32_t u = uAt(D, x, y, z);
+  const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
+  bool in[6];
+  uint8_t na[6];
+  T ne[6];
+#pragma unroll
+  for (int d = 0; d < 6; d++) {
+    in[d] = !(nx[d] < 0 || ny[d] < 0 || nz[d] < 0 || nx[d] >= D.w || ny[d] >= D.h || nz[d] >= D.d);
+    const uint32_t q = in[d] ? uAt(D, nx[d], ny[d], nz[d]) : u;
+    na[d] = act[q];
+    ne[d] = e[q];
+  }
+  const bool a = ok && act[u] != 0;
+  const T ec = e[u], rc = rho[u];
+  // uStencil's sum and diagonal, in its order
+  T sum = (T)0, diag = (T)6;
+#pragma unroll
+  for (int d = 0; d < 6; d++) {
+    if (!in[d])
+      diag -= (T)1;
+    else if (na[d])
+      sum += ne[d];
+    else
+      diag -= (T)gm;
+  }
+  const T r = a ? rc - (diag * ec - sum) * ((T)1 / (T)h2) : (T)0;
+  const int base = (int)(threadIdx.x & ~7u);
   T acc = (T)0;
   bool any = false;
-  for (int j = 0; j < 8; j++) {
-    const int x = 2 * X + ((j >> 2) & 1), y = 2 * Y + ((j >> 1) & 1), z = 2 * Z + (j & 1);
-    const uint32_t u = uAt(D, x, y, z);
-    if (!act[u]) continue;
-    any = true;
-    T sum, diag;
-    uStencil<T>(D, x, y, z, e, act, (T)gm, sum, diag);
-    acc += rho[u] - (diag * e[u] - sum) * inv;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const T rk = __shfl(r, base + k, 64);
+    const int ak = __shfl(a ? 1 : 0, base + k, 64);
+    if (ak) acc += rk;
+    any |= ak != 0;
   }
-  rho[Dc.off + i] = acc * (T).125;
-  act[Dc.off + i] = any ? 1 : 0;
+  if (ok && j == 0) {
+    rho[Dc.off + i] = acc * (T).125;
+    act[Dc.off + i] = any ? 1 : 0;
+  }
 }
 
 template <class T>
