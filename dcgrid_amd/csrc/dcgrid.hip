@@ -632,6 +632,9 @@ struct DCGridSim : SimBase {
   SolveCtl* ctlMirrorDev = nullptr;  // the device address of that copy
   std::vector<UDimArg> pyr;    // pyramid grids below the top level (cell size doubling)
   int pyrCoarse = 0;           // first pyramid grid of the single-workgroup coarse kernel
+  uint8_t* coarseCache = nullptr;  // per level solve: what the coarse kernel derives from the activity
+  uint32_t* coarseTags = nullptr;  // the solve each slot belongs to
+  uint32_t solveSerial = 0;        // tag of the current projection solve (never 0)
   dcg_solve_info rep{};
   int predCycles[DCG_MAX_LEVELS] = {};  // cycles the last solve of each level took
   bool warmValid = false;  // the pressure field holds a projection's solution (warm start)
@@ -690,6 +693,9 @@ struct DCGridSim : SimBase {
     DCG_TRY(alloc(&ueBuf, off + 1));
     DCG_TRY(alloc(&urhoBuf, off + 1));
     DCG_TRY(alloc(&uactBuf, off + 1));
+    DCG_TRY(alloc(&coarseCache, (size_t)L * kCoarseCacheBytes));
+    DCG_TRY(alloc(&coarseTags, (size_t)L));
+    DCG_TRY(fill32(coarseTags, 0u, (size_t)L));
     return DCG_OK;
   }
   // Ghost factor of a Dirichlet face on the grid m levels coarser than the solved level: the
@@ -830,8 +836,11 @@ struct DCGridSim : SimBase {
         // coarsest grid: 8 sweeps converge 2^3 (as 40 did), flat grids such as 8x2x8 need 40
         const UDimArg& Db = pyr[J - 1];
         ca.bottom = std::max(Db.w, std::max(Db.h, Db.d)) <= 2 ? 8 : 40;
+        ca.slot = lmin;
+        ca.tag = solveSerial;
         DCG_LAUNCH(DCG_K_SOLVE, "k_svu_coarse",
-                   (k_svu_coarse<T><<<1, kCoarseThreads, 0, stream>>>(ca, ue, ur, uactBuf, ctl)));
+                   (k_svu_coarse<T><<<1, kCoarseThreads, 0, stream>>>(ca, ue, ur, uactBuf, ctl, coarseCache,
+                                                                       coarseTags)));
       } else {
         for (int i = 0; i < 20; i++) DCG_TRY(smoothPyr<T>(jd, lmin, nu, i == 0, ctl));
       }
@@ -959,6 +968,7 @@ struct DCGridSim : SimBase {
   }
   int projectSolve() override {
     DCG_TRY(solveSetup());
+    if (++solveSerial == 0) solveSerial = 1;  // the coarse kernel's cached setup is per solve
     svPasses = smoothSweeps();
     svInner = kSvInner;
     rep = dcg_solve_info{};

@@ -687,6 +687,8 @@ struct CoarseArgs {
   float gm[kMaxPyr];
   uint32_t goff;        // offset of grid jc in the pyramid arrays
   int n, sweeps, bottom;
+  int slot;             // cache slot of the level solve (its lmin)
+  uint32_t tag;         // the solve's tag: a slot holding it is current
 };
 __device__ __forceinline__ void cCoord(const UDimArg& D, uint32_t i, int& x, int& y, int& z) {
   x = (int)(i % (uint32_t)D.w);
@@ -694,53 +696,103 @@ __device__ __forceinline__ void cCoord(const UDimArg& D, uint32_t i, int& x, int
   y = (int)(q % (uint32_t)D.h);
   z = (int)(q / (uint32_t)D.h);
 }
-// Position of the calling lane's entry in a list counted by *cnt (LDS) for the lanes whose
-// `take` is set: one atomic per wavefront (the lanes below in the wavefront come first).
-__device__ __forceinline__ uint32_t cWaveAppend(uint32_t* cnt, bool take) {
-  const unsigned long long m = __ballot(take);
-  const int leader = __ffsll((long long)__ballot(1)) - 1;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-  uint32_t base = 0;
-  if ((int)lane == leader && m != 0ull) base = atomicAdd(cnt, (uint32_t)__popcll(m));
-  return (uint32_t)__shfl((int)base, leader, 64) + below;
-}
 // fl: bits 0-5 neighbour d (x-, x+, y-, y+, z-, z+) coupled (inside the grid and active), bit 6
 // parity, bit 7 active; inv = 1 / diagonal (walls Neumann, inactive neighbours ghosts gm e_c).
 // The active cells of each parity are listed in lst (the grid's range of the LDS arrays: parity
 // 0 from its start, parity 1 from its end; cnt[0], cnt[1] their numbers, zero on entry). The
 // order within a list does not matter: a half-sweep's cells read only cells of the other parity.
+// A thread takes cells threadIdx.x + k * blockDim.x, k < K (K = ceil(n / blockDim.x)): the loads
+// of all its cells are issued together, and each wavefront reserves its list entries of all K
+// rounds with one LDS atomic per parity.
+template <class T, int K>
+__device__ __forceinline__ void cPrepareK(const UDimArg& D, const uint8_t* act, uint8_t* fl, T* inv, T gm,
+                                          uint16_t* lst, uint32_t* cnt) {
+  const uint32_t n = (uint32_t)D.w * D.h * D.d;
+  uint32_t f[K];
+  T diag[K];
+  bool valid[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint32_t i = threadIdx.x + (uint32_t)k * blockDim.x;
+    valid[k] = i < n;
+    int x, y, z;
+    cCoord(D, valid[k] ? i : 0u, x, y, z);
+    const uint32_t u = D.off + (valid[k] ? i : 0u);
+    const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
+    bool in[6];
+    uint8_t na[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      in[d] = !(nx[d] < 0 || ny[d] < 0 || nz[d] < 0 || nx[d] >= D.w || ny[d] >= D.h || nz[d] >= D.d);
+      na[d] = act[in[d] ? uAt(D, nx[d], ny[d], nz[d]) : u];
+    }
+    const uint8_t self = act[u];
+    uint32_t fk = 0;
+    T dg = (T)6;
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      if (!in[d]) {
+        dg -= (T)1;
+      } else if (na[d]) {
+        fk |= 1u << d;
+      } else {
+        dg -= gm;
+      }
+    }
+    fk |= (uint32_t)((x + y + z) & 1) << 6;
+    fk |= (valid[k] && self) ? 0x80u : 0u;
+    f[k] = fk;
+    diag[k] = dg;
+  }
+  const uint32_t lane = threadIdx.x & 63u;
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int par = 0; par < 2; par++) {
+    unsigned long long m[K];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      m[k] = __ballot((f[k] & 0x80u) && ((f[k] >> 6) & 1u) == (uint32_t)par);
+      tot += (uint32_t)__popcll(m[k]);
+    }
+    uint32_t base = 0;
+    if (tot && lane == (uint32_t)(__ffsll((long long)__ballot(1)) - 1)) base = atomicAdd(&cnt[par], tot);
+    base = __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint32_t i = threadIdx.x + (uint32_t)k * blockDim.x;
+      if ((m[k] >> lane) & 1ull) {
+        const uint32_t pos = base + (uint32_t)__popcll(m[k] & below);
+        lst[par ? D.off + n - 1 - pos : D.off + pos] = (uint16_t)(D.off + i);
+      }
+      base += (uint32_t)__popcll(m[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    if (!valid[k]) continue;
+    const uint32_t u = D.off + threadIdx.x + (uint32_t)k * blockDim.x;
+    fl[u] = (uint8_t)f[k];
+    inv[u] = (T)1 / diag[k];
+  }
+}
+static constexpr int kCPrepRounds = (kCoarseCells + kCoarseThreads - 1) / kCoarseThreads;
+static_assert(kCPrepRounds == 5, "one case per round count");
 template <class T>
 __device__ __forceinline__ void cPrepare(const UDimArg& D, const uint8_t* act, uint8_t* fl, T* inv, T gm,
                                          uint16_t* lst, uint32_t* cnt) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    int x, y, z;
-    cCoord(D, i, x, y, z);
-    const int nx[6] = {x - 1, x + 1, x, x, x, x}, ny[6] = {y, y, y - 1, y + 1, y, y}, nz[6] = {z, z, z, z, z - 1, z + 1};
-    uint32_t f = 0;
-    T diag = (T)6;
-#pragma unroll
-    for (int d = 0; d < 6; d++) {
-      if (nx[d] < 0 || ny[d] < 0 || nz[d] < 0 || nx[d] >= D.w || ny[d] >= D.h || nz[d] >= D.d) {
-        diag -= (T)1;
-      } else if (act[uAt(D, nx[d], ny[d], nz[d])]) {
-        f |= 1u << d;
-      } else {
-        diag -= gm;
-      }
-    }
-    const uint32_t u = D.off + i;
-    const uint32_t par = (uint32_t)((x + y + z) & 1);
-    const bool on = act[u] != 0;
-    f |= par << 6;
-    f |= on ? 0x80u : 0u;
-    fl[u] = (uint8_t)f;
-    inv[u] = (T)1 / diag;
-    const uint32_t p0 = cWaveAppend(&cnt[0], on && par == 0);
-    const uint32_t p1 = cWaveAppend(&cnt[1], on && par == 1);
-    if (on) lst[par ? D.off + n - 1 - p1 : D.off + p0] = (uint16_t)u;
-  }
+  const uint32_t rounds = (n + blockDim.x - 1) / blockDim.x;
+  if (rounds <= 1)
+    cPrepareK<T, 1>(D, act, fl, inv, gm, lst, cnt);
+  else if (rounds == 2)
+    cPrepareK<T, 2>(D, act, fl, inv, gm, lst, cnt);
+  else if (rounds == 3)
+    cPrepareK<T, 3>(D, act, fl, inv, gm, lst, cnt);
+  else if (rounds == 4)
+    cPrepareK<T, 4>(D, act, fl, inv, gm, lst, cnt);
+  else
+    cPrepareK<T, 5>(D, act, fl, inv, gm, lst, cnt);
 }
 // Sum of the coupled neighbours of LDS cell u: all six loads issued unconditionally (indices
 // clamped into the arrays) and selected by the flags, so a cell costs one round of LDS latency.
@@ -758,21 +810,25 @@ __device__ __forceinline__ T cNeighbourSum(const UDimArg& D, uint32_t u, uint32_
   sum += (f & 32u) ? a5 : (T)0;
   return sum;
 }
-// Red-black sweeps of one grid over its parity lists: each thread keeps its (at most kCSlots)
-// cells of each parity in registers (LDS index, flags, h^2 rho, inverse diagonal), so a
-// half-sweep is one round of LDS loads for all of them, then the stores.
+// Red-black sweeps of one grid over its parity lists: each thread keeps its (at most S) cells of
+// each parity in registers (LDS index, flags, h^2 rho, inverse diagonal), so a half-sweep is one
+// round of LDS loads for all of them, then the stores. A grid whose parities have at most
+// kCWaveSlots<T> * 64 active cells each is swept by the first wavefront alone, with wavefront
+// ordering instead of workgroup barriers (two slots in FP64 keep the kernel within its registers).
 static constexpr int kCSlots = 3;
-static_assert(kCSlots * kCoarseThreads >= (kCoarseCells + 1) / 2, "a parity of a coarse grid exceeds the slots");
 template <class T>
+constexpr int kCWaveSlots = sizeof(T) == 4 ? 4 : 2;
+static_assert(kCSlots * kCoarseThreads >= (kCoarseCells + 1) / 2, "a parity of a coarse grid exceeds the slots");
+template <class T, int S>
 struct CParity {
-  uint32_t u[kCSlots], f[kCSlots];
-  T hr[kCSlots], iv[kCSlots];
+  uint32_t u[S], f[S];
+  T hr[S], iv[S];
   int ns;  // slots in use (the same in every thread): loads are issued for these only
   __device__ __forceinline__ void load(const UDimArg& D, uint32_t n, uint32_t nt, int par, uint32_t count,
                                        const uint16_t* lst, const uint8_t* fl, const T* rho, const T* inv, T h2) {
     ns = (int)((count + nt - 1) / nt);
 #pragma unroll
-    for (int k = 0; k < kCSlots; k++) {
+    for (int k = 0; k < S; k++) {
       const uint32_t j = threadIdx.x + (uint32_t)k * nt;
       const bool ok = j < count;
       const uint32_t li = ok ? (par ? D.off + n - 1 - j : D.off + j) : D.off;
@@ -793,26 +849,23 @@ struct CParity {
       if (f[k] & 0x80u) e[u[k]] = v[k];
   }
   __device__ __forceinline__ void sweep(const UDimArg& D, T* e) const {
-    static_assert(kCSlots == 3, "one case per slot count");
+    static_assert(S >= 2 && S <= 4, "one case per slot count");
     if (ns == 1)
       sweepN<1>(D, e);
-    else if (ns == 2)
+    else if (ns == 2 || (S == 2 && ns > 2))
       sweepN<2>(D, e);
-    else if (ns >= 3)
-      sweepN<3>(D, e);
+    else if (ns == 3 || (S == 3 && ns > 3))
+      sweepN<(S < 3 ? S : 3)>(D, e);
+    else if (ns >= 4)
+      sweepN<S>(D, e);
   }
 };
-template <class T>
-__device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, const uint8_t* fl, const T* inv,
-                                        const uint16_t* lst, const uint32_t* cnt, T h2, int sweeps) {
+template <class T, int S>
+__device__ __forceinline__ void cSweepLoop(const UDimArg& D, T* e, const T* rho, const uint8_t* fl, const T* inv,
+                                           const uint16_t* lst, const uint32_t* cnt, T h2, int sweeps, bool wave) {
   const uint32_t n = (uint32_t)D.w * D.h * D.d;
-  const bool wave = n <= 64;  // one wavefront, ordered LDS accesses: no workgroup barrier
-  if (wave && threadIdx.x >= 64) {
-    __syncthreads();
-    return;
-  }
   const uint32_t nt = wave ? 64u : blockDim.x;
-  CParity<T> p0, p1;
+  CParity<T, S> p0, p1;
   p0.load(D, n, nt, 0, cnt[0], lst, fl, rho, inv, h2);
   p1.load(D, n, nt, 1, cnt[1], lst, fl, rho, inv, h2);
   auto sync = [&]() {
@@ -830,34 +883,89 @@ __device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, co
     p1.sweep(D, e);
     sync();
   }
-  if (wave) __syncthreads();
+}
+template <class T>
+__device__ __forceinline__ void cSweeps(const UDimArg& D, T* e, const T* rho, const uint8_t* fl, const T* inv,
+                                        const uint16_t* lst, const uint32_t* cnt, T h2, int sweeps) {
+  const bool wave = max(cnt[0], cnt[1]) <= (uint32_t)(kCWaveSlots<T> * 64);
+  if (wave) {
+    if (threadIdx.x < 64) cSweepLoop<T, kCWaveSlots<T>>(D, e, rho, fl, inv, lst, cnt, h2, sweeps, true);
+    __syncthreads();
+  } else {
+    cSweepLoop<T, kCSlots>(D, e, rho, fl, inv, lst, cnt, h2, sweeps, false);
+  }
+}
+// What a coarse call computes from the activity alone (couplings, inverse diagonals, parity
+// lists, the activity of every grid) is the same in every cycle of a level solve: the first
+// call of a solve stores that part of its LDS in a slot of device memory tagged with the solve,
+// later calls copy it back (16-byte loads, all issued at once) instead of recomputing it.
+template <class T>
+struct alignas(16) CoarseLds {
+  T e[kCoarseCells];
+  T rho[kCoarseCells];
+  // the cached image: from inv to the end
+  T inv[kCoarseCells];
+  uint8_t act[kCoarseCells];
+  uint8_t fl[kCoarseCells];
+  uint16_t lst[kCoarseCells];
+  uint32_t cnt[kMaxPyr][2];
+};
+template <class T>
+constexpr size_t kCoarseImage = sizeof(T) * kCoarseCells + (size_t)kCoarseCells * 4 + sizeof(uint32_t) * 2 * kMaxPyr;
+static constexpr size_t kCoarseCacheBytes = (kCoarseImage<double> + 255) & ~(size_t)255;
+static_assert(kCoarseImage<float> % 16 == 0 && kCoarseImage<double> % 16 == 0, "cached image in 16-byte units");
+template <class T>
+__device__ __forceinline__ void cCopyImage(const uint4* __restrict__ src, uint4* __restrict__ dst) {
+  constexpr int N = (int)(kCoarseImage<T> / 16), R = (N + kCoarseThreads - 1) / kCoarseThreads;
+  uint4 v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = (int)threadIdx.x + r * kCoarseThreads;
+    if (i < N) v[r] = src[i];
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = (int)threadIdx.x + r * kCoarseThreads;
+    if (i < N) dst[i] = v[r];
+  }
 }
 template <class T>
 __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T* __restrict__ ue, const T* __restrict__ urho,
                                                                const uint8_t* __restrict__ uact,
-                                                               const SolveCtl* __restrict__ ctl) {
-  __shared__ T e[kCoarseCells];
-  __shared__ T rho[kCoarseCells];
-  __shared__ T inv[kCoarseCells];
-  __shared__ uint8_t act[kCoarseCells];
-  __shared__ uint8_t fl[kCoarseCells];
-  __shared__ uint16_t lst[kCoarseCells];
-  __shared__ uint32_t cnt[kMaxPyr][2];
+                                                               const SolveCtl* __restrict__ ctl,
+                                                               uint8_t* __restrict__ cache, uint32_t* __restrict__ tags) {
+  __shared__ CoarseLds<T> S;
+  static_assert(offsetof(CoarseLds<T>, inv) % 16 == 0 &&
+                    offsetof(CoarseLds<T>, cnt) + sizeof(S.cnt) - offsetof(CoarseLds<T>, inv) == kCoarseImage<T>,
+                "the cached fields are one contiguous image");
+  T* e = S.e;
+  T* rho = S.rho;
+  T* inv = S.inv;
+  uint8_t* act = S.act;
+  uint8_t* fl = S.fl;
+  uint16_t* lst = S.lst;
+  uint32_t(*cnt)[2] = S.cnt;
+  const uint32_t tagNow = tags[ca.slot];
   if (solveDone(ctl)) return;
+  const bool hit = tagNow == ca.tag;
+  uint4* img = (uint4*)(cache + (size_t)ca.slot * kCoarseCacheBytes);
   const UDimArg D0 = ca.D[0];
   const uint32_t n0 = (uint32_t)D0.w * D0.h * D0.d;
+  if (hit) cCopyImage<T>(img, (uint4*)inv);
   for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x) {
     e[i] = (T)0;
     rho[i] = urho[ca.goff + i];
-    act[i] = uact[ca.goff + i];
+    if (!hit) act[i] = uact[ca.goff + i];
   }
-  if (threadIdx.x < 2 * kMaxPyr) cnt[threadIdx.x / 2][threadIdx.x % 2] = 0u;
+  if (!hit && threadIdx.x < 2 * kMaxPyr) cnt[threadIdx.x / 2][threadIdx.x % 2] = 0u;
   __syncthreads();
   for (int j = 0; j + 1 < ca.n; j++) {
     const UDimArg F = ca.D[j], Cg = ca.D[j + 1];
     const T h2j = (T)ca.h2[j];
-    cPrepare<T>(F, act, fl, inv, (T)ca.gm[j], lst, cnt[j]);
-    __syncthreads();
+    if (!hit) {
+      cPrepare<T>(F, act, fl, inv, (T)ca.gm[j], lst, cnt[j]);
+      __syncthreads();
+    }
     cSweeps<T>(F, e, rho, fl, inv, lst, cnt[j], h2j, ca.sweeps);
     // residual restricted into the next grid (mean over the 2x2x2 children, inactive ones 0)
     const uint32_t nc = (uint32_t)Cg.w * Cg.h * Cg.d;
@@ -878,15 +986,17 @@ __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T*
       }
       rho[Cg.off + i] = acc * (T).125;
       e[Cg.off + i] = (T)0;
-      act[Cg.off + i] = any ? 1 : 0;
+      if (!hit) act[Cg.off + i] = any ? 1 : 0;
     }
     __syncthreads();
   }
   {
     const int l = ca.n - 1;
     const UDimArg Dl = ca.D[l];
-    cPrepare<T>(Dl, act, fl, inv, (T)ca.gm[l], lst, cnt[l]);
-    __syncthreads();
+    if (!hit) {
+      cPrepare<T>(Dl, act, fl, inv, (T)ca.gm[l], lst, cnt[l]);
+      __syncthreads();
+    }
     cSweeps<T>(Dl, e, rho, fl, inv, lst, cnt[l], (T)ca.h2[l], ca.bottom);
   }
   for (int j = ca.n - 2; j >= 0; j--) {
@@ -904,6 +1014,10 @@ __global__ void __launch_bounds__(kCoarseThreads) k_svu_coarse(CoarseArgs ca, T*
   }
   for (uint32_t i = threadIdx.x; i < n0; i += blockDim.x)
     if (act[i]) ue[ca.goff + i] = e[i];
+  if (!hit) {
+    cCopyImage<T>((const uint4*)inv, img);
+    if (threadIdx.x == 0) tags[ca.slot] = ca.tag;
+  }
 }
 
 // ---------------------------------------------------------------- refinement loop of a level
