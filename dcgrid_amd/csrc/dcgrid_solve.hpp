@@ -20,11 +20,11 @@
 // sweeps there. One smoothing pass is two launches (one per colour) instead of two launches per
 // sweep, and the result does not depend on the order in which workgroups run.
 //
-// The refinement loop runs on the device: the residual kernel's last workgroup sums the
-// partial norms, decides convergence (or the stall of the working precision) and publishes it
-// in the level's SolveCtl; every later kernel of the level's loop returns at once when the
-// level is done, so the host enqueues cycles ahead and reads the control block once it expects
-// the level to have converged.
+// The refinement loop runs on the device: the residual kernel writes per-workgroup partial
+// norms, and the next launch (k_sv_decide, one workgroup) sums them in workgroup order, decides
+// convergence (or the stall of the working precision) and publishes it in the level's SolveCtl;
+// every later kernel of the level's loop returns at once when the level is done, so the host
+// enqueues cycles ahead and reads the control block once it expects the level to have converged.
 #pragma once
 
 #include "dcgrid_fluid.hpp"
@@ -46,8 +46,7 @@ struct SolveCtl {
   double delta;   // (mean - shift) / h^2 still to take from rho on the level's cells
   double mean;    // mean of x after the last update (normalisation of a singular level)
   uint32_t cycles, stall, done, singular;
-  uint32_t ticket;  // arrivals of the residual's workgroups (last one reduces)
-  uint32_t pad[3];
+  uint32_t pad[4];
 };
 static_assert(sizeof(SolveCtl) == 80, "SolveCtl layout");
 
