@@ -111,6 +111,59 @@ def test_fixed_point_of_the_reference_jacobi_sweep():
         assert err <= 3e-7  # measured about 6e-8 on every level
 
 
+def test_full_size_config2_small_regions():
+    """Configuration 2 at its full 512^3 size, where sparse levels' regions (the top-level blocks
+    holding their cells) cover at most 1/16 of the top level, so their V-cycles end at the top
+    level without the pyramid: every level still reaches 1e-8, and the solution is (to FP32
+    rounding) a fixed point of the reference Jacobi half-sweep on every level."""
+    w = scenes.config(2)
+    p = w.params.copy()
+    p.project_mode = PROJECT_SOLVE
+    p.solve_precision = SOLVE_MIXED
+    p.solve_tol = 1e-8
+    say("[case] config 2 at 512^3, 6 solved steps")
+    g = dcgrid_amd.Simulation(w.kind, w.size, p, w.max_num_blocks)
+    g.seed_scene()
+    g.step(6)
+    inf = g.info()
+    t = g.export_tables()
+    lev, par = t["block_levels"], t["parent_indices"]
+    L, top = inf.levels, inf.levels - 1
+    full = [(512 // (8 << l)) ** 3 for l in range(L)]
+    loads = list(inf.block_loads)[:L]
+    none = np.uint64(0xFFFFFFFFFFFFFFFF)
+    small = []
+    for l in range(top):
+        if loads[l] >= full[l]:
+            continue  # dense level
+        b = np.nonzero(lev == l)[0]
+        for _ in range(top - l):
+            b = (par[b] // 8).astype(np.int64)
+        tops = np.unique(b)
+        say("sparse level", l, "region covers", tops.size, "of", full[top], "top-level blocks")
+        if tops.size * 16 <= full[top]:
+            small.append(l)
+    assert small, "no sparse level with a small region: the case does not exercise that cycle"
+    assert not np.any(par[lev == top] != none)
+    g.project_solve()
+    rep = g.solve_report()
+    say("cycles", rep["cycles"], "rel", rep["rel_residual"])
+    assert rep["converged"] and all(r <= 1e-8 for r in rep["rel_residual"])
+    pr = g.export_field("pressure")
+    for level in range(L):
+        g.run_stage(STAGE_JACOBI, level)
+        tp = g.export_field("t_pressure")
+        m = level_cells(g, level)
+        d = tp[m].astype(np.float64) - pr[m]
+        scale = np.linalg.norm(pr[m]) / np.sqrt(m.sum())
+        if rep["singular"][level]:
+            d = d - d.mean()
+        err = np.sqrt(np.mean(d * d)) / scale
+        say("level", level, "small region" if level in small else "", "jacobi change", err)
+        assert err <= 3e-7
+    g.close()
+
+
 def test_precision_variants():
     """FP64 throughout also reaches 1e-8 and agrees with the mixed solve; FP32 throughout stops
     where its arithmetic runs out (reported, not converged at 1e-8) with a larger error."""
