@@ -424,8 +424,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the sharding design is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported "
+                                   "(the step has no multi-GPU path)"}))
         sys.exit(2)
 
     import dcgrid_amd

@@ -370,7 +370,7 @@ def test_gpu_error_paths():
     p = scenes.reference_params(256)
     p.max_levels = 3
     with pytest.raises(dcgrid_amd.DCGError) as e:
-        dcgrid_amd.Simulation("dcgrid", (256, 256, 256), p, 4096)  # BW=4: survey M1
+        dcgrid_amd.Simulation("dcgrid", (256, 256, 256), p, 4096)  # BW 4, 3 levels: 4096 blocks are too few
     assert e.value.code == -2
     p.block_width = 6
     with pytest.raises(dcgrid_amd.DCGError) as e:

@@ -1,6 +1,7 @@
-"""Pins of the CPU oracle against facts fixed independently of its code: the survey's sizing
-KATs, textbook Morton order, brute-force geometry on tiny grids, closed forms of the stencils and
-transfer operators, and structural invariants of the block tables. CPU only.
+"""Pins of the CPU oracle against facts fixed independently of its code: known-answer tests of
+the level and budget sizing, textbook Morton order, brute-force geometry on tiny grids, closed
+forms of the stencils and transfer operators, and structural invariants of the block tables. CPU
+only.
 """
 import numpy as np
 import pytest
@@ -52,7 +53,7 @@ def test_planner_errors():
     p = scenes.reference_params(256)
     p.block_width = 4
     p.max_levels = 3
-    with pytest.raises(DCGError) as e:  # survey M1: 3 levels of 4^3 blocks need 4096 coarse blocks
+    with pytest.raises(DCGError) as e:  # 3 levels of 4^3 blocks at 256^3 need more than 4096 blocks
         oracle.plan((256, 256, 256), 4096, p)
     assert e.value.code == DCG_E_BUDGET
     p.block_width = 8
