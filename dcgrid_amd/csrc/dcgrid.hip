@@ -803,11 +803,12 @@ struct DCGridSim : SimBase {
     DCG_TRY(smoothBlocks<T>(top, lmin, nu, true, ctl));
     // The pyramid below the top level serves a sparse level's solve only where the level's region
     // covers a sizeable part of the top level. Where the region's ancestors are at most 1/16 of the
-    // top level's blocks, the top level closes the cycle with two more smoothing passes instead
-    // (about 9 launches and the coarse kernel fewer per cycle). Measured (cycles per step, ms per
-    // step): configuration 2 at 512^3, whose sparse regions cover 5 % of the top level, 34.8 ->
-    // 34.4, 6.8 -> 5.8 ms; where a region covered 1/8 of the top level (configuration 1, its level
-    // 1) its solve took 25 cycles without the pyramid instead of 13, so the pyramid stays there.
+    // top level's blocks, the cycle turns at the top level after its smoothing pass (about 9
+    // launches and the coarse kernel fewer per cycle). Measured (cycles per step, ms per step):
+    // configuration 2 at 512^3, whose sparse regions cover 5 % of the top level, 34.8 -> 33.8,
+    // 6.8 -> 5.2 ms (two more passes at the top level took 34.4 cycles, 5.7 ms); where a region
+    // covered 1/8 of the top level (configuration 1, its level 1) its solve took 25 cycles without
+    // the pyramid instead of 13, so the pyramid stays there.
     const uint32_t topActive = countOf(lmin, top, 0) + countOf(lmin, top, 1);
     const bool smallRegion = lmin < g.S && (size_t)topActive * 16 <= (size_t)maxL[top];
     if (!pyr.empty() && !smallRegion) {
@@ -867,9 +868,10 @@ struct DCGridSim : SimBase {
                                                                              ctl)));
       DCG_TRY(smoothBlocks<T>(top, lmin, nu, false, ctl));
     } else {
-      // no pyramid (a top level too small to halve) or a small region: more passes on the top level
-      const int passes = pyr.empty() ? 20 : 3;
-      for (int i = 1; i < passes; i++) DCG_TRY(smoothBlocks<T>(top, lmin, nu, false, ctl));
+      // no pyramid (a top level too small to halve): the top level is the bottom grid, smoothed
+      // 19 more passes; a small region: nothing more
+      if (pyr.empty())
+        for (int i = 1; i < 20; i++) DCG_TRY(smoothBlocks<T>(top, lmin, nu, false, ctl));
     }
     for (int k = top - 1; k >= lmin; k--) {
       const uint32_t n0 = countOf(lmin, k, 0), n1 = countOf(lmin, k, 1);
