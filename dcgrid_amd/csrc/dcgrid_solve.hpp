@@ -102,6 +102,18 @@ __device__ __forceinline__ void svFaceDescs(const DGBase& g, const int4* __restr
   }
 }
 
+// svFaceDescs for a listed block: the descriptors beside its list entry (no load depends on the
+// block index).
+template <int BW>
+__device__ __forceinline__ void svFaceDescsListed(const int4* __restrict__ faces, uint32_t entry, int t, int4* dd) {
+  constexpr int BV = Geo<BW>::BV, FACE = Geo<BW>::FACE, FPT = (FACE + BV - 1) / BV;
+#pragma unroll
+  for (int j = 0; j < FPT; j++) {
+    const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
+    dd[j] = faces[(size_t)entry * 6 + fc / (BW * BW)];
+  }
+}
+
 // Block origin and colour (parity of the block coordinates) of block b of `level`.
 template <int BW>
 __device__ __forceinline__ int3 svOrigin(const DGBase& g, int level, uint32_t b) {
@@ -189,37 +201,48 @@ __global__ void k_sv_mirror_counts(const uint32_t* __restrict__ count, uint32_t*
 // diagonal loses gm). Tile entry 0 (a corner, never a face) holds 0 for uncoupled neighbours.
 enum : int { SV_NEU = -1, SV_GHOST = -2 };
 
-// Stages the correction e of block b around local cell t: V (values as T), K (codes), and for
-// the calling thread its own cell's reach and rho. Loads of a thread are issued together.
+// What a thread loads to stage block b around its local cell t: its own cell's correction and
+// reach, and for each of its face cells the neighbour cell, its level shift, reach and correction.
 template <int BW, class T>
-__device__ __forceinline__ void svStageBlock(const DGBase& g, uint32_t b, int t, int lmin, const T* __restrict__ e,
-                                             const uint8_t* __restrict__ reach, bool ownZero, bool haloZero,
-                                             bool valid, T* V, int16_t* K, const int4* dd) {
-  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE, BB = Geo<BW>::BB;
-  constexpr int FPT = (FACE + BV - 1) / BV;
-  const uint32_t c = b * BV + t;
-  const int a = apronIndexOfLocal<BW>(t);
+struct SvStage {
+  static constexpr int FPT = (Geo<BW>::FACE + Geo<BW>::BV - 1) / Geo<BW>::BV;
   uint32_t nb[FPT], sh[FPT];
-  int ai[FPT], aj[FPT], ak[FPT];
+  int rc, rn[FPT];
+  T vc, vn[FPT];
+};
+
+// The loads of svStageBlock, all issued together (no LDS access).
+template <int BW, class T>
+__device__ __forceinline__ void svLoadBlock(uint32_t b, int t, const T* __restrict__ e, const uint8_t* __restrict__ reach,
+                                            bool ownZero, bool haloZero, const int4* dd, SvStage<BW, T>& S) {
+  constexpr int BV = Geo<BW>::BV, FACE = Geo<BW>::FACE, FPT = SvStage<BW, T>::FPT;
+  const uint32_t c = b * BV + t;
 #pragma unroll
   for (int j = 0; j < FPT; j++) {
     const int fc = t + j * BV < FACE ? t + j * BV : FACE - 1;
-    faceCell<BW>(fc, ai[j], aj[j], ak[j]);
-    nb[j] = borderApronCellShiftD<BW>(b, ai[j], aj[j], ak[j], dd[j], sh[j]);
+    int ai, aj, ak;
+    faceCell<BW>(fc, ai, aj, ak);
+    S.nb[j] = borderApronCellShiftD<BW>(b, ai, aj, ak, dd[j], S.sh[j]);
   }
-  const int rc = reach[c];
-  const T vc = ownZero ? (T)0 : e[c];
-  int rn[FPT];
-  T vn[FPT];
+  S.rc = reach[c];
+  S.vc = ownZero ? (T)0 : e[c];
 #pragma unroll
   for (int j = 0; j < FPT; j++) {
-    rn[j] = reach[nb[j]];
-    vn[j] = haloZero ? (T)0 : e[nb[j]];
+    S.rn[j] = reach[S.nb[j]];
+    S.vn[j] = haloZero ? (T)0 : e[S.nb[j]];
   }
-  __builtin_amdgcn_sched_barrier(0);
+}
+
+// The LDS half of svStageBlock: V (values as T) and K (codes) of the block's tile from what
+// svLoadBlock loaded.
+template <int BW, class T>
+__device__ __forceinline__ void svWriteBlock(uint32_t b, int t, int lmin, bool valid, T* V, int16_t* K,
+                                             const SvStage<BW, T>& S) {
+  constexpr int BV = Geo<BW>::BV, A = Geo<BW>::A, FACE = Geo<BW>::FACE, BB = Geo<BW>::BB, FPT = SvStage<BW, T>::FPT;
   if (!valid) return;
-  V[a] = vc;
-  K[a] = rc <= lmin ? (int16_t)a : (int16_t)SV_GHOST;
+  const int a = apronIndexOfLocal<BW>(t);
+  V[a] = S.vc;
+  K[a] = S.rc <= lmin ? (int16_t)a : (int16_t)SV_GHOST;
   if (t == 0) {
     V[0] = (T)0;
     K[0] = (int16_t)SV_GHOST;
@@ -227,21 +250,35 @@ __device__ __forceinline__ void svStageBlock(const DGBase& g, uint32_t b, int t,
 #pragma unroll
   for (int j = 0; j < FPT; j++) {
     if (t + j * BV >= FACE) continue;
-    const int fa = (ai[j] * A + aj[j]) * A + ak[j];
+    int ai, aj, ak;
+    faceCell<BW>(t + j * BV, ai, aj, ak);
+    const int fa = (ai * A + aj) * A + ak;
     int code;
-    if (sh[j] == ND_OUT) {
+    if (S.sh[j] == ND_OUT) {
       // the clamp onto an own cell: the asking cell itself (Neumann) or another own cell
-      const uint32_t own = nb[j] - b * BV;
-      const uint32_t ask = loc<BB>(clampi(ai[j], 1, BW) - 1, clampi(aj[j], 1, BW) - 1, clampi(ak[j], 1, BW) - 1);
-      code = own == ask ? SV_NEU : (rn[j] <= lmin ? apronIndexOfLocal<BW>((int)own) : SV_GHOST);
-    } else if (sh[j] != 0) {
+      const uint32_t own = S.nb[j] - b * BV;
+      const uint32_t ask = loc<BB>(clampi(ai, 1, BW) - 1, clampi(aj, 1, BW) - 1, clampi(ak, 1, BW) - 1);
+      code = own == ask ? SV_NEU : (S.rn[j] <= lmin ? apronIndexOfLocal<BW>((int)own) : SV_GHOST);
+    } else if (S.sh[j] != 0) {
       code = SV_GHOST;
     } else {
-      code = rn[j] <= lmin ? fa : SV_GHOST;
-      V[fa] = vn[j];
+      code = S.rn[j] <= lmin ? fa : SV_GHOST;
+      V[fa] = S.vn[j];
     }
     K[fa] = (int16_t)code;
   }
+}
+
+// Stages the correction e of block b around local cell t: V (values as T), K (codes). Loads of
+// a thread are issued together.
+template <int BW, class T>
+__device__ __forceinline__ void svStageBlock(uint32_t b, int t, int lmin, const T* __restrict__ e,
+                                             const uint8_t* __restrict__ reach, bool ownZero, bool haloZero,
+                                             bool valid, T* V, int16_t* K, const int4* dd) {
+  SvStage<BW, T> S;
+  svLoadBlock<BW, T>(b, t, e, reach, ownZero, haloZero, dd, S);
+  __builtin_amdgcn_sched_barrier(0);
+  svWriteBlock<BW, T>(b, t, lmin, valid, V, K, S);
 }
 
 // Neighbour tile entries (uncoupled ones point at the zero entry) and diagonal of the cell at
@@ -280,28 +317,29 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_smooth(DGBase g, int level, 
   const int par = (lx + ly + lz) & 1;
   T* V = Vs[sub];
   int16_t* K = Ks[sub];
-  T delta = (T)0;
-  bool first = true;
-  for (uint32_t base = blockIdx.x * BPW; base < count; base += gridDim.x * BPW) {
-    const uint32_t i = base + sub;
-    const bool work = i < count;
-    const uint32_t entry = g.offs[level] + (work ? i : count - 1);
-    const uint32_t b = list[entry];
-    int4 dd[FPT];
-    svFaceDescs<BW>(g, faces, entry, b, t, dd);
-    if (first) {  // the control block, read while the list loads are in flight
-      if (solveDone(ctl)) return;
-      delta = level == lmin ? (T)ctl->delta : (T)0;
-      first = false;
-    }
-    __syncthreads();  // the previous block's tile is no longer read
-    const T r = work ? rho[b * BV + t] - delta : (T)0;
-    svStageBlock<BW, T>(g, b, t, lmin, e, reach, ownZero != 0, haloZero != 0, work, V, K, dd);
-    __syncthreads();
+  const uint32_t stride = gridDim.x * BPW;
+  uint32_t base = blockIdx.x * BPW;
+  if (base >= count) return;
+  // The loads of a block are issued before the sweeps of the block before it (same colour: the
+  // blocks of one launch never read each other's cells, so the early loads see the same values).
+  // Nothing is computed from the loaded values before the sweeps, so no wait for them precedes
+  // the sweeps; a workgroup's last block runs a copy of the sweeps without loads.
+  uint32_t i = base + sub;
+  bool work = i < count;
+  uint32_t entry = g.offs[level] + (work ? i : count - 1);
+  uint32_t b = list[entry];
+  int4 dd[FPT];
+  svFaceDescsListed<BW>(faces, entry, t, dd);
+  if (solveDone(ctl)) return;  // the control block, read while the list loads are in flight
+  const T delta = level == lmin ? (T)ctl->delta : (T)0;
+  T rv = rho[b * BV + t];
+  SvStage<BW, T> S;
+  svLoadBlock<BW, T>(b, t, e, reach, ownZero != 0, haloZero != 0, dd, S);
+  // nu red-black sweeps of the staged block cb and the store of its active cells
+  const auto sweep = [&](uint32_t cb, bool cwork, T hr) {
     int idx[6];
     const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
-    const bool act = work && K[a] >= 0;
-    const T hr = (T)h2 * r;
+    const bool act = cwork && K[a] >= 0;
     const T inv = (T)1 / diag;
     for (int s = 0; s < nu; s++) {
 #pragma unroll
@@ -310,7 +348,28 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_smooth(DGBase g, int level, 
         __syncthreads();
       }
     }
-    if (act) e[b * BV + t] = V[a];
+    if (act) e[cb * BV + t] = V[a];
+  };
+  for (;;) {
+    __syncthreads();  // the previous block's tile is no longer read
+    svWriteBlock<BW, T>(b, t, lmin, work, V, K, S);
+    __syncthreads();
+    const uint32_t cb = b;
+    const bool cwork = work;
+    const T hr = (T)h2 * (cwork ? rv - delta : (T)0);
+    base += stride;
+    if (base >= count) {  // the workgroup's last block: nothing to load
+      sweep(cb, cwork, hr);
+      break;
+    }
+    i = base + sub;
+    work = i < count;
+    entry = g.offs[level] + (work ? i : count - 1);
+    b = list[entry];
+    svFaceDescsListed<BW>(faces, entry, t, dd);
+    rv = rho[b * BV + t];
+    svLoadBlock<BW, T>(b, t, e, reach, ownZero != 0, haloZero != 0, dd, S);
+    sweep(cb, cwork, hr);
   }
 }
 
@@ -356,7 +415,7 @@ __global__ void __launch_bounds__(Geo<BW>::WG) k_sv_restrict(DGBase g, int level
     if (!TOP && work) work = g.par[b] != NFU;
     __syncthreads();
     const T r = work ? rho[b * BV + t] - delta : (T)0;
-    svStageBlock<BW, T>(g, b, t, lmin, e, reach, false, false, work, V, K, dd);
+    svStageBlock<BW, T>(b, t, lmin, e, reach, false, false, work, V, K, dd);
     __syncthreads();
     int idx[6];
     const T diag = svCoupling<A, T>(K, a, (T)gm, idx);
